@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Benchmark of the force-simulation tick on one MI355X.
+
+A step is one tick (gravity -> many-body -> link IN -> link OUT, each with its
+integrate pass, then alpha decay) over the resident 1M-point / 5M-link graph.
+Prints one JSON line: whole-job ticks/s with inputs already in HBM, the
+roofline of the kernel that dominates the tick (timed with HIP events on the
+library's stream over the timed region), and the oracle's CPU tick timed on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "ticks/s at 1M pts/5M links; batched layouts/s + p50/p99 latency, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+VALU_FP32_PEAK_TFLOPS = 157.3
+
+# single-kernel stages whose events bracket exactly one kernel launch
+SINGLE_KERNEL = ("gravity", "mb_force", "mouse")
+
+
+def stage_work(stage: str, wl: dict, ctx_info: dict) -> tuple[str, float]:
+    """Algorithmic bytes (or flops) of one launch of a stage; see DESIGN.md."""
+    n, np_, L, gf = ctx_info["n"], ctx_info["np"], ctx_info["L"], ctx_info["gf"]
+    if stage in ("gravity", "center", "mouse"):
+        # present index 4, position in 8, flags 1, position out 8, key 4, value 4
+        return "hbm", 29.0 * np_
+    if stage == "mb_force":
+        # sorted index 4, sorted position 8, jitter 8, flags 1, position out 8 per point;
+        # flops: 20 per far-field cell evaluation (7 + 27*(L-1) per interior point)
+        # and 20 per sampled near-field pair (<= 72)
+        evals = 7 + 27 * (L - 1) + ctx_info.get("near_pairs", 72)
+        return "valu", 20.0 * evals * np_
+    if stage in ("link_in", "link_out"):
+        E = ctx_info["E_live"][0 if stage == "link_in" else 1]
+        # entry 16 + gathered neighbour position 8 per entry; per point 25
+        return "hbm", 24.0 * E + 25.0 * np_
+    return "hbm", 0.0
+
+
+def cpu_baseline(wl: dict, seconds: float) -> dict:
+    from oracle import oracle as O
+
+    p = wl["params"]
+    pd = dict(space_size=wl["S"], gravity=p.gravity, center=0.0, repulsion=p.repulsion,
+              link_spring=p.link_spring, link_distance=p.link_distance, lo=1.0, hi=1.2,
+              friction=p.friction, decay=p.decay)
+    Lk = O.Links(wl["n"], wl["links"], wl["r_in"], wl["r_out"])
+    pos, a = wl["xy"].astype(np.float64), 1.0
+    ticks = 0
+    t0 = time.perf_counter()
+    while True:
+        pos, a = O.tick(pos, a, 0.5 + 0.01 * ticks, pd, links=Lk, jitter=wl["jitter"])
+        ticks += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": ticks / el, "unit": "ticks/s", "cores": 1, "kind": "port",
+            "sample": f"{ticks} full ticks of the same 1M-point/5M-link workload through the FP64 "
+                      f"oracle (oracle/cosmos_oracle.c, 1 thread) in {el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
+              "(each graph or request would be assigned to one GPU; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from graph_amd import _lib, datagen, workloads
+
+    wl = workloads.cfg1()
+    n = wl["n"]
+    ctx = _lib.Context(device=0)
+    ctx.set_positions(wl["xy"])
+    ctx.set_jitter(wl["jitter"])
+    ctx.set_links(wl["links"], None, wl["r_in"], wl["r_out"])
+    ctx.set_alpha(1.0)
+    p = wl["params"]
+    seeds = datagen.uniform(args.warmup + args.steps + 1, 0.0, 1.0, 99)
+
+    # warmup with every stage bracketed: per-stage breakdown, picks the dominant kernel
+    ctx.set_profiling(_lib.STAGES)
+    ctx.step(p, args.warmup, seeds=seeds[: args.warmup])
+    breakdown = ctx.get_profile()
+    single = {s: v for s, v in breakdown.items() if s in SINGLE_KERNEL}
+    dominant = max(single, key=lambda s: single[s][0]) if single else "mb_force"
+
+    ctx.set_profiling([dominant])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(p, args.steps, seeds=seeds[args.warmup: args.warmup + args.steps])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timed = ctx.get_profile()
+    el = t1 - t0
+    ticks_per_s = args.steps / el
+
+    gf, L = 512, 8
+    info = dict(n=n, np=n, L=L, gf=gf, E_live=[int(wl["links"].size // 2)] * 2)
+    kind, work = stage_work(dominant, wl, info)
+    k_ms, k_cnt = timed.get(dominant, (0.0, 0))
+    k_avg_s = (k_ms / max(k_cnt, 1)) / 1000.0
+    if kind == "hbm":
+        achieved = work / k_avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    else:
+        achieved = work / k_avg_s / 1e12
+        roof = {"bound": "valu", "achieved": achieved, "peak": VALU_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / VALU_FP32_PEAK_TFLOPS, "traffic": None}
+    roof["kernel"] = {"gravity": "k_pointwise<GRAVITY>", "mb_force": "k_mb_force"}.get(dominant, dominant)
+    roof["avg_ms"] = k_avg_s * 1e3
+
+    out = {
+        "metric": METRIC,
+        "value": ticks_per_s,
+        "unit": "ticks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl["name"], "points": n, "links": int(wl["links"].size // 2),
+                   "space_size": wl["S"], "finest_grid": gf, "levels": L, "ticks_per_step": 1},
+        "point_ticks_per_s": ticks_per_s * n,
+        "stage_ms_per_tick": {s: v[0] / max(v[1], 1) for s, v in breakdown.items()},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,281 @@
+"""ctypes binding of ``include/cosmosgl.h`` (``graph_amd/libcosmosgl.so``).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+HEADER = PKG.parent / "include" / "cosmosgl.h"
+LIB_PATH = PKG / "libcosmosgl.so"
+
+CG_OK = 0
+CG_ERR_INVALID = -1
+CG_ERR_DEVICE = -2
+CG_ERR_BUDGET = -3
+CG_ERR_STATE = -4
+
+FORCE_GRAVITY = 0
+FORCE_CENTER = 1
+FORCE_MOUSE = 2
+FORCE_MANYBODY = 3
+FORCE_LINK_IN = 4
+FORCE_LINK_OUT = 5
+FORCE_COLLISION = 6
+
+
+class CgCreateParams(C.Structure):
+    _fields_ = [("device", C.c_int32), ("hbm_budget", C.c_uint64)]
+
+
+class CgSimParams(C.Structure):
+    _fields_ = [
+        ("space_size", C.c_float),
+        ("decay", C.c_float),
+        ("gravity", C.c_float),
+        ("center", C.c_float),
+        ("repulsion", C.c_float),
+        ("link_spring", C.c_float),
+        ("link_distance", C.c_float),
+        ("link_dist_random_variation_range", C.c_float * 2),
+        ("repulsion_from_mouse", C.c_float),
+        ("friction", C.c_float),
+        ("collision", C.c_float),
+        ("collision_radius", C.c_float),
+        ("collision_padding", C.c_float),
+        ("point_default_size", C.c_float),
+        ("enable_simulation", C.c_int32),
+        ("run_forces", C.c_int32),
+        ("mouse_repulsion", C.c_int32),
+        ("mouse_position", C.c_float * 2),
+    ]
+
+
+class CosmosGLError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cosmosgl error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+_F = C.POINTER(C.c_float)
+_U = C.POINTER(C.c_uint32)
+_I = C.POINTER(C.c_int32)
+_D = C.POINTER(C.c_double)
+_CTX = C.c_void_p
+
+_SIGNATURES = {
+    "cg_default_params": (None, [C.POINTER(CgSimParams)]),
+    "cg_create": (C.c_int, [C.POINTER(_CTX), C.POINTER(CgCreateParams)]),
+    "cg_destroy": (C.c_int, [_CTX]),
+    "cg_set_positions": (C.c_int, [_CTX, _F, C.c_uint32]),
+    "cg_set_links": (C.c_int, [_CTX, _F, C.c_uint32, _F, _F, _F]),
+    "cg_set_jitter": (C.c_int, [_CTX, _F, C.c_uint32]),
+    "cg_set_point_sizes": (C.c_int, [_CTX, _F, C.c_uint32]),
+    "cg_set_pinned": (C.c_int, [_CTX, _U, C.c_uint32]),
+    "cg_set_alpha": (C.c_int, [_CTX, C.c_double]),
+    "cg_get_alpha": (C.c_int, [_CTX, _D]),
+    "cg_step": (C.c_int, [_CTX, C.POINTER(CgSimParams), C.c_uint32, _F, _D]),
+    "cg_get_positions": (C.c_int, [_CTX, _F, C.c_uint32]),
+    "cg_last_error": (C.c_char_p, [_CTX]),
+    "cg_eval_force": (C.c_int, [_CTX, C.POINTER(CgSimParams), C.c_int32, C.c_float, C.c_float, _F]),
+    "cg_debug_slots": (C.c_int, [_CTX, _I, C.c_uint32, _U, _U]),
+    "cg_debug_link_csr": (C.c_int, [_CTX, C.c_int32, _U, _U, _F, _F, _U, C.c_uint32, _U]),
+    "cg_device_bytes": (C.c_uint64, [_CTX]),
+    "cg_set_profiling": (C.c_int, [_CTX, C.c_uint32]),
+    "cg_get_profile": (C.c_int, [_CTX, _D, _U, C.c_uint32]),
+}
+
+STAGES = ["mouse", "gravity", "center", "mb_sort", "mb_cells", "mb_pyramid", "mb_force", "link_in",
+          "link_out", "collision"]
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/cosmosgl.h."""
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:[\w\s\*]+?)\b(cg_\w+)\s*\(", text, re.M)))
+
+
+def lib() -> C.CDLL:
+    """Loads libcosmosgl.so (once)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m graph_amd.build` "
+                "(the simulation has no CPU fallback)"
+            )
+        l = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def fptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_F)
+
+
+def uptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.uint32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_U)
+
+
+def iptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_I)
+
+
+def dptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def default_params() -> CgSimParams:
+    p = CgSimParams()
+    lib().cg_default_params(C.byref(p))
+    return p
+
+
+class Context:
+    """One simulated graph on one GPU (owner of a ``cg_ctx``)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self._l = lib()
+        self._ctx = _CTX()
+        cp = CgCreateParams(device, hbm_budget)
+        rc = self._l.cg_create(C.byref(self._ctx), C.byref(cp))
+        if rc != CG_OK:
+            raise CosmosGLError(rc, "cg_create failed")
+        self.n = 0
+
+    def _check(self, rc: int) -> None:
+        if rc != CG_OK:
+            raise CosmosGLError(rc, (self._l.cg_last_error(self._ctx) or b"").decode())
+
+    def close(self) -> None:
+        if self._ctx:
+            self._l.cg_destroy(self._ctx)
+            self._ctx = _CTX()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- ingest ---------------------------------------------------------------
+    def set_positions(self, xy: np.ndarray) -> None:
+        xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
+        self.n = xy.size // 2
+        self._check(self._l.cg_set_positions(self._ctx, fptr(xy), self.n))
+
+    def set_links(self, links, strength=None, rand_in=None, rand_out=None) -> None:
+        links = np.ascontiguousarray(links, dtype=np.float32).reshape(-1)
+        m = links.size // 2
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+        strength, rand_in, rand_out = conv(strength), conv(rand_in), conv(rand_out)
+        self._check(self._l.cg_set_links(self._ctx, fptr(links), m, fptr(strength),
+                                         fptr(rand_in), fptr(rand_out)))
+
+    def set_jitter(self, xy: np.ndarray) -> None:
+        xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
+        self._check(self._l.cg_set_jitter(self._ctx, fptr(xy), xy.size // 2))
+
+    def set_point_sizes(self, sizes) -> None:
+        if sizes is None:
+            self._check(self._l.cg_set_point_sizes(self._ctx, None, 0))
+            return
+        s = np.ascontiguousarray(sizes, dtype=np.float32)
+        self._check(self._l.cg_set_point_sizes(self._ctx, fptr(s), s.size))
+
+    def set_pinned(self, indices) -> None:
+        idx = np.ascontiguousarray(indices if indices is not None else [], dtype=np.uint32)
+        self._check(self._l.cg_set_pinned(self._ctx, uptr(idx) if idx.size else None, idx.size))
+
+    def set_alpha(self, alpha: float) -> None:
+        self._check(self._l.cg_set_alpha(self._ctx, float(alpha)))
+
+    def get_alpha(self) -> float:
+        a = C.c_double()
+        self._check(self._l.cg_get_alpha(self._ctx, C.byref(a)))
+        return a.value
+
+    # --- tick -----------------------------------------------------------------
+    def step(self, params: CgSimParams, nticks: int = 1, seeds=None, trace: bool = False):
+        s = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.float32)
+        tr = np.zeros(nticks, dtype=np.float64) if trace else None
+        self._check(self._l.cg_step(self._ctx, C.byref(params), nticks, fptr(s), dptr(tr)))
+        return tr
+
+    def get_positions(self) -> np.ndarray:
+        out = np.empty(2 * self.n, dtype=np.float32)
+        self._check(self._l.cg_get_positions(self._ctx, fptr(out), self.n))
+        return out
+
+    # --- inspection -----------------------------------------------------------
+    def eval_force(self, params: CgSimParams, force: int, alpha: float, seed: float = 0.0) -> np.ndarray:
+        out = np.zeros(2 * self.n, dtype=np.float32)
+        self._check(self._l.cg_eval_force(self._ctx, C.byref(params), force, alpha, seed, fptr(out)))
+        return out
+
+    def debug_slots(self):
+        gf, L = C.c_uint32(), C.c_uint32()
+        self._check(self._l.cg_debug_slots(self._ctx, None, 0, C.byref(gf), C.byref(L)))
+        out = np.empty(gf.value * gf.value * 8, dtype=np.int32)
+        self._check(self._l.cg_debug_slots(self._ctx, iptr(out), out.size, C.byref(gf), C.byref(L)))
+        return out, gf.value, L.value
+
+    def debug_link_csr(self, direction: int):
+        E = C.c_uint32()
+        self._check(self._l.cg_debug_link_csr(self._ctx, direction, None, None, None, None, None, 0,
+                                              C.byref(E)))
+        n = self.n
+        off = np.empty(n + 1, dtype=np.uint32)
+        nbr = np.empty(max(E.value, 1), dtype=np.uint32)
+        bias = np.empty(max(E.value, 1), dtype=np.float32)
+        strength = np.empty(max(E.value, 1), dtype=np.float32)
+        deg = np.empty(max(n, 1), dtype=np.uint32)
+        self._check(self._l.cg_debug_link_csr(self._ctx, direction, uptr(off), uptr(nbr), fptr(bias),
+                                              fptr(strength), uptr(deg), nbr.size, C.byref(E)))
+        e = E.value
+        return off, nbr[:e], bias[:e], strength[:e], deg[:n]
+
+    def set_profiling(self, stages) -> None:
+        mask = 0
+        for s in stages:
+            mask |= 1 << (STAGES.index(s) if isinstance(s, str) else int(s))
+        self._check(self._l.cg_set_profiling(self._ctx, mask))
+
+    def get_profile(self) -> dict:
+        ms = np.zeros(len(STAGES), np.float64)
+        cnt = np.zeros(len(STAGES), np.uint32)
+        self._check(self._l.cg_get_profile(self._ctx, dptr(ms), uptr(cnt), len(STAGES)))
+        return {s: (float(ms[i]), int(cnt[i])) for i, s in enumerate(STAGES) if cnt[i]}
+
+    def device_bytes(self) -> int:
+        return int(self._l.cg_device_bytes(self._ctx))

@@ -1,0 +1,179 @@
+/*
+ * cosmosgl.h — C ABI of the MI355X force-simulation tick.
+ *
+ * One context = one simulated graph on one GPU, bound to one HIP stream.
+ * A context is not re-entrant; separate contexts may be driven from separate
+ * threads.  Inputs are copied on call, outputs are caller-allocated, the
+ * context owns all device memory.  No function throws; each returns CG_OK or
+ * a negative code, and cg_last_error() holds the message of the last failure.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * cosmos.gl 3.4.1 source tree):
+ *   cg_create          Graph constructor + module construction, src/index.ts:422-434
+ *   cg_set_positions   Graph.setPointPositions src/index.ts:548 -> Points.updatePositions
+ *                      Points/index.ts:390-483, updateExit :1313-1386
+ *   cg_set_links       Graph.setLinks src/index.ts:723 + setLinkStrength :836 ->
+ *                      GraphData._createAdjacencyLists/_calculateDegrees
+ *                      GraphData/index.ts:525-568 -> ForceLink.create x2 ForceLink/index.ts:40-171
+ *   cg_set_jitter      ForceManyBody.create random texture, ForceManyBody/index.ts:132-163
+ *   cg_set_point_sizes Graph.setPointSizes src/index.ts:627 (collision radii,
+ *                      ForceCollision/index.ts:121-148)
+ *   cg_set_pinned      Graph.setPinnedPoints src/index.ts:912 -> Points.updatePinnedStatus
+ *                      Points/index.ts:1264-1304
+ *   cg_set_alpha       Graph.start(alpha) / render(alpha) src/index.ts:1457, 936
+ *   cg_step            Graph.runSimulationStep src/index.ts:2006-2099 (nticks times)
+ *   cg_get_positions   Graph.getPointPositions src/index.ts:1106-1130 (readPixels)
+ *   cg_get_alpha       Store.alpha, reported through onSimulationTick src/index.ts:2090
+ */
+#ifndef COSMOSGL_H
+#define COSMOSGL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CG_OK 0
+#define CG_ERR_INVALID (-1)   /* bad argument */
+#define CG_ERR_DEVICE (-2)    /* HIP runtime failure */
+#define CG_ERR_BUDGET (-3)    /* would exceed the context's HBM budget */
+#define CG_ERR_STATE (-4)     /* call out of order (e.g. step before positions) */
+
+typedef struct cg_ctx cg_ctx;
+
+typedef struct cg_create_params {
+    int32_t device;        /* HIP device ordinal */
+    uint64_t hbm_budget;   /* max device bytes this context may hold; 0 = no limit */
+} cg_create_params;
+
+/* Mirror of the reference's `simulation*` config keys (src/config.ts:368-456,
+ * defaults src/variables.ts:68-83) plus the tick gates of runSimulationStep.
+ * Parameters are re-read on every cg_step, so a change takes effect on the next
+ * tick as in the reference. */
+typedef struct cg_sim_params {
+    float space_size;                   /* Store.adjustedSpaceSize (Store/index.ts:209-233) */
+    float decay;                        /* simulationDecay */
+    float gravity;                      /* simulationGravity (0 = pass skipped) */
+    float center;                       /* simulationCenter (0 = pass skipped) */
+    float repulsion;                    /* simulationRepulsion */
+    float link_spring;                  /* simulationLinkSpring */
+    float link_distance;                /* simulationLinkDistance */
+    float link_dist_random_variation_range[2]; /* simulationLinkDistRandomVariationRange */
+    float repulsion_from_mouse;         /* simulationRepulsionFromMouse */
+    float friction;                     /* simulationFriction */
+    float collision;                    /* simulationCollision (0 = pass skipped) */
+    float collision_radius;             /* simulationCollisionRadius; <= 0 = per-point size/2 */
+    float collision_padding;            /* simulationCollisionPadding */
+    float point_default_size;           /* pointDefaultSize (size of points without one) */
+    int32_t enable_simulation;          /* enableSimulation: 0 makes cg_step a no-op */
+    int32_t run_forces;                 /* forceExecution || (running && !zoom gate) */
+    int32_t mouse_repulsion;            /* right button held && enableRightClickRepulsion */
+    float mouse_position[2];            /* Store.mousePosition, space coordinates */
+} cg_sim_params;
+
+/* Fills the reference defaults (src/variables.ts:10-19, 68-83: spaceSize 4096). */
+void cg_default_params(cg_sim_params* p);
+
+int cg_create(cg_ctx** out, const cg_create_params* params);
+int cg_destroy(cg_ctx* ctx);
+
+/* xy = [x0, y0, x1, y1, ...], n points.  A point with a NaN coordinate is
+ * absent (helper.ts:157-159): it is never binned, never integrated and reads
+ * back as NaN.  Resets jitter, sizes and pinned state when n changes, and
+ * rebuilds the link CSRs from the last cg_set_links input. */
+int cg_set_positions(cg_ctx* ctx, const float* xy, uint32_t n);
+
+/* links = [s0, t0, s1, t1, ...] as the reference's Float32Array, m links.
+ * Endpoints that are not integers in [0, n) invalidate the link, which keeps
+ * its index (GraphData/index.ts:537-547).  strength: m per-link strengths or
+ * NULL (degree-derived default, ForceLink/index.ts:66-73).  rand_in / rand_out:
+ * m U(0,1) draws, rand_in[l] for link l's entry in the INCOMING instance and
+ * rand_out[l] for its OUTGOING entry (ForceLink/index.ts:76); NULL = drawn
+ * from the context's own generator. */
+int cg_set_links(cg_ctx* ctx, const float* links, uint32_t m, const float* strength,
+                 const float* rand_in, const float* rand_out);
+
+/* Per-point many-body jitter r_i = (U(-1,1)*1e-5, U(-1,1)*1e-5), n pairs. */
+int cg_set_jitter(cg_ctx* ctx, const float* xy, uint32_t n);
+
+/* Per-point sizes for collision (NaN = pointDefaultSize); NULL clears. */
+int cg_set_point_sizes(cg_ctx* ctx, const float* sizes, uint32_t n);
+
+/* Pinned point indices (indices outside [0, n) are ignored); count 0 clears. */
+int cg_set_pinned(cg_ctx* ctx, const uint32_t* indices, uint32_t count);
+
+int cg_set_alpha(cg_ctx* ctx, double alpha);
+int cg_get_alpha(cg_ctx* ctx, double* alpha);
+
+/* Runs nticks ticks.  seeds: nticks per-tick near-field seeds in [0,1) (the
+ * reference's store.getRandomFloat(0, 1) per tick) or NULL for the context's
+ * own generator.  alpha_trace: NULL or nticks doubles receiving alpha after
+ * each tick (what onSimulationTick reports).  Asynchronous on the context's
+ * stream; cg_get_positions synchronises. */
+int cg_step(cg_ctx* ctx, const cg_sim_params* params, uint32_t nticks,
+            const float* seeds, double* alpha_trace);
+
+/* Copies n positions to xy (2n floats); absent points read back NaN. */
+int cg_get_positions(cg_ctx* ctx, float* xy, uint32_t n);
+
+const char* cg_last_error(const cg_ctx* ctx);
+
+/* ---------------------------------------------------------------------- */
+/* Inspection entry points.  They have no reference counterpart: they expose */
+/* one force pass or one intermediate structure so the result can be        */
+/* compared with the oracle from an identical input state.                  */
+/* ---------------------------------------------------------------------- */
+
+#define CG_FORCE_GRAVITY 0
+#define CG_FORCE_CENTER 1
+#define CG_FORCE_MOUSE 2
+#define CG_FORCE_MANYBODY 3
+#define CG_FORCE_LINK_IN 4
+#define CG_FORCE_LINK_OUT 5
+#define CG_FORCE_COLLISION 6
+
+/* Velocity that one force pass writes from the current positions (nothing is
+ * integrated).  v: 2n floats; absent points get 0. */
+int cg_eval_force(cg_ctx* ctx, const cg_sim_params* params, int32_t force, float alpha,
+                  float seed, float* v);
+
+/* Near-field slots of the last many-body evaluation: finest^2 * 8 point
+ * indices (row-major cells, ascending ticket, -1 = empty).  finest/levels
+ * receive the pyramid geometry. */
+int cg_debug_slots(cg_ctx* ctx, int32_t* slots, uint32_t capacity, uint32_t* finest,
+                   uint32_t* levels);
+
+/* Link CSR of one direction (0 = INCOMING instance, 1 = OUTGOING) in the
+ * reference's entry order: offsets[n+1], nbr[E], bias[E], strength[E], and
+ * degree[n].  Returns E through *entries. */
+int cg_debug_link_csr(cg_ctx* ctx, int32_t dir, uint32_t* offsets, uint32_t* nbr, float* bias,
+                      float* strength, uint32_t* degree, uint32_t capacity, uint32_t* entries);
+
+/* Bytes of device memory the context holds. */
+uint64_t cg_device_bytes(const cg_ctx* ctx);
+
+/* Stage timing on the context's stream (no reference counterpart; used by the
+ * benchmark).  cg_set_profiling(mask) brackets every launch of the stages in
+ * mask (bit s = stage s) with HIP events during later cg_step calls;
+ * cg_get_profile synchronises and returns, per stage, the summed
+ * milliseconds and the number of bracketed launches since the last call. */
+#define CG_STAGE_MOUSE 0
+#define CG_STAGE_GRAVITY 1
+#define CG_STAGE_CENTER 2
+#define CG_STAGE_MB_SORT 3
+#define CG_STAGE_MB_CELLS 4
+#define CG_STAGE_MB_PYRAMID 5
+#define CG_STAGE_MB_FORCE 6
+#define CG_STAGE_LINK_IN 7
+#define CG_STAGE_LINK_OUT 8
+#define CG_STAGE_COLLISION 9
+#define CG_STAGES 10
+int cg_set_profiling(cg_ctx* ctx, uint32_t stage_mask);
+int cg_get_profile(cg_ctx* ctx, double* ms, uint32_t* launches, uint32_t nstages);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COSMOSGL_H */

@@ -1,0 +1,256 @@
+"""GPU parity: every force pass of the HIP path against the oracle from an
+identical input state, integer structures bit-exact, through the C ABI.
+
+Tolerance (per point, vector norm): |v_gpu - v_oracle| <= 1e-5 * spaceSize
+for forces evaluated in f32 on the GPU vs f64 in the oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from graph_amd import _lib, datagen
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # relative to spaceSize
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def params(S, **kw):
+    p = _lib.default_params()
+    p.space_size = S
+    for k, v in kw.items():
+        if k == "range":
+            p.link_dist_random_variation_range[0], p.link_dist_random_variation_range[1] = v
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def ctx_for(xy, links=None, jitter=None, r_in=None, r_out=None, strength=None):
+    c = _lib.Context()
+    c.set_positions(xy)
+    if jitter is not None:
+        c.set_jitter(jitter)
+    if links is not None:
+        c.set_links(links, strength, r_in, r_out)
+    return c
+
+
+def err_norm(v_gpu, v_or):
+    d = v_gpu.reshape(-1, 2).astype(np.float64) - v_or.reshape(-1, 2)
+    return np.hypot(d[:, 0], d[:, 1])
+
+
+def check(v_gpu, v_or, S, what, idx=None):
+    e = err_norm(v_gpu, v_or)
+    if idx is not None:
+        e = e[idx]
+    worst = float(np.nanmax(e)) if e.size else 0.0
+    say(f"{what}: max |dv| = {worst:.3e} ({worst / S:.2e} S)")
+    assert np.isfinite(e).all()
+    assert worst <= TOL * S, f"{what}: {worst / S:.3e} S > {TOL} S"
+
+
+def test_pointwise_forces():
+    S, n = 4096.0, 5000
+    xy = datagen.uniform_square(n, S, 1)
+    xy[10:14] = np.nan
+    absent = O.absent_mask(xy)
+    c = ctx_for(xy)
+    p = params(S, gravity=0.25, center=0.3, repulsion_from_mouse=2.0)
+    p.mouse_position[0], p.mouse_position[1] = 1000.0, 2000.0
+    pres = absent == 0
+    xy0 = np.where(np.isnan(xy), 0, xy)
+    for force, want in [
+        (_lib.FORCE_GRAVITY, O.gravity(xy0, S, 0.7, 0.25)),
+        (_lib.FORCE_CENTER, O.center(xy, 0.7, 0.3, absent)),
+        (_lib.FORCE_MOUSE, O.mouse(xy0, 1000.0, 2000.0, 2.0)),
+    ]:
+        got = c.eval_force(p, force, 0.7)
+        check(got, want, S, f"force {force}", idx=pres)
+        assert (got.reshape(-1, 2)[~pres] == 0).all()
+
+
+@pytest.mark.parametrize("case", ["cfg0", "clustered", "tiny"])
+def test_manybody_slots_bit_exact_and_forces(case):
+    rng = np.random.default_rng(0)
+    if case == "cfg0":
+        S, n = 4096.0, 10_000
+        xy = datagen.uniform_square(n, S, 42)
+    elif case == "clustered":
+        # a dense Gaussian clump: finest cells far above 64 points (wave-per-cell path)
+        S, n = 4096.0, 20_000
+        xy = np.clip(rng.normal(2048, 40, 2 * n), 0, S).astype(np.float32)
+        xy[: 2 * 500] = rng.uniform(0, S, 1000)
+        xy[2 * 7: 2 * 9] = np.nan
+    else:
+        S, n = 1024.0, 7
+        xy = rng.uniform(0, S, 2 * n).astype(np.float32)
+    jit = datagen.jitter(n, 5)
+    absent = O.absent_mask(xy)
+    c = ctx_for(xy, jitter=jit)
+    p = params(S, repulsion=0.5)
+    for seed in (0.125, 0.8765):
+        got = c.eval_force(p, _lib.FORCE_MANYBODY, 0.9, seed)
+        slots, gf, L = c.debug_slots()
+        want_slots = O.slots(xy, S, seed, absent)
+        assert (gf, L) == O.geometry(n)
+        assert np.array_equal(slots, want_slots), f"slots differ in {np.sum(slots != want_slots)} entries"
+        want = O.manybody(xy, S, 0.9, 0.5, seed, jitter=jit, absent=absent)
+        check(got, want, S, f"manybody {case} seed {seed}", idx=absent == 0)
+
+
+def test_manybody_cfg1_shape():
+    # 1M points uniform in a disk of radius 2048 centred in S = 8192: L = 8, g_f = 512
+    S, n = 8192.0, 1_000_000
+    xy = datagen.uniform_disk(n, 4096, 4096, 2048, 7)
+    jit = datagen.jitter(n, 8)
+    c = ctx_for(xy, jitter=jit)
+    p = params(S, repulsion=0.5)
+    seed = 0.4242
+    got = c.eval_force(p, _lib.FORCE_MANYBODY, 1.0, seed)
+    slots, gf, L = c.debug_slots()
+    assert (gf, L) == (512, 8)
+    want_slots = O.slots(xy, S, seed)
+    assert np.array_equal(slots, want_slots)
+    idx = np.random.default_rng(1).choice(n, 20_000, replace=False).astype(np.uint32)
+    want = O.manybody(xy, S, 1.0, 0.5, seed, jitter=jit, eval_idx=idx)
+    check(got, want, S, "manybody cfg1 (20k-point sample)", idx=idx)
+
+
+def test_link_csr_bit_exact():
+    n = 10_000
+    links = datagen.barabasi_albert(n, 2, 42)
+    bad = np.array([3, 3, 1.5, 2, -1, 0, n, 1, 5, 5, 17, 4], np.float32)
+    links = np.concatenate([links, bad])
+    m = links.size // 2
+    user = datagen.uniform(m, -0.5, 2.0, 3)
+    user[::7] = np.nan
+    xy = datagen.uniform_square(n, 4096.0, 1)
+    for strength in (None, user):
+        c = ctx_for(xy, links=links, r_in=datagen.uniform(m, 0, 1, 1), r_out=datagen.uniform(m, 0, 1, 2),
+                    strength=strength)
+        for d in (0, 1):
+            off, nbr, bias, st, deg = c.debug_link_csr(d)
+            o_off, o_nbr, o_lof, o_deg = O.link_csr(n, links, d)
+            o_bias, o_st = O.link_props(n, o_off, o_nbr, o_lof, o_deg, strength)
+            assert np.array_equal(off, o_off)
+            assert np.array_equal(nbr, o_nbr)
+            assert np.array_equal(deg, o_deg)
+            assert np.array_equal(bias.view(np.uint32), o_bias.view(np.uint32))
+            assert np.array_equal(st.view(np.uint32), o_st.view(np.uint32))
+
+
+@pytest.mark.parametrize("case", ["ba_cfg0", "chung_lu_hubs", "absent_neighbours"])
+def test_link_forces_vs_serial_oracle(case):
+    if case == "chung_lu_hubs":
+        S, n = 8192.0, 200_000
+        links = datagen.chung_lu(n, 2.1, 1.0, 1_000_000, 7)
+        xy = datagen.uniform_disk(n, 4096, 4096, 2048, 3)
+    else:
+        S, n = 4096.0, 10_000
+        links = datagen.barabasi_albert(n, 2, 42)
+        xy = datagen.uniform_square(n, S, 42)
+        if case == "absent_neighbours":
+            xy[2 * 5: 2 * 40] = np.nan
+    m = links.size // 2
+    r_in, r_out = datagen.uniform(m, 0, 1, 11), datagen.uniform(m, 0, 1, 12)
+    absent = O.absent_mask(xy)
+    c = ctx_for(xy, links=links, r_in=r_in, r_out=r_out)
+    p = params(S, link_spring=1.0, link_distance=10.0, range=(1.0, 1.2))
+    L = O.Links(n, links, r_in, r_out)
+    for alpha in (1.0, 0.3):
+        for d in (0, 1):
+            got = c.eval_force(p, _lib.FORCE_LINK_IN + d, alpha)
+            want = L.force(xy, d, alpha, 1.0, 10.0, 1.0, 1.2, absent)
+            check(got, want, S, f"link {case} dir {d} alpha {alpha}", idx=absent == 0)
+
+
+def test_collision_vs_oracle():
+    S, n = 4096.0, 20_000
+    rng = np.random.default_rng(4)
+    xy = np.clip(rng.normal(2048, 300, 2 * n), 0, S).astype(np.float32)
+    sizes = 2 * datagen.lognormal(n, np.log(2), 0.5, 9)
+    c = ctx_for(xy)
+    c.set_point_sizes(sizes)
+    p = params(S, collision=0.7)
+    got = c.eval_force(p, _lib.FORCE_COLLISION, 0.8)
+    want = O.collision(xy, sizes, S, 0.8, 0.7, max_size=max(4.0, float(sizes.max())))
+    check(got, want, S, "collision")
+
+
+def test_ticks_deterministic_alpha_and_masks():
+    S, n = 4096.0, 10_000
+    xy = datagen.uniform_square(n, S, 42)
+    xy[2 * 3: 2 * 6] = np.nan
+    links = datagen.barabasi_albert(n, 2, 42)
+    m = links.size // 2
+    r_in, r_out = datagen.uniform(m, 0, 1, 1), datagen.uniform(m, 0, 1, 2)
+    jit = datagen.jitter(n, 3)
+    seeds = datagen.uniform(40, 0, 1, 4)
+    outs = []
+    for _ in range(2):
+        c = ctx_for(xy, links=links, jitter=jit, r_in=r_in, r_out=r_out)
+        c.set_pinned([0, 1, 2, 10])
+        c.set_alpha(1.0)
+        p = params(S, repulsion=0.5, gravity=0.25, friction=0.85, decay=5000)
+        tr = c.step(p, 40, seeds=seeds, trace=True)
+        outs.append(c.get_positions())
+        a = 1.0
+        for t in range(40):
+            a = O.alpha_step(a, 5000)
+            assert tr[t] == a
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    out = outs[0].reshape(-1, 2)
+    assert np.isnan(out[3:6]).all()
+    assert np.array_equal(out[[0, 1, 2, 10]], xy.reshape(-1, 2)[[0, 1, 2, 10]])
+    live = np.ones(n, bool)
+    live[3:6] = False
+    assert (out[live] >= 0).all() and (out[live] <= S).all()
+    assert not np.array_equal(out[live], xy.reshape(-1, 2)[live])
+
+
+def edge_stats(xy, links, r, lo=1.0, hi=1.2, dist=10.0):
+    P = xy.reshape(-1, 2).astype(np.float64)
+    s, t = links[0::2].astype(np.int64), links[1::2].astype(np.int64)
+    L = np.hypot(*(P[s] - P[t]).T)
+    D = (r.astype(np.float64) * (hi - lo) + lo) * dist
+    stress = float(np.sum((L - D) ** 2) / np.sum(D**2))
+    return np.percentile(L, [10, 50, 90]), stress
+
+
+def test_layout_quality_vs_oracle_cfg0():
+    """cfg0 for 300 ticks on the GPU and in the FP64 oracle from the same inputs:
+    trajectories diverge chaotically, so the layouts are compared by their
+    edge-length distribution and edge stress (within 2%)."""
+    S, n, ticks = 4096.0, 10_000, 300
+    xy = datagen.uniform_square(n, S, 42)
+    links = datagen.barabasi_albert(n, 2, 42)
+    m = links.size // 2
+    r_in, r_out = datagen.uniform(m, 0, 1, 1), datagen.uniform(m, 0, 1, 2)
+    jit = datagen.jitter(n, 3)
+    seeds = datagen.uniform(ticks, 0, 1, 4)
+    c = ctx_for(xy, links=links, jitter=jit, r_in=r_in, r_out=r_out)
+    c.set_alpha(1.0)
+    p = params(S, repulsion=0.5, gravity=0.25, friction=0.85, decay=5000, link_spring=1.0,
+               link_distance=10.0)
+    c.step(p, ticks, seeds=seeds)
+    gpu = c.get_positions()
+    pd = dict(space_size=S, gravity=0.25, center=0, repulsion=0.5, link_spring=1.0, link_distance=10.0,
+              lo=1.0, hi=1.2, friction=0.85, decay=5000.0)
+    Lk = O.Links(n, links, r_in, r_out)
+    pos, a = xy.astype(np.float64), 1.0
+    for t in range(ticks):
+        pos, a = O.tick(pos, a, float(seeds[t]), pd, links=Lk, jitter=jit)
+    q_g, s_g = edge_stats(gpu, links, r_in)
+    q_o, s_o = edge_stats(pos, links, r_in)
+    say(f"edge length p10/p50/p90 gpu {q_g} oracle {q_o}; stress gpu {s_g:.4f} oracle {s_o:.4f}")
+    np.testing.assert_allclose(q_g, q_o, rtol=0.02)
+    assert abs(s_g - s_o) <= 0.02 * s_o

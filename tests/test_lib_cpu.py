@@ -1,0 +1,102 @@
+"""CPU-side checks of the native library and the host logic (no GPU calls)."""
+import ctypes
+import math
+import subprocess
+
+import numpy as np
+import pytest
+
+from graph_amd import _lib, datagen
+
+
+def test_header_declares_the_boundary():
+    syms = _lib.header_symbols()
+    for name in ["cg_create", "cg_set_positions", "cg_set_links", "cg_step", "cg_get_positions",
+                 "cg_destroy", "cg_last_error", "cg_set_jitter", "cg_set_point_sizes",
+                 "cg_set_pinned", "cg_get_alpha", "cg_set_alpha"]:
+        assert name in syms
+
+
+def test_library_exports_every_header_symbol():
+    l = _lib.lib()
+    missing = [s for s in _lib.header_symbols() if not hasattr(l, s)]
+    assert not missing, missing
+
+
+def test_default_params_mirror_reference_defaults():
+    # src/variables.ts:10-19, 68-83
+    p = _lib.default_params()
+    assert p.space_size == 4096 and p.decay == 5000
+    assert p.gravity == pytest.approx(0.25) and p.center == 0
+    assert p.repulsion == 1.0 and p.link_spring == 1 and p.link_distance == 10
+    assert list(p.link_dist_random_variation_range) == pytest.approx([1, 1.2])
+    assert p.repulsion_from_mouse == 2 and p.friction == pytest.approx(0.85)
+    assert p.collision == 0 and p.collision_padding == 0 and p.point_default_size == 4
+    assert p.enable_simulation == 1
+
+
+def test_struct_layout_matches_header(tmp_path):
+    # the ctypes mirror must agree with the C compiler's layout of the header structs
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "cosmosgl.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu %zu\", sizeof(cg_sim_params),"
+        " offsetof(cg_sim_params, friction), offsetof(cg_sim_params, enable_simulation),"
+        " offsetof(cg_sim_params, mouse_position), sizeof(cg_create_params));return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(_lib.HEADER.parent), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    P = _lib.CgSimParams
+    assert got == [ctypes.sizeof(P), P.friction.offset, P.enable_simulation.offset,
+                   P.mouse_position.offset, ctypes.sizeof(_lib.CgCreateParams)]
+
+
+def test_barabasi_albert():
+    links = datagen.barabasi_albert(10_000, 2, 42)
+    m = links.size // 2
+    assert m == sum(min(2, t) for t in range(1, 10_000))  # 19,997
+    s, t = links[0::2].astype(np.int64), links[1::2].astype(np.int64)
+    assert (s > t).all()                      # new node -> earlier node
+    assert np.array_equal(links, datagen.barabasi_albert(10_000, 2, 42))
+    pairs = set(zip(s.tolist(), t.tolist()))
+    assert len(pairs) == m                    # distinct targets per new node
+    deg = np.bincount(np.concatenate([s, t]), minlength=10_000)
+    assert deg.max() > 50                     # preferential attachment grows hubs
+
+
+def test_chung_lu_power_law():
+    n, m = 200_000, 1_000_000
+    links = datagen.chung_lu(n, 2.1, 1.0, m, 7)
+    assert links.size == 2 * m
+    s, t = links[0::2].astype(np.int64), links[1::2].astype(np.int64)
+    assert (s != t).all() and s.min() >= 0 and max(s.max(), t.max()) < n
+    deg = np.bincount(np.concatenate([s, t]), minlength=n)
+    assert deg.max() > 5_000                  # hubs for the chunked link path
+
+
+def test_sbm_cross_fraction():
+    n = 100_000
+    links = datagen.sbm(n, 100, 1.1, 5.0, 0.1, 11)
+    m = links.size // 2
+    assert 0.9 * n * 2.5 / 0.9 < m < 1.1 * n * 2.5 / 0.9
+
+
+def test_positions_generators():
+    xy = datagen.uniform_disk(100_000, 4096, 4096, 2048, 7).reshape(-1, 2)
+    r = np.hypot(xy[:, 0] - 4096, xy[:, 1] - 4096)
+    assert r.max() <= 2048.01
+    assert abs(np.median(r) - 2048 / math.sqrt(2)) < 20
+    sq = datagen.uniform_square(10_000, 4096, 42)
+    assert sq.min() >= 0 and sq.max() < 4096
+    j = datagen.jitter(1000, 3)
+    assert np.abs(j).max() <= 1e-5
+
+
+def test_space_size_adjustment():
+    from graph_amd.simulation import adjust_space_size
+
+    # Store/index.ts:209-233
+    assert adjust_space_size(16384, 16384) == 8192
+    assert adjust_space_size(8192, 16384) == 8192
+    assert adjust_space_size(-5, 16384) == 4096
+    assert adjust_space_size(1, None) == 2

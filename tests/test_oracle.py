@@ -1,0 +1,410 @@
+"""Pins the oracle (oracle/cosmos_oracle.c) before it is trusted as the checker.
+
+The reference ships no tests, fixtures or golden vectors for this path, so
+every check here comes from mathematics the reference's formulas fix: closed
+forms, published inverses, brute force on tiny inputs, exactness in special
+configurations, unbiasedness of the Monte-Carlo near field, hand-built CSR
+fixtures.  Each test names the mistake it would catch.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+M32 = 0xFFFFFFFF
+
+
+def lowbias32_inverse(x: int) -> int:
+    # published inverse of lowbias32 (hash-prospector, "lowbias32_r")
+    x ^= x >> 16
+    x = (x * 0x43021123) & M32
+    x ^= (x >> 15) ^ (x >> 30)
+    x = (x * 0x1D69E2A5) & M32
+    x ^= x >> 16
+    return x
+
+
+def test_lowbias32_round_trips_through_published_inverse(oracle):
+    # a wrong constant or shift in the forward hash breaks the round trip
+    rng = np.random.default_rng(1)
+    for x in [0, 1, 2, 0xFFFFFFFF, 0x80000000, *rng.integers(0, 2**32, 200).tolist()]:
+        assert lowbias32_inverse(oracle.lowbias32(int(x))) == int(x)
+
+
+def test_ticket_range_and_value(oracle):
+    # (t + 0.5)/2^24 scaled into (0.001, 0.999): within 2 ulp of the real value
+    for idx in range(0, 5000, 7):
+        for seed in (0.0, 0.25, 0.7071):
+            tk = oracle.ticket(idx, seed)
+            sb = int(np.float32(seed).view(np.uint32))
+            t = oracle.lowbias32(idx ^ sb) >> 8
+            exact = 0.001 + (t + 0.5) / 2**24 * 0.998
+            assert 0.001 < tk < 0.999
+            assert abs(tk - exact) <= 2 * np.spacing(np.float32(exact))
+
+
+def test_ticket_is_monotone_in_hash(oracle):
+    # peel order must follow the hash order (a transposed step breaks it)
+    ts = []
+    for idx in range(20000):
+        t = oracle.lowbias32(idx) >> 8
+        ts.append((t, oracle.ticket(idx, 0.0)))
+    ts.sort()
+    vals = [v for _, v in ts]
+    assert all(a <= b for a, b in zip(vals, vals[1:]))
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 15, 16, 17, 100, 10000, 16384, 16385, 65536, 65537,
+                               1000000, 4000000, 123456789])
+def test_geometry_matches_reference_formula(oracle, n):
+    # ForceManyBody/index.ts:650-655 evaluated with JS double semantics
+    target = 2 * math.sqrt(n)
+    p = 0.0 if n == 0 else 2.0 ** math.ceil(math.log2(target))
+    gf = int(min(512, max(8, p)))
+    L = int(math.log2(gf)) - 1
+    assert oracle.geometry(n) == (gf, L)
+
+
+def test_geometry_config_values(oracle):
+    assert oracle.geometry(10_000) == (256, 7)      # cfg0
+    assert oracle.geometry(1_000_000) == (512, 8)   # cfg1: the 512 cap, not 13 levels
+    assert oracle.geometry(1_000) == (64, 5)        # smallest cfg3 graph
+
+
+def test_cell_brute_force(oracle):
+    # clamp(floor(p / h), 0, g-1) against exact rational arithmetic
+    rng = np.random.default_rng(2)
+    for S, g in [(4096.0, 256), (8192.0, 512), (16384.0, 4), (1000.0, 8), (4096.0, 287)]:
+        h = oracle.cell_size(S, g)
+        assert h == np.float32(S / g)
+        pts = np.concatenate([rng.uniform(-10, S + 10, 500), [0.0, S, -0.0, S - 1e-3, h, 2 * h, -h]])
+        for p in pts.astype(np.float32):
+            q = np.float32(p) / np.float32(h)  # IEEE f32 division
+            want = min(max(math.floor(float(q)), 0), g - 1)
+            if float(h) in (1.0, 2.0, 4.0, 8.0, 16.0, 32.0, 64.0, 1024.0, 4096.0):
+                # power-of-two cell: the f32 quotient is exact
+                want_exact = min(max(math.floor(Fraction(float(p)) / Fraction(float(h))), 0), g - 1)
+                assert want == want_exact
+            assert oracle.cell(float(p), float(h), g) == want
+
+
+def peel_slots(xy, S, seed, oracle_mod):
+    """The depth-peel passes of build-nearfield-slots.vert restated as K
+    argmin passes (not as a sort): pass k keeps, per cell, the smallest
+    (ticket, index) among points whose ticket exceeds the previous slot's."""
+    xy = np.asarray(xy, np.float32).reshape(-1, 2)
+    n = len(xy)
+    gf, _ = oracle_mod.geometry(n)
+    h = oracle_mod.cell_size(S, gf)
+    cells = {}
+    for i, (x, y) in enumerate(xy):
+        if np.isnan(x) or np.isnan(y):
+            continue
+        c = oracle_mod.cell(float(y), h, gf) * gf + oracle_mod.cell(float(x), h, gf)
+        cells.setdefault(c, []).append((oracle_mod.ticket(i, seed), i))
+    out = -np.ones(gf * gf * 8, np.int32)
+    for c, pts in cells.items():
+        prev = None
+        for k in range(8):
+            elig = [p for p in pts if prev is None or p[0] > prev]
+            if not elig:
+                break
+            best = min(elig)
+            out[c * 8 + k] = best[1]
+            prev = best[0]
+    return out
+
+
+@pytest.mark.parametrize("n,S,seed", [(60, 64.0, 0.5), (300, 128.0, 0.123), (2000, 512.0, 0.9),
+                                      (40, 4096.0, 0.3)])
+def test_slots_equal_iterative_peel(oracle, n, S, seed):
+    rng = np.random.default_rng(n)
+    xy = rng.uniform(0, S, 2 * n).astype(np.float32)
+    xy[6:8] = np.nan  # an absent point is never captured
+    want = peel_slots(xy, S, seed, oracle)
+    got = oracle.slots(xy, S, seed, absent=oracle.absent_mask(xy))
+    assert np.array_equal(got, want)
+
+
+def test_slots_exhaustive_when_cell_small(oracle):
+    # a cell with <= 8 points holds all of them in ascending ticket order
+    rng = np.random.default_rng(5)
+    n, S = 40, 4096.0
+    xy = rng.uniform(0, S, 2 * n).astype(np.float32)
+    gf, _ = oracle.geometry(n)
+    slots = oracle.slots(xy, S, 0.77).reshape(-1, 8)
+    h = oracle.cell_size(S, gf)
+    counts = {}
+    for i in range(n):
+        c = oracle.cell(float(xy[2 * i + 1]), h, gf) * gf + oracle.cell(float(xy[2 * i]), h, gf)
+        counts.setdefault(c, []).append(i)
+    for c, members in counts.items():
+        assert len(members) <= 8
+        got = [s for s in slots[c] if s >= 0]
+        assert sorted(got) == sorted(members)
+        tks = [oracle.ticket(i, 0.77) for i in got]
+        assert tks == sorted(tks)
+
+
+def exact_pairwise(xy, alpha, k, present):
+    """sum over j != i of the clamped inverse-distance term (force-level.frag:43-56)."""
+    xy = np.asarray(xy, np.float64).reshape(-1, 2)
+    v = np.zeros_like(xy)
+    for i in present:
+        for j in present:
+            if i == j:
+                continue
+            d = xy[i] - xy[j]
+            l = d @ d
+            if l <= 0:
+                continue
+            dist = math.sqrt(l)
+            if l < 1:
+                l = math.sqrt(l)
+            v[i] += alpha * k / math.sqrt(l) * d / dist
+    return v
+
+
+@pytest.mark.parametrize("n_total,n_present", [(10, 10), (200, 12), (5000, 16), (1_000_000, 16)])
+def test_manybody_exact_when_sparse(oracle, n_total, n_present):
+    """One present point per level-0 cell makes every far-field centroid a
+    single point and every near-field cell exhaustive, so the pyramid must
+    reproduce the exact all-pairs sum.  Absent points only deepen the pyramid
+    (its size follows the total count).  Catches a cell counted twice or
+    missed at any level, a wrong 6x6 base, a wrong exclusion shell, or a
+    wrong falloff."""
+    rng = np.random.default_rng(n_total)
+    S = 4096.0
+    xy = np.full(2 * n_total, np.nan, np.float32)
+    cells = rng.permutation(16)[:n_present]
+    for t, c in enumerate(cells):
+        cx, cy = c % 4, c // 4
+        xy[2 * t] = (cx + rng.uniform(0.05, 0.95)) * 1024
+        xy[2 * t + 1] = (cy + rng.uniform(0.05, 0.95)) * 1024
+    absent = oracle.absent_mask(xy)
+    present = list(range(n_present))
+    got = oracle.manybody(xy, S, 0.8, 0.5, 0.31, jitter=None, absent=absent,
+                          eval_idx=np.arange(n_present)).reshape(-1, 2)
+    want = exact_pairwise(xy, 0.8, 0.5, present)
+    np.testing.assert_allclose(got[:n_present], want[:n_present], rtol=1e-12, atol=1e-15)
+
+
+def test_manybody_close_pairs_use_sqrt_clamp(oracle):
+    # two points 0.5 apart: l = 0.25 < 1 -> l = sqrt(l) = 0.5, |v| = alpha*k/sqrt(0.5)
+    xy = np.array([100.0, 100.0, 100.5, 100.0], np.float32)
+    v = oracle.manybody(xy, 4096.0, 1.0, 1.0, 0.5).reshape(-1, 2)
+    assert v[0][0] == pytest.approx(-1 / math.sqrt(0.5), rel=1e-12)
+    assert v[1][0] == pytest.approx(1 / math.sqrt(0.5), rel=1e-12)
+    # far apart (l >= 1): |v| = alpha*k/d
+    xy = np.array([100.0, 100.0, 3100.0, 2100.0], np.float32)
+    v = oracle.manybody(xy, 4096.0, 0.5, 2.0, 0.5).reshape(-1, 2)
+    d = math.hypot(3000, 2000)
+    assert np.hypot(*v[0]) == pytest.approx(1.0 / d, rel=1e-12)
+
+
+def test_near_field_is_unbiased(oracle):
+    """30 points in one finest cell: K=8 slots sample it, so one tick is a
+    random estimate; averaged over seeds it must approach the exact pair sum
+    (Horvitz-Thompson weight others/sampled, force-nearfield.frag:142-146)."""
+    rng = np.random.default_rng(9)
+    n, S = 30, 4096.0
+    gf, _ = oracle.geometry(n)
+    h = S / gf
+    xy = (h * 3 + rng.uniform(0.1, 0.9, 2 * n) * h).astype(np.float32)
+    exact = exact_pairwise(xy, 1.0, 1.0, range(n))
+    acc = np.zeros((n, 2))
+    seeds = rng.uniform(0, 1, 1500).astype(np.float32)
+    for s in seeds:
+        acc += oracle.manybody(xy, S, 1.0, 1.0, float(s)).reshape(-1, 2)
+    mean = acc / len(seeds)
+    err = np.linalg.norm(mean - exact, axis=1) / np.linalg.norm(exact, axis=1)
+    assert np.median(err) < 0.05
+    # one tick is NOT exact (the estimator is really active)
+    one = oracle.manybody(xy, S, 1.0, 1.0, 0.5).reshape(-1, 2)
+    assert np.max(np.abs(one - exact)) > 1e-3
+
+
+def test_near_field_jitter_and_clamp(oracle):
+    # coincident points kick along the point's own jitter vector r_i, and the
+    # near-field velocity is then scaled by (1 + r) and capped at 2h
+    xy = np.array([500.0, 500.0, 500.0, 500.0], np.float32)
+    r = np.array([3e-6, -4e-6, 1e-6, 1e-6], np.float32)
+    v = oracle.manybody(xy, 4096.0, 1.0, 1.0, 0.1, jitter=r).reshape(-1, 2)
+    rr = r.astype(np.float64).reshape(-1, 2)[0]
+    l = math.sqrt(rr @ rr)  # l < 1 -> sqrt
+    w = 1.0 / math.sqrt(l) * rr / math.sqrt(rr @ rr)
+    w = w + w * rr
+    h = 4096.0 / 8
+    if np.linalg.norm(w) > 2 * h:
+        w = w * 2 * h / np.linalg.norm(w)
+    np.testing.assert_allclose(v[0], w, rtol=1e-9)
+
+
+def test_gravity_center_mouse_match_shader_trig(oracle):
+    # the shaders write these with atan2/cos/sin; the oracle uses the closed form
+    rng = np.random.default_rng(3)
+    xy = rng.uniform(0, 4096, 64)
+    P = xy.reshape(-1, 2)
+    S, a, g = 4096.0, 0.7, 0.25
+    v = oracle.gravity(xy, S, a, g).reshape(-1, 2)
+    for (x, y), w in zip(P, v):
+        dx, dy = S / 2 - x, S / 2 - y
+        d, th = math.hypot(dx, dy), math.atan2(dy, dx)
+        np.testing.assert_allclose(w, [a * g * d * 0.1 * math.cos(th), a * g * d * 0.1 * math.sin(th)],
+                                   rtol=1e-9, atol=1e-9)
+    v = oracle.center(xy, a, 0.3).reshape(-1, 2)
+    cm = P.mean(axis=0)
+    for (x, y), w in zip(P, v):
+        dx, dy = cm[0] - x, cm[1] - y
+        d, th = math.hypot(dx, dy), math.atan2(dy, dx)
+        np.testing.assert_allclose(w, [a * 0.3 * d * 0.01 * math.cos(th), a * 0.3 * d * 0.01 * math.sin(th)],
+                                   rtol=1e-9, atol=1e-9)
+    mx, my, k = 2000.0, 2100.0, 2.0
+    P2 = np.vstack([P, [[mx, my], [mx + 3, my]]])
+    v = oracle.mouse(P2.reshape(-1), mx, my, k).reshape(-1, 2)
+    for (x, y), w in zip(P2, v):
+        dx, dy = mx - x, my - y
+        d = max(math.hypot(dx, dy), 10.0)
+        th = math.atan2(dy, dx)
+        np.testing.assert_allclose(w, [-100 * k / d**2 * math.cos(th), -100 * k / d**2 * math.sin(th)],
+                                   rtol=1e-9, atol=1e-12)
+
+
+def test_center_skips_absent(oracle):
+    xy = np.array([0.0, 0.0, 10.0, 0.0, np.nan, np.nan])
+    v = oracle.center(xy, 1.0, 1.0, absent=np.array([0, 0, 1], np.uint8)).reshape(-1, 2)
+    np.testing.assert_allclose(v[0], [0.05, 0.0])
+    np.testing.assert_allclose(v[1], [-0.05, 0.0])
+
+
+LINKS = np.array([0, 1, 1, 2, 2, 0, 0, 1, 3, 3, 1.5, 2, -1, 0, 7, 1, 4, 0], np.float32)
+
+
+def test_link_csr_hand_fixture(oracle):
+    # self-loop (3,3), duplicate (0,1), fractional (1.5), negative (-1) and
+    # out-of-range (7) endpoints; n = 5.  Invalid links keep their index.
+    n = 5
+    off, nbr, lof, deg = oracle.link_csr(n, LINKS, 0)  # INCOMING instance: source -> targets
+    assert off.tolist() == [0, 2, 3, 4, 5, 6]
+    assert nbr.tolist() == [1, 1, 2, 0, 3, 0]
+    assert lof.tolist() == [0, 3, 1, 2, 4, 8]
+    assert deg.tolist() == [4, 3, 2, 2, 1]
+    off, nbr, lof, _ = oracle.link_csr(n, LINKS, 1)  # OUTGOING instance: target -> sources
+    assert off.tolist() == [0, 2, 4, 5, 6, 6]
+    assert nbr.tolist() == [2, 4, 0, 0, 1, 3]
+    assert lof.tolist() == [2, 8, 0, 3, 1, 4]
+
+
+def test_link_props_hand_fixture(oracle):
+    n = 5
+    off, nbr, lof, deg = oracle.link_csr(n, LINKS, 0)
+    bias, st = oracle.link_props(n, off, nbr, lof, deg)
+    # entry 0: p=0 (deg 4) -> q=1 (deg 3): bias 3/7, strength sqrt(1/min(3,4))
+    assert bias[0] == np.float32(3 / 7)
+    assert st[0] == np.float32(math.sqrt(1 / 3))
+    # self-loop p=3 -> q=3: bias 1/2, strength sqrt(1/2)
+    assert bias[4] == np.float32(0.5) and st[4] == np.float32(math.sqrt(0.5))
+    # user strengths: finite and >= 0 used as is (sqrt stored); else the default
+    user = np.array([4.0, np.nan, 9.0, -1.0, 0.0, 1, 1, 1, np.inf], np.float32)
+    bias, st = oracle.link_props(n, off, nbr, lof, deg, strength=user)
+    # entries in IN order are links 0, 3, 1, 2, 4, 8 (link 1: p1 deg 3 -> q2 deg 2)
+    assert st.tolist() == pytest.approx([2.0, math.sqrt(1 / 3), math.sqrt(1 / 2), 3.0, 0.0, 1.0], rel=1e-7)
+
+
+def test_link_single_spring_closed_form(oracle):
+    # one link shorter than 0.99*D: l = 0.99*D, t = -1/99 * spring * alpha * s * b
+    xy = np.array([100.0, 100.0, 103.0, 104.0])
+    links = np.array([0, 1], np.float32)
+    r = np.array([0.5], np.float32)
+    D = (0.5 * (1.2 - 1.0) + 1.0) * 10.0
+    v = oracle.link(xy, links, 0, r, alpha=0.9, spring=1.0, distance=10.0).reshape(-1, 2)
+    t = (0.99 * D - D) / (0.99 * D) * 1.0 * 0.9 * 1.0 * 0.5  # strength sqrt(1/1), bias 1/2
+    np.testing.assert_allclose(v[0], np.array([3.0, 4.0]) * t, rtol=1e-7)
+    # longer than D: attractive, t = (|x| - D)/|x| * ...
+    xy = np.array([100.0, 100.0, 130.0, 140.0])
+    v = oracle.link(xy, links, 1, r, alpha=0.9).reshape(-1, 2)  # OUT instance moves the target
+    t = (50.0 - D) / 50.0 * 0.9 * 0.5
+    np.testing.assert_allclose(v[1], np.array([-30.0, -40.0]) * t, rtol=1e-7)
+
+
+def test_link_gauss_seidel_order(oracle):
+    # a point with two links: the second term sees the first term's velocity
+    xy = np.array([0.0, 0.0, 100.0, 0.0, 0.0, 100.0])
+    links = np.array([0, 1, 0, 2], np.float32)
+    r = np.array([0.0, 1.0], np.float32)
+    v = oracle.link(xy, links, 0, r, alpha=1.0, spring=1.0, distance=10.0, lo=1.0, hi=1.2).reshape(-1, 2)
+    # degrees: p0 = 2, p1 = p2 = 1; bias = 1/3; strength = sqrt(1/1)
+    b = 1 / 3
+    x1 = np.array([100.0, 0.0])
+    t1 = (100.0 - 10.0) / 100.0 * b
+    v1 = x1 * t1
+    x2 = np.array([0.0, 100.0]) - v1
+    D2 = 12.0
+    l2 = np.linalg.norm(x2)
+    t2 = (l2 - D2) / l2 * b
+    np.testing.assert_allclose(v[0], v1 + x2 * t2, rtol=1e-6)
+
+
+def test_link_skips_absent_neighbour(oracle):
+    xy = np.array([0.0, 0.0, 100.0, 0.0, np.nan, np.nan])
+    links = np.array([0, 1, 0, 2], np.float32)
+    r = np.array([0.0, 0.0], np.float32)
+    absent = np.array([0, 0, 1], np.uint8)
+    v = oracle.link(xy, links, 0, r, alpha=1.0, absent=absent).reshape(-1, 2)
+    # degree still counts the link to the absent point: deg p0 = 2 -> bias 1/3
+    t = (100.0 - 10.0) / 100.0 * (1 / 3)
+    np.testing.assert_allclose(v[0], [100.0 * t, 0.0], rtol=1e-6)
+
+
+def test_alpha_decay(oracle):
+    a, d = 1.0, 5000.0
+    f = 1 - 0.001 ** (1 / d)
+    assert f == pytest.approx(1.380597e-3, rel=1e-6)
+    trace = []
+    for _ in range(5000):
+        a = oracle.alpha_step(a, d)
+        trace.append(a)
+    assert trace[299] == pytest.approx(0.660693, rel=1e-5)
+    assert trace[999] == pytest.approx(0.251189, rel=1e-5)
+    first = next(t for t, x in enumerate(trace, 1) if x < 0.001)
+    assert first == 5000
+    assert trace[499] == pytest.approx((1 - f) ** 500, rel=1e-12)
+
+
+def test_collision_two_point_closed_form(oracle):
+    S = 4096.0
+    xy = np.array([100.0, 100.0, 103.0, 100.0])
+    sizes = np.array([4.0, 4.0], np.float32)
+    G, cs = oracle.collision_grid(S, 0.0, 0.0, 4.0)
+    assert (G, cs) == (512, 8.0)
+    v = oracle.collision(xy, sizes, S, alpha=1.0, strength=1.0).reshape(-1, 2)
+    rho, d = 4.0, 3.0
+    f = min(1.0 * 1.0 * math.sqrt((rho - d) / rho) * rho * 0.5 * 0.25 * 1, rho / 2)
+    u = min(f, 0.1 * 2.0)  # per-pass clamp to 0.1 * own radius
+    np.testing.assert_allclose(v[0], [-4 * u, 0.0], rtol=1e-12)
+    np.testing.assert_allclose(v[1], [4 * u, 0.0], rtol=1e-12)
+
+
+def test_collision_coincident_golden_angle(oracle):
+    xy = np.array([200.0, 200.0, 200.0, 200.0])
+    sizes = np.array([4.0, 4.0], np.float32)
+    v = oracle.collision(xy, sizes, 4096.0, alpha=1.0, strength=0.5).reshape(-1, 2)
+    rho = 4.0
+    f = min(1.0 * 0.5 * rho * 0.1, rho * 0.3)
+    for i in range(2):
+        ang = float(np.float32(i) * np.float32(0.618033988749895))
+        u = np.array([math.cos(ang), math.sin(ang)]) * f
+        nu = np.linalg.norm(u)
+        if nu > 0.2:
+            u *= 0.2 / nu
+        np.testing.assert_allclose(v[i], 4 * u, rtol=1e-6)
+
+
+def test_integrate(oracle):
+    xy = np.array([10.0, 10.0, 4090.0, 5.0, 1.0, 1.0, np.nan, np.nan])
+    v = np.array([1.0, -2.0, 20.0, -20.0, 5.0, 5.0, 1.0, 1.0])
+    out = oracle.integrate(xy, v, 0.85, 4096.0, pinned=np.array([0, 0, 1, 0], np.uint8),
+                           absent=np.array([0, 0, 0, 1], np.uint8))
+    np.testing.assert_allclose(out[:6], [10.85, 8.3, 4096.0, 0.0, 1.0, 1.0])
+    assert np.isnan(out[6:]).all()
