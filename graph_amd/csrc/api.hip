@@ -326,7 +326,7 @@ int cg_step(cg_ctx* x, const cg_sim_params* p, uint32_t nticks, const float* see
     CG_HIP(c, hipSetDevice(c->device));
     const double decay_factor = 1.0 - std::pow(0.001, 1.0 / (double)p->decay);
     const bool have_links = c->links_ready && c->m > 0;  // linksTextureSize != 0
-    int rc;
+    int rc = CG_OK;
     for (uint32_t t = 0; t < nticks; ++t) {
         const float a = (float)c->alpha;
         const float seed = seeds ? seeds[t] : rng_unit(c);

@@ -205,7 +205,7 @@ int col_prepare(Ctx* c, const cg_sim_params* p) {
 int col_run(Ctx* c, const cg_sim_params* p, float alpha, const float2* pin, float2* pout,
             float2* vout) {
     if (c->np == 0) return CG_OK;
-    int rc;
+    int rc = CG_OK;
     // re-derive grid and sizes when the inputs that shape them change
     {
         double R = p->collision_radius, pad = p->collision_padding;

@@ -421,7 +421,7 @@ int mb_alloc(Ctx* c) {
 int mb_run(Ctx* c, const cg_sim_params* p, float alpha, float seed, const float2* pin,
            float2* pout, float2* vout, bool keys_ready) {
     if (c->np == 0) return CG_OK;
-    int rc;
+    int rc = CG_OK;
     prof_begin(c, CG_STAGE_MB_SORT);
     if (!keys_ready) {
         rc = keys_only(c, pin, p->space_size);

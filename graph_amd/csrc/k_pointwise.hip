@@ -153,7 +153,7 @@ int pw_run(Ctx* c, const cg_sim_params* p, int force, float alpha, const float2*
     unsigned nb = blocks_for(c->np, 256);
     if (force == CG_FORCE_CENTER) {
         if (!c->center_part) {
-            int rc;
+            int rc = CG_OK;
             c->center_part = (double*)dalloc(c, sizeof(double) * 2 * CENTER_BLOCKS, &rc);
             if (rc) return rc;
             c->center_sum = (double*)dalloc(c, sizeof(double) * 4, &rc);
