@@ -90,6 +90,7 @@ _SIGNATURES = {
     "cg_debug_link_csr": (C.c_int, [_CTX, C.c_int32, _U, _U, _F, _F, _U, C.c_uint32, _U]),
     "cg_device_bytes": (C.c_uint64, [_CTX]),
     "cg_set_profiling": (C.c_int, [_CTX, C.c_uint32]),
+    "cg_debug_link_stats": (C.c_int, [_CTX, _U]),
     "cg_get_profile": (C.c_int, [_CTX, _D, _U, C.c_uint32]),
 }
 
@@ -265,8 +266,8 @@ class Context:
         e = E.value
         return off, nbr[:e], bias[:e], strength[:e], deg[:n]
 
-    def set_profiling(self, stages) -> None:
-        mask = 0
+    def set_profiling(self, stages, link_stats: bool = False) -> None:
+        mask = (1 << 31) if link_stats else 0
         for s in stages:
             mask |= 1 << (STAGES.index(s) if isinstance(s, str) else int(s))
         self._check(self._l.cg_set_profiling(self._ctx, mask))
@@ -276,6 +277,15 @@ class Context:
         cnt = np.zeros(len(STAGES), np.uint32)
         self._check(self._l.cg_get_profile(self._ctx, dptr(ms), uptr(cnt), len(STAGES)))
         return {s: (float(ms[i]), int(cnt[i])) for i, s in enumerate(STAGES) if cnt[i]}
+
+    def debug_link_stats(self) -> dict:
+        out = np.zeros(8, np.uint32)
+        self._check(self._l.cg_debug_link_stats(self._ctx, uptr(out)))
+        t = {}
+        for name, o in (("wave", 0), ("block", 4)):
+            s, n, mx, cap = (int(v) for v in out[o:o + 4])
+            t[name] = {"mean_sweeps": s / n if n else 0.0, "points": n, "max_sweeps": mx, "at_cap": cap}
+        return t
 
     def device_bytes(self) -> int:
         return int(self._l.cg_device_bytes(self._ctx))

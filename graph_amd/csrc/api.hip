@@ -170,6 +170,9 @@ int cg_create(cg_ctx** out, const cg_create_params* params) {
     c->budget = params ? params->hbm_budget : 0;
     hipError_t e = hipSetDevice(c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
         delete x;
         return CG_ERR_DEVICE;
@@ -184,6 +187,10 @@ int cg_destroy(cg_ctx* x) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     for (auto& a : c->allocs) (void)hipFree(a.first);
     c->allocs.clear();
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -475,9 +482,21 @@ int cg_debug_link_csr(cg_ctx* x, int32_t dir, uint32_t* offsets, uint32_t* nbr, 
     return CG_OK;
 }
 
+int cg_debug_link_stats(cg_ctx* x, uint32_t* out8) {
+    if (!x || !out8) return CG_ERR_INVALID;
+    Ctx* c = &x->c;
+    std::memset(out8, 0, 8 * sizeof(uint32_t));
+    if (!c->link_stats) return CG_OK;
+    CG_HIP(c, hipSetDevice(c->device));
+    CG_HIP(c, hipMemcpyAsync(out8, c->link_stats, 32, hipMemcpyDeviceToHost, c->stream));
+    CG_HIP(c, hipMemsetAsync(c->link_stats, 0, 64, c->stream));
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    return CG_OK;
+}
+
 int cg_set_profiling(cg_ctx* x, uint32_t mask) {
     if (!x) return CG_ERR_INVALID;
-    x->c.prof_mask = mask & ((1u << CG_STAGES) - 1);
+    x->c.prof_mask = mask & (((1u << CG_STAGES) - 1) | CG_PROFILE_LINK_STATS);
     return CG_OK;
 }
 

@@ -34,11 +34,17 @@ struct LinkDir {
     uint32_t* soff = nullptr;    // [n_sched+1] offsets of each schedule slot into ent
     uint32_t n_sched = 0;
     uint32_t n_lane = 0, n_wave = 0, n_block = 0;  // tier sizes, in schedule order
+    uint32_t e_chunk0 = 0;       // first entry (in ent) of the chunked tiers
+    uint32_t* poff = nullptr;    // [n_wave+n_block+1] step-major offsets into link_pre
+    uint32_t* eslot = nullptr;   // schedule slot of each chunked-tier entry
+    float2* warm = nullptr;      // per chunk-tier lane: last converged chunk start
 };
 
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;     // second stream (default priority) for the hub link tier
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint64_t budget = 0;
     uint64_t held = 0;
     std::string err;
@@ -94,6 +100,8 @@ struct Ctx {
     uint32_t* degree = nullptr;
     LinkDir dir[2];
     bool links_ready = false;
+    float4* link_pre = nullptr;     // per-entry (q.x, q.y, D, K) of the chunked tiers
+    uint32_t* link_stats = nullptr; // multiple-shooting sweep counters (cg_debug_link_stats)
 
     // stage timing (cg_set_profiling)
     uint32_t prof_mask = 0;

@@ -44,6 +44,11 @@ __global__ void k_validate(const float2* __restrict__ links, uint32_t m, uint32_
     }
 }
 
+__global__ void k_fill_words(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 __global__ void k_degree(const uint32_t* __restrict__ outdeg, const uint32_t* __restrict__ indeg,
                          uint32_t n, uint32_t* __restrict__ degree) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -60,12 +65,13 @@ __global__ void k_iota_keys(const uint32_t* __restrict__ owner, uint32_t m,
 
 // Entry e of the point-ordered CSR (ForceLink/index.ts:58-79): neighbour,
 // random draw, strength, bias — bias and strength in f64, stored as f32.
+// Also records the entry's owner and whether its neighbour is present.
 __global__ void k_entries(const uint32_t* __restrict__ sorted_owner,
                           const uint32_t* __restrict__ sorted_link, uint32_t E,
                           const uint32_t* __restrict__ other_of, const uint32_t* __restrict__ degree,
                           const float* __restrict__ strength, const float* __restrict__ rnd,
                           const uint8_t* __restrict__ flags, uint4* __restrict__ ent,
-                          uint32_t* __restrict__ live) {
+                          uint32_t* __restrict__ owner_out, uint32_t* __restrict__ live_flag) {
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     uint32_t l = sorted_link[e];
@@ -85,41 +91,52 @@ __global__ void k_entries(const uint32_t* __restrict__ sorted_owner,
     }
     ent[e] = make_uint4(q, __float_as_uint(rnd[l]), __float_as_uint((float)sqrt(s)),
                         __float_as_uint((float)bias));
-    if (!(flags[q] & F_ABSENT)) atomicAdd(&live[p], 1u);
+    owner_out[e] = p;
+    // only entries of a present owner toward a present neighbour are scheduled
+    // (an absent owner has no schedule slot; an absent neighbour is skipped,
+    // force-spring.ts:69-74)
+    live_flag[e] = ((flags[q] | flags[p]) & F_ABSENT) ? 0u : 1u;
 }
 
+// Live (present-neighbour) entry count per present point, from the scan of live flags.
 __global__ void k_sched_keys(const uint32_t* __restrict__ present, uint32_t np,
-                             const uint32_t* __restrict__ live, uint32_t* __restrict__ keys,
-                             uint32_t* __restrict__ vals) {
+                             const uint32_t* __restrict__ off_pt, const uint32_t* __restrict__ lrank,
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np) return;
     uint32_t p = present[k];
-    keys[k] = live[p];
+    keys[k] = lrank[off_pt[p + 1]] - lrank[off_pt[p]];
     vals[k] = p;
 }
 
-__global__ void k_tier_count(const uint32_t* __restrict__ skeys, uint32_t ns,
-                             uint32_t* __restrict__ counts) {
+// Tier boundaries in the degree-sorted schedule: first position above LANE_MAX
+// and first above WAVE_MAX (counts pre-filled with ns).  One writer each.
+__global__ void k_tier_bounds(const uint32_t* __restrict__ skeys, uint32_t ns,
+                              uint32_t* __restrict__ counts) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
-    uint32_t d = skeys[k];
-    if (d <= LANE_MAX) atomicAdd(&counts[0], 1u);
-    else if (d <= WAVE_MAX) atomicAdd(&counts[1], 1u);
+    uint32_t d = skeys[k], prev = k ? skeys[k - 1] : 0u;
+    if (d > LANE_MAX && (k == 0 || prev <= LANE_MAX)) counts[0] = k;
+    if (d > WAVE_MAX && (k == 0 || prev <= WAVE_MAX)) counts[1] = k;
 }
 
-// Copy each scheduled point's live entries, in link order, to its schedule slot.
-__global__ void k_permute(const uint32_t* __restrict__ sched, uint32_t ns,
-                          const uint32_t* __restrict__ soff, const uint32_t* __restrict__ off_pt,
-                          const uint4* __restrict__ ent_pt, const uint8_t* __restrict__ flags,
-                          uint4* __restrict__ ent) {
+__global__ void k_slot_of(const uint32_t* __restrict__ sched, uint32_t ns,
+                          uint32_t* __restrict__ slot_of) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ns) return;
-    uint32_t p = sched[j];
-    uint32_t dst = soff[j];
-    for (uint32_t e = off_pt[p]; e < off_pt[p + 1]; ++e) {
-        uint4 v = ent_pt[e];
-        if (!(flags[v.x] & F_ABSENT)) ent[dst++] = v;
-    }
+    if (j < ns) slot_of[sched[j]] = j;
+}
+
+// Entry-parallel copy of live entries, in link order, to their schedule slot.
+__global__ void k_permute(const uint4* __restrict__ ent_pt, uint32_t E,
+                          const uint32_t* __restrict__ owner, const uint32_t* __restrict__ live_flag,
+                          const uint32_t* __restrict__ lrank, const uint32_t* __restrict__ off_pt,
+                          const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ soff,
+                          uint4* __restrict__ ent) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E || !live_flag[e]) return;
+    uint32_t p = owner[e];
+    uint32_t dst = soff[slot_of[p]] + (lrank[e] - lrank[off_pt[p]]);
+    ent[dst] = ent_pt[e];
 }
 
 // ---- spring pass ------------------------------------------------------------------
@@ -161,74 +178,164 @@ __global__ void __launch_bounds__(256) k_link_lane(LkArgs a, const uint32_t* __r
     const uint32_t b = soff[j], e = soff[j + 1];
     const float2 pp = pin[p];
     float2 v = make_float2(0.f, 0.f);
-    for (uint32_t k = b; k < e; ++k) {
-        uint4 en = ent[k];
-        float2 q = pin[en.x];
-        float x, y;
-        float t = spring_term(a, en, pp, v, q, x, y);
-        v.x += x * t;
-        v.y += y * t;
+    constexpr int G = 8;  // entries fetched together: G entry loads, then G neighbour gathers
+    for (uint32_t k = b; k < e; k += G) {
+        uint4 en[G];
+        float2 q[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) en[u] = ent[min(k + u, e - 1)];
+#pragma unroll
+        for (int u = 0; u < G; ++u) q[u] = pin[en[u].x];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            float x, y;
+            float t = spring_term(a, en[u], pp, v, q[u], x, y);
+            t = (k + u < e) ? t : 0.f;
+            v.x += x * t;
+            v.y += y * t;
+        }
     }
     if (vout) vout[p] = v;
     if (pout) pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, v, a.friction, a.S);
 }
 
-constexpr int MAX_SWEEPS = 12;
+constexpr int MAX_SWEEPS = 8;
+constexpr int PREFETCH = 16;  // entries in flight per lane in the chunk loop
+
+// Per-entry terms that do not depend on the running velocity, gathered in one
+// coalesced pass for the chunked tiers: (q.x, q.y, D, K) with D the rest length
+// and K = linkSpring * alpha * strength * bias.  Stored step-major per point:
+// entry k of a point with d entries run by B lanes (chunk c = ceil(d/B)) sits at
+// poff[slot] + (k % c) * B + k / c, so lane t reads its step-s term at s*B + t
+// and a wave's loads are one contiguous segment.
+__global__ void __launch_bounds__(256) k_link_pre(LkArgs a, const uint4* __restrict__ ent,
+                                                  uint32_t e0, uint32_t e1,
+                                                  const uint32_t* __restrict__ eslot,
+                                                  const uint32_t* __restrict__ soff,
+                                                  const uint32_t* __restrict__ poff,
+                                                  uint32_t first_slot, uint32_t block_slot,
+                                                  const float2* __restrict__ pin,
+                                                  float4* __restrict__ pre) {
+    uint32_t e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    const uint32_t j = eslot[e - e0];
+    const uint32_t local = e - soff[j], d = soff[j + 1] - soff[j];
+    const uint32_t B = j < block_slot ? 64u : 1024u;
+    const uint32_t c = (d + B - 1) / B;
+    const uint32_t dst = poff[j - first_slot] + (local % c) * B + local / c;
+    uint4 en = ent[e];
+    float2 q = pin[en.x];
+    float r = __uint_as_float(en.y), s = __uint_as_float(en.z), b = __uint_as_float(en.w);
+    float D = (r * a.span + a.lo) * a.dist;
+    pre[dst] = make_float4(q.x, q.y, D, a.spring_alpha * s * b);
+}
+
+// Schedule slot of every chunked-tier entry (built once per link set): the
+// last slot j in [first, last) with soff[j] <= e, by binary search.
+__global__ void k_eslot(const uint32_t* __restrict__ soff, uint32_t first, uint32_t last,
+                        uint32_t e0, uint32_t e1, uint32_t* __restrict__ eslot) {
+    uint32_t e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    uint32_t lo = first, hi = last;  // invariant: soff[lo] <= e < soff[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (soff[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    eslot[e - e0] = lo;
+}
+
+// One serial spring step from a pre-gathered entry (no change when !live),
+// also applying the step's Jacobian to J (column-major 2x2, J <- M J):
+//   u' = u + x t(u),  x = q - u,  t = K (1 - D / max(|x|, 0.99 D))
+//   M = du'/du = (1 - t) I - (K D / |x|^3) x x^T   (the last term is 0 when clamped)
+__device__ inline void spring_step(float4 e, float2 pp, float2& v, float (&J)[4], bool live) {
+    float x = e.x - (pp.x + v.x), y = e.y - (pp.y + v.y);
+    float l0 = __builtin_amdgcn_sqrtf(x * x + y * y);
+    bool clamp = l0 < e.z * 0.99f;
+    float l = clamp ? e.z * 0.99f : l0;
+    float r = __builtin_amdgcn_rcpf(l);
+    float t = (1.f - e.z * r) * e.w;
+    float w = clamp ? 0.f : e.w * e.z * r * r * r;
+    t = live ? t : 0.f;
+    w = live ? w : 0.f;
+    v.x += x * t;
+    v.y += y * t;
+    float a = 1.f - t;
+    float m00 = a - w * x * x, m01 = -w * x * y, m11 = a - w * y * y;
+    float j0 = m00 * J[0] + m01 * J[1], j1 = m01 * J[0] + m11 * J[1];
+    float j2 = m00 * J[2] + m01 * J[3], j3 = m01 * J[2] + m11 * J[3];
+    J[0] = j0; J[1] = j1; J[2] = j2; J[3] = j3;
+}
 
 template <int B>
 __global__ void __launch_bounds__(B) k_link_chunk(LkArgs a, const uint32_t* __restrict__ sched,
                                                   const uint32_t* __restrict__ soff,
-                                                  const uint4* __restrict__ ent, uint32_t first,
+                                                  const uint32_t* __restrict__ poff,
+                                                  const float4* __restrict__ pre,
+                                                  uint32_t first_slot, uint32_t first,
                                                   const float2* __restrict__ pin,
                                                   const uint8_t* __restrict__ flags,
+                                                  float2* __restrict__ warm,
+                                                  uint32_t* __restrict__ stats,
                                                   float2* __restrict__ pout,
                                                   float2* __restrict__ vout) {
-    __shared__ float sA[B], sX[B], sY[B];
+    __shared__ float sJ[4][B], sX[B], sY[B];
     __shared__ float s_delta[B / 64];
     __shared__ float2 s_final;
     const int t = threadIdx.x;
     const uint32_t j = first + blockIdx.x;
+    // chunk starts converged on the previous call: the first guesses
+    float2* __restrict__ wslot = warm + (size_t)blockIdx.x * B + t;
     const uint32_t p = sched[j];
-    const uint32_t base = soff[j];
-    const uint32_t d = soff[j + 1] - base;
+    const uint32_t d = soff[j + 1] - soff[j];
     const uint32_t c = (d + B - 1) / B;
     const uint32_t lo = min(d, (uint32_t)t * c), hi = min(d, lo + c);
+    const uint32_t steps = hi - lo;
     const int last = (int)((d - 1) / c);
     const float2 pp = pin[p];
-    float2 g = make_float2(0.f, 0.f);
-    for (int sweep = 0; sweep <= MAX_SWEEPS; ++sweep) {
+    const float4* __restrict__ mine = pre + poff[j - first_slot] + t;
+    float2 g = *wslot;
+    float prev = INFINITY;
+    for (int sweep = 0;; ++sweep) {
         float2 v = g;
-        float A = 1.f;
-        for (uint32_t k = lo; k < hi; ++k) {
-            uint4 en = ent[base + k];
-            float2 q = pin[en.x];
-            float x, y;
-            float tk = spring_term(a, en, pp, v, q, x, y);
-            v.x += x * tk;
-            v.y += y * tk;
-            A *= (1.f - tk);
+        float J[4] = {1.f, 0.f, 0.f, 1.f};
+        for (uint32_t k = 0; k < steps; k += PREFETCH) {
+            // unconditional loads (index clamped into the chunk) so all are in flight together
+            float4 buf[PREFETCH];
+#pragma unroll
+            for (int u = 0; u < PREFETCH; ++u) buf[u] = mine[(size_t)min(k + u, steps - 1) * B];
+#pragma unroll
+            for (int u = 0; u < PREFETCH; ++u) spring_step(buf[u], pp, v, J, k + u < steps);
         }
-        if (sweep == MAX_SWEEPS || (sweep > 0 && s_delta[0] <= a.tol)) {
-            // g is converged: v is this chunk's exact serial end from g
-            if (t == last) s_final = v;
-            break;
-        }
-        // chunk map  start -> A*start + (v - A*g); inclusive scan of the composition
-        float cA = A, cx = v.x - A * g.x, cy = v.y - A * g.y;
-        sA[t] = cA;
+        // chunk map  s -> J (s - g) + v  =  J s + b;  inclusive scan of the composition
+        float cJ[4] = {J[0], J[1], J[2], J[3]};
+        float cx = v.x - (J[0] * g.x + J[2] * g.y), cy = v.y - (J[1] * g.x + J[3] * g.y);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sJ[q][t] = cJ[q];
         sX[t] = cx;
         sY[t] = cy;
         __syncthreads();
         for (int off = 1; off < B; off <<= 1) {
-            float pA = 1.f, px = 0.f, py = 0.f;
-            if (t >= off) { pA = sA[t - off]; px = sX[t - off]; py = sY[t - off]; }
+            float pJ[4] = {1.f, 0.f, 0.f, 1.f}, px = 0.f, py = 0.f;
+            if (t >= off) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) pJ[q] = sJ[q][t - off];
+                px = sX[t - off];
+                py = sY[t - off];
+            }
             __syncthreads();
             if (t >= off) {
-                // mine after theirs: s -> cA*(pA*s + p) + c
-                cx = cA * px + cx;
-                cy = cA * py + cy;
-                cA = cA * pA;
-                sA[t] = cA;
+                // mine after theirs: s -> cJ (pJ s + p) + c
+                float nx = cJ[0] * px + cJ[2] * py + cx;
+                float ny = cJ[1] * px + cJ[3] * py + cy;
+                float n0 = cJ[0] * pJ[0] + cJ[2] * pJ[1], n1 = cJ[1] * pJ[0] + cJ[3] * pJ[1];
+                float n2 = cJ[0] * pJ[2] + cJ[2] * pJ[3], n3 = cJ[1] * pJ[2] + cJ[3] * pJ[3];
+                cJ[0] = n0; cJ[1] = n1; cJ[2] = n2; cJ[3] = n3;
+                cx = nx;
+                cy = ny;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) sJ[q][t] = cJ[q];
                 sX[t] = cx;
                 sY[t] = cy;
             }
@@ -236,20 +343,35 @@ __global__ void __launch_bounds__(B) k_link_chunk(LkArgs a, const uint32_t* __re
         }
         float2 start = (t == 0) ? make_float2(0.f, 0.f) : make_float2(sX[t - 1], sY[t - 1]);
         float dlt = fmaxf(fabsf(start.x - g.x), fabsf(start.y - g.y));
-        if (hi <= lo) dlt = 0.f;
-        g = start;
+        if (steps == 0) dlt = 0.f;
         // block max of the start corrections
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) dlt = fmaxf(dlt, __shfl_xor(dlt, o, 64));
-        __syncthreads();
         if ((t & 63) == 0) s_delta[t >> 6] = dlt;
         __syncthreads();
-        if (t == 0) {
-            float mx = s_delta[0];
-            for (int w = 1; w < B / 64; ++w) mx = fmaxf(mx, s_delta[w]);
-            s_delta[0] = mx;
+        float mx = s_delta[0];
+        for (int w = 1; w < B / 64; ++w) mx = fmaxf(mx, s_delta[w]);
+        // converged, at the f32 noise floor (no longer halving), or at the cap
+        if (mx <= a.tol || (sweep >= 2 && mx > 0.5f * prev) || sweep == MAX_SWEEPS) {
+            // the last chunk's end, moved to its corrected start
+            if (t == last) {
+                float ex = start.x - g.x, ey = start.y - g.y;
+                s_final = make_float2(v.x + J[0] * ex + J[2] * ey, v.y + J[1] * ex + J[3] * ey);
+            }
+            *wslot = start;
+            if (stats && t == 0) {
+                // sweeps used, points, max sweeps, points not within tolerance (per tier)
+                const int o = B == 64 ? 0 : 4;
+                atomicAdd(&stats[o + 0], (uint32_t)(sweep + 1));
+                atomicAdd(&stats[o + 1], 1u);
+                atomicMax(&stats[o + 2], (uint32_t)(sweep + 1));
+                if (mx > a.tol) atomicAdd(&stats[o + 3], 1u);
+            }
+            break;
         }
-        __syncthreads();
+        prev = mx;
+        g = start;
+        __syncthreads();  // s_delta is rewritten next sweep
     }
     __syncthreads();
     if (t == 0) {
@@ -267,19 +389,18 @@ int link_build(Ctx* c) {
     for (int d = 0; d < 2; ++d) {
         LinkDir& L = c->dir[d];
         dfree(c, L.ent_pt); dfree(c, L.off_pt); dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff);
+        dfree(c, L.poff); dfree(c, L.eslot); dfree(c, L.warm);
         L = LinkDir();
     }
     dfree(c, c->degree);
+    dfree(c, c->link_pre);
     c->degree = nullptr;
+    c->link_pre = nullptr;
     if (!c->links_set || c->n == 0) return CG_OK;
     const uint32_t n = c->n, m = c->m;
     int rc = CG_OK;
     c->degree = (uint32_t*)dalloc(c, (size_t)n * 4, &rc);
     if (rc) return rc;
-    if (m == 0) {
-        CG_HIP(c, hipMemsetAsync(c->degree, 0, (size_t)n * 4, c->stream));
-    }
-    // temporaries
     std::vector<void*> tmp;
     auto talloc = [&](size_t bytes) -> void* {
         void* p = dalloc(c, bytes ? bytes : 16, &rc);
@@ -288,6 +409,7 @@ int link_build(Ctx* c) {
     };
     auto cleanup = [&]() { for (void* p : tmp) dfree(c, p); };
     const size_t mm = m ? m : 1;
+    const size_t kk = std::max<size_t>(mm, (size_t)c->np) + 1;  // CSR sort and schedule sort
     float2* d_links = (float2*)talloc(mm * 8);
     float* d_str = c->h_strength.empty() ? nullptr : (float*)talloc(mm * 4);
     float* d_rin = (float*)talloc(mm * 4);
@@ -296,24 +418,25 @@ int link_build(Ctx* c) {
     uint32_t* tgt = (uint32_t*)talloc(mm * 4);
     uint32_t* outdeg = (uint32_t*)talloc(((size_t)n + 1) * 4);
     uint32_t* indeg = (uint32_t*)talloc(((size_t)n + 1) * 4);
-    const size_t kk = std::max<size_t>(mm, (size_t)c->np) + 1;  // CSR sort and schedule sort
     uint32_t* k0 = (uint32_t*)talloc(kk * 4);
     uint32_t* k1 = (uint32_t*)talloc(kk * 4);
     uint32_t* v0 = (uint32_t*)talloc(kk * 4);
     uint32_t* v1 = (uint32_t*)talloc(kk * 4);
-    uint32_t* live = (uint32_t*)talloc(((size_t)n + 1) * 4);
+    uint32_t* eown = (uint32_t*)talloc(mm * 4);
+    uint32_t* lflag = (uint32_t*)talloc((mm + 1) * 4);
+    uint32_t* lrank = (uint32_t*)talloc((mm + 1) * 4);
+    uint32_t* slot_of = (uint32_t*)talloc(((size_t)n + 1) * 4);
     uint32_t* counts = (uint32_t*)talloc(64);
     if (rc) { cleanup(); return rc; }
+    CG_HIP(c, hipMemsetAsync(outdeg, 0, ((size_t)n + 1) * 4, c->stream));
+    CG_HIP(c, hipMemsetAsync(indeg, 0, ((size_t)n + 1) * 4, c->stream));
+    CG_HIP(c, hipMemsetAsync(c->degree, 0, (size_t)n * 4, c->stream));
     if (m) {
         CG_HIP(c, hipMemcpyAsync(d_links, c->h_links.data(), (size_t)m * 8, hipMemcpyHostToDevice, c->stream));
         if (d_str)
             CG_HIP(c, hipMemcpyAsync(d_str, c->h_strength.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
         CG_HIP(c, hipMemcpyAsync(d_rin, c->h_rin.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
         CG_HIP(c, hipMemcpyAsync(d_rout, c->h_rout.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
-    }
-    CG_HIP(c, hipMemsetAsync(outdeg, 0, ((size_t)n + 1) * 4, c->stream));
-    CG_HIP(c, hipMemsetAsync(indeg, 0, ((size_t)n + 1) * 4, c->stream));
-    if (m) {
         hipLaunchKernelGGL(k_validate, dim3(blocks_for(m, 256)), dim3(256), 0, c->stream, d_links, m,
                            n, src, tgt, outdeg, indeg);
         hipLaunchKernelGGL(k_degree, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, outdeg, indeg,
@@ -321,18 +444,19 @@ int link_build(Ctx* c) {
     }
     int nbits = 1;
     while (nbits < 32 && (1ull << nbits) <= (unsigned long long)n) ++nbits;  // n itself must fit
-    // sort / scan temp storage sized for the largest use
-    size_t sort_bytes = 0, scan_bytes = 0, sched_bytes = 0;
+    size_t sort_bytes = 0, scan_bytes = 0, scan2_bytes = 0, sched_bytes = 0;
     CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, v0, v1, (int)mm, 0,
                                                  nbits, c->stream));
     CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, outdeg, outdeg, (int)n + 1,
                                                c->stream));
+    CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan2_bytes, lflag, lrank, (int)mm + 1,
+                                               c->stream));
     CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sched_bytes, k0, k1, v0, v1,
                                                  (int)(c->np ? c->np : 1), 0, 32, c->stream));
-    size_t tbytes = std::max(sort_bytes, std::max(scan_bytes, sched_bytes));
+    size_t tbytes = std::max(std::max(sort_bytes, scan_bytes), std::max(scan2_bytes, sched_bytes));
     void* d_tmp = talloc(tbytes);
     if (rc) { cleanup(); return rc; }
-    uint32_t E = 0;
+    size_t pre_need = 0;
     for (int d = 0; d < 2; ++d) {
         LinkDir& L = c->dir[d];
         uint32_t* owner = d == 0 ? src : tgt;
@@ -340,15 +464,15 @@ int link_build(Ctx* c) {
         uint32_t* cnt = d == 0 ? outdeg : indeg;  // entries owned per point
         L.off_pt = (uint32_t*)dalloc(c, ((size_t)n + 1) * 4, &rc);
         if (rc) { cleanup(); return rc; }
-        // offsets: exclusive scan of the per-point counts (cnt[n] == 0)
         size_t sb = tbytes;
         CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, cnt, L.off_pt, (int)n + 1, c->stream));
+        uint32_t E = 0;
         CG_HIP(c, hipMemcpyAsync(&E, L.off_pt + n, 4, hipMemcpyDeviceToHost, c->stream));
         CG_HIP(c, hipStreamSynchronize(c->stream));
         L.E = E;
         L.ent_pt = (uint4*)dalloc(c, (size_t)(E ? E : 1) * 16, &rc);
         if (rc) { cleanup(); return rc; }
-        CG_HIP(c, hipMemsetAsync(live, 0, ((size_t)n + 1) * 4, c->stream));
+        CG_HIP(c, hipMemsetAsync(lflag, 0, (mm + 1) * 4, c->stream));
         if (m) {
             hipLaunchKernelGGL(k_iota_keys, dim3(blocks_for(m, 256)), dim3(256), 0, c->stream, owner,
                                m, k0, v0);
@@ -358,8 +482,10 @@ int link_build(Ctx* c) {
             if (E)
                 hipLaunchKernelGGL(k_entries, dim3(blocks_for(E, 256)), dim3(256), 0, c->stream, k1,
                                    v1, E, other, c->degree, d_str, d == 0 ? d_rin : d_rout,
-                                   c->flags, L.ent_pt, live);
+                                   c->flags, L.ent_pt, eown, lflag);
         }
+        sb = tbytes;
+        CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, lflag, lrank, (int)E + 1, c->stream));
         // schedule: present points by ascending live degree
         const uint32_t ns = c->np;
         L.n_sched = ns;
@@ -368,30 +494,59 @@ int link_build(Ctx* c) {
         if (rc) { cleanup(); return rc; }
         if (ns) {
             hipLaunchKernelGGL(k_sched_keys, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream,
-                               c->present, ns, live, k0, v0);
+                               c->present, ns, L.off_pt, lrank, k0, v0);
             sb = tbytes;
             CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(d_tmp, sb, k0, k1, v0, L.sched, (int)ns, 0,
                                                          32, c->stream));
             CG_HIP(c, hipMemsetAsync(k1 + ns, 0, 4, c->stream));
             sb = tbytes;
             CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, k1, L.soff, (int)ns + 1, c->stream));
-            CG_HIP(c, hipMemsetAsync(counts, 0, 64, c->stream));
-            hipLaunchKernelGGL(k_tier_count, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, k1,
+            hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, c->stream, counts, 2u, ns);
+            hipLaunchKernelGGL(k_tier_bounds, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, k1,
                                ns, counts);
-            uint32_t h[3] = {0, 0, 0};
+            uint32_t h[2] = {0, 0};
             CG_HIP(c, hipMemcpyAsync(h, counts, 8, hipMemcpyDeviceToHost, c->stream));
-            CG_HIP(c, hipMemcpyAsync(&h[2], L.soff + ns, 4, hipMemcpyDeviceToHost, c->stream));
+            CG_HIP(c, hipMemcpyAsync(&L.E_live, L.soff + ns, 4, hipMemcpyDeviceToHost, c->stream));
             CG_HIP(c, hipStreamSynchronize(c->stream));
             L.n_lane = h[0];
-            L.n_wave = h[1];
-            L.n_block = ns - h[0] - h[1];
-            L.E_live = h[2];
+            L.n_wave = h[1] - h[0];
+            L.n_block = ns - h[1];
+            // chunked tiers: padded step-major offsets (host) and entry -> slot map
+            const uint32_t nch = ns - L.n_lane;
+            std::vector<uint32_t> hs(nch + 1), hp(nch + 1, 0);
+            CG_HIP(c, hipMemcpy(hs.data(), L.soff + L.n_lane, (size_t)(nch + 1) * 4, hipMemcpyDeviceToHost));
+            L.e_chunk0 = hs[0];
+            for (uint32_t k = 0; k < nch; ++k) {
+                uint32_t dd = hs[k + 1] - hs[k];
+                uint32_t B = (k < L.n_wave) ? 64u : 1024u;
+                hp[k + 1] = hp[k] + ((dd + B - 1) / B) * B;
+            }
+            pre_need = std::max<size_t>(pre_need, (size_t)hp[nch]);
+            const size_t nwarm = (size_t)L.n_wave * 64 + (size_t)L.n_block * 1024;
+            L.warm = (float2*)dalloc(c, (nwarm ? nwarm : 1) * 8, &rc);
+            if (rc) { cleanup(); return rc; }
+            CG_HIP(c, hipMemsetAsync(L.warm, 0, (nwarm ? nwarm : 1) * 8, c->stream));
+            L.poff = (uint32_t*)dalloc(c, (size_t)(nch + 1) * 4, &rc);
+            L.eslot = (uint32_t*)dalloc(c, (size_t)(L.E_live - L.e_chunk0 + 1) * 4, &rc);
+            if (rc) { cleanup(); return rc; }
+            CG_HIP(c, hipMemcpy(L.poff, hp.data(), (size_t)(nch + 1) * 4, hipMemcpyHostToDevice));
+            if (L.E_live > L.e_chunk0)
+                hipLaunchKernelGGL(k_eslot, dim3(blocks_for(L.E_live - L.e_chunk0, 256)), dim3(256), 0,
+                                   c->stream, L.soff, L.n_lane, ns, L.e_chunk0, L.E_live, L.eslot);
             L.ent = (uint4*)dalloc(c, (size_t)(L.E_live ? L.E_live : 1) * 16, &rc);
             if (rc) { cleanup(); return rc; }
-            hipLaunchKernelGGL(k_permute, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream,
-                               L.sched, ns, L.soff, L.off_pt, L.ent_pt, c->flags, L.ent);
+            hipLaunchKernelGGL(k_slot_of, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.sched,
+                               ns, slot_of);
+            if (E)
+                hipLaunchKernelGGL(k_permute, dim3(blocks_for(E, 256)), dim3(256), 0, c->stream,
+                                   L.ent_pt, E, eown, lflag, lrank, L.off_pt, slot_of, L.soff, L.ent);
         }
         CG_HIP(c, hipGetLastError());
+    }
+    c->link_pre = (float4*)dalloc(c, (pre_need ? pre_need : 1) * 16, &rc);
+    if (!c->link_stats) {
+        c->link_stats = (uint32_t*)dalloc(c, 64, &rc);
+        if (!rc) CG_HIP(c, hipMemsetAsync(c->link_stats, 0, 64, c->stream));
     }
     CG_HIP(c, hipStreamSynchronize(c->stream));
     cleanup();
@@ -411,16 +566,36 @@ int link_run(Ctx* c, const cg_sim_params* p, int dir, float alpha, const float2*
     a.span = p->link_dist_random_variation_range[1] - p->link_dist_random_variation_range[0];
     a.friction = p->friction;
     a.S = p->space_size;
-    a.tol = 1e-7f * p->space_size;
+    a.tol = 2.5e-6f * p->space_size;  // a quarter of the 1e-5 * spaceSize force tolerance
+    // The hub tier (one 1024-lane block per hub) runs on the side stream while the
+    // lane and 64-lane tiers fill the other CUs on the main stream.
+    const uint32_t e0 = L.e_chunk0, e1 = L.E_live;
+    const uint32_t first = L.n_lane, block_slot = L.n_lane + L.n_wave;
+    // sweep counters only while requested (same-address atomics per block)
+    uint32_t* stats = (c->prof_mask & CG_PROFILE_LINK_STATS) ? c->link_stats : nullptr;
+    if (e1 > e0)
+        hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(e1 - e0, 256)), dim3(256), 0, c->stream, a, L.ent,
+                           e0, e1, L.eslot, L.soff, L.poff, first, block_slot, pin, c->link_pre);
+    const bool fork = L.n_block && c->side;
+    if (L.n_block) {
+        hipStream_t s = c->stream;
+        if (fork) {
+            CG_HIP(c, hipEventRecord(c->ev_fork, c->stream));
+            CG_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+            s = c->side;
+        }
+        hipLaunchKernelGGL(k_link_chunk<1024>, dim3(L.n_block), dim3(1024), 0, s, a, L.sched, L.soff,
+                           L.poff, c->link_pre, first, block_slot, pin, c->flags,
+                           L.warm + (size_t)L.n_wave * 64, stats, pout, vout);
+        if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
+    }
+    if (L.n_wave)
+        hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_wave), dim3(64), 0, c->stream, a, L.sched, L.soff,
+                           L.poff, c->link_pre, first, first, pin, c->flags, L.warm, stats, pout, vout);
     if (L.n_lane)
         hipLaunchKernelGGL(k_link_lane, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, a,
                            L.sched, L.soff, L.ent, L.n_lane, pin, c->flags, pout, vout);
-    if (L.n_wave)
-        hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_wave), dim3(64), 0, c->stream, a, L.sched,
-                           L.soff, L.ent, L.n_lane, pin, c->flags, pout, vout);
-    if (L.n_block)
-        hipLaunchKernelGGL(k_link_chunk<1024>, dim3(L.n_block), dim3(1024), 0, c->stream, a, L.sched,
-                           L.soff, L.ent, L.n_lane + L.n_wave, pin, c->flags, pout, vout);
+    if (fork) CG_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     CG_HIP(c, hipGetLastError());
     return CG_OK;
 }

@@ -151,6 +151,12 @@ int cg_debug_slots(cg_ctx* ctx, int32_t* slots, uint32_t capacity, uint32_t* fin
 int cg_debug_link_csr(cg_ctx* ctx, int32_t dir, uint32_t* offsets, uint32_t* nbr, float* bias,
                       float* strength, uint32_t* degree, uint32_t capacity, uint32_t* entries);
 
+/* Multiple-shooting counters of the chunked link tiers since the last call
+ * (then reset), collected while cg_set_profiling has CG_PROFILE_LINK_STATS: out[0..3] for the 64-lane tier and out[4..7] for the 1024-lane
+ * tier = {sweeps summed over points, points, max sweeps, points stopped at the
+ * sweep cap}. */
+int cg_debug_link_stats(cg_ctx* ctx, uint32_t* out8);
+
 /* Bytes of device memory the context holds. */
 uint64_t cg_device_bytes(const cg_ctx* ctx);
 
@@ -170,6 +176,7 @@ uint64_t cg_device_bytes(const cg_ctx* ctx);
 #define CG_STAGE_LINK_OUT 8
 #define CG_STAGE_COLLISION 9
 #define CG_STAGES 10
+#define CG_PROFILE_LINK_STATS (1u << 31) /* also count link sweeps (cg_debug_link_stats) */
 int cg_set_profiling(cg_ctx* ctx, uint32_t stage_mask);
 int cg_get_profile(cg_ctx* ctx, double* ms, uint32_t* launches, uint32_t nstages);
 
