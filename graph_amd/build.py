@@ -3,6 +3,7 @@
 - ``graph_amd/libcosmosgl.so``: the HIP kernels and the C ABI of
   ``include/cosmosgl.h``, compiled for gfx950 only.
 - ``graph_amd/libcgdata.so``: host-only synthetic input generators.
+- ``graph_amd/napi/cosmosgl.node``: the Node-API addon over the C ABI.
 
 Run ``python -m graph_amd.build`` (or ``__graft_entry__.build()``).  Objects go
 to ``graph_amd/_build/``; every call recompiles.
@@ -76,9 +77,18 @@ def build_datagen() -> Path:
     return out
 
 
+def build_napi() -> Path | None:
+    """Node-API addon (graph_amd/napi/cosmosgl.node) when the Node headers are present."""
+    if not Path("/usr/include/node/node_api.h").exists():
+        return None
+    _run(["make", "-s", "-B", "-C", str(PKG / "napi")])
+    return PKG / "napi" / "cosmosgl.node"
+
+
 def build_all() -> None:
     build_datagen()
     build_cosmosgl()
+    build_napi()
 
 
 if __name__ == "__main__":

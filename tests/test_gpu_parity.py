@@ -254,3 +254,34 @@ def test_layout_quality_vs_oracle_cfg0():
     say(f"edge length p10/p50/p90 gpu {q_g} oracle {q_o}; stress gpu {s_g:.4f} oracle {s_o:.4f}")
     np.testing.assert_allclose(q_g, q_o, rtol=0.02)
     assert abs(s_g - s_o) <= 0.02 * s_o
+
+
+def test_napi_addon_matches_ctypes(tmp_path):
+    """The Node-API addon drives the same ABI: identical inputs give identical bits."""
+    import json
+    import shutil
+    import subprocess
+
+    node = shutil.which("node")
+    addon = _lib.PKG / "napi" / "cosmosgl.node"
+    if node is None or not addon.exists():
+        pytest.skip("node or the addon is not available")
+    S, n = 4096.0, 10_000
+    xy = datagen.uniform_square(n, S, 42)
+    links = datagen.barabasi_albert(n, 2, 42)
+    m = links.size // 2
+    r_in, r_out = datagen.uniform(m, 0, 1, 1), datagen.uniform(m, 0, 1, 2)
+    jit = datagen.jitter(n, 3)
+    seeds = datagen.uniform(25, 0, 1, 4)
+    for name, arr in (("xy", xy), ("links", links), ("rin", r_in), ("rout", r_out), ("jitter", jit),
+                      ("seeds", seeds)):
+        arr.astype(np.float32).tofile(tmp_path / f"{name}.f32")
+    (tmp_path / "params.json").write_text(json.dumps({"spaceSize": S, "simulationRepulsion": 0.5}))
+    subprocess.run([node, str(_lib.PKG / "napi" / "run_ticks.js"), str(tmp_path)], check=True, timeout=120)
+    got = np.fromfile(tmp_path / "out.f32", np.float32)
+    c = ctx_for(xy, links=links, jitter=jit, r_in=r_in, r_out=r_out)
+    c.set_alpha(1.0)
+    tr = c.step(params(S, repulsion=0.5), 25, seeds=seeds, trace=True)
+    want = c.get_positions()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(np.fromfile(tmp_path / "alpha.f64", np.float64), tr)

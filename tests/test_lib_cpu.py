@@ -100,3 +100,24 @@ def test_space_size_adjustment():
     assert adjust_space_size(8192, 16384) == 8192
     assert adjust_space_size(-5, 16384) == 4096
     assert adjust_space_size(1, None) == 2
+
+
+def test_napi_addon_exports():
+    import shutil
+
+    node = shutil.which("node")
+    addon = _lib.PKG / "napi" / "cosmosgl.node"
+    if node is None or not addon.exists():
+        pytest.skip("node or the addon is not available")
+    code = ("const a=require(process.argv[1]);"
+            "console.log(Object.keys(a).sort().join(','));"
+            "console.log(JSON.stringify(a.defaultParams()))")
+    out = subprocess.run([node, "-e", code, str(addon)], capture_output=True, text=True, check=True).stdout
+    names, params = out.strip().split("\n")
+    assert names.split(",") == sorted(["create", "destroy", "setPositions", "setLinks", "setJitter",
+                                        "setPointSizes", "setPinned", "setAlpha", "getAlpha", "step",
+                                        "getPositions", "defaultParams"])
+    import json
+
+    d = json.loads(params)
+    assert d["spaceSize"] == 4096 and d["simulationDecay"] == 5000
