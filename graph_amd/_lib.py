@@ -92,6 +92,20 @@ _SIGNATURES = {
     "cg_set_profiling": (C.c_int, [_CTX, C.c_uint32]),
     "cg_debug_link_stats": (C.c_int, [_CTX, _U]),
     "cg_get_profile": (C.c_int, [_CTX, _D, _U, C.c_uint32]),
+    "cg_graph_add": (C.c_int, [_CTX, _U]),
+    "cg_graph_remove": (C.c_int, [_CTX, C.c_uint32]),
+    "cg_graph_set_params": (C.c_int, [_CTX, C.c_uint32, C.POINTER(CgSimParams)]),
+    "cg_graph_set_seed": (C.c_int, [_CTX, C.c_uint32, C.c_uint64]),
+    "cg_graph_set_positions": (C.c_int, [_CTX, C.c_uint32, _F, C.c_uint32]),
+    "cg_graph_set_links": (C.c_int, [_CTX, C.c_uint32, _F, C.c_uint32, _F, _F, _F]),
+    "cg_graph_set_jitter": (C.c_int, [_CTX, C.c_uint32, _F, C.c_uint32]),
+    "cg_graph_set_point_sizes": (C.c_int, [_CTX, C.c_uint32, _F, C.c_uint32]),
+    "cg_graph_set_pinned": (C.c_int, [_CTX, C.c_uint32, _U, C.c_uint32]),
+    "cg_graph_set_alpha": (C.c_int, [_CTX, C.c_uint32, C.c_double]),
+    "cg_graph_get_alpha": (C.c_int, [_CTX, C.c_uint32, _D]),
+    "cg_graph_get_positions": (C.c_int, [_CTX, C.c_uint32, _F, C.c_uint32]),
+    "cg_step_graphs": (C.c_int, [_CTX, _U, C.c_uint32, C.c_uint32]),
+    "cg_sync": (C.c_int, [_CTX]),
 }
 
 STAGES = ["mouse", "gravity", "center", "mb_sort", "mb_cells", "mb_pyramid", "mb_force", "link_in",
@@ -157,7 +171,10 @@ def default_params() -> CgSimParams:
 
 
 class Context:
-    """One simulated graph on one GPU (owner of a ``cg_ctx``)."""
+    """A set of simulated graphs on one GPU (owner of a ``cg_ctx``).
+
+    The single-graph methods (set_positions ... get_positions) act on graph 0;
+    the ``graph_*`` methods manage further graphs packed beside it."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 0):
         self._l = lib()
@@ -167,6 +184,7 @@ class Context:
         if rc != CG_OK:
             raise CosmosGLError(rc, "cg_create failed")
         self.n = 0
+        self._sizes: dict[int, int] = {}  # point count of each graph_* graph
 
     def _check(self, rc: int) -> None:
         if rc != CG_OK:
@@ -289,3 +307,68 @@ class Context:
 
     def device_bytes(self) -> int:
         return int(self._l.cg_device_bytes(self._ctx))
+
+    # --- graph set ------------------------------------------------------------
+    def graph_add(self) -> int:
+        gid = C.c_uint32()
+        self._check(self._l.cg_graph_add(self._ctx, C.byref(gid)))
+        self._sizes[gid.value] = 0
+        return gid.value
+
+    def graph_remove(self, gid: int) -> None:
+        self._check(self._l.cg_graph_remove(self._ctx, gid))
+        self._sizes.pop(gid, None)
+
+    def graph_set_params(self, gid: int, params: CgSimParams) -> None:
+        self._check(self._l.cg_graph_set_params(self._ctx, gid, C.byref(params)))
+
+    def graph_set_seed(self, gid: int, seed: int) -> None:
+        self._check(self._l.cg_graph_set_seed(self._ctx, gid, int(seed)))
+
+    def graph_set_positions(self, gid: int, xy: np.ndarray) -> None:
+        xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
+        self._sizes[gid] = xy.size // 2
+        self._check(self._l.cg_graph_set_positions(self._ctx, gid, fptr(xy), xy.size // 2))
+
+    def graph_set_links(self, gid: int, links, strength=None, rand_in=None, rand_out=None) -> None:
+        links = np.ascontiguousarray(links, dtype=np.float32).reshape(-1)
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+        strength, rand_in, rand_out = conv(strength), conv(rand_in), conv(rand_out)
+        self._check(self._l.cg_graph_set_links(self._ctx, gid, fptr(links), links.size // 2,
+                                               fptr(strength), fptr(rand_in), fptr(rand_out)))
+
+    def graph_set_jitter(self, gid: int, xy: np.ndarray) -> None:
+        xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
+        self._check(self._l.cg_graph_set_jitter(self._ctx, gid, fptr(xy), xy.size // 2))
+
+    def graph_set_point_sizes(self, gid: int, sizes) -> None:
+        if sizes is None:
+            self._check(self._l.cg_graph_set_point_sizes(self._ctx, gid, None, 0))
+            return
+        s = np.ascontiguousarray(sizes, dtype=np.float32)
+        self._check(self._l.cg_graph_set_point_sizes(self._ctx, gid, fptr(s), s.size))
+
+    def graph_set_pinned(self, gid: int, indices) -> None:
+        idx = np.ascontiguousarray(indices if indices is not None else [], dtype=np.uint32)
+        self._check(self._l.cg_graph_set_pinned(self._ctx, gid, uptr(idx) if idx.size else None, idx.size))
+
+    def graph_set_alpha(self, gid: int, alpha: float) -> None:
+        self._check(self._l.cg_graph_set_alpha(self._ctx, gid, float(alpha)))
+
+    def graph_get_alpha(self, gid: int) -> float:
+        a = C.c_double()
+        self._check(self._l.cg_graph_get_alpha(self._ctx, gid, C.byref(a)))
+        return a.value
+
+    def graph_get_positions(self, gid: int) -> np.ndarray:
+        n = self._sizes.get(gid, 0)
+        out = np.empty(2 * n, dtype=np.float32)
+        self._check(self._l.cg_graph_get_positions(self._ctx, gid, fptr(out), n))
+        return out
+
+    def step_graphs(self, gids, nticks: int) -> None:
+        ids = np.ascontiguousarray(list(gids), dtype=np.uint32)
+        self._check(self._l.cg_step_graphs(self._ctx, uptr(ids) if ids.size else None, ids.size, nticks))
+
+    def sync(self) -> None:
+        self._check(self._l.cg_sync(self._ctx))

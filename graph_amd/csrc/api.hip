@@ -1,11 +1,15 @@
-// api.hip — the extern "C" boundary: context lifetime, ingest, the tick
+// api.hip — the extern "C" boundary: context lifetime, per-graph ingest, the
+// packer that lays every live graph out in the shared device arrays, the tick
 // driver and the inspection entry points.
 //
 // The tick driver restates Graph.runSimulationStep (src/index.ts:2006-2099):
 // [mouse -> integrate] -> gravity -> center -> many-body -> link IN ->
 // link OUT -> collision, each force followed by its own integrate pass (fused
 // into the force kernel here), then the alpha decay of Store.addAlpha
-// (Store/index.ts:328-332).  Positions ping-pong between two device buffers.
+// (Store/index.ts:328-332).  Positions ping-pong between two device buffers;
+// each graph records which buffer holds its current positions, so graphs that
+// skip a pass (no links, collision off, not stepped) stay consistent.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <new>
@@ -86,13 +90,17 @@ void prof_end(Ctx* c, int stage) {
 
 namespace {
 
+constexpr uint64_t GRAPH_SEED = 0x5eed5eedULL;   // every graph's generator starts here
+constexpr uint32_t DESC_ENTRIES = 1u << 16;      // descriptors per upload (ticks x graphs)
+constexpr uint32_t DESC_MAX_TICKS = 256;
+
 uint64_t splitmix(uint64_t& x) {
     uint64_t z = (x += 0x9e3779b97f4a7c15ull);
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
     return z ^ (z >> 31);
 }
-float rng_unit(Ctx* c) { return (float)(splitmix(c->rng) >> 40) * 0x1.0p-24f; }
+float rng_unit(Graph& g) { return (float)(splitmix(g.rng) >> 40) * 0x1.0p-24f; }
 
 // Pyramid geometry — ForceManyBody/index.ts:646-656: finest = the smallest
 // power of two >= 2*sqrt(n), clamped to [8, 512]; levels = log2(finest) - 1.
@@ -107,33 +115,568 @@ void geometry(uint32_t n, uint32_t* gf, uint32_t* L) {
     *L = lg - 1;
 }
 
-void free_points(Ctx* c) {
-    dfree(c, c->pos[0]); dfree(c, c->pos[1]);
-    dfree(c, c->flags); dfree(c, c->present); dfree(c, c->jitter);
-    for (int b = 0; b < 2; ++b) { dfree(c, c->keys[b]); dfree(c, c->vals[b]); c->keys[b] = c->vals[b] = nullptr; }
-    dfree(c, c->sort_tmp); dfree(c, c->cell_begin); dfree(c, c->cell_end);
-    for (int l = 0; l < MAX_LEVELS; ++l) { dfree(c, c->sums[l]); dfree(c, c->cent[l]); c->sums[l] = nullptr; c->cent[l] = nullptr; }
-    dfree(c, c->sorted_pos); dfree(c, c->slot_idx); dfree(c, c->slot_pos);
-    dfree(c, c->big_list); dfree(c, c->big_count);
-    dfree(c, c->col_begin); dfree(c, c->col_end); dfree(c, c->col_cell); dfree(c, c->col_size);
-    c->pos[0] = c->pos[1] = nullptr;
-    c->flags = nullptr; c->present = nullptr; c->jitter = nullptr;
-    c->sort_tmp = nullptr; c->cell_begin = c->cell_end = nullptr;
-    c->sorted_pos = nullptr; c->slot_idx = nullptr; c->slot_pos = nullptr;
-    c->big_list = c->big_count = nullptr;
-    c->col_begin = c->col_end = nullptr; c->col_cell = nullptr; c->col_size = nullptr;
-    c->col_G = 0;
+bool on(float v) { return v != 0.f && !std::isnan(v); }  // JS truthiness of a number
+
+bool same_bits(float a, float b) {
+    uint32_t x, y;
+    std::memcpy(&x, &a, 4);
+    std::memcpy(&y, &b, 4);
+    return x == y;
 }
 
-int upload_flags(Ctx* c) {
-    CG_HIP(c, hipMemcpyAsync(c->flags, c->h_flags.data(), c->n, hipMemcpyHostToDevice, c->stream));
+// The inputs that shape a graph's collision grid or its membership in the
+// collision arrays; a change repacks.
+bool same_collision(const cg_sim_params& a, const cg_sim_params& b) {
+    return on(a.collision) == on(b.collision) && same_bits(a.collision_radius, b.collision_radius) &&
+           same_bits(a.collision_padding, b.collision_padding) &&
+           same_bits(a.point_default_size, b.point_default_size) && same_bits(a.space_size, b.space_size);
+}
+
+bool collides(const Graph& g) { return on(g.params.collision) && g.np > 0; }
+
+Graph* lookup(Ctx* c, uint32_t id) {
+    if (id >= c->graphs.size() || !c->graphs[id].live) return nullptr;
+    return &c->graphs[id];
+}
+
+void reset_graph(Graph& g, uint32_t id) {
+    g = Graph();
+    g.id = id;
+    g.live = true;
+    g.rng = GRAPH_SEED;
+    cg_default_params(&g.params);
+}
+
+// Graph 0 backs the single-graph entry points; it joins the set on first use.
+Graph& default_graph(Ctx* c) {
+    if (c->graphs.empty()) c->graphs.emplace_back();
+    Graph& g = c->graphs[0];
+    if (!g.live) {
+        reset_graph(g, 0);
+        c->dirty = true;
+    }
+    return g;
+}
+
+void set_params(Ctx* c, Graph& g, const cg_sim_params& p) {
+    if (!same_collision(g.params, p)) {
+        g.col_dirty = true;
+        c->dirty = true;
+    }
+    g.params = p;
+}
+
+template <typename T>
+void release(Ctx* c, T*& p) {
+    dfree(c, (void*)p);
+    p = nullptr;
+}
+
+// Frees the packed arrays except the position buffers and the link schedules.
+void free_packed(Ctx* c) {
+    release(c, c->flags); release(c, c->present); release(c, c->gid); release(c, c->jitter);
+    release(c, c->size); release(c, c->desc_base); c->desc = nullptr;
+    release(c, c->f0s); release(c, c->colc0s);
+    for (int b = 0; b < 2; ++b) {
+        release(c, c->keys[b]); release(c, c->vals[b]);
+        release(c, c->col_keys[b]); release(c, c->col_vals[b]);
+        release(c, c->pyr[b]);
+        c->n_pyr[b] = 0;
+    }
+    release(c, c->sort_tmp); release(c, c->cell_begin); release(c, c->cell_end);
+    release(c, c->sums); release(c, c->cent); release(c, c->sorted_pos);
+    release(c, c->slot_idx); release(c, c->slot_pos); release(c, c->big_list); release(c, c->big_count);
+    release(c, c->center_part); release(c, c->center_sum); release(c, c->center_chunks);
+    release(c, c->center_first);
+    c->n_center_chunks = 0;
+    release(c, c->cpresent); release(c, c->col_sort_tmp);
+    release(c, c->col_begin); release(c, c->col_end); release(c, c->col_cell);
+}
+
+template <typename T>
+int upload(Ctx* c, T* dst, const std::vector<T>& src) {
+    if (!src.empty())
+        CG_HIP(c, hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
     return CG_OK;
+}
+
+// Lays every live graph out in fresh packed arrays (id order), carrying the
+// positions of graphs that were already packed over on the device.
+int pack(Ctx* c) {
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->side) CG_HIP(c, hipStreamSynchronize(c->side));
+    c->desc_pending = false;
+    int rc = CG_OK;
+    c->order.clear();
+    for (auto& g : c->graphs)
+        if (g.live) c->order.push_back(g.id);
+    const uint32_t G = (uint32_t)c->order.size();
+    for (uint32_t id : c->order) {
+        Graph& g = c->graphs[id];
+        if (collides(g) && (g.col_dirty || g.col_size.size() != g.n)) {
+            rc = col_prepare_graph(c, g);
+            if (rc) return rc;
+        }
+    }
+    uint64_t N = 0, NP = 0, F = 0, C = 0, CG = 0, NPC = 0;
+    for (uint32_t s = 0; s < G; ++s) {
+        const Graph& g = c->graphs[c->order[s]];
+        N += g.n;
+        NP += g.np;
+        F += (uint64_t)g.gf * g.gf;
+        C += level_off(g.L);
+        if (collides(g)) {
+            CG += 4ull * g.col_G * g.col_G;
+            NPC += g.np;
+        }
+    }
+    if (N >= (1ull << 31) || F * SLOTS >= (1ull << 32) || C >= (1ull << 31) || CG >= (1ull << 32) ||
+        4 * NPC >= (1ull << 31))
+        return fail(c, CG_ERR_INVALID, "the graph set exceeds the 32-bit index space of one context");
+    std::vector<uint32_t> old_p0(G);
+    std::vector<GDesc> st(G);
+    {
+        uint32_t p0 = 0, k0 = 0, f0 = 0, c0 = 0, col0 = 0;
+        for (uint32_t s = 0; s < G; ++s) {
+            Graph& g = c->graphs[c->order[s]];
+            old_p0[s] = g.p0;
+            g.slot = s;
+            g.p0 = p0; g.k0 = k0; g.f0 = f0; g.c0 = c0; g.col_c0 = col0;
+            p0 += g.n; k0 += g.np; f0 += g.gf * g.gf; c0 += level_off(g.L);
+            if (collides(g)) col0 += 4 * g.col_G * g.col_G;
+            GDesc d{};
+            d.p0 = g.p0; d.n = g.n; d.k0 = g.k0; d.np = g.np; d.f0 = g.f0; d.c0 = g.c0;
+            d.gf = g.gf; d.L = g.L; d.col_c0 = g.col_c0;
+            d.col_G = collides(g) ? g.col_G : 0;
+            d.col_h = collides(g) ? g.col_h : 1.f;
+            st[s] = d;
+        }
+    }
+    float2* old_pos[2] = {c->pos[0], c->pos[1]};
+    c->pos[0] = c->pos[1] = nullptr;
+    free_packed(c);
+    LinkDir old_dirs[2] = {c->dir[0], c->dir[1]};
+    c->G = G;
+    c->N = (uint32_t)N;
+    c->NP = (uint32_t)NP;
+    c->F = (uint32_t)F;
+    c->C = (uint32_t)C;
+    c->CG = (uint32_t)CG;
+    c->NPC = (uint32_t)NPC;
+    const size_t nn = N ? N : 1, gg = G ? G : 1;
+    c->pos[0] = (float2*)dalloc(c, nn * 8, &rc);
+    c->pos[1] = (float2*)dalloc(c, nn * 8, &rc);
+    c->flags = (uint8_t*)dalloc(c, nn, &rc);
+    c->gid = (uint32_t*)dalloc(c, nn * 4, &rc);
+    c->jitter = (float2*)dalloc(c, nn * 8, &rc);
+    c->size = (float*)dalloc(c, nn * 4, &rc);
+    c->present = (uint32_t*)dalloc(c, (NP ? NP : 1) * 4, &rc);
+    c->cpresent = (uint32_t*)dalloc(c, (NPC ? NPC : 1) * 4, &rc);
+    c->f0s = (uint32_t*)dalloc(c, gg * 4, &rc);
+    c->colc0s = (uint32_t*)dalloc(c, gg * 4, &rc);
+    if (rc) return rc;
+
+    // host staging of the per-point arrays
+    std::vector<uint8_t> hflags(N);
+    std::vector<uint32_t> hgid(N), hpres, hcpres, hf0(G), hcol(G), k0s(G), nps(G);
+    std::vector<float> hjit(2 * N), hsize(N, 0.f);
+    std::vector<uint4> w0, w1;
+    hpres.reserve(NP);
+    hcpres.reserve(NPC);
+    for (uint32_t s = 0; s < G; ++s) {
+        const Graph& g = c->graphs[c->order[s]];
+        const bool col = collides(g);
+        if (g.n) {
+            std::memcpy(&hflags[g.p0], g.flags.data(), g.n);
+            std::fill(hgid.begin() + g.p0, hgid.begin() + g.p0 + g.n, s);
+            std::memcpy(&hjit[2 * (size_t)g.p0], g.jitter.data(), (size_t)g.n * 8);
+            if (col) std::memcpy(&hsize[g.p0], g.col_size.data(), (size_t)g.n * 4);
+        }
+        for (uint32_t i = 0; i < g.n; ++i) {
+            if (g.flags[i] & F_ABSENT) continue;
+            hpres.push_back(g.p0 + i);
+            if (col) hcpres.push_back(g.p0 + i);
+        }
+        hf0[s] = g.f0;
+        hcol[s] = g.col_c0;
+        k0s[s] = g.k0;
+        nps[s] = g.np;
+        if (g.np) pyramid_work_host(s, g.gf, g.L, w0, w1);
+    }
+    rc = upload(c, c->flags, hflags); if (rc) return rc;
+    rc = upload(c, c->gid, hgid); if (rc) return rc;
+    rc = upload(c, (float*)c->jitter, hjit); if (rc) return rc;
+    rc = upload(c, c->size, hsize); if (rc) return rc;
+    rc = upload(c, c->present, hpres); if (rc) return rc;
+    rc = upload(c, c->cpresent, hcpres); if (rc) return rc;
+    rc = upload(c, c->f0s, hf0); if (rc) return rc;
+    rc = upload(c, c->colc0s, hcol); if (rc) return rc;
+
+    // positions: host input for newly set graphs, device copy for the others
+    for (uint32_t s = 0; s < G; ++s) {
+        Graph& g = c->graphs[c->order[s]];
+        if (g.n == 0) {
+            g.xy_pending = false;
+            g.cur = 0;
+            continue;
+        }
+        const size_t bytes = (size_t)g.n * 8;
+        for (int b = 0; b < 2; ++b) {
+            if (g.xy_pending || !old_pos[0])
+                CG_HIP(c, hipMemcpyAsync(c->pos[b] + g.p0, g.xy.data(), bytes, hipMemcpyHostToDevice, c->stream));
+            else
+                CG_HIP(c, hipMemcpyAsync(c->pos[b] + g.p0, old_pos[g.cur] + old_p0[s], bytes,
+                                         hipMemcpyDeviceToDevice, c->stream));
+        }
+        g.cur = 0;
+    }
+
+    // centroid chunks and pyramid work lists
+    {
+        std::vector<uint4> chunks;
+        std::vector<uint32_t> first;
+        center_chunks_host(k0s, nps, chunks, first);
+        c->n_center_chunks = (uint32_t)chunks.size();
+        c->center_chunks = (uint4*)dalloc(c, (chunks.size() + 1) * 16, &rc);
+        c->center_first = (uint32_t*)dalloc(c, first.size() * 4, &rc);
+        c->center_part = (double*)dalloc(c, (chunks.size() + 1) * 2 * sizeof(double), &rc);
+        c->center_sum = (double*)dalloc(c, gg * 3 * sizeof(double), &rc);
+        for (int b = 0; b < 2; ++b) {
+            const auto& w = b ? w1 : w0;
+            c->n_pyr[b] = (uint32_t)w.size();
+            c->pyr[b] = (uint4*)dalloc(c, (w.size() + 1) * 16, &rc);
+        }
+        if (rc) return rc;
+        rc = upload(c, c->center_chunks, chunks); if (rc) return rc;
+        rc = upload(c, c->center_first, first); if (rc) return rc;
+        rc = upload(c, c->pyr[0], w0); if (rc) return rc;
+        rc = upload(c, c->pyr[1], w1); if (rc) return rc;
+    }
+
+    // descriptor tables (device and pinned staging)
+    c->desc_ticks = std::max<uint32_t>(1, std::min<uint32_t>(DESC_MAX_TICKS, DESC_ENTRIES / (uint32_t)gg));
+    c->desc_base = (GDesc*)dalloc(c, (size_t)c->desc_ticks * gg * sizeof(GDesc), &rc);
+    if (rc) return rc;
+    if (c->h_desc_pinned) (void)hipHostFree(c->h_desc_pinned);
+    c->h_desc_pinned = nullptr;
+    CG_HIP(c, hipHostMalloc((void**)&c->h_desc_pinned, (size_t)c->desc_ticks * gg * sizeof(GDesc)));
+    c->h_desc = st;
+    c->desc = c->desc_base;
+    rc = upload(c, c->desc_base, st); if (rc) return rc;
+
+    rc = mb_alloc(c); if (rc) return rc;
+    rc = col_alloc(c); if (rc) return rc;
+    rc = link_pack(c, old_dirs); if (rc) return rc;
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(c, old_pos[0]);
+    dfree(c, old_pos[1]);
+    for (uint32_t id : c->order) {
+        Graph& g = c->graphs[id];
+        if (g.xy_pending) {
+            g.xy_pending = false;
+            std::vector<float>().swap(g.xy);
+        }
+    }
+    c->dirty = false;
+    return CG_OK;
+}
+
+// The parameter-dependent part of a descriptor (alpha, seed and the force constants).
+void fill_dynamic(GDesc& d, const cg_sim_params& p, float a, float seed) {
+    const float S = p.space_size;
+    d.S = S;
+    for (uint32_t l = 0; l < (uint32_t)MAX_LEVELS; ++l)
+        d.h[l] = (l < d.L) ? (float)((double)S / (double)(4u << l)) : 1.f;
+    d.alpha = a;
+    d.seed = seed;
+    d.ak = a * p.repulsion;
+    d.friction = p.friction;
+    d.gravity = p.gravity;
+    d.center = p.center;
+    d.mouse_k = p.repulsion_from_mouse;
+    d.mx = p.mouse_position[0];
+    d.my = p.mouse_position[1];
+    d.spring_alpha = p.link_spring * a;
+    d.dist = p.link_distance;
+    d.lo = p.link_dist_random_variation_range[0];
+    d.span = p.link_dist_random_variation_range[1] - p.link_dist_random_variation_range[0];
+    d.tol = 2.5e-6f * S;  // a quarter of the 1e-5 * spaceSize force tolerance
+    d.col_strength = p.collision;
+    d.col_radius = p.collision_radius;
+    d.col_padding = p.collision_padding;
+}
+
+// Passes a graph runs in one tick — the gates of src/index.ts:2012-2078.
+uint32_t graph_passes(const Graph& g) {
+    const cg_sim_params& p = g.params;
+    if (!p.enable_simulation || g.np == 0) return 0;
+    uint32_t passes = 0;
+    if (p.mouse_repulsion) passes |= 1u << P_MOUSE;
+    if (p.run_forces) {
+        if (on(p.gravity)) passes |= 1u << P_GRAVITY;
+        if (on(p.center)) passes |= 1u << P_CENTER;
+        passes |= 1u << P_MANYBODY;
+        if (g.links_ready && g.m > 0) passes |= (1u << P_LINK_IN) | (1u << P_LINK_OUT);
+        if (collides(g)) passes |= 1u << P_COLLISION;
+    }
+    return passes;
+}
+
+struct TickPlan {
+    uint32_t any;   // union of the passes of every graph
+    int keys_pass;  // per-point pass that also writes the many-body cell keys, or -1
+};
+
+int launch_tick(Ctx* c, const TickPlan& tp) {
+    const uint32_t a = tp.any;
+    int rc = CG_OK;
+    if (a & (1u << P_MOUSE)) {
+        prof_begin(c, CG_STAGE_MOUSE);
+        rc = pw_run(c, P_MOUSE, false, nullptr);
+        prof_end(c, CG_STAGE_MOUSE);
+        if (rc) return rc;
+    }
+    if (a & (1u << P_GRAVITY)) {
+        prof_begin(c, CG_STAGE_GRAVITY);
+        rc = pw_run(c, P_GRAVITY, tp.keys_pass == P_GRAVITY, nullptr);
+        prof_end(c, CG_STAGE_GRAVITY);
+        if (rc) return rc;
+    }
+    if (a & (1u << P_CENTER)) {
+        prof_begin(c, CG_STAGE_CENTER);
+        rc = pw_run(c, P_CENTER, tp.keys_pass == P_CENTER, nullptr);
+        prof_end(c, CG_STAGE_CENTER);
+        if (rc) return rc;
+    }
+    if (a & (1u << P_MANYBODY)) {
+        rc = mb_run(c, tp.keys_pass >= 0, nullptr);
+        if (rc) return rc;
+    }
+    for (int d = 0; d < 2; ++d) {
+        if (!(a & (1u << (P_LINK_IN + d)))) continue;
+        prof_begin(c, CG_STAGE_LINK_IN + d);
+        rc = link_run(c, d, nullptr);
+        prof_end(c, CG_STAGE_LINK_IN + d);
+        if (rc) return rc;
+    }
+    if (a & (1u << P_COLLISION)) {
+        prof_begin(c, CG_STAGE_COLLISION);
+        rc = col_run(c, nullptr);
+        prof_end(c, CG_STAGE_COLLISION);
+        if (rc) return rc;
+    }
+    return CG_OK;
+}
+
+// Advances the graphs in `ids` by nticks.  The graph `sel` (or none) takes its
+// near-field seeds from `seeds` and reports alpha after each tick in `trace`.
+int run_ticks(Ctx* c, const std::vector<uint32_t>& ids, uint32_t nticks, int64_t sel,
+              const float* seeds, double* trace) {
+    int rc = CG_OK;
+    if (c->dirty) {
+        rc = pack(c);
+        if (rc) return rc;
+    }
+    const uint32_t G = c->G;
+    if (G == 0 || nticks == 0) return CG_OK;
+    std::vector<uint8_t> run(G, 0);
+    for (uint32_t id : ids) run[c->graphs[id].slot] = 1;
+    const int64_t sel_slot = sel >= 0 ? (int64_t)c->graphs[sel].slot : -1;
+    std::vector<TickPlan> plan;
+    for (uint32_t t0 = 0; t0 < nticks; t0 += c->desc_ticks) {
+        const uint32_t T = std::min(c->desc_ticks, nticks - t0);
+        if (c->desc_pending) {
+            CG_HIP(c, hipEventSynchronize(c->ev_desc));
+            c->desc_pending = false;
+        }
+        plan.assign(T, TickPlan{0, -1});
+        for (uint32_t tt = 0; tt < T; ++tt) {
+            GDesc* D = c->h_desc_pinned + (size_t)tt * G;
+            uint32_t any = 0, all = ~0u;
+            for (uint32_t s = 0; s < G; ++s) {
+                Graph& g = c->graphs[c->order[s]];
+                const cg_sim_params& p = g.params;
+                GDesc d = c->h_desc[s];
+                const bool live = run[s] && p.enable_simulation;
+                const uint32_t passes = live ? graph_passes(g) : 0u;
+                uint32_t cur = g.cur, inbuf = 0;
+                for (uint32_t q = 0; q < P_COUNT; ++q) {
+                    if (cur) inbuf |= 1u << q;
+                    if ((passes >> q) & 1u) cur ^= 1u;
+                }
+                float seed = 0.f;
+                if (live) seed = ((int64_t)s == sel_slot && seeds) ? seeds[t0 + tt] : rng_unit(g);
+                fill_dynamic(d, p, (float)g.alpha, seed);
+                d.passes = passes;
+                d.inbuf = inbuf;
+                D[s] = d;
+                if (live) {
+                    g.cur = cur;
+                    if (p.run_forces) {
+                        const double decay = 1.0 - std::pow(0.001, 1.0 / (double)p.decay);
+                        g.alpha += (0.0 - g.alpha) * decay;
+                        if (p.mouse_repulsion) g.alpha = std::max(g.alpha, 0.1);
+                    }
+                }
+                any |= passes;
+                if (g.np) all &= passes;
+            }
+            // One per-point pass writes the cell keys when every graph with points runs it
+            // last before the many-body pass; otherwise a separate key pass does.
+            const int K = (any & (1u << P_CENTER)) ? P_CENTER : (any & (1u << P_GRAVITY)) ? P_GRAVITY : -1;
+            plan[tt] = TickPlan{any, (K >= 0 && ((all >> K) & 1u)) ? K : -1};
+            if (trace && sel_slot >= 0) trace[t0 + tt] = c->graphs[sel].alpha;
+        }
+        CG_HIP(c, hipMemcpyAsync(c->desc_base, c->h_desc_pinned, (size_t)T * G * sizeof(GDesc),
+                                 hipMemcpyHostToDevice, c->stream));
+        CG_HIP(c, hipEventRecord(c->ev_desc, c->stream));
+        c->desc_pending = true;
+        for (uint32_t tt = 0; tt < T; ++tt) {
+            c->desc = c->desc_base + (size_t)tt * G;
+            rc = launch_tick(c, plan[tt]);
+            if (rc) {
+                c->desc = c->desc_base;
+                return rc;
+            }
+        }
+    }
+    c->desc = c->desc_base;
+    CG_HIP(c, hipGetLastError());
+    return CG_OK;
+}
+
+// Writes one descriptor table (slot 0 of the device tables) synchronously.
+int write_table(Ctx* c, const std::vector<GDesc>& table) {
+    if (c->desc_pending) {
+        CG_HIP(c, hipEventSynchronize(c->ev_desc));
+        c->desc_pending = false;
+    }
+    CG_HIP(c, hipMemcpyAsync(c->desc_base, table.data(), table.size() * sizeof(GDesc),
+                             hipMemcpyHostToDevice, c->stream));
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    c->desc = c->desc_base;
+    return CG_OK;
+}
+
+int graph_set_positions(Ctx* c, Graph& g, const float* xy, uint32_t n) {
+    if (n && !xy) return fail(c, CG_ERR_INVALID, "set_positions: null positions");
+    if (n >= (1u << 31)) return fail(c, CG_ERR_INVALID, "set_positions: too many points");
+    const bool same_n = n == g.n;
+    g.n = n;
+    // Points/position-utils.ts:17-24: a half-NaN position becomes fully NaN (absent)
+    g.xy.assign(2 * (size_t)n, 0.f);
+    g.flags.assign(n, 0);
+    uint32_t np = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        float px = xy[2 * i], py = xy[2 * i + 1];
+        bool absent = std::isnan(px) || std::isnan(py);
+        g.xy[2 * i] = absent ? NAN : px;
+        g.xy[2 * i + 1] = absent ? NAN : py;
+        if (absent) g.flags[i] = F_ABSENT;
+        else ++np;
+    }
+    for (uint32_t k : g.pinned)
+        if (k < n) g.flags[k] |= F_PINNED;
+    g.np = np;
+    geometry(n, &g.gf, &g.L);
+    // default jitter from the graph's generator: (U(-1,1)*1e-5, U(-1,1)*1e-5)
+    g.jitter.resize(2 * (size_t)n);
+    for (auto& v : g.jitter) v = (rng_unit(g) * 2.f - 1.f) * 0.00001f;
+    if (!same_n) g.sizes.clear();
+    g.xy_pending = true;
+    g.col_dirty = true;
+    c->dirty = true;
+    if (g.links_set) return link_build_graph(c, g);
+    return CG_OK;
+}
+
+int graph_set_links(Ctx* c, Graph& g, const float* links, uint32_t m, const float* strength,
+                    const float* rand_in, const float* rand_out) {
+    if (m && !links) return fail(c, CG_ERR_INVALID, "set_links: null links");
+    g.m = m;
+    g.links.assign(links, links + 2 * (size_t)m);
+    if (strength) g.strength.assign(strength, strength + m);
+    else g.strength.clear();
+    g.rin.resize(m);
+    g.rout.resize(m);
+    for (uint32_t l = 0; l < m; ++l) g.rin[l] = rand_in ? rand_in[l] : rng_unit(g);
+    for (uint32_t l = 0; l < m; ++l) g.rout[l] = rand_out ? rand_out[l] : rng_unit(g);
+    g.links_set = true;
+    c->dirty = true;
+    return link_build_graph(c, g);
+}
+
+int graph_set_jitter(Ctx* c, Graph& g, const float* xy, uint32_t n) {
+    if (n != g.n || (n && !xy)) return fail(c, CG_ERR_INVALID, "set_jitter: expected one pair per point");
+    g.jitter.assign(xy, xy + 2 * (size_t)n);
+    if (!c->dirty && n) {
+        CG_HIP(c, hipMemcpyAsync(c->jitter + g.p0, g.jitter.data(), (size_t)n * 8, hipMemcpyHostToDevice,
+                                 c->stream));
+        CG_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return CG_OK;
+}
+
+int graph_set_point_sizes(Ctx* c, Graph& g, const float* sizes, uint32_t n) {
+    if (sizes && n != g.n) return fail(c, CG_ERR_INVALID, "set_point_sizes: expected one size per point");
+    if (sizes) g.sizes.assign(sizes, sizes + n);
+    else g.sizes.clear();
+    g.col_dirty = true;
+    if (collides(g)) c->dirty = true;
+    return CG_OK;
+}
+
+int graph_set_pinned(Ctx* c, Graph& g, const uint32_t* idx, uint32_t count) {
+    if (count && !idx) return fail(c, CG_ERR_INVALID, "set_pinned: null indices");
+    g.pinned.assign(idx, idx + count);
+    for (auto& f : g.flags) f &= (uint8_t)~F_PINNED;
+    for (uint32_t k = 0; k < count; ++k)
+        if (idx[k] < g.n) g.flags[idx[k]] |= F_PINNED;
+    if (!c->dirty && g.n) {
+        CG_HIP(c, hipMemcpyAsync(c->flags + g.p0, g.flags.data(), g.n, hipMemcpyHostToDevice, c->stream));
+        CG_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return CG_OK;
+}
+
+int graph_get_positions(Ctx* c, Graph& g, float* xy, uint32_t n) {
+    if (n != g.n || (n && !xy)) return fail(c, CG_ERR_INVALID, "get_positions: size mismatch");
+    if (n == 0) return CG_OK;
+    if (g.xy_pending) {
+        std::memcpy(xy, g.xy.data(), (size_t)n * 8);
+        return CG_OK;
+    }
+    CG_HIP(c, hipMemcpyAsync(xy, c->pos[g.cur] + g.p0, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    return CG_OK;
+}
+
+int pass_of_force(int32_t force) {
+    switch (force) {
+        case CG_FORCE_GRAVITY: return P_GRAVITY;
+        case CG_FORCE_CENTER: return P_CENTER;
+        case CG_FORCE_MOUSE: return P_MOUSE;
+        case CG_FORCE_MANYBODY: return P_MANYBODY;
+        case CG_FORCE_LINK_IN: return P_LINK_IN;
+        case CG_FORCE_LINK_OUT: return P_LINK_OUT;
+        case CG_FORCE_COLLISION: return P_COLLISION;
+        default: return -1;
+    }
 }
 
 }  // namespace
 }  // namespace cg
 
 using namespace cg;
+
+#define CG_CTX(x)                       \
+    if (!(x)) return CG_ERR_INVALID;    \
+    Ctx* c = &(x)->c;                   \
+    CG_HIP(c, hipSetDevice(c->device))
+
+#define CG_GRAPH(g, id)                                                        \
+    Graph* g = lookup(c, (id));                                                \
+    if (!g) return fail(c, CG_ERR_INVALID, "unknown graph id " + std::to_string(id))
 
 extern "C" {
 
@@ -173,8 +716,9 @@ int cg_create(cg_ctx** out, const cg_create_params* params) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_desc, hipEventDisableTiming);
     if (e != hipSuccess) {
-        delete x;
+        cg_destroy(x);
         return CG_ERR_DEVICE;
     }
     *out = x;
@@ -186,13 +730,15 @@ int cg_destroy(cg_ctx* x) {
     Ctx* c = &x->c;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_desc) (void)hipEventDestroy(c->ev_desc);
     if (c->side) (void)hipStreamDestroy(c->side);
     for (auto& a : c->allocs) (void)hipFree(a.first);
     c->allocs.clear();
+    if (c->h_desc_pinned) (void)hipHostFree(c->h_desc_pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete x;
     return CG_OK;
@@ -202,128 +748,45 @@ const char* cg_last_error(const cg_ctx* x) { return x ? x->c.err.c_str() : "null
 
 uint64_t cg_device_bytes(const cg_ctx* x) { return x ? x->c.held : 0; }
 
+// ---- single-graph entry points (graph 0) ---------------------------------
+
 int cg_set_positions(cg_ctx* x, const float* xy, uint32_t n) {
-    if (!x) return CG_ERR_INVALID;
-    Ctx* c = &x->c;
-    if (n && !xy) return fail(c, CG_ERR_INVALID, "cg_set_positions: null positions");
-    CG_HIP(c, hipSetDevice(c->device));
-    CG_HIP(c, hipStreamSynchronize(c->stream));
-    const bool same_n = (n == c->n) && c->pos[0];
-    std::vector<float> keep_jitter;
-    free_points(c);
-    c->n = n;
-    c->cur = 0;
-    // Points/position-utils.ts:17-24: a half-NaN position becomes fully NaN (absent)
-    std::vector<float> pos(2 * (size_t)n);
-    std::vector<uint32_t> present;
-    c->h_flags.assign(n, 0);
-    present.reserve(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        float px = xy[2 * i], py = xy[2 * i + 1];
-        bool absent = std::isnan(px) || std::isnan(py);
-        pos[2 * i] = absent ? NAN : px;
-        pos[2 * i + 1] = absent ? NAN : py;
-        if (absent) c->h_flags[i] = F_ABSENT;
-        else present.push_back(i);
-    }
-    for (uint32_t k : c->h_pinned)
-        if (k < n) c->h_flags[k] |= F_PINNED;
-    c->np = (uint32_t)present.size();
-    geometry(n, &c->gf, &c->L);
-    int rc = CG_OK;
-    const size_t nn = n ? n : 1;
-    c->pos[0] = (float2*)dalloc(c, nn * 8, &rc); if (rc) return rc;
-    c->pos[1] = (float2*)dalloc(c, nn * 8, &rc); if (rc) return rc;
-    c->flags = (uint8_t*)dalloc(c, nn, &rc); if (rc) return rc;
-    c->present = (uint32_t*)dalloc(c, (c->np ? c->np : 1) * 4, &rc); if (rc) return rc;
-    c->jitter = (float2*)dalloc(c, nn * 8, &rc); if (rc) return rc;
-    if (n) {
-        CG_HIP(c, hipMemcpyAsync(c->pos[0], pos.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
-        CG_HIP(c, hipMemcpyAsync(c->pos[1], pos.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
-        rc = upload_flags(c); if (rc) return rc;
-        if (c->np)
-            CG_HIP(c, hipMemcpyAsync(c->present, present.data(), (size_t)c->np * 4, hipMemcpyHostToDevice, c->stream));
-        // default jitter from the context generator: (U(-1,1)*1e-5, U(-1,1)*1e-5)
-        std::vector<float> jit(2 * (size_t)n);
-        for (auto& v : jit) v = (rng_unit(c) * 2.f - 1.f) * 0.00001f;
-        CG_HIP(c, hipMemcpyAsync(c->jitter, jit.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
-        CG_HIP(c, hipStreamSynchronize(c->stream));
-    }
-    if (!same_n) c->h_sizes.clear();
-    rc = mb_alloc(c); if (rc) return rc;
-    if (c->links_set) { rc = link_build(c); if (rc) return rc; }
-    return CG_OK;
+    CG_CTX(x);
+    return graph_set_positions(c, default_graph(c), xy, n);
 }
 
 int cg_set_links(cg_ctx* x, const float* links, uint32_t m, const float* strength,
                  const float* rand_in, const float* rand_out) {
-    if (!x) return CG_ERR_INVALID;
-    Ctx* c = &x->c;
-    if (m && !links) return fail(c, CG_ERR_INVALID, "cg_set_links: null links");
-    CG_HIP(c, hipSetDevice(c->device));
-    c->m = m;
-    c->h_links.assign(links, links + 2 * (size_t)m);
-    if (strength) c->h_strength.assign(strength, strength + m);
-    else c->h_strength.clear();
-    c->h_rin.resize(m);
-    c->h_rout.resize(m);
-    for (uint32_t l = 0; l < m; ++l) c->h_rin[l] = rand_in ? rand_in[l] : rng_unit(c);
-    for (uint32_t l = 0; l < m; ++l) c->h_rout[l] = rand_out ? rand_out[l] : rng_unit(c);
-    c->links_set = true;
-    return link_build(c);
+    CG_CTX(x);
+    return graph_set_links(c, default_graph(c), links, m, strength, rand_in, rand_out);
 }
 
 int cg_set_jitter(cg_ctx* x, const float* xy, uint32_t n) {
-    if (!x) return CG_ERR_INVALID;
-    Ctx* c = &x->c;
-    if (n != c->n || (n && !xy)) return fail(c, CG_ERR_INVALID, "cg_set_jitter: expected one pair per point");
-    CG_HIP(c, hipSetDevice(c->device));
-    if (n) {
-        CG_HIP(c, hipMemcpyAsync(c->jitter, xy, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
-        CG_HIP(c, hipStreamSynchronize(c->stream));
-    }
-    return CG_OK;
+    CG_CTX(x);
+    return graph_set_jitter(c, default_graph(c), xy, n);
 }
 
 int cg_set_point_sizes(cg_ctx* x, const float* sizes, uint32_t n) {
-    if (!x) return CG_ERR_INVALID;
-    Ctx* c = &x->c;
-    if (sizes && n != c->n) return fail(c, CG_ERR_INVALID, "cg_set_point_sizes: expected one size per point");
-    if (sizes) c->h_sizes.assign(sizes, sizes + n);
-    else c->h_sizes.clear();
-    c->col_last_default = -1.f;  // re-resolve sizes on the next collision pass
-    return CG_OK;
+    CG_CTX(x);
+    return graph_set_point_sizes(c, default_graph(c), sizes, n);
 }
 
 int cg_set_pinned(cg_ctx* x, const uint32_t* idx, uint32_t count) {
-    if (!x) return CG_ERR_INVALID;
-    Ctx* c = &x->c;
-    if (count && !idx) return fail(c, CG_ERR_INVALID, "cg_set_pinned: null indices");
-    CG_HIP(c, hipSetDevice(c->device));
-    c->h_pinned.assign(idx, idx + count);
-    for (auto& f : c->h_flags) f &= (uint8_t)~F_PINNED;
-    for (uint32_t k = 0; k < count; ++k)
-        if (idx[k] < c->n) c->h_flags[idx[k]] |= F_PINNED;
-    if (c->n) {
-        int rc = upload_flags(c); if (rc) return rc;
-        CG_HIP(c, hipStreamSynchronize(c->stream));
-    }
-    return CG_OK;
+    CG_CTX(x);
+    return graph_set_pinned(c, default_graph(c), idx, count);
 }
 
 int cg_set_alpha(cg_ctx* x, double alpha) {
     if (!x) return CG_ERR_INVALID;
-    x->c.alpha = alpha;
+    default_graph(&x->c).alpha = alpha;
     return CG_OK;
 }
 
 int cg_get_alpha(cg_ctx* x, double* alpha) {
     if (!x || !alpha) return CG_ERR_INVALID;
-    *alpha = x->c.alpha;
+    *alpha = default_graph(&x->c).alpha;
     return CG_OK;
 }
-
-static bool on(float v) { return v != 0.f && !std::isnan(v); }  // JS truthiness of a number
 
 int cg_step(cg_ctx* x, const cg_sim_params* p, uint32_t nticks, const float* seeds,
             double* alpha_trace) {
@@ -331,110 +794,176 @@ int cg_step(cg_ctx* x, const cg_sim_params* p, uint32_t nticks, const float* see
     Ctx* c = &x->c;
     if (!p->enable_simulation) return CG_OK;
     CG_HIP(c, hipSetDevice(c->device));
-    const double decay_factor = 1.0 - std::pow(0.001, 1.0 / (double)p->decay);
-    const bool have_links = c->links_ready && c->m > 0;  // linksTextureSize != 0
-    int rc = CG_OK;
-    for (uint32_t t = 0; t < nticks; ++t) {
-        const float a = (float)c->alpha;
-        const float seed = seeds ? seeds[t] : rng_unit(c);
-        if (c->np && p->mouse_repulsion) {
-            prof_begin(c, CG_STAGE_MOUSE);
-            rc = pw_run(c, p, CG_FORCE_MOUSE, a, c->pos[c->cur], c->pos[c->cur ^ 1], nullptr, false);
-            prof_end(c, CG_STAGE_MOUSE);
-            if (rc) return rc;
-            c->cur ^= 1;
-        }
-        if (p->run_forces) {
-            if (c->np) {
-                const bool grav = on(p->gravity), cent = on(p->center);
-                if (grav) {
-                    prof_begin(c, CG_STAGE_GRAVITY);
-                    rc = pw_run(c, p, CG_FORCE_GRAVITY, a, c->pos[c->cur], c->pos[c->cur ^ 1], nullptr, !cent);
-                    prof_end(c, CG_STAGE_GRAVITY);
-                    if (rc) return rc;
-                    c->cur ^= 1;
-                }
-                if (cent) {
-                    prof_begin(c, CG_STAGE_CENTER);
-                    rc = pw_run(c, p, CG_FORCE_CENTER, a, c->pos[c->cur], c->pos[c->cur ^ 1], nullptr, true);
-                    prof_end(c, CG_STAGE_CENTER);
-                    if (rc) return rc;
-                    c->cur ^= 1;
-                }
-                rc = mb_run(c, p, a, seed, c->pos[c->cur], c->pos[c->cur ^ 1], nullptr, grav || cent);
-                if (rc) return rc;
-                c->cur ^= 1;
-                if (have_links) {
-                    for (int d = 0; d < 2; ++d) {
-                        prof_begin(c, CG_STAGE_LINK_IN + d);
-                        rc = link_run(c, p, d, a, c->pos[c->cur], c->pos[c->cur ^ 1], nullptr);
-                        prof_end(c, CG_STAGE_LINK_IN + d);
-                        if (rc) return rc;
-                        c->cur ^= 1;
-                    }
-                }
-                if (on(p->collision)) {
-                    prof_begin(c, CG_STAGE_COLLISION);
-                    rc = col_run(c, p, a, c->pos[c->cur], c->pos[c->cur ^ 1], nullptr);
-                    prof_end(c, CG_STAGE_COLLISION);
-                    if (rc) return rc;
-                    c->cur ^= 1;
-                }
-            }
-            c->alpha += (0.0 - c->alpha) * decay_factor;
-            if (p->mouse_repulsion) c->alpha = std::max(c->alpha, 0.1);
-        }
-        if (alpha_trace) alpha_trace[t] = c->alpha;
-    }
-    CG_HIP(c, hipGetLastError());
-    return CG_OK;
+    Graph& g = default_graph(c);
+    set_params(c, g, *p);
+    return run_ticks(c, {0u}, nticks, 0, seeds, alpha_trace);
 }
 
 int cg_get_positions(cg_ctx* x, float* xy, uint32_t n) {
-    if (!x) return CG_ERR_INVALID;
-    Ctx* c = &x->c;
-    if (n != c->n || (n && !xy)) return fail(c, CG_ERR_INVALID, "cg_get_positions: size mismatch");
-    CG_HIP(c, hipSetDevice(c->device));
-    if (n) {
-        CG_HIP(c, hipMemcpyAsync(xy, c->pos[c->cur], (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    CG_CTX(x);
+    if (c->graphs.empty() || !c->graphs[0].live) {
+        if (n) return fail(c, CG_ERR_INVALID, "cg_get_positions: size mismatch");
+        return CG_OK;
     }
+    return graph_get_positions(c, c->graphs[0], xy, n);
+}
+
+// ---- packed graph set ------------------------------------------------------
+
+int cg_graph_add(cg_ctx* x, uint32_t* id) {
+    CG_CTX(x);
+    if (!id) return fail(c, CG_ERR_INVALID, "cg_graph_add: null id");
+    if (c->graphs.empty()) c->graphs.emplace_back();  // id 0 stays with the single-graph calls
+    const uint32_t nid = (uint32_t)c->graphs.size();
+    c->graphs.emplace_back();
+    reset_graph(c->graphs.back(), nid);
+    c->dirty = true;
+    *id = nid;
+    return CG_OK;
+}
+
+int cg_graph_remove(cg_ctx* x, uint32_t id) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    link_free_graph(c, *g);
+    *g = Graph();
+    g->id = id;
+    g->live = false;
+    c->dirty = true;
+    return CG_OK;
+}
+
+int cg_graph_set_params(cg_ctx* x, uint32_t id, const cg_sim_params* p) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    if (!p) return fail(c, CG_ERR_INVALID, "cg_graph_set_params: null params");
+    set_params(c, *g, *p);
+    return CG_OK;
+}
+
+int cg_graph_set_seed(cg_ctx* x, uint32_t id, uint64_t seed) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    g->rng = seed;
+    return CG_OK;
+}
+
+int cg_graph_set_positions(cg_ctx* x, uint32_t id, const float* xy, uint32_t n) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    return graph_set_positions(c, *g, xy, n);
+}
+
+int cg_graph_set_links(cg_ctx* x, uint32_t id, const float* links, uint32_t m, const float* strength,
+                       const float* rand_in, const float* rand_out) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    return graph_set_links(c, *g, links, m, strength, rand_in, rand_out);
+}
+
+int cg_graph_set_jitter(cg_ctx* x, uint32_t id, const float* xy, uint32_t n) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    return graph_set_jitter(c, *g, xy, n);
+}
+
+int cg_graph_set_point_sizes(cg_ctx* x, uint32_t id, const float* sizes, uint32_t n) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    return graph_set_point_sizes(c, *g, sizes, n);
+}
+
+int cg_graph_set_pinned(cg_ctx* x, uint32_t id, const uint32_t* idx, uint32_t count) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    return graph_set_pinned(c, *g, idx, count);
+}
+
+int cg_graph_set_alpha(cg_ctx* x, uint32_t id, double alpha) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    g->alpha = alpha;
+    return CG_OK;
+}
+
+int cg_graph_get_alpha(cg_ctx* x, uint32_t id, double* alpha) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    if (!alpha) return fail(c, CG_ERR_INVALID, "cg_graph_get_alpha: null output");
+    *alpha = g->alpha;
+    return CG_OK;
+}
+
+int cg_graph_get_positions(cg_ctx* x, uint32_t id, float* xy, uint32_t n) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    return graph_get_positions(c, *g, xy, n);
+}
+
+int cg_step_graphs(cg_ctx* x, const uint32_t* ids, uint32_t count, uint32_t nticks) {
+    CG_CTX(x);
+    if (count && !ids) return fail(c, CG_ERR_INVALID, "cg_step_graphs: null ids");
+    std::vector<uint32_t> v(ids, ids + count);
+    for (uint32_t id : v) {
+        CG_GRAPH(g, id);
+        (void)g;
+    }
+    return run_ticks(c, v, nticks, -1, nullptr, nullptr);
+}
+
+int cg_sync(cg_ctx* x) {
+    CG_CTX(x);
     CG_HIP(c, hipStreamSynchronize(c->stream));
     return CG_OK;
 }
+
+// ---- inspection (graph 0) --------------------------------------------------
 
 int cg_eval_force(cg_ctx* x, const cg_sim_params* p, int32_t force, float alpha, float seed,
                   float* v) {
     if (!x || !p) return CG_ERR_INVALID;
     Ctx* c = &x->c;
-    if (c->n && !v) return fail(c, CG_ERR_INVALID, "cg_eval_force: null output");
     CG_HIP(c, hipSetDevice(c->device));
-    if (c->n == 0) return CG_OK;
+    Graph& g0 = default_graph(c);
+    if (g0.n && !v) return fail(c, CG_ERR_INVALID, "cg_eval_force: null output");
+    const int pass = pass_of_force(force);
+    if (pass < 0) return fail(c, CG_ERR_INVALID, "cg_eval_force: unknown force");
+    set_params(c, g0, *p);
     int rc = CG_OK;
-    float2* dv = (float2*)dalloc(c, (size_t)c->n * 8, &rc);
+    if (c->dirty) {
+        rc = pack(c);
+        if (rc) return rc;
+    }
+    Graph& g = c->graphs[0];
+    if (g.n == 0) return CG_OK;
+    std::memset(v, 0, (size_t)g.n * 8);
+    const bool runs = g.np && ((pass < P_LINK_IN) || (pass == P_COLLISION && collides(g)) ||
+                               ((pass == P_LINK_IN || pass == P_LINK_OUT) && g.links_ready && g.m > 0));
+    if (!runs) return CG_OK;
+    std::vector<GDesc> table(c->G);
+    for (uint32_t s = 0; s < c->G; ++s) {
+        const Graph& h = c->graphs[c->order[s]];
+        GDesc d = c->h_desc[s];
+        fill_dynamic(d, h.params, alpha, seed);
+        d.passes = (s == g.slot) ? (1u << pass) : 0u;
+        d.inbuf = h.cur ? ((1u << P_COUNT) - 1) : 0u;
+        table[s] = d;
+    }
+    rc = write_table(c, table);
     if (rc) return rc;
-    CG_HIP(c, hipMemsetAsync(dv, 0, (size_t)c->n * 8, c->stream));
-    const float2* pin = c->pos[c->cur];
-    switch (force) {
-        case CG_FORCE_GRAVITY:
-        case CG_FORCE_CENTER:
-        case CG_FORCE_MOUSE:
-            rc = pw_run(c, p, force, alpha, pin, nullptr, dv, false);
-            break;
-        case CG_FORCE_MANYBODY:
-            rc = mb_run(c, p, alpha, seed, pin, nullptr, dv, false);
-            break;
-        case CG_FORCE_LINK_IN:
-        case CG_FORCE_LINK_OUT:
-            rc = link_run(c, p, force - CG_FORCE_LINK_IN, alpha, pin, nullptr, dv);
-            break;
-        case CG_FORCE_COLLISION:
-            rc = col_run(c, p, alpha, pin, nullptr, dv);
-            break;
-        default:
-            rc = fail(c, CG_ERR_INVALID, "cg_eval_force: unknown force");
+    float2* dv = (float2*)dalloc(c, (size_t)c->N * 8, &rc);
+    if (rc) return rc;
+    hipError_t e = hipMemsetAsync(dv, 0, (size_t)c->N * 8, c->stream);
+    if (e != hipSuccess) rc = hip_fail(c, e, "cg_eval_force clear");
+    if (!rc) {
+        if (pass <= P_CENTER) rc = pw_run(c, pass, false, dv);
+        else if (pass == P_MANYBODY) rc = mb_run(c, false, dv);
+        else if (pass == P_COLLISION) rc = col_run(c, dv);
+        else rc = link_run(c, pass - P_LINK_IN, dv);
     }
     if (rc == CG_OK) {
-        hipError_t e = hipMemcpyAsync(v, dv, (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream);
+        e = hipMemcpyAsync(v, dv + g.p0, (size_t)g.n * 8, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = hip_fail(c, e, "cg_eval_force readback");
     }
@@ -445,16 +974,30 @@ int cg_eval_force(cg_ctx* x, const cg_sim_params* p, int32_t force, float alpha,
 
 int cg_debug_slots(cg_ctx* x, int32_t* slots, uint32_t capacity, uint32_t* finest,
                    uint32_t* levels) {
-    if (!x) return CG_ERR_INVALID;
-    Ctx* c = &x->c;
-    if (finest) *finest = c->gf;
-    if (levels) *levels = c->L;
-    const size_t need = (size_t)c->gf * c->gf * SLOTS;
+    CG_CTX(x);
+    if (c->graphs.empty() || !c->graphs[0].live) return fail(c, CG_ERR_STATE, "cg_debug_slots: no graph");
+    const Graph& g = c->graphs[0];
+    if (finest) *finest = g.gf;
+    if (levels) *levels = g.L;
+    const size_t need = (size_t)g.gf * g.gf * SLOTS;
     if (!slots) return CG_OK;
-    if (capacity < need || !c->slot_idx) return fail(c, CG_ERR_INVALID, "cg_debug_slots: capacity");
-    CG_HIP(c, hipSetDevice(c->device));
-    CG_HIP(c, hipMemcpyAsync(slots, c->slot_idx, need * 4, hipMemcpyDeviceToHost, c->stream));
+    if (capacity < need || !c->slot_idx || c->dirty)
+        return fail(c, CG_ERR_INVALID, "cg_debug_slots: capacity or no many-body pass yet");
+    // the kernels skip the slots of empty cells: report them as -1 from the cell counts
+    const size_t cells = (size_t)g.gf * g.gf;
+    std::vector<double> sums(3 * cells);
+    CG_HIP(c, hipMemcpyAsync(slots, c->slot_idx + (size_t)g.f0 * SLOTS, need * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+    CG_HIP(c, hipMemcpyAsync(sums.data(), c->sums + 3 * ((size_t)g.c0 + level_off(g.L - 1)),
+                             sums.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CG_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t rm = 0; rm < cells; ++rm) {
+        const bool empty = !(sums[3 * rm + 2] > 0.0);
+        for (size_t s = 0; s < SLOTS; ++s) {
+            int32_t& v = slots[rm * SLOTS + s];
+            v = (empty || v < 0) ? -1 : v - (int32_t)g.p0;
+        }
+    }
     return CG_OK;
 }
 
@@ -462,18 +1005,20 @@ int cg_debug_link_csr(cg_ctx* x, int32_t dir, uint32_t* offsets, uint32_t* nbr, 
                       float* strength, uint32_t* degree, uint32_t capacity, uint32_t* entries) {
     if (!x || dir < 0 || dir > 1) return CG_ERR_INVALID;
     Ctx* c = &x->c;
-    const LinkDir& L = c->dir[dir];
-    if (entries) *entries = L.E;
-    if (!c->links_ready) return fail(c, CG_ERR_STATE, "cg_debug_link_csr: no links");
     CG_HIP(c, hipSetDevice(c->device));
-    const uint32_t n = c->n;
-    if (offsets) CG_HIP(c, hipMemcpy(offsets, L.off_pt, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
-    if (degree) CG_HIP(c, hipMemcpy(degree, c->degree, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (c->graphs.empty() || !c->graphs[0].live) return fail(c, CG_ERR_STATE, "cg_debug_link_csr: no graph");
+    const Graph& g = c->graphs[0];
+    const GraphLinks& lk = g.lk;
+    if (entries) *entries = lk.E[dir];
+    if (!g.links_ready) return fail(c, CG_ERR_STATE, "cg_debug_link_csr: no links");
+    const uint32_t n = g.n, E = lk.E[dir];
+    if (offsets) CG_HIP(c, hipMemcpy(offsets, lk.off_pt[dir], ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+    if (degree) CG_HIP(c, hipMemcpy(degree, lk.degree, (size_t)n * 4, hipMemcpyDeviceToHost));
     if (nbr || bias || strength) {
-        if (capacity < L.E) return fail(c, CG_ERR_INVALID, "cg_debug_link_csr: capacity");
-        std::vector<uint32_t> ent((size_t)L.E * 4);
-        if (L.E) CG_HIP(c, hipMemcpy(ent.data(), L.ent_pt, (size_t)L.E * 16, hipMemcpyDeviceToHost));
-        for (uint32_t e = 0; e < L.E; ++e) {
+        if (capacity < E) return fail(c, CG_ERR_INVALID, "cg_debug_link_csr: capacity");
+        std::vector<uint32_t> ent((size_t)E * 4);
+        if (E) CG_HIP(c, hipMemcpy(ent.data(), lk.ent_pt[dir], (size_t)E * 16, hipMemcpyDeviceToHost));
+        for (uint32_t e = 0; e < E; ++e) {
             if (nbr) nbr[e] = ent[4 * (size_t)e];
             if (strength) std::memcpy(&strength[e], &ent[4 * (size_t)e + 2], 4);
             if (bias) std::memcpy(&bias[e], &ent[4 * (size_t)e + 3], 4);
@@ -501,9 +1046,7 @@ int cg_set_profiling(cg_ctx* x, uint32_t mask) {
 }
 
 int cg_get_profile(cg_ctx* x, double* ms, uint32_t* launches, uint32_t nstages) {
-    if (!x) return CG_ERR_INVALID;
-    Ctx* c = &x->c;
-    CG_HIP(c, hipSetDevice(c->device));
+    CG_CTX(x);
     CG_HIP(c, hipStreamSynchronize(c->stream));
     for (uint32_t s = 0; s < nstages; ++s) {
         if (ms) ms[s] = 0.0;

@@ -1,4 +1,11 @@
 // cg_internal.h — shared state and device helpers of the force-simulation library.
+//
+// One context holds a set of graphs packed into shared device arrays: points of
+// graph g occupy [p0, p0 + n), its present points [k0, k0 + np) of the present
+// list, its finest cells [f0, f0 + gf^2) of the finest-cell index space and its
+// pyramid levels a block of the level arena starting at c0.  Every kernel looks
+// up its graph's descriptor, so one launch per pass serves every graph, and a
+// graph computes the same bits alone or packed with others.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,20 +31,101 @@ constexpr uint32_t BIG_CELL = 64; // finest cells above this many points use a w
 constexpr uint32_t LANE_MAX = 32;
 constexpr uint32_t WAVE_MAX = 4096;
 
+// Passes of one tick, in order (src/index.ts:2012-2078).  Bit p of GDesc::passes
+// says the graph runs pass p this tick; bit p of GDesc::inbuf names the position
+// buffer (0/1) holding the graph's positions when pass p starts.
+enum Pass : uint32_t {
+    P_MOUSE = 0,
+    P_GRAVITY = 1,
+    P_CENTER = 2,
+    P_MANYBODY = 3,
+    P_LINK_IN = 4,
+    P_LINK_OUT = 5,
+    P_COLLISION = 6,
+    P_COUNT = 7
+};
+
+// Level l of a graph's pyramid starts this many cells after c0 (levels 4^2, 8^2, ...).
+__host__ __device__ inline uint32_t level_off(uint32_t l) { return 16u * ((1u << (2 * l)) - 1u) / 3u; }
+
+// Per-graph descriptor, one per graph per tick (device table).
+struct GDesc {
+    uint32_t p0, n;         // packed point range
+    uint32_t k0, np;        // packed present-list range
+    uint32_t f0;            // first finest cell
+    uint32_t c0;            // level arena base
+    uint32_t gf, L;         // finest grid size, level count
+    uint32_t passes;        // passes run this tick
+    uint32_t inbuf;         // input position buffer per pass
+    uint32_t col_c0;        // collision cell base (4 grids of col_G^2)
+    uint32_t col_G;
+    float h[MAX_LEVELS];    // S / (4 << l) as f32
+    float S;
+    float alpha;            // f32 copy of this tick's alpha
+    float seed;             // near-field seed of this tick
+    float ak;               // alpha * repulsion
+    float friction;
+    float gravity, center;
+    float mouse_k, mx, my;
+    float spring_alpha, dist, lo, span;
+    float tol;              // multiple-shooting tolerance (2.5e-6 S)
+    float col_strength, col_radius, col_padding, col_h;
+};
+
+// Per-graph link CSR built once per cg_set_links, in the reference's entry
+// order with graph-local indices (ForceLink/index.ts:40-83).
+struct GraphLinks {
+    uint32_t E[2] = {0, 0};
+    uint4* ent_pt[2] = {nullptr, nullptr};   // {local neighbour, draw, strength, bias}
+    uint32_t* off_pt[2] = {nullptr, nullptr};// [n+1]
+    uint32_t* live[2] = {nullptr, nullptr};  // [n] entries toward present neighbours
+    uint32_t* degree = nullptr;              // [n]
+};
+
+struct Graph {
+    uint32_t id = 0;
+    bool live = false;
+    uint32_t n = 0, np = 0, m = 0;
+    cg_sim_params params{};
+    double alpha = 1.0;
+    uint64_t rng = 0x5eed5eedULL;
+    // host copies
+    std::vector<float> xy;                   // positions as set (absent = NaN), until packed
+    bool xy_pending = false;
+    std::vector<uint8_t> flags;
+    std::vector<uint32_t> pinned;
+    std::vector<float> jitter;
+    std::vector<float> links, strength, rin, rout;
+    bool links_set = false;
+    std::vector<float> sizes;                // raw sizes (NaN = default); empty = none
+    // derived
+    uint32_t gf = 0, L = 0;
+    GraphLinks lk;
+    bool links_ready = false;
+    uint32_t col_G = 0;
+    float col_h = 0.f;
+    bool col_dirty = true;
+    std::vector<float> col_size;             // resolved sizes
+    // placement in the packed arrays (valid after pack)
+    uint32_t slot = 0;                       // index in the descriptor table
+    uint32_t p0 = 0, k0 = 0, f0 = 0, c0 = 0, col_c0 = 0;
+    uint32_t cur = 0;                        // buffer holding current positions
+    uint32_t w0[2] = {0, 0}, wn[2] = {0, 0}; // warm-start region per link direction
+};
+
 struct LinkDir {
-    uint32_t E = 0;              // entries (valid links) in this direction
-    uint32_t E_live = 0;         // entries whose neighbour is present
-    uint4* ent_pt = nullptr;     // point-ordered entries {nbr, rand, strength, bias} (reference order)
-    uint32_t* off_pt = nullptr;  // [n+1] point-ordered offsets
-    uint4* ent = nullptr;        // schedule-ordered live entries used by the kernels
-    uint32_t* sched = nullptr;   // schedule: present points by ascending live degree
-    uint32_t* soff = nullptr;    // [n_sched+1] offsets of each schedule slot into ent
+    uint32_t E_live = 0;         // live entries over all graphs
+    uint4* ent = nullptr;        // schedule-ordered live entries (global neighbour indices)
+    uint32_t* sched = nullptr;   // present points by ascending live degree
+    uint32_t* soff = nullptr;    // [n_sched+1]
     uint32_t n_sched = 0;
-    uint32_t n_lane = 0, n_wave = 0, n_block = 0;  // tier sizes, in schedule order
-    uint32_t e_chunk0 = 0;       // first entry (in ent) of the chunked tiers
+    uint32_t n_lane = 0, n_wave = 0, n_block = 0;
+    uint32_t e_chunk0 = 0;       // first entry of the chunked tiers
     uint32_t* poff = nullptr;    // [n_wave+n_block+1] step-major offsets into link_pre
     uint32_t* eslot = nullptr;   // schedule slot of each chunked-tier entry
-    float2* warm = nullptr;      // per chunk-tier lane: last converged chunk start
+    uint32_t* woff = nullptr;    // [n_wave+n_block] warm-start offset of each chunked slot
+    float2* warm = nullptr;      // last converged chunk starts, per graph regions
+    size_t warm_len = 0;
 };
 
 struct Ctx {
@@ -50,58 +138,68 @@ struct Ctx {
     std::string err;
     std::vector<std::pair<void*, size_t>> allocs;
 
-    uint32_t n = 0;       // points
-    uint32_t np = 0;      // present points
-    double alpha = 1.0;
-    uint64_t rng = 0x5eed5eedULL;
+    std::vector<Graph> graphs;      // by id
+    bool dirty = true;              // the graph set changed: repack before the next tick
 
-    // host copies of the inputs needed to rebuild derived state
-    std::vector<float> h_links, h_strength, h_rin, h_rout;
-    uint32_t m = 0;
-    bool links_set = false;
-    std::vector<uint8_t> h_flags;
-    std::vector<float> h_sizes;     // raw sizes (NaN = default), empty = none set
+    // packed sizes
+    uint32_t G = 0;                 // live graphs
+    uint32_t N = 0, NP = 0;         // points, present points
+    uint32_t F = 0, C = 0;          // finest cells, level cells
+    uint32_t CG = 0;                // collision cells (all grids)
+    std::vector<uint32_t> order;    // live graph ids in packing order (slot -> id)
 
-    // point state
+    // packed point state
     float2* pos[2] = {nullptr, nullptr};
-    int cur = 0;
     uint8_t* flags = nullptr;
     uint32_t* present = nullptr;
+    uint32_t* gid = nullptr;        // graph slot of every packed point
     float2* jitter = nullptr;
+    float* size = nullptr;          // resolved collision sizes
+    GDesc* desc = nullptr;          // descriptor table of the tick being launched
+    std::vector<GDesc> h_desc;      // placement part of every descriptor, by slot
+    uint32_t* f0s = nullptr;        // [G] finest-cell bases
+    uint32_t* colc0s = nullptr;     // [G] collision cell bases
 
     // many-body
-    uint32_t gf = 0, L = 0;
     uint32_t* keys[2] = {nullptr, nullptr};
     uint32_t* vals[2] = {nullptr, nullptr};
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
-    uint32_t* cell_begin = nullptr;
+    int key_bits = 1;
+    uint32_t* cell_begin = nullptr; // [F]
     uint32_t* cell_end = nullptr;
-    double* sums[MAX_LEVELS] = {};
-    float4* cent[MAX_LEVELS] = {};
-    float2* sorted_pos = nullptr;
-    int32_t* slot_idx = nullptr;
+    double* sums = nullptr;         // [3 C]
+    float4* cent = nullptr;         // [C]
+    float2* sorted_pos = nullptr;   // [NP]
+    int32_t* slot_idx = nullptr;    // [8 F] global point index or -1
     float2* slot_pos = nullptr;
     uint32_t* big_list = nullptr;
     uint32_t* big_count = nullptr;
-    double* center_part = nullptr;  // block partial sums for the centroid
-    double* center_sum = nullptr;   // [3]
+    uint4* pyr[2] = {nullptr, nullptr};  // pyramid work lists {graph slot, level, tile, levels}
+    uint32_t n_pyr[2] = {0, 0};
+    double* center_part = nullptr;  // [2 chunks] per-chunk partial sums
+    double* center_sum = nullptr;   // [3 G]
+    uint4* center_chunks = nullptr; // {k begin, k end, graph slot, -} per chunk
+    uint32_t* center_first = nullptr;  // [G + 1] first chunk of each graph
+    uint32_t n_center_chunks = 0;
 
-    // collision
-    uint32_t col_G = 0;
-    uint32_t* col_begin = nullptr;  // [G*G]
+    // collision (graphs with simulationCollision on)
+    uint32_t NPC = 0;               // their present points
+    uint32_t* cpresent = nullptr;   // [NPC]
+    uint32_t* col_keys[2] = {nullptr, nullptr};  // [4 NPC]
+    uint32_t* col_vals[2] = {nullptr, nullptr};
+    void* col_sort_tmp = nullptr;
+    size_t col_sort_bytes = 0;
+    uint32_t* col_begin = nullptr;  // [CG]
     uint32_t* col_end = nullptr;
-    float4* col_cell = nullptr;     // [4][G*G] {sum x, sum y, sum size, count}
-    float* col_size = nullptr;      // resolved sizes [n]
-    float col_max_size = 0.f;
-    float col_last_default = -1.f;
+    float4* col_cell = nullptr;     // [CG]
+    int col_bits = 1;
 
     // links
-    uint32_t* degree = nullptr;
     LinkDir dir[2];
-    bool links_ready = false;
-    float4* link_pre = nullptr;     // per-entry (q.x, q.y, D, K) of the chunked tiers
-    uint32_t* link_stats = nullptr; // multiple-shooting sweep counters (cg_debug_link_stats)
+    bool any_links = false;
+    float4* link_pre = nullptr;
+    uint32_t* link_stats = nullptr;
 
     // stage timing (cg_set_profiling)
     uint32_t prof_mask = 0;
@@ -110,7 +208,13 @@ struct Ctx {
     hipEvent_t prof_open[CG_STAGES] = {};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_rec;
 
-    std::vector<uint32_t> h_pinned;  // pinned indices as given (re-applied when n changes)
+    // per-tick descriptor tables: device [desc_ticks * G], pinned host staging of
+    // the same size, and the event after the last upload from that staging
+    GDesc* desc_base = nullptr;
+    GDesc* h_desc_pinned = nullptr;
+    uint32_t desc_ticks = 0;
+    hipEvent_t ev_desc = nullptr;
+    bool desc_pending = false;
 };
 
 void prof_begin(Ctx* c, int stage);
@@ -128,19 +232,24 @@ void dfree(Ctx* c, void* p);
         if (e_ != hipSuccess) return cg::hip_fail(c, e_, #expr); \
     } while (0)
 
-// launchers (k_*.hip)
-int mb_alloc(Ctx* c);                                  // k_manybody.hip
-int mb_run(Ctx* c, const cg_sim_params* p, float alpha, float seed, const float2* pin,
-           float2* pout, float2* vout, bool keys_ready);
-int pw_run(Ctx* c, const cg_sim_params* p, int force, float alpha, const float2* pin,
-           float2* pout, float2* vout, bool write_keys);   // k_pointwise.hip
-int keys_only(Ctx* c, const float2* pin, float space_size);
-int link_build(Ctx* c);                                // k_link.hip
-int link_run(Ctx* c, const cg_sim_params* p, int dir, float alpha, const float2* pin,
-             float2* pout, float2* vout);
-int col_prepare(Ctx* c, const cg_sim_params* p);       // k_collision.hip
-int col_run(Ctx* c, const cg_sim_params* p, float alpha, const float2* pin, float2* pout,
-            float2* vout);
+// Launchers.  Input and output position buffers come from GDesc::inbuf; vout
+// (non-null: evaluation mode) receives the pass's velocity instead of integrating.
+int pw_run(Ctx* c, int pass, bool write_keys, float2* vout);       // k_pointwise.hip
+int keys_run(Ctx* c);
+int center_run(Ctx* c);
+void center_chunks_host(const std::vector<uint32_t>& k0, const std::vector<uint32_t>& np,
+                        std::vector<uint4>& chunks, std::vector<uint32_t>& first);
+int mb_alloc(Ctx* c);                                             // k_manybody.hip
+int mb_run(Ctx* c, bool keys_ready, float2* vout);
+void pyramid_work_host(uint32_t slot, uint32_t gf, uint32_t L, std::vector<uint4>& w0,
+                       std::vector<uint4>& w1);
+int link_build_graph(Ctx* c, Graph& g);                           // k_link.hip
+void link_free_graph(Ctx* c, Graph& g);
+int link_pack(Ctx* c, LinkDir (&old)[2]);
+int link_run(Ctx* c, int dir, float2* vout);
+int col_prepare_graph(Ctx* c, Graph& g);                          // k_collision.hip
+int col_run(Ctx* c, float2* vout);
+int col_alloc(Ctx* c);
 
 // ---------------------------------------------------------------------------
 // device helpers
@@ -176,6 +285,7 @@ __device__ inline int cell_of(float p, float h, int g) {
 }
 
 // lowbias32 and the near-field ticket bits — build-nearfield-slots.vert:69-79.
+// `index` is the point's index within its graph.
 __device__ inline uint32_t ticket_bits(uint32_t index, float seed) {
     uint32_t h = index ^ __float_as_uint(seed);
     h ^= h >> 16;
@@ -199,6 +309,17 @@ __device__ inline float2 integrate(float2 p, float2 v, float friction, float S) 
     x = fminf(fmaxf(x, 0.f), S);
     y = fminf(fmaxf(y, 0.f), S);
     return make_float2(x, y);
+}
+
+// Largest j in [0, G) with base[j] <= x (base ascending, base[0] == 0).
+__device__ inline uint32_t find_graph(const uint32_t* __restrict__ base, uint32_t G, uint32_t x) {
+    uint32_t lo = 0, hi = G;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
 }
 
 inline unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }

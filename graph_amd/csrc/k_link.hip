@@ -1,5 +1,6 @@
-// k_link.hip — link springs: CSR build on device and the per-direction
-// Gauss-Seidel spring pass fused with the integrate pass.
+// k_link.hip — link springs: per-graph CSR build on device, the packed
+// schedule over all graphs, and the per-direction Gauss-Seidel spring pass
+// fused with the integrate pass.
 //
 // Reference: GraphData/index.ts:286-289 (isPointIndex), :525-568 (adjacency,
 // degrees); ForceLink/index.ts:40-171 (create: CSR, bias, strength, random
@@ -8,24 +9,33 @@
 // The reference evaluates each point's links serially, every term seeing the
 // velocity accumulated so far.  Points are scheduled by live degree:
 //   - up to LANE_MAX entries: one lane per point, the serial loop as written;
-//   - more: one workgroup per point.  The chain is cut into one chunk per lane;
-//     each sweep runs every chunk serially from a guessed start velocity,
-//     records the chunk's end velocity and its slope prod(1 - t_k), and an
-//     affine prefix scan over the chunks yields the next guesses (multiple
-//     shooting).  Sweeps repeat until the guesses stop moving; a last sweep
-//     from the converged starts gives the point's velocity.
+//   - more: one workgroup per point (64 or 1024 lanes).  The chain is cut into
+//     one chunk per lane; each sweep runs every chunk serially from a guessed
+//     start velocity, carrying the chunk's end velocity and 2x2 Jacobian, and
+//     an affine prefix scan over the chunks yields the next guesses (multiple
+//     shooting with Newton-accurate chunk maps).  Sweeps repeat until no start
+//     moves more than the tolerance; the last chunk's end, moved to its
+//     corrected start, is the point's velocity.  Converged starts are kept per
+//     point as the next call's first guesses.
 #include <hipcub/hipcub.hpp>
+
+#include <algorithm>
 
 #include "cg_internal.h"
 
 namespace cg {
 namespace {
 
-// ---- CSR build ------------------------------------------------------------------
+// ---- per-graph CSR build ------------------------------------------------------
 
 // isPointIndex (GraphData/index.ts:286-289): an integer in [0, n).
 __device__ inline bool valid_index(float v, uint32_t n) {
     return v >= 0.f && floorf(v) == v && (double)v < (double)n;
+}
+
+__global__ void k_fill_words(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
 }
 
 __global__ void k_validate(const float2* __restrict__ links, uint32_t m, uint32_t n,
@@ -44,11 +54,6 @@ __global__ void k_validate(const float2* __restrict__ links, uint32_t m, uint32_
     }
 }
 
-__global__ void k_fill_words(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
-}
-
 __global__ void k_degree(const uint32_t* __restrict__ outdeg, const uint32_t* __restrict__ indeg,
                          uint32_t n, uint32_t* __restrict__ degree) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,14 +69,14 @@ __global__ void k_iota_keys(const uint32_t* __restrict__ owner, uint32_t m,
 }
 
 // Entry e of the point-ordered CSR (ForceLink/index.ts:58-79): neighbour,
-// random draw, strength, bias — bias and strength in f64, stored as f32.
-// Also records the entry's owner and whether its neighbour is present.
+// random draw, strength, bias — bias and strength in f64, stored as f32 —
+// and whether it is live (owner and neighbour present).
 __global__ void k_entries(const uint32_t* __restrict__ sorted_owner,
                           const uint32_t* __restrict__ sorted_link, uint32_t E,
                           const uint32_t* __restrict__ other_of, const uint32_t* __restrict__ degree,
                           const float* __restrict__ strength, const float* __restrict__ rnd,
                           const uint8_t* __restrict__ flags, uint4* __restrict__ ent,
-                          uint32_t* __restrict__ owner_out, uint32_t* __restrict__ live_flag) {
+                          uint32_t* __restrict__ live_flag) {
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     uint32_t l = sorted_link[e];
@@ -91,21 +96,36 @@ __global__ void k_entries(const uint32_t* __restrict__ sorted_owner,
     }
     ent[e] = make_uint4(q, __float_as_uint(rnd[l]), __float_as_uint((float)sqrt(s)),
                         __float_as_uint((float)bias));
-    owner_out[e] = p;
     // only entries of a present owner toward a present neighbour are scheduled
-    // (an absent owner has no schedule slot; an absent neighbour is skipped,
-    // force-spring.ts:69-74)
+    // (force-spring.ts:69-74 skips an absent neighbour; an absent owner never moves)
     live_flag[e] = ((flags[q] | flags[p]) & F_ABSENT) ? 0u : 1u;
 }
 
-// Live (present-neighbour) entry count per present point, from the scan of live flags.
+// Live entries per point, from the scan of live flags.
+__global__ void k_live_count(const uint32_t* __restrict__ off_pt, const uint32_t* __restrict__ lrank,
+                             uint32_t n, uint32_t* __restrict__ live) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) live[p] = lrank[off_pt[p + 1]] - lrank[off_pt[p]];
+}
+
+// ---- packed schedule ------------------------------------------------------------
+
+struct GLinkPtr {
+    const uint4* ent_pt;
+    const uint32_t* off_pt;
+    const uint32_t* live;
+};
+
 __global__ void k_sched_keys(const uint32_t* __restrict__ present, uint32_t np,
-                             const uint32_t* __restrict__ off_pt, const uint32_t* __restrict__ lrank,
-                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                             const uint32_t* __restrict__ gid, const GDesc* __restrict__ desc,
+                             const GLinkPtr* __restrict__ tab, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np) return;
-    uint32_t p = present[k];
-    keys[k] = lrank[off_pt[p + 1]] - lrank[off_pt[p]];
+    const uint32_t p = present[k];
+    const uint32_t g = gid[p];
+    const GLinkPtr t = tab[g];
+    keys[k] = t.live ? t.live[p - desc[g].p0] : 0u;
     vals[k] = p;
 }
 
@@ -120,61 +140,76 @@ __global__ void k_tier_bounds(const uint32_t* __restrict__ skeys, uint32_t ns,
     if (d > WAVE_MAX && (k == 0 || prev <= WAVE_MAX)) counts[1] = k;
 }
 
-__global__ void k_slot_of(const uint32_t* __restrict__ sched, uint32_t ns,
-                          uint32_t* __restrict__ slot_of) {
+// Copy each scheduled point's live entries, in link order, to its slot, with
+// global neighbour indices.
+__global__ void k_assemble(const uint32_t* __restrict__ sched, uint32_t ns,
+                           const uint32_t* __restrict__ soff, const uint32_t* __restrict__ gid,
+                           const GDesc* __restrict__ desc, const GLinkPtr* __restrict__ tab,
+                           const uint8_t* __restrict__ flags, uint4* __restrict__ ent) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < ns) slot_of[sched[j]] = j;
+    if (j >= ns) return;
+    const uint32_t p = sched[j];
+    const uint32_t g = gid[p];
+    const GLinkPtr t = tab[g];
+    if (!t.ent_pt) return;
+    const uint32_t p0 = desc[g].p0, lp = p - p0;
+    uint32_t dst = soff[j];
+    for (uint32_t e = t.off_pt[lp]; e < t.off_pt[lp + 1]; ++e) {
+        uint4 v = t.ent_pt[e];
+        v.x += p0;
+        if (!(flags[v.x] & F_ABSENT)) ent[dst++] = v;
+    }
 }
 
-// Entry-parallel copy of live entries, in link order, to their schedule slot.
-__global__ void k_permute(const uint4* __restrict__ ent_pt, uint32_t E,
-                          const uint32_t* __restrict__ owner, const uint32_t* __restrict__ live_flag,
-                          const uint32_t* __restrict__ lrank, const uint32_t* __restrict__ off_pt,
-                          const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ soff,
-                          uint4* __restrict__ ent) {
-    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E || !live_flag[e]) return;
-    uint32_t p = owner[e];
-    uint32_t dst = soff[slot_of[p]] + (lrank[e] - lrank[off_pt[p]]);
-    ent[dst] = ent_pt[e];
+// Schedule slot of every chunked-tier entry: the last slot j in [first, last)
+// with soff[j] <= e, by binary search.
+__global__ void k_eslot(const uint32_t* __restrict__ soff, uint32_t first, uint32_t last,
+                        uint32_t e0, uint32_t e1, uint32_t* __restrict__ eslot) {
+    uint32_t e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    uint32_t lo = first, hi = last;  // invariant: soff[lo] <= e < soff[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (soff[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    eslot[e - e0] = lo;
 }
 
 // ---- spring pass ------------------------------------------------------------------
 
-struct LkArgs {
-    float spring_alpha;  // linkSpring * alpha
-    float dist;          // linkDistance
-    float lo, span;      // variation range [lo, hi): lo and hi - lo
-    float friction, S;
-    float tol;           // multiple-shooting convergence threshold on chunk starts
-};
-
 // One term of force-spring.ts:61-90.  Returns the step fraction t (v += x*t).
-__device__ inline float spring_term(const LkArgs& a, uint4 e, float2 pp, float2 v, float2 q,
+__device__ inline float spring_term(const GDesc& d, uint4 e, float2 pp, float2 v, float2 q,
                                     float& x, float& y) {
     float r = __uint_as_float(e.y), s = __uint_as_float(e.z), b = __uint_as_float(e.w);
-    float D = (r * a.span + a.lo) * a.dist;
+    float D = (r * d.span + d.lo) * d.dist;
     x = q.x - (pp.x + v.x);
     y = q.y - (pp.y + v.y);
     float l = sqrtf(x * x + y * y);
     l = fmaxf(l, D * 0.99f);
     l = (l - D) / l;
-    l *= a.spring_alpha;
+    l *= d.spring_alpha;
     l *= s;
     l *= b;
     return l;
 }
 
-__global__ void __launch_bounds__(256) k_link_lane(LkArgs a, const uint32_t* __restrict__ sched,
+__global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __restrict__ sched,
                                                    const uint32_t* __restrict__ soff,
                                                    const uint4* __restrict__ ent, uint32_t count,
-                                                   const float2* __restrict__ pin,
+                                                   const uint32_t* __restrict__ gid,
+                                                   const GDesc* __restrict__ desc,
+                                                   float2* __restrict__ pos0,
+                                                   float2* __restrict__ pos1,
                                                    const uint8_t* __restrict__ flags,
-                                                   float2* __restrict__ pout,
                                                    float2* __restrict__ vout) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= count) return;
     const uint32_t p = sched[j];
+    const GDesc& d = desc[gid[p]];
+    if (!(d.passes & (1u << pass))) return;
+    const bool bsel = (d.inbuf >> pass) & 1u;
+    const float2* __restrict__ pin = bsel ? pos1 : pos0;
     const uint32_t b = soff[j], e = soff[j + 1];
     const float2 pp = pin[p];
     float2 v = make_float2(0.f, 0.f);
@@ -189,14 +224,18 @@ __global__ void __launch_bounds__(256) k_link_lane(LkArgs a, const uint32_t* __r
 #pragma unroll
         for (int u = 0; u < G; ++u) {
             float x, y;
-            float t = spring_term(a, en[u], pp, v, q[u], x, y);
+            float t = spring_term(d, en[u], pp, v, q[u], x, y);
             t = (k + u < e) ? t : 0.f;
             v.x += x * t;
             v.y += y * t;
         }
     }
-    if (vout) vout[p] = v;
-    if (pout) pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, v, a.friction, a.S);
+    if (vout) {
+        vout[p] = v;
+        return;
+    }
+    float2* __restrict__ pout = bsel ? pos0 : pos1;
+    pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, v, d.friction, d.S);
 }
 
 constexpr int MAX_SWEEPS = 8;
@@ -208,41 +247,33 @@ constexpr int PREFETCH = 16;  // entries in flight per lane in the chunk loop
 // entry k of a point with d entries run by B lanes (chunk c = ceil(d/B)) sits at
 // poff[slot] + (k % c) * B + k / c, so lane t reads its step-s term at s*B + t
 // and a wave's loads are one contiguous segment.
-__global__ void __launch_bounds__(256) k_link_pre(LkArgs a, const uint4* __restrict__ ent,
+__global__ void __launch_bounds__(256) k_link_pre(int pass, const uint4* __restrict__ ent,
                                                   uint32_t e0, uint32_t e1,
                                                   const uint32_t* __restrict__ eslot,
                                                   const uint32_t* __restrict__ soff,
                                                   const uint32_t* __restrict__ poff,
+                                                  const uint32_t* __restrict__ sched,
                                                   uint32_t first_slot, uint32_t block_slot,
-                                                  const float2* __restrict__ pin,
+                                                  const uint32_t* __restrict__ gid,
+                                                  const GDesc* __restrict__ desc,
+                                                  const float2* __restrict__ pos0,
+                                                  const float2* __restrict__ pos1,
                                                   float4* __restrict__ pre) {
     uint32_t e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= e1) return;
     const uint32_t j = eslot[e - e0];
-    const uint32_t local = e - soff[j], d = soff[j + 1] - soff[j];
+    const GDesc& d = desc[gid[sched[j]]];
+    if (!(d.passes & (1u << pass))) return;
+    const float2* __restrict__ pin = ((d.inbuf >> pass) & 1u) ? pos1 : pos0;
+    const uint32_t local = e - soff[j], deg = soff[j + 1] - soff[j];
     const uint32_t B = j < block_slot ? 64u : 1024u;
-    const uint32_t c = (d + B - 1) / B;
+    const uint32_t c = (deg + B - 1) / B;
     const uint32_t dst = poff[j - first_slot] + (local % c) * B + local / c;
     uint4 en = ent[e];
     float2 q = pin[en.x];
     float r = __uint_as_float(en.y), s = __uint_as_float(en.z), b = __uint_as_float(en.w);
-    float D = (r * a.span + a.lo) * a.dist;
-    pre[dst] = make_float4(q.x, q.y, D, a.spring_alpha * s * b);
-}
-
-// Schedule slot of every chunked-tier entry (built once per link set): the
-// last slot j in [first, last) with soff[j] <= e, by binary search.
-__global__ void k_eslot(const uint32_t* __restrict__ soff, uint32_t first, uint32_t last,
-                        uint32_t e0, uint32_t e1, uint32_t* __restrict__ eslot) {
-    uint32_t e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= e1) return;
-    uint32_t lo = first, hi = last;  // invariant: soff[lo] <= e < soff[hi]
-    while (hi - lo > 1) {
-        uint32_t mid = lo + (hi - lo) / 2;
-        if (soff[mid] <= e) lo = mid;
-        else hi = mid;
-    }
-    eslot[e - e0] = lo;
+    float D = (r * d.span + d.lo) * d.dist;
+    pre[dst] = make_float4(q.x, q.y, D, d.spring_alpha * s * b);
 }
 
 // One serial spring step from a pre-gathered entry (no change when !live),
@@ -269,30 +300,37 @@ __device__ inline void spring_step(float4 e, float2 pp, float2& v, float (&J)[4]
 }
 
 template <int B>
-__global__ void __launch_bounds__(B) k_link_chunk(LkArgs a, const uint32_t* __restrict__ sched,
+__global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint32_t* __restrict__ sched,
                                                   const uint32_t* __restrict__ soff,
                                                   const uint32_t* __restrict__ poff,
+                                                  const uint32_t* __restrict__ woff,
                                                   const float4* __restrict__ pre,
                                                   uint32_t first_slot, uint32_t first,
-                                                  const float2* __restrict__ pin,
+                                                  const uint32_t* __restrict__ gid,
+                                                  const GDesc* __restrict__ desc,
+                                                  float2* __restrict__ pos0,
+                                                  float2* __restrict__ pos1,
                                                   const uint8_t* __restrict__ flags,
                                                   float2* __restrict__ warm,
                                                   uint32_t* __restrict__ stats,
-                                                  float2* __restrict__ pout,
                                                   float2* __restrict__ vout) {
     __shared__ float sJ[4][B], sX[B], sY[B];
     __shared__ float s_delta[B / 64];
     __shared__ float2 s_final;
     const int t = threadIdx.x;
     const uint32_t j = first + blockIdx.x;
-    // chunk starts converged on the previous call: the first guesses
-    float2* __restrict__ wslot = warm + (size_t)blockIdx.x * B + t;
     const uint32_t p = sched[j];
-    const uint32_t d = soff[j + 1] - soff[j];
-    const uint32_t c = (d + B - 1) / B;
-    const uint32_t lo = min(d, (uint32_t)t * c), hi = min(d, lo + c);
+    const GDesc& d = desc[gid[p]];
+    if (!(d.passes & (1u << pass))) return;  // uniform over the block
+    const bool bsel = (d.inbuf >> pass) & 1u;
+    const float2* __restrict__ pin = bsel ? pos1 : pos0;
+    // chunk starts converged on the previous call: the first guesses
+    float2* __restrict__ wslot = warm + woff[j - first_slot] + t;
+    const uint32_t deg = soff[j + 1] - soff[j];
+    const uint32_t c = (deg + B - 1) / B;
+    const uint32_t lo = min(deg, (uint32_t)t * c), hi = min(deg, lo + c);
     const uint32_t steps = hi - lo;
-    const int last = (int)((d - 1) / c);
+    const int last = (int)((deg - 1) / c);
     const float2 pp = pin[p];
     const float4* __restrict__ mine = pre + poff[j - first_slot] + t;
     float2 g = *wslot;
@@ -352,7 +390,7 @@ __global__ void __launch_bounds__(B) k_link_chunk(LkArgs a, const uint32_t* __re
         float mx = s_delta[0];
         for (int w = 1; w < B / 64; ++w) mx = fmaxf(mx, s_delta[w]);
         // converged, at the f32 noise floor (no longer halving), or at the cap
-        if (mx <= a.tol || (sweep >= 2 && mx > 0.5f * prev) || sweep == MAX_SWEEPS) {
+        if (mx <= d.tol || (sweep >= 2 && mx > 0.5f * prev) || sweep == MAX_SWEEPS) {
             // the last chunk's end, moved to its corrected start
             if (t == last) {
                 float ex = start.x - g.x, ey = start.y - g.y;
@@ -365,7 +403,7 @@ __global__ void __launch_bounds__(B) k_link_chunk(LkArgs a, const uint32_t* __re
                 atomicAdd(&stats[o + 0], (uint32_t)(sweep + 1));
                 atomicAdd(&stats[o + 1], 1u);
                 atomicMax(&stats[o + 2], (uint32_t)(sweep + 1));
-                if (mx > a.tol) atomicAdd(&stats[o + 3], 1u);
+                if (mx > d.tol) atomicAdd(&stats[o + 3], 1u);
             }
             break;
         }
@@ -376,30 +414,44 @@ __global__ void __launch_bounds__(B) k_link_chunk(LkArgs a, const uint32_t* __re
     __syncthreads();
     if (t == 0) {
         float2 v = s_final;
-        if (vout) vout[p] = v;
-        if (pout) pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, v, a.friction, a.S);
+        if (vout) {
+            vout[p] = v;
+        } else {
+            float2* __restrict__ pout = bsel ? pos0 : pos1;
+            pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, v, d.friction, d.S);
+        }
     }
+}
+
+__global__ void k_copy_f2(const float2* __restrict__ src, float2* __restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 }  // namespace
 
-// Builds both directions' CSR, degrees and schedules from the host link copy.
-int link_build(Ctx* c) {
-    c->links_ready = false;
+void link_free_graph(Ctx* c, Graph& g) {
     for (int d = 0; d < 2; ++d) {
-        LinkDir& L = c->dir[d];
-        dfree(c, L.ent_pt); dfree(c, L.off_pt); dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff);
-        dfree(c, L.poff); dfree(c, L.eslot); dfree(c, L.warm);
-        L = LinkDir();
+        dfree(c, g.lk.ent_pt[d]);
+        dfree(c, g.lk.off_pt[d]);
+        dfree(c, g.lk.live[d]);
     }
-    dfree(c, c->degree);
-    dfree(c, c->link_pre);
-    c->degree = nullptr;
-    c->link_pre = nullptr;
-    if (!c->links_set || c->n == 0) return CG_OK;
-    const uint32_t n = c->n, m = c->m;
+    dfree(c, g.lk.degree);
+    g.lk = GraphLinks();
+    g.links_ready = false;
+    g.wn[0] = g.wn[1] = 0;
+}
+
+// Builds one graph's CSR in both directions (graph-local indices) from its
+// host link copy: validation, degrees, the reference's entry order, bias,
+// strength, rest-length draws, live counts.
+int link_build_graph(Ctx* c, Graph& gr) {
+    link_free_graph(c, gr);
+    if (!gr.links_set || gr.n == 0) return CG_OK;
+    const uint32_t n = gr.n, m = gr.m;
     int rc = CG_OK;
-    c->degree = (uint32_t*)dalloc(c, (size_t)n * 4, &rc);
+    GraphLinks& lk = gr.lk;
+    lk.degree = (uint32_t*)dalloc(c, (size_t)n * 4, &rc);
     if (rc) return rc;
     std::vector<void*> tmp;
     auto talloc = [&](size_t bytes) -> void* {
@@ -409,68 +461,63 @@ int link_build(Ctx* c) {
     };
     auto cleanup = [&]() { for (void* p : tmp) dfree(c, p); };
     const size_t mm = m ? m : 1;
-    const size_t kk = std::max<size_t>(mm, (size_t)c->np) + 1;  // CSR sort and schedule sort
     float2* d_links = (float2*)talloc(mm * 8);
-    float* d_str = c->h_strength.empty() ? nullptr : (float*)talloc(mm * 4);
+    float* d_str = gr.strength.empty() ? nullptr : (float*)talloc(mm * 4);
     float* d_rin = (float*)talloc(mm * 4);
     float* d_rout = (float*)talloc(mm * 4);
+    uint8_t* d_flags = (uint8_t*)talloc(n);
     uint32_t* src = (uint32_t*)talloc(mm * 4);
     uint32_t* tgt = (uint32_t*)talloc(mm * 4);
     uint32_t* outdeg = (uint32_t*)talloc(((size_t)n + 1) * 4);
     uint32_t* indeg = (uint32_t*)talloc(((size_t)n + 1) * 4);
-    uint32_t* k0 = (uint32_t*)talloc(kk * 4);
-    uint32_t* k1 = (uint32_t*)talloc(kk * 4);
-    uint32_t* v0 = (uint32_t*)talloc(kk * 4);
-    uint32_t* v1 = (uint32_t*)talloc(kk * 4);
-    uint32_t* eown = (uint32_t*)talloc(mm * 4);
+    uint32_t* k0 = (uint32_t*)talloc(mm * 4);
+    uint32_t* k1 = (uint32_t*)talloc(mm * 4);
+    uint32_t* v0 = (uint32_t*)talloc(mm * 4);
+    uint32_t* v1 = (uint32_t*)talloc(mm * 4);
     uint32_t* lflag = (uint32_t*)talloc((mm + 1) * 4);
     uint32_t* lrank = (uint32_t*)talloc((mm + 1) * 4);
-    uint32_t* slot_of = (uint32_t*)talloc(((size_t)n + 1) * 4);
-    uint32_t* counts = (uint32_t*)talloc(64);
     if (rc) { cleanup(); return rc; }
+    CG_HIP(c, hipMemcpyAsync(d_flags, gr.flags.data(), n, hipMemcpyHostToDevice, c->stream));
     CG_HIP(c, hipMemsetAsync(outdeg, 0, ((size_t)n + 1) * 4, c->stream));
     CG_HIP(c, hipMemsetAsync(indeg, 0, ((size_t)n + 1) * 4, c->stream));
-    CG_HIP(c, hipMemsetAsync(c->degree, 0, (size_t)n * 4, c->stream));
+    CG_HIP(c, hipMemsetAsync(lk.degree, 0, (size_t)n * 4, c->stream));
     if (m) {
-        CG_HIP(c, hipMemcpyAsync(d_links, c->h_links.data(), (size_t)m * 8, hipMemcpyHostToDevice, c->stream));
+        CG_HIP(c, hipMemcpyAsync(d_links, gr.links.data(), (size_t)m * 8, hipMemcpyHostToDevice, c->stream));
         if (d_str)
-            CG_HIP(c, hipMemcpyAsync(d_str, c->h_strength.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
-        CG_HIP(c, hipMemcpyAsync(d_rin, c->h_rin.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
-        CG_HIP(c, hipMemcpyAsync(d_rout, c->h_rout.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
+            CG_HIP(c, hipMemcpyAsync(d_str, gr.strength.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
+        CG_HIP(c, hipMemcpyAsync(d_rin, gr.rin.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
+        CG_HIP(c, hipMemcpyAsync(d_rout, gr.rout.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_validate, dim3(blocks_for(m, 256)), dim3(256), 0, c->stream, d_links, m,
                            n, src, tgt, outdeg, indeg);
         hipLaunchKernelGGL(k_degree, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, outdeg, indeg,
-                           n, c->degree);
+                           n, lk.degree);
     }
     int nbits = 1;
     while (nbits < 32 && (1ull << nbits) <= (unsigned long long)n) ++nbits;  // n itself must fit
-    size_t sort_bytes = 0, scan_bytes = 0, scan2_bytes = 0, sched_bytes = 0;
+    size_t sort_bytes = 0, scan_bytes = 0, scan2_bytes = 0;
     CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, v0, v1, (int)mm, 0,
                                                  nbits, c->stream));
     CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, outdeg, outdeg, (int)n + 1,
                                                c->stream));
     CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan2_bytes, lflag, lrank, (int)mm + 1,
                                                c->stream));
-    CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sched_bytes, k0, k1, v0, v1,
-                                                 (int)(c->np ? c->np : 1), 0, 32, c->stream));
-    size_t tbytes = std::max(std::max(sort_bytes, scan_bytes), std::max(scan2_bytes, sched_bytes));
+    size_t tbytes = std::max(sort_bytes, std::max(scan_bytes, scan2_bytes));
     void* d_tmp = talloc(tbytes);
     if (rc) { cleanup(); return rc; }
-    size_t pre_need = 0;
     for (int d = 0; d < 2; ++d) {
-        LinkDir& L = c->dir[d];
         uint32_t* owner = d == 0 ? src : tgt;
         uint32_t* other = d == 0 ? tgt : src;
         uint32_t* cnt = d == 0 ? outdeg : indeg;  // entries owned per point
-        L.off_pt = (uint32_t*)dalloc(c, ((size_t)n + 1) * 4, &rc);
+        lk.off_pt[d] = (uint32_t*)dalloc(c, ((size_t)n + 1) * 4, &rc);
+        lk.live[d] = (uint32_t*)dalloc(c, (size_t)n * 4, &rc);
         if (rc) { cleanup(); return rc; }
         size_t sb = tbytes;
-        CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, cnt, L.off_pt, (int)n + 1, c->stream));
+        CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, cnt, lk.off_pt[d], (int)n + 1, c->stream));
         uint32_t E = 0;
-        CG_HIP(c, hipMemcpyAsync(&E, L.off_pt + n, 4, hipMemcpyDeviceToHost, c->stream));
+        CG_HIP(c, hipMemcpyAsync(&E, lk.off_pt[d] + n, 4, hipMemcpyDeviceToHost, c->stream));
         CG_HIP(c, hipStreamSynchronize(c->stream));
-        L.E = E;
-        L.ent_pt = (uint4*)dalloc(c, (size_t)(E ? E : 1) * 16, &rc);
+        lk.E[d] = E;
+        lk.ent_pt[d] = (uint4*)dalloc(c, (size_t)(E ? E : 1) * 16, &rc);
         if (rc) { cleanup(); return rc; }
         CG_HIP(c, hipMemsetAsync(lflag, 0, (mm + 1) * 4, c->stream));
         if (m) {
@@ -481,92 +528,178 @@ int link_build(Ctx* c) {
                                                          c->stream));
             if (E)
                 hipLaunchKernelGGL(k_entries, dim3(blocks_for(E, 256)), dim3(256), 0, c->stream, k1,
-                                   v1, E, other, c->degree, d_str, d == 0 ? d_rin : d_rout,
-                                   c->flags, L.ent_pt, eown, lflag);
+                                   v1, E, other, lk.degree, d_str, d == 0 ? d_rin : d_rout, d_flags,
+                                   lk.ent_pt[d], lflag);
         }
         sb = tbytes;
         CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, lflag, lrank, (int)E + 1, c->stream));
-        // schedule: present points by ascending live degree
-        const uint32_t ns = c->np;
-        L.n_sched = ns;
-        L.sched = (uint32_t*)dalloc(c, (size_t)(ns ? ns : 1) * 4, &rc);
-        L.soff = (uint32_t*)dalloc(c, ((size_t)ns + 1) * 4, &rc);
-        if (rc) { cleanup(); return rc; }
-        if (ns) {
-            hipLaunchKernelGGL(k_sched_keys, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream,
-                               c->present, ns, L.off_pt, lrank, k0, v0);
-            sb = tbytes;
-            CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(d_tmp, sb, k0, k1, v0, L.sched, (int)ns, 0,
-                                                         32, c->stream));
-            CG_HIP(c, hipMemsetAsync(k1 + ns, 0, 4, c->stream));
-            sb = tbytes;
-            CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, k1, L.soff, (int)ns + 1, c->stream));
-            hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, c->stream, counts, 2u, ns);
-            hipLaunchKernelGGL(k_tier_bounds, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, k1,
-                               ns, counts);
-            uint32_t h[2] = {0, 0};
-            CG_HIP(c, hipMemcpyAsync(h, counts, 8, hipMemcpyDeviceToHost, c->stream));
-            CG_HIP(c, hipMemcpyAsync(&L.E_live, L.soff + ns, 4, hipMemcpyDeviceToHost, c->stream));
-            CG_HIP(c, hipStreamSynchronize(c->stream));
-            L.n_lane = h[0];
-            L.n_wave = h[1] - h[0];
-            L.n_block = ns - h[1];
-            // chunked tiers: padded step-major offsets (host) and entry -> slot map
-            const uint32_t nch = ns - L.n_lane;
-            std::vector<uint32_t> hs(nch + 1), hp(nch + 1, 0);
-            CG_HIP(c, hipMemcpy(hs.data(), L.soff + L.n_lane, (size_t)(nch + 1) * 4, hipMemcpyDeviceToHost));
-            L.e_chunk0 = hs[0];
-            for (uint32_t k = 0; k < nch; ++k) {
-                uint32_t dd = hs[k + 1] - hs[k];
-                uint32_t B = (k < L.n_wave) ? 64u : 1024u;
-                hp[k + 1] = hp[k] + ((dd + B - 1) / B) * B;
-            }
-            pre_need = std::max<size_t>(pre_need, (size_t)hp[nch]);
-            const size_t nwarm = (size_t)L.n_wave * 64 + (size_t)L.n_block * 1024;
-            L.warm = (float2*)dalloc(c, (nwarm ? nwarm : 1) * 8, &rc);
-            if (rc) { cleanup(); return rc; }
-            CG_HIP(c, hipMemsetAsync(L.warm, 0, (nwarm ? nwarm : 1) * 8, c->stream));
-            L.poff = (uint32_t*)dalloc(c, (size_t)(nch + 1) * 4, &rc);
-            L.eslot = (uint32_t*)dalloc(c, (size_t)(L.E_live - L.e_chunk0 + 1) * 4, &rc);
-            if (rc) { cleanup(); return rc; }
-            CG_HIP(c, hipMemcpy(L.poff, hp.data(), (size_t)(nch + 1) * 4, hipMemcpyHostToDevice));
-            if (L.E_live > L.e_chunk0)
-                hipLaunchKernelGGL(k_eslot, dim3(blocks_for(L.E_live - L.e_chunk0, 256)), dim3(256), 0,
-                                   c->stream, L.soff, L.n_lane, ns, L.e_chunk0, L.E_live, L.eslot);
-            L.ent = (uint4*)dalloc(c, (size_t)(L.E_live ? L.E_live : 1) * 16, &rc);
-            if (rc) { cleanup(); return rc; }
-            hipLaunchKernelGGL(k_slot_of, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.sched,
-                               ns, slot_of);
-            if (E)
-                hipLaunchKernelGGL(k_permute, dim3(blocks_for(E, 256)), dim3(256), 0, c->stream,
-                                   L.ent_pt, E, eown, lflag, lrank, L.off_pt, slot_of, L.soff, L.ent);
-        }
+        hipLaunchKernelGGL(k_live_count, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream,
+                           lk.off_pt[d], lrank, n, lk.live[d]);
         CG_HIP(c, hipGetLastError());
-    }
-    c->link_pre = (float4*)dalloc(c, (pre_need ? pre_need : 1) * 16, &rc);
-    if (!c->link_stats) {
-        c->link_stats = (uint32_t*)dalloc(c, 64, &rc);
-        if (!rc) CG_HIP(c, hipMemsetAsync(c->link_stats, 0, 64, c->stream));
     }
     CG_HIP(c, hipStreamSynchronize(c->stream));
     cleanup();
     if (rc) return rc;
-    c->links_ready = true;
+    gr.links_ready = true;
     return CG_OK;
 }
 
-int link_run(Ctx* c, const cg_sim_params* p, int dir, float alpha, const float2* pin,
-             float2* pout, float2* vout) {
-    if (!c->links_ready) return CG_OK;
+// Packed schedules over all live graphs, from their per-graph CSRs.  Warm-start
+// regions are laid out graph by graph, each graph's chunked points in index
+// order, so their layout depends only on the graph; a graph whose links did
+// not change carries its region over from the previous packing (old).
+int link_pack(Ctx* c, LinkDir (&old)[2]) {
+    auto free_dir = [&](LinkDir& L) {
+        dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff); dfree(c, L.poff);
+        dfree(c, L.eslot); dfree(c, L.woff); dfree(c, L.warm);
+        L = LinkDir();
+    };
+    for (int d = 0; d < 2; ++d) c->dir[d] = LinkDir();
+    dfree(c, c->link_pre);
+    c->link_pre = nullptr;
+    c->any_links = false;
+    const uint32_t G = c->G, NP = c->NP;
+    for (uint32_t s = 0; s < G; ++s) {
+        const Graph& g = c->graphs[c->order[s]];
+        if (g.links_ready && g.m > 0) c->any_links = true;
+    }
+    if (!c->any_links || NP == 0) {
+        for (int d = 0; d < 2; ++d) free_dir(old[d]);
+        return CG_OK;
+    }
+    int rc = CG_OK;
+    std::vector<void*> tmp;
+    auto talloc = [&](size_t bytes) -> void* {
+        void* p = dalloc(c, bytes ? bytes : 16, &rc);
+        if (p) tmp.push_back(p);
+        return p;
+    };
+    auto cleanup = [&]() { for (void* p : tmp) dfree(c, p); };
+    uint32_t* k0 = (uint32_t*)talloc(((size_t)NP + 1) * 4);
+    uint32_t* k1 = (uint32_t*)talloc(((size_t)NP + 1) * 4);
+    uint32_t* v0 = (uint32_t*)talloc(((size_t)NP + 1) * 4);
+    uint32_t* counts = (uint32_t*)talloc(64);
+    GLinkPtr* tab = (GLinkPtr*)talloc((size_t)G * sizeof(GLinkPtr));
+    if (rc) { cleanup(); return rc; }
+    size_t sort_bytes = 0, scan_bytes = 0;
+    CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, v0, v0, (int)NP, 0, 32,
+                                                 c->stream));
+    CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, k1, k0, (int)NP + 1, c->stream));
+    const size_t tbytes = std::max(sort_bytes, scan_bytes);
+    void* d_tmp = talloc(tbytes);
+    if (rc) { cleanup(); return rc; }
+    std::vector<uint32_t> h_p0(G);
+    for (uint32_t s = 0; s < G; ++s) h_p0[s] = c->graphs[c->order[s]].p0;
+    size_t pre_need = 0;
+    for (int d = 0; d < 2; ++d) {
+        LinkDir& L = c->dir[d];
+        std::vector<GLinkPtr> h_tab(G);
+        for (uint32_t s = 0; s < G; ++s) {
+            Graph& g = c->graphs[c->order[s]];
+            if (g.links_ready && g.m > 0) h_tab[s] = {g.lk.ent_pt[d], g.lk.off_pt[d], g.lk.live[d]};
+            else h_tab[s] = {nullptr, nullptr, nullptr};
+        }
+        CG_HIP(c, hipMemcpyAsync(tab, h_tab.data(), (size_t)G * sizeof(GLinkPtr), hipMemcpyHostToDevice,
+                                 c->stream));
+        const uint32_t ns = NP;
+        L.n_sched = ns;
+        L.sched = (uint32_t*)dalloc(c, (size_t)ns * 4, &rc);
+        L.soff = (uint32_t*)dalloc(c, ((size_t)ns + 1) * 4, &rc);
+        if (rc) { cleanup(); return rc; }
+        hipLaunchKernelGGL(k_sched_keys, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, c->present,
+                           ns, c->gid, c->desc, tab, k0, v0);
+        size_t sb = tbytes;
+        CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(d_tmp, sb, k0, k1, v0, L.sched, (int)ns, 0, 32,
+                                                     c->stream));
+        CG_HIP(c, hipMemsetAsync(k1 + ns, 0, 4, c->stream));
+        sb = tbytes;
+        CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, k1, L.soff, (int)ns + 1, c->stream));
+        hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, c->stream, counts, 2u, ns);
+        hipLaunchKernelGGL(k_tier_bounds, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, k1, ns,
+                           counts);
+        uint32_t h[2] = {0, 0};
+        CG_HIP(c, hipMemcpyAsync(h, counts, 8, hipMemcpyDeviceToHost, c->stream));
+        CG_HIP(c, hipMemcpyAsync(&L.E_live, L.soff + ns, 4, hipMemcpyDeviceToHost, c->stream));
+        CG_HIP(c, hipStreamSynchronize(c->stream));
+        L.n_lane = h[0];
+        L.n_wave = h[1] - h[0];
+        L.n_block = ns - h[1];
+        L.ent = (uint4*)dalloc(c, (size_t)(L.E_live ? L.E_live : 1) * 16, &rc);
+        if (rc) { cleanup(); return rc; }
+        hipLaunchKernelGGL(k_assemble, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.sched, ns,
+                           L.soff, c->gid, c->desc, tab, c->flags, L.ent);
+        // chunked tiers: step-major offsets, entry -> slot map, warm-start offsets
+        const uint32_t nch = ns - L.n_lane;
+        std::vector<uint32_t> hs(nch + 1), hp(nch + 1, 0), hsched(nch ? nch : 1), hw(nch ? nch : 1, 0);
+        CG_HIP(c, hipMemcpyAsync(hs.data(), L.soff + L.n_lane, (size_t)(nch + 1) * 4,
+                                 hipMemcpyDeviceToHost, c->stream));
+        if (nch)
+            CG_HIP(c, hipMemcpyAsync(hsched.data(), L.sched + L.n_lane, (size_t)nch * 4,
+                                     hipMemcpyDeviceToHost, c->stream));
+        CG_HIP(c, hipStreamSynchronize(c->stream));
+        L.e_chunk0 = hs[0];
+        for (uint32_t k = 0; k < nch; ++k) {
+            uint32_t dd = hs[k + 1] - hs[k];
+            uint32_t B = (k < L.n_wave) ? 64u : 1024u;
+            hp[k + 1] = hp[k] + ((dd + B - 1) / B) * B;
+        }
+        pre_need = std::max<size_t>(pre_need, (size_t)hp[nch]);
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> per(G);  // (local point, chunk slot)
+        for (uint32_t k = 0; k < nch; ++k) {
+            uint32_t p = hsched[k];
+            uint32_t s = (uint32_t)(std::upper_bound(h_p0.begin(), h_p0.end(), p) - h_p0.begin()) - 1;
+            per[s].push_back({p - h_p0[s], k});
+        }
+        size_t wpos = 0;
+        std::vector<std::pair<size_t, size_t>> region(G);  // (offset, length) per graph slot
+        for (uint32_t s = 0; s < G; ++s) {
+            std::sort(per[s].begin(), per[s].end());
+            size_t start = wpos;
+            for (auto& pk : per[s]) {
+                hw[pk.second] = (uint32_t)wpos;
+                wpos += (pk.second < L.n_wave) ? 64 : 1024;
+            }
+            region[s] = {start, wpos - start};
+        }
+        L.warm_len = wpos;
+        L.poff = (uint32_t*)dalloc(c, (size_t)(nch + 1) * 4, &rc);
+        L.woff = (uint32_t*)dalloc(c, (size_t)(nch ? nch : 1) * 4, &rc);
+        L.eslot = (uint32_t*)dalloc(c, (size_t)(L.E_live - L.e_chunk0 + 1) * 4, &rc);
+        L.warm = (float2*)dalloc(c, (wpos ? wpos : 1) * 8, &rc);
+        if (rc) { cleanup(); return rc; }
+        CG_HIP(c, hipMemcpyAsync(L.poff, hp.data(), (size_t)(nch + 1) * 4, hipMemcpyHostToDevice, c->stream));
+        CG_HIP(c, hipMemcpyAsync(L.woff, hw.data(), (size_t)(nch ? nch : 1) * 4, hipMemcpyHostToDevice,
+                                 c->stream));
+        CG_HIP(c, hipMemsetAsync(L.warm, 0, (wpos ? wpos : 1) * 8, c->stream));
+        for (uint32_t s = 0; s < G; ++s) {
+            Graph& g = c->graphs[c->order[s]];
+            if (old[d].warm && g.wn[d] && g.wn[d] == region[s].second)
+                hipLaunchKernelGGL(k_copy_f2, dim3(blocks_for(g.wn[d], 256)), dim3(256), 0, c->stream,
+                                   old[d].warm + g.w0[d], L.warm + region[s].first, (size_t)g.wn[d]);
+            g.w0[d] = (uint32_t)region[s].first;
+            g.wn[d] = (uint32_t)region[s].second;
+        }
+        if (L.E_live > L.e_chunk0)
+            hipLaunchKernelGGL(k_eslot, dim3(blocks_for(L.E_live - L.e_chunk0, 256)), dim3(256), 0,
+                               c->stream, L.soff, L.n_lane, ns, L.e_chunk0, L.E_live, L.eslot);
+        CG_HIP(c, hipGetLastError());
+        // keep the staging vectors alive until the copies above have run
+        CG_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    for (int d = 0; d < 2; ++d) free_dir(old[d]);
+    c->link_pre = (float4*)dalloc(c, (pre_need ? pre_need : 1) * 16, &rc);
+    if (!c->link_stats && !rc) {
+        c->link_stats = (uint32_t*)dalloc(c, 64, &rc);
+        if (!rc) CG_HIP(c, hipMemsetAsync(c->link_stats, 0, 64, c->stream));
+    }
+    cleanup();
+    return rc;
+}
+
+int link_run(Ctx* c, int dir, float2* vout) {
+    if (!c->any_links) return CG_OK;
     const LinkDir& L = c->dir[dir];
-    LkArgs a;
-    a.spring_alpha = p->link_spring * alpha;
-    a.dist = p->link_distance;
-    a.lo = p->link_dist_random_variation_range[0];
-    a.span = p->link_dist_random_variation_range[1] - p->link_dist_random_variation_range[0];
-    a.friction = p->friction;
-    a.S = p->space_size;
-    a.tol = 2.5e-6f * p->space_size;  // a quarter of the 1e-5 * spaceSize force tolerance
+    const int pass = P_LINK_IN + dir;
     // The hub tier (one 1024-lane block per hub) runs on the side stream while the
     // lane and 64-lane tiers fill the other CUs on the main stream.
     const uint32_t e0 = L.e_chunk0, e1 = L.E_live;
@@ -574,8 +707,9 @@ int link_run(Ctx* c, const cg_sim_params* p, int dir, float alpha, const float2*
     // sweep counters only while requested (same-address atomics per block)
     uint32_t* stats = (c->prof_mask & CG_PROFILE_LINK_STATS) ? c->link_stats : nullptr;
     if (e1 > e0)
-        hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(e1 - e0, 256)), dim3(256), 0, c->stream, a, L.ent,
-                           e0, e1, L.eslot, L.soff, L.poff, first, block_slot, pin, c->link_pre);
+        hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(e1 - e0, 256)), dim3(256), 0, c->stream, pass,
+                           L.ent, e0, e1, L.eslot, L.soff, L.poff, L.sched, first, block_slot, c->gid,
+                           c->desc, c->pos[0], c->pos[1], c->link_pre);
     const bool fork = L.n_block && c->side;
     if (L.n_block) {
         hipStream_t s = c->stream;
@@ -584,17 +718,19 @@ int link_run(Ctx* c, const cg_sim_params* p, int dir, float alpha, const float2*
             CG_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
             s = c->side;
         }
-        hipLaunchKernelGGL(k_link_chunk<1024>, dim3(L.n_block), dim3(1024), 0, s, a, L.sched, L.soff,
-                           L.poff, c->link_pre, first, block_slot, pin, c->flags,
-                           L.warm + (size_t)L.n_wave * 64, stats, pout, vout);
+        hipLaunchKernelGGL(k_link_chunk<1024>, dim3(L.n_block), dim3(1024), 0, s, pass, L.sched, L.soff,
+                           L.poff, L.woff, c->link_pre, first, block_slot, c->gid, c->desc, c->pos[0],
+                           c->pos[1], c->flags, L.warm, stats, vout);
         if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
     if (L.n_wave)
-        hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_wave), dim3(64), 0, c->stream, a, L.sched, L.soff,
-                           L.poff, c->link_pre, first, first, pin, c->flags, L.warm, stats, pout, vout);
+        hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_wave), dim3(64), 0, c->stream, pass, L.sched, L.soff,
+                           L.poff, L.woff, c->link_pre, first, first, c->gid, c->desc, c->pos[0],
+                           c->pos[1], c->flags, L.warm, stats, vout);
     if (L.n_lane)
-        hipLaunchKernelGGL(k_link_lane, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, a,
-                           L.sched, L.soff, L.ent, L.n_lane, pin, c->flags, pout, vout);
+        hipLaunchKernelGGL(k_link_lane, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, pass,
+                           L.sched, L.soff, L.ent, L.n_lane, c->gid, c->desc, c->pos[0], c->pos[1],
+                           c->flags, vout);
     if (fork) CG_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     CG_HIP(c, hipGetLastError());
     return CG_OK;

@@ -1,7 +1,12 @@
 /*
  * cosmosgl.h — C ABI of the MI355X force-simulation tick.
  *
- * One context = one simulated graph on one GPU, bound to one HIP stream.
+ * One context = a set of simulated graphs on one GPU, bound to one HIP
+ * stream.  The single-graph entry points (cg_set_positions ... cg_get_positions)
+ * act on graph 0 of the set; the cg_graph_* entry points add further graphs,
+ * which are packed into the same device arrays so that one launch per force
+ * pass advances all of them (batched layouts, continuous batching).  A graph
+ * computes the same bits alone and packed with others.
  * A context is not re-entrant; separate contexts may be driven from separate
  * threads.  Inputs are copied on call, outputs are caller-allocated, the
  * context owns all device memory.  No function throws; each returns CG_OK or
@@ -179,6 +184,38 @@ uint64_t cg_device_bytes(const cg_ctx* ctx);
 #define CG_PROFILE_LINK_STATS (1u << 31) /* also count link sweeps (cg_debug_link_stats) */
 int cg_set_profiling(cg_ctx* ctx, uint32_t stage_mask);
 int cg_get_profile(cg_ctx* ctx, double* ms, uint32_t* launches, uint32_t nstages);
+
+/* ---------------------------------------------------------------------- */
+/* Graph set.  Each call mirrors the single-graph call of the same name on   */
+/* graph `id` — in the reference one Graph instance per layout              */
+/* (src/index.ts:422-434), each with its own config and Store.alpha; here   */
+/* many layouts share the device.  Ids are never reused; id 0 belongs to the */
+/* single-graph entry points.  Every graph has its own generator (jitter,    */
+/* rest-length draws, near-field seeds), seeded identically unless           */
+/* cg_graph_set_seed says otherwise, so a graph gives the same result alone  */
+/* and in any mix.  Adding, removing or resizing a graph repacks the set on  */
+/* the next step; positions of the other graphs stay on the device.         */
+int cg_graph_add(cg_ctx* ctx, uint32_t* id);
+int cg_graph_remove(cg_ctx* ctx, uint32_t id);
+int cg_graph_set_params(cg_ctx* ctx, uint32_t id, const cg_sim_params* params);
+int cg_graph_set_seed(cg_ctx* ctx, uint32_t id, uint64_t seed);
+int cg_graph_set_positions(cg_ctx* ctx, uint32_t id, const float* xy, uint32_t n);
+int cg_graph_set_links(cg_ctx* ctx, uint32_t id, const float* links, uint32_t m,
+                       const float* strength, const float* rand_in, const float* rand_out);
+int cg_graph_set_jitter(cg_ctx* ctx, uint32_t id, const float* xy, uint32_t n);
+int cg_graph_set_point_sizes(cg_ctx* ctx, uint32_t id, const float* sizes, uint32_t n);
+int cg_graph_set_pinned(cg_ctx* ctx, uint32_t id, const uint32_t* indices, uint32_t count);
+int cg_graph_set_alpha(cg_ctx* ctx, uint32_t id, double alpha);
+int cg_graph_get_alpha(cg_ctx* ctx, uint32_t id, double* alpha);
+int cg_graph_get_positions(cg_ctx* ctx, uint32_t id, float* xy, uint32_t n);
+
+/* Advances the listed graphs by nticks ticks each with their stored params
+ * (Graph.runSimulationStep per graph); the other graphs of the set keep
+ * their state.  Asynchronous like cg_step. */
+int cg_step_graphs(cg_ctx* ctx, const uint32_t* ids, uint32_t count, uint32_t nticks);
+
+/* Waits for the context's queued work. */
+int cg_sync(cg_ctx* ctx);
 
 #ifdef __cplusplus
 }

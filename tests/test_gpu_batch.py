@@ -1,0 +1,133 @@
+"""GPU tests of the packed graph set (batched layouts, continuous batching).
+
+A graph's result must not depend on what else shares the device: every kernel
+looks its graph up and works in graph-local order, so the check is bit-exact
+(positions compared as raw f32 bits, NaNs included), not a tolerance.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from graph_amd import _lib, datagen, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def spec(kind: str, seed: int) -> dict:
+    """Small graphs covering every pass and the link tiers."""
+    if kind == "ba":
+        n, S = 3000, 2048.0
+        links = datagen.barabasi_albert(n, 2, seed)
+        p = workloads.sim_params(S)
+        sizes = None
+    elif kind == "hubs":  # chung-lu: lane, 64-lane and 1024-lane link tiers
+        n, S = 20000, 4096.0
+        links = datagen.chung_lu(n, 2.1, 1.0, 5 * n, seed)
+        p = workloads.sim_params(S, center=0.3)
+        sizes = None
+    elif kind == "collide":
+        n, S = 2500, 1024.0
+        links = datagen.barabasi_albert(n, 1, seed)
+        p = workloads.sim_params(S, collision=1.0)
+        sizes = (2.0 * datagen.lognormal(n, np.log(2.0), 0.5, seed)).astype(np.float32)
+    elif kind == "nolinks":
+        n, S = 1500, 1024.0
+        links = None
+        p = workloads.sim_params(S, gravity=0.0)
+        sizes = None
+    else:
+        raise ValueError(kind)
+    xy = datagen.uniform_square(n, S, seed)
+    if kind == "ba":
+        xy = xy.reshape(-1, 2).copy()
+        xy[::97] = np.nan  # absent points
+        xy[5::97, 1] = np.nan  # half-NaN: absent too (Points/position-utils.ts:17-24)
+    r_in, r_out = (None, None) if links is None else workloads._draws(links.size // 2, seed)
+    return dict(kind=kind, n=n, xy=xy, links=links, r_in=r_in, r_out=r_out, params=p, sizes=sizes,
+                pinned=np.arange(0, n, 211, dtype=np.uint32))
+
+
+def load(ctx: _lib.Context, s: dict) -> int:
+    gid = ctx.graph_add()
+    ctx.graph_set_params(gid, s["params"])
+    ctx.graph_set_positions(gid, s["xy"])
+    if s["links"] is not None:
+        ctx.graph_set_links(gid, s["links"], None, s["r_in"], s["r_out"])
+    if s["sizes"] is not None:
+        ctx.graph_set_point_sizes(gid, s["sizes"])
+    ctx.graph_set_pinned(gid, s["pinned"])
+    return gid
+
+
+def alone(s: dict, ticks: int) -> np.ndarray:
+    with _lib.Context() as ctx:
+        gid = load(ctx, s)
+        ctx.step_graphs([gid], ticks)
+        return ctx.graph_get_positions(gid)
+
+
+def bits(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def test_alone_equals_continuous_mix():
+    kinds = ["ba", "hubs", "collide", "nolinks", "ba"]
+    specs = [spec(k, 100 + i) for i, k in enumerate(kinds)]
+    # schedule: (tick block, graphs admitted before it, graphs removed after it)
+    with _lib.Context() as ctx:
+        ids, done, ticks_run = {}, {}, {}
+        say("admit 0,1")
+        for i in (0, 1):
+            ids[i] = load(ctx, specs[i])
+            ticks_run[i] = 0
+        for block, admit, leave in ((20, (2,), ()), (15, (3,), (0,)), (25, (4,), ()), (10, (), (1, 2, 3, 4))):
+            live = [i for i in ids if i not in done]
+            ctx.step_graphs([ids[i] for i in live], block)
+            for i in live:
+                ticks_run[i] += block
+            for i in leave:
+                done[i] = ctx.graph_get_positions(ids[i])
+                ctx.graph_remove(ids[i])
+            for i in admit:
+                say(f"admit {i} after a block of {block}")
+                ids[i] = load(ctx, specs[i])
+                ticks_run[i] = 0
+    for i, s in enumerate(specs):
+        ref = alone(s, ticks_run[i])
+        same = np.array_equal(bits(ref), bits(done[i]))
+        say(f"graph {i} ({s['kind']}, n={s['n']}, {ticks_run[i]} ticks): bit-identical={same}")
+        assert same, f"graph {i}: mixed result differs from the graph run alone"
+        present = ~np.isnan(s["xy"].reshape(-1, 2)).any(axis=1)
+        assert np.isfinite(ref.reshape(-1, 2)[present]).all()
+        assert np.isnan(ref.reshape(-1, 2)[~present]).all()
+
+
+def test_unstepped_graphs_keep_state_and_alpha():
+    a, b = spec("ba", 7), spec("nolinks", 8)
+    with _lib.Context() as ctx:
+        ga, gb = load(ctx, a), load(ctx, b)
+        ctx.step_graphs([ga, gb], 5)
+        xb = ctx.graph_get_positions(gb)
+        alpha_b = ctx.graph_get_alpha(gb)
+        ctx.step_graphs([ga], 12)
+        assert np.array_equal(bits(xb), bits(ctx.graph_get_positions(gb)))
+        assert ctx.graph_get_alpha(gb) == alpha_b
+        assert ctx.graph_get_alpha(ga) < alpha_b
+
+
+def test_default_graph_api_matches_graph_set_api():
+    s = spec("hubs", 3)
+    with _lib.Context() as ctx:
+        ctx.set_positions(s["xy"])
+        ctx.set_links(s["links"], None, s["r_in"], s["r_out"])
+        ctx.set_pinned(s["pinned"])
+        ctx.step(s["params"], 30)
+        x0 = ctx.get_positions()
+    x1 = alone(s, 30)
+    assert np.array_equal(bits(x0), bits(x1))
