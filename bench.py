@@ -71,6 +71,100 @@ def cpu_baseline(wl: dict, seconds: float) -> dict:
                       f"oracle (oracle/cosmos_oracle.c, 1 thread) in {el:.1f} s"}
 
 
+def bench_batched(count: int, ticks: int, sample: int, log) -> dict:
+    """Batched layouts: `count` independent BA graphs packed into one graph set,
+    `ticks` ticks each, against the plain one-graph-at-a-time loop (the
+    single-graph entry points, one context per graph) timed on an evenly
+    strided sample of the same graphs and scaled to the whole batch."""
+    from graph_amd import _lib, workloads
+
+    ns, ms = workloads.cfg3_sizes(count)
+    graphs = [workloads.cfg3_graph(i, int(ns[i]), int(ms[i])) for i in range(count)]
+    ctx = _lib.Context(device=0)
+    ids = []
+    for g in graphs:
+        gid = ctx.graph_add()
+        ids.append(gid)
+        ctx.graph_set_params(gid, g["params"])
+        ctx.graph_set_positions(gid, g["xy"])
+        ctx.graph_set_jitter(gid, g["jitter"])
+        ctx.graph_set_links(gid, g["links"], None, g["r_in"], g["r_out"])
+    t0 = time.perf_counter()
+    ctx.step_graphs(ids, 0)  # packs the set (no tick)
+    ctx.sync()
+    pack_s = time.perf_counter() - t0
+    dev_gib = ctx.device_bytes() / 2**30
+    log(f"batched: {count} graphs, {int(ns.sum())} points packed in {pack_s:.2f}s ({dev_gib:.1f} GiB)")
+    t0 = time.perf_counter()
+    ctx.step_graphs(ids, ticks)
+    ctx.sync()
+    batch_s = time.perf_counter() - t0
+    picks = workloads.cfg3_plain_sample(count, sample)
+    xb = {i: ctx.graph_get_positions(ids[i]) for i in picks}
+    ctx.close()
+    log(f"batched: {ticks} ticks in {batch_s:.2f}s")
+    plain_s, same = [], True
+    for i in picks:
+        g = graphs[i]
+        with _lib.Context(device=0) as c:
+            c.set_positions(g["xy"])
+            c.set_jitter(g["jitter"])
+            c.set_links(g["links"], None, g["r_in"], g["r_out"])
+            c.step(g["params"], 0)
+            c.get_positions()
+            t0 = time.perf_counter()
+            c.step(g["params"], ticks)
+            x = c.get_positions()
+            plain_s.append(time.perf_counter() - t0)
+        same &= bool(np.array_equal(x.view(np.uint32), xb[i].view(np.uint32)))
+    plain_total = float(np.mean(plain_s)) * count
+    log(f"batched: plain loop {np.mean(plain_s) * 1e3:.1f} ms per graph on {len(picks)} graphs")
+    return {
+        "workload": f"{count} Barabasi-Albert graphs, n log-uniform in [1000, 200000] (seed 23), "
+                    f"m uniform in 1..4, {ticks} ticks each",
+        "graphs": count, "points": int(ns.sum()), "ticks": ticks,
+        "layouts_per_s": count / batch_s, "ms_per_tick": batch_s / ticks * 1e3,
+        "point_ticks_per_s": float(ns.sum()) * ticks / batch_s,
+        "p50_latency_s": batch_s, "p99_latency_s": batch_s,
+        "pack_s": pack_s, "device_gib": dev_gib,
+        "plain_layouts_per_s": count / plain_total,
+        "plain_sample": f"{len(picks)} evenly strided graphs of the batch run one at a time through "
+                        f"cg_step, mean {np.mean(plain_s) * 1e3:.1f} ms per {ticks}-tick layout, "
+                        f"scaled to {count} graphs",
+        "speedup_vs_plain": plain_total / batch_s,
+        "plain_equals_batched": same,
+    }
+
+
+def bench_serving(window_s: float, budget_gib: float, segment: int, log) -> dict:
+    """Continuous batching: Poisson arrivals at 40/s over `window_s` seconds,
+    each job laid out until alpha < 0.001, against the plain first-come
+    first-served path over the same jobs."""
+    from graph_amd import serving, workloads
+
+    jobs = workloads.cfg4_jobs(duration=window_s)
+    log(f"serving: {len(jobs)} jobs, {sum(j.n for j in jobs)} points, {sum(j.m for j in jobs)} links")
+    srv = serving.BatchServer(budget=int(budget_gib * 2**30), segment=segment)
+    res, st = srv.run(jobs)
+    srv.close()
+    log(f"serving: batched done in {st['wall_s']:.1f}s")
+    seq, seq_st = serving.serve_sequential(jobs)
+    log(f"serving: sequential done, busy {seq_st['busy_s']:.1f}s")
+    same = all(np.array_equal(a.xy.view(np.uint32), b.xy.view(np.uint32)) for a, b in zip(res, seq))
+    b, p = serving.latency_summary(res), serving.latency_summary(seq)
+    return {
+        "workload": f"Poisson arrivals 40/s over {window_s:g}s (seed 31), n log-uniform in [500, 500000], "
+                    f"Chung-Lu links 2-8x points, 10% with collision, until alpha < 0.001 "
+                    f"({serving.alpha_ticks(5000.0)} ticks)",
+        "jobs": len(jobs), "hbm_budget_gib": budget_gib, "segment_ticks": segment,
+        "batched": b, "sequential": p,
+        "speedup_layouts_per_s": b["layouts_per_s"] / p["layouts_per_s"],
+        "max_active": st["max_active"], "segments": st["segments"],
+        "point_ticks_per_s": st["point_ticks"] / st["wall_s"],
+        "sequential_equals_batched": bool(same),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +172,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-graphs", type=int, default=2048)
+    ap.add_argument("--batch-ticks", type=int, default=300)
+    ap.add_argument("--batch-sample", type=int, default=32)
+    ap.add_argument("--serve-seconds", type=float, default=0.5)
+    ap.add_argument("--serve-budget-gib", type=float, default=200.0)
+    ap.add_argument("--serve-segment", type=int, default=100)
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: batched,serving")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -151,9 +252,15 @@ def main() -> int:
         "stage_ms_per_tick": {s: v[0] / max(v[1], 1) for s, v in breakdown.items()},
         "roofline": roof,
     }
+    ctx.close()
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    skip = set(filter(None, args.skip.split(",")))
+    if "batched" not in skip:
+        out["batched"] = bench_batched(args.batch_graphs, args.batch_ticks, args.batch_sample, log)
+    if "serving" not in skip:
+        out["serving"] = bench_serving(args.serve_seconds, args.serve_budget_gib, args.serve_segment, log)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
-    ctx.close()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -1,0 +1,211 @@
+"""Layout serving: continuous batching of layout jobs over one device.
+
+Every job is one graph laid out from alpha = 1 until alpha < ALPHA_MIN, the
+point where the reference's frame loop ends the simulation
+(src/index.ts:2164-2169, Store/index.ts:8).  ``BatchServer`` keeps every admitted
+job resident in one context's graph set (positions, velocities and alpha stay
+on the device between ticks), advances all of them together, one launch per
+force pass per tick, admits arrivals at segment boundaries while the HBM
+budget allows, and retires each job on the tick its alpha crosses the cutoff.
+``serve_sequential`` is the plain path: one job at a time through the
+single-graph entry points, in arrival order.
+
+A job computes the same bits in either path (the graph set packs graphs
+without changing any graph's arithmetic), so the two differ only in time.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+ALPHA_MIN = 0.001
+
+
+@dataclass
+class Job:
+    """One layout request: the graph, its parameters and its arrival time (s)."""
+    jid: int
+    arrival: float
+    xy: np.ndarray
+    links: np.ndarray | None
+    params: _lib.CgSimParams
+    r_in: np.ndarray | None = None
+    r_out: np.ndarray | None = None
+    jitter: np.ndarray | None = None
+    sizes: np.ndarray | None = None
+
+    @property
+    def n(self) -> int:
+        return self.xy.size // 2
+
+    @property
+    def m(self) -> int:
+        return 0 if self.links is None else self.links.size // 2
+
+
+@dataclass
+class Result:
+    jid: int
+    arrival: float
+    start: float
+    finish: float
+    ticks: int
+    xy: np.ndarray = field(repr=False)
+
+    @property
+    def latency(self) -> float:
+        return self.finish - self.arrival
+
+
+def finest_grid(n: int) -> int:
+    """ForceManyBody/index.ts:646-656: smallest power of two >= 2 sqrt(n), in [8, 512]."""
+    p = 1
+    while (p // 2) * (p // 2) < n:
+        p <<= 1
+    return min(512, max(8, p))
+
+
+def footprint_bytes(n: int, m: int, collision: bool) -> int:
+    """Device bytes one job holds in a packed set, with the transient copy a
+    repack makes (positions and link schedules exist twice while it runs)."""
+    gf = finest_grid(n)
+    cells = gf * gf
+    per_point = 16 + 1 + 4 + 8 + 4 + 4 + 16 + 8 + 32   # positions x2, flags, gid, jitter, size,
+    #                                                     present, keys/values x2, sorted, link schedule
+    per_cell = 8 + 96 + 4 + 54                          # bounds, 8 slots, crowded list, levels
+    per_link = 2 * (16 + 16 + 16) + 24                  # CSR + schedule + pre-gathered terms, build
+    col = (4 * n * 16 + 4 * 512 * 512 * 24) if collision else 0
+    return int(1.3 * (per_point * n + per_cell * cells + per_link * m + col)) + (1 << 20)
+
+
+def alpha_ticks(decay: float, alpha0: float = 1.0) -> int:
+    """Ticks until alpha < ALPHA_MIN under Store.addAlpha (Store/index.ts:328-332)."""
+    f = 1.0 - math.pow(ALPHA_MIN, 1.0 / decay)
+    a, t = alpha0, 0
+    while a >= ALPHA_MIN and t < 10_000_000:
+        a += (0.0 - a) * f
+        t += 1
+    return t
+
+
+def _load(ctx: _lib.Context, gid: int, job: Job) -> None:
+    ctx.graph_set_params(gid, job.params)
+    ctx.graph_set_positions(gid, job.xy)
+    if job.jitter is not None:
+        ctx.graph_set_jitter(gid, job.jitter)
+    if job.links is not None:
+        ctx.graph_set_links(gid, job.links, None, job.r_in, job.r_out)
+    if job.sizes is not None:
+        ctx.graph_set_point_sizes(gid, job.sizes)
+    ctx.graph_set_alpha(gid, 1.0)
+
+
+class BatchServer:
+    """Continuous batching over one context.
+
+    ``segment`` caps the ticks between two admission points; a segment also
+    ends on the tick the first active job reaches its cutoff, so no job runs
+    past it.  ``budget`` is the HBM budget admission works against."""
+
+    def __init__(self, budget: int, segment: int = 100, device: int = 0):
+        self.budget = int(budget)
+        self.segment = int(segment)
+        self.ctx = _lib.Context(device=device, hbm_budget=self.budget)
+
+    def close(self) -> None:
+        self.ctx.close()
+
+    def run(self, jobs: list[Job], log=None) -> tuple[list[Result], dict]:
+        ctx = self.ctx
+        pending = sorted(jobs, key=lambda j: j.arrival)
+        active: dict[int, tuple[Job, int, int, float, int]] = {}  # gid -> job, left, total, start, bytes
+        held = 0
+        results: list[Result] = []
+        stats = dict(segments=0, ticks=0, max_active=0, point_ticks=0, admitted_late=0)
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        while pending or active:
+            t = now()
+            # admit every arrived job that fits, in arrival order (no overtaking)
+            while pending and pending[0].arrival <= t:
+                job = pending[0]
+                need = footprint_bytes(job.n, job.m, job.sizes is not None)
+                if active and held + need > self.budget:
+                    break
+                pending.pop(0)
+                gid = ctx.graph_add()
+                _load(ctx, gid, job)
+                total = alpha_ticks(job.params.decay)
+                active[gid] = (job, total, total, t, need)
+                held += need
+            if not active:
+                # idle until the next arrival
+                time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
+                continue
+            stats["max_active"] = max(stats["max_active"], len(active))
+            k = min(self.segment, min(v[1] for v in active.values()))
+            ids = list(active)
+            ctx.step_graphs(ids, k)
+            ctx.sync()
+            stats["segments"] += 1
+            stats["ticks"] += k
+            stats["point_ticks"] += k * sum(v[0].n for v in active.values())
+            for gid in ids:
+                job, left, total, start, need = active[gid]
+                left -= k
+                if left > 0:
+                    active[gid] = (job, left, total, start, need)
+                    continue
+                xy = ctx.graph_get_positions(gid)
+                fin = now()
+                ctx.graph_remove(gid)
+                del active[gid]
+                held -= need
+                results.append(Result(job.jid, job.arrival, start, fin, total, xy))
+                if log:
+                    log(f"job {job.jid} n={job.n} done at {fin:.2f}s latency {fin - job.arrival:.2f}s")
+        stats["wall_s"] = now()
+        return sorted(results, key=lambda r: r.jid), stats
+
+
+def serve_sequential(jobs: list[Job], device: int = 0, log=None) -> tuple[list[Result], dict]:
+    """The plain path: each job alone through the single-graph entry points,
+    first come first served.  Service times are measured; a job starts when it
+    has arrived and the previous one has finished."""
+    results = []
+    clock = 0.0
+    busy = 0.0
+    for job in sorted(jobs, key=lambda j: j.arrival):
+        with _lib.Context(device=device) as ctx:
+            ctx.set_positions(job.xy)
+            if job.jitter is not None:
+                ctx.set_jitter(job.jitter)
+            if job.links is not None:
+                ctx.set_links(job.links, None, job.r_in, job.r_out)
+            if job.sizes is not None:
+                ctx.set_point_sizes(job.sizes)
+            ctx.set_alpha(1.0)
+            total = alpha_ticks(job.params.decay)
+            s0 = time.perf_counter()
+            ctx.step(job.params, total)
+            xy = ctx.get_positions()
+            service = time.perf_counter() - s0
+        start = max(clock, job.arrival)
+        clock = start + service
+        busy += service
+        results.append(Result(job.jid, job.arrival, start, clock, total, xy))
+        if log:
+            log(f"sequential job {job.jid} n={job.n} service {service:.3f}s")
+    return sorted(results, key=lambda r: r.jid), dict(busy_s=busy, makespan_s=clock)
+
+
+def latency_summary(results: list[Result]) -> dict:
+    lat = np.array([r.latency for r in results], dtype=np.float64)
+    fin = max(r.finish for r in results) - min(r.arrival for r in results)
+    return dict(jobs=len(results), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                mean_s=float(lat.mean()), makespan_s=float(fin), layouts_per_s=len(results) / fin)

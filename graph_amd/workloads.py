@@ -82,3 +82,44 @@ def cfg3_sizes(count: int = 2048, seed: int = 23):
     n = np.exp(np.log(1000) + u[:count] * (np.log(200_000) - np.log(1000))).astype(np.int64)
     m = 1 + np.minimum((u[count:] * 4).astype(np.int64), 3)
     return n, m
+
+
+def cfg3_plain_sample(count: int, sample: int) -> list[int]:
+    """Graph indices of the plain one-graph-at-a-time timing sample: every
+    (count // sample)-th graph.  The point counts are i.i.d. log-uniform, so an
+    evenly strided sample is a uniform random sample of the batch."""
+    stride = max(1, count // max(1, sample))
+    return list(range(0, count, stride))[:sample]
+
+
+def cfg4_jobs(rate: float = 40.0, duration: float = 120.0, seed: int = 31, n_lo: int = 500,
+              n_hi: int = 500_000, collision_frac: float = 0.1):
+    """Layout requests: Poisson arrivals at `rate`/s over `duration` s; point
+    counts log-uniform in [n_lo, n_hi]; links 2-8x points from Chung-Lu with
+    exponent uniform in [2.0, 2.8]; a `collision_frac` share with collision on
+    and log-normal radii (mu = ln 2, sigma = 0.5); positions uniform in a
+    spaceSize of next_pow2(4 sqrt(n) 8); parameters as in the 10k workload."""
+    from .serving import Job
+
+    k = int(rate * duration * 3) + 16
+    u = datagen.uniform(6 * k, 0.0, 1.0, seed).astype(np.float64).reshape(6, k)
+    gaps = -np.log(1.0 - u[0]) / rate
+    arrivals = np.cumsum(gaps)
+    jobs = []
+    for i in range(k):
+        if arrivals[i] > duration:
+            break
+        n = int(math.exp(math.log(n_lo) + u[1, i] * (math.log(n_hi) - math.log(n_lo))))
+        ratio = 2.0 + 6.0 * u[2, i]
+        gamma = 2.0 + 0.8 * u[3, i]
+        collide = u[4, i] < collision_frac
+        s = seed * 100_003 + i
+        S = float(next_pow2(4 * math.sqrt(n) * 8))
+        m = int(ratio * n)
+        links = datagen.chung_lu(n, gamma, 1.0, m, s)
+        r_in, r_out = _draws(m, s)
+        sizes = (2.0 * datagen.lognormal(n, math.log(2.0), 0.5, s)).astype(np.float32) if collide else None
+        p = sim_params(S, collision=1.0) if collide else sim_params(S)
+        jobs.append(Job(jid=i, arrival=float(arrivals[i]), xy=datagen.uniform_square(n, S, s), links=links,
+                        params=p, r_in=r_in, r_out=r_out, jitter=datagen.jitter(n, s), sizes=sizes))
+    return jobs
