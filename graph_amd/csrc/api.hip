@@ -134,6 +134,12 @@ bool same_collision(const cg_sim_params& a, const cg_sim_params& b) {
 
 bool collides(const Graph& g) { return on(g.params.collision) && g.np > 0; }
 
+// Finest cells a graph occupies in the packed cell arrays (rounded up to whole blocks).
+uint64_t pad_cells(uint32_t gf) {
+    const uint64_t n = (uint64_t)gf * gf;
+    return (n + CELL_BLOCK - 1) / CELL_BLOCK * CELL_BLOCK;
+}
+
 Graph* lookup(Ctx* c, uint32_t id) {
     if (id >= c->graphs.size() || !c->graphs[id].live) return nullptr;
     return &c->graphs[id];
@@ -176,16 +182,19 @@ void release(Ctx* c, T*& p) {
 void free_packed(Ctx* c) {
     release(c, c->flags); release(c, c->present); release(c, c->gid); release(c, c->jitter);
     release(c, c->size); release(c, c->desc_base); c->desc = nullptr;
-    release(c, c->f0s); release(c, c->colc0s);
+    release(c, c->f0s); release(c, c->colc0s); release(c, c->blk_graph);
     for (int b = 0; b < 2; ++b) {
-        release(c, c->keys[b]); release(c, c->vals[b]);
         release(c, c->col_keys[b]); release(c, c->col_vals[b]);
         release(c, c->pyr[b]);
         c->n_pyr[b] = 0;
     }
-    release(c, c->sort_tmp); release(c, c->cell_begin); release(c, c->cell_end);
+    for (int q = 0; q < 2; ++q) { release(c, c->rk[q]); release(c, c->rv[q]); }
+    c->sorted = nullptr;
+    release(c, c->hist); release(c, c->cell_begin); release(c, c->cell_end); release(c, c->scan_part);
+    c->n_scan_tiles = 0;
     release(c, c->sums); release(c, c->cent); release(c, c->sorted_pos);
-    release(c, c->slot_idx); release(c, c->slot_pos); release(c, c->big_list); release(c, c->big_count);
+    release(c, c->slot_idx); release(c, c->slot_pos); release(c, c->mid_cells); release(c, c->big_cells); release(c, c->big_chunks); release(c, c->chunk_top);
+    release(c, c->big_count);
     release(c, c->center_part); release(c, c->center_sum); release(c, c->center_chunks);
     release(c, c->center_first);
     c->n_center_chunks = 0;
@@ -223,7 +232,7 @@ int pack(Ctx* c) {
         const Graph& g = c->graphs[c->order[s]];
         N += g.n;
         NP += g.np;
-        F += (uint64_t)g.gf * g.gf;
+        F += pad_cells(g.gf);
         C += level_off(g.L);
         if (collides(g)) {
             CG += 4ull * g.col_G * g.col_G;
@@ -242,7 +251,7 @@ int pack(Ctx* c) {
             old_p0[s] = g.p0;
             g.slot = s;
             g.p0 = p0; g.k0 = k0; g.f0 = f0; g.c0 = c0; g.col_c0 = col0;
-            p0 += g.n; k0 += g.np; f0 += g.gf * g.gf; c0 += level_off(g.L);
+            p0 += g.n; k0 += g.np; f0 += (uint32_t)pad_cells(g.gf); c0 += level_off(g.L);
             if (collides(g)) col0 += 4 * g.col_G * g.col_G;
             GDesc d{};
             d.p0 = g.p0; d.n = g.n; d.k0 = g.k0; d.np = g.np; d.f0 = g.f0; d.c0 = g.c0;
@@ -274,11 +283,12 @@ int pack(Ctx* c) {
     c->cpresent = (uint32_t*)dalloc(c, (NPC ? NPC : 1) * 4, &rc);
     c->f0s = (uint32_t*)dalloc(c, gg * 4, &rc);
     c->colc0s = (uint32_t*)dalloc(c, gg * 4, &rc);
+    c->blk_graph = (uint32_t*)dalloc(c, (F / CELL_BLOCK + 1) * 4, &rc);
     if (rc) return rc;
 
     // host staging of the per-point arrays
     std::vector<uint8_t> hflags(N);
-    std::vector<uint32_t> hgid(N), hpres, hcpres, hf0(G), hcol(G), k0s(G), nps(G);
+    std::vector<uint32_t> hgid(N), hpres, hcpres, hf0(G), hcol(G), k0s(G), nps(G), hblk(F / CELL_BLOCK);
     std::vector<float> hjit(2 * N), hsize(N, 0.f);
     std::vector<uint4> w0, w1;
     hpres.reserve(NP);
@@ -298,6 +308,7 @@ int pack(Ctx* c) {
             if (col) hcpres.push_back(g.p0 + i);
         }
         hf0[s] = g.f0;
+        std::fill(hblk.begin() + g.f0 / CELL_BLOCK, hblk.begin() + (g.f0 + pad_cells(g.gf)) / CELL_BLOCK, s);
         hcol[s] = g.col_c0;
         k0s[s] = g.k0;
         nps[s] = g.np;
@@ -311,6 +322,7 @@ int pack(Ctx* c) {
     rc = upload(c, c->cpresent, hcpres); if (rc) return rc;
     rc = upload(c, c->f0s, hf0); if (rc) return rc;
     rc = upload(c, c->colc0s, hcol); if (rc) return rc;
+    rc = upload(c, c->blk_graph, hblk); if (rc) return rc;
 
     // positions: host input for newly set graphs, device copy for the others
     for (uint32_t s = 0; s < G; ++s) {
@@ -404,6 +416,13 @@ void fill_dynamic(GDesc& d, const cg_sim_params& p, float a, float seed) {
     d.col_strength = p.collision;
     d.col_radius = p.collision_radius;
     d.col_padding = p.collision_padding;
+    // fixed point of the cell sums: np * 16 S * 2^k < 2^63
+    int bits_s = 0, bits_n = 0;
+    while (bits_s < 128 && std::ldexp(1.0, bits_s) < (double)S) ++bits_s;
+    while (bits_n < 32 && (1ull << bits_n) < (unsigned long long)d.np + 1) ++bits_n;
+    const int k = std::max(0, std::min(40, 59 - bits_s - bits_n));
+    d.fx_scale = std::ldexp(1.f, k);
+    d.fx_inv = std::ldexp(1.f, -k);
 }
 
 // Passes a graph runs in one tick — the gates of src/index.ts:2012-2078.
@@ -424,7 +443,6 @@ uint32_t graph_passes(const Graph& g) {
 
 struct TickPlan {
     uint32_t any;   // union of the passes of every graph
-    int keys_pass;  // per-point pass that also writes the many-body cell keys, or -1
 };
 
 int launch_tick(Ctx* c, const TickPlan& tp) {
@@ -432,24 +450,24 @@ int launch_tick(Ctx* c, const TickPlan& tp) {
     int rc = CG_OK;
     if (a & (1u << P_MOUSE)) {
         prof_begin(c, CG_STAGE_MOUSE);
-        rc = pw_run(c, P_MOUSE, false, nullptr);
+        rc = pw_run(c, P_MOUSE, nullptr);
         prof_end(c, CG_STAGE_MOUSE);
         if (rc) return rc;
     }
     if (a & (1u << P_GRAVITY)) {
         prof_begin(c, CG_STAGE_GRAVITY);
-        rc = pw_run(c, P_GRAVITY, tp.keys_pass == P_GRAVITY, nullptr);
+        rc = pw_run(c, P_GRAVITY, nullptr);
         prof_end(c, CG_STAGE_GRAVITY);
         if (rc) return rc;
     }
     if (a & (1u << P_CENTER)) {
         prof_begin(c, CG_STAGE_CENTER);
-        rc = pw_run(c, P_CENTER, tp.keys_pass == P_CENTER, nullptr);
+        rc = pw_run(c, P_CENTER, nullptr);
         prof_end(c, CG_STAGE_CENTER);
         if (rc) return rc;
     }
     if (a & (1u << P_MANYBODY)) {
-        rc = mb_run(c, tp.keys_pass >= 0, nullptr);
+        rc = mb_run(c, nullptr);
         if (rc) return rc;
     }
     for (int d = 0; d < 2; ++d) {
@@ -489,10 +507,10 @@ int run_ticks(Ctx* c, const std::vector<uint32_t>& ids, uint32_t nticks, int64_t
             CG_HIP(c, hipEventSynchronize(c->ev_desc));
             c->desc_pending = false;
         }
-        plan.assign(T, TickPlan{0, -1});
+        plan.assign(T, TickPlan{0});
         for (uint32_t tt = 0; tt < T; ++tt) {
             GDesc* D = c->h_desc_pinned + (size_t)tt * G;
-            uint32_t any = 0, all = ~0u;
+            uint32_t any = 0;
             for (uint32_t s = 0; s < G; ++s) {
                 Graph& g = c->graphs[c->order[s]];
                 const cg_sim_params& p = g.params;
@@ -519,12 +537,8 @@ int run_ticks(Ctx* c, const std::vector<uint32_t>& ids, uint32_t nticks, int64_t
                     }
                 }
                 any |= passes;
-                if (g.np) all &= passes;
             }
-            // One per-point pass writes the cell keys when every graph with points runs it
-            // last before the many-body pass; otherwise a separate key pass does.
-            const int K = (any & (1u << P_CENTER)) ? P_CENTER : (any & (1u << P_GRAVITY)) ? P_GRAVITY : -1;
-            plan[tt] = TickPlan{any, (K >= 0 && ((all >> K) & 1u)) ? K : -1};
+            plan[tt] = TickPlan{any};
             if (trace && sel_slot >= 0) trace[t0 + tt] = c->graphs[sel].alpha;
         }
         CG_HIP(c, hipMemcpyAsync(c->desc_base, c->h_desc_pinned, (size_t)T * G * sizeof(GDesc),
@@ -957,8 +971,8 @@ int cg_eval_force(cg_ctx* x, const cg_sim_params* p, int32_t force, float alpha,
     hipError_t e = hipMemsetAsync(dv, 0, (size_t)c->N * 8, c->stream);
     if (e != hipSuccess) rc = hip_fail(c, e, "cg_eval_force clear");
     if (!rc) {
-        if (pass <= P_CENTER) rc = pw_run(c, pass, false, dv);
-        else if (pass == P_MANYBODY) rc = mb_run(c, false, dv);
+        if (pass <= P_CENTER) rc = pw_run(c, pass, dv);
+        else if (pass == P_MANYBODY) rc = mb_run(c, dv);
         else if (pass == P_COLLISION) rc = col_run(c, dv);
         else rc = link_run(c, pass - P_LINK_IN, dv);
     }
@@ -983,16 +997,15 @@ int cg_debug_slots(cg_ctx* x, int32_t* slots, uint32_t capacity, uint32_t* fines
     if (!slots) return CG_OK;
     if (capacity < need || !c->slot_idx || c->dirty)
         return fail(c, CG_ERR_INVALID, "cg_debug_slots: capacity or no many-body pass yet");
-    // the kernels skip the slots of empty cells: report them as -1 from the cell counts
+    // the kernels skip the slots of empty cells (end 0): report them as -1
     const size_t cells = (size_t)g.gf * g.gf;
-    std::vector<double> sums(3 * cells);
+    std::vector<uint32_t> cnt(cells);
     CG_HIP(c, hipMemcpyAsync(slots, c->slot_idx + (size_t)g.f0 * SLOTS, need * 4, hipMemcpyDeviceToHost,
                              c->stream));
-    CG_HIP(c, hipMemcpyAsync(sums.data(), c->sums + 3 * ((size_t)g.c0 + level_off(g.L - 1)),
-                             sums.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CG_HIP(c, hipMemcpyAsync(cnt.data(), c->cell_end + g.f0, cells * 4, hipMemcpyDeviceToHost, c->stream));
     CG_HIP(c, hipStreamSynchronize(c->stream));
     for (size_t rm = 0; rm < cells; ++rm) {
-        const bool empty = !(sums[3 * rm + 2] > 0.0);
+        const bool empty = cnt[morton2((uint32_t)(rm % g.gf), (uint32_t)(rm / g.gf))] == 0;
         for (size_t s = 0; s < SLOTS; ++s) {
             int32_t& v = slots[rm * SLOTS + s];
             v = (empty || v < 0) ? -1 : v - (int32_t)g.p0;

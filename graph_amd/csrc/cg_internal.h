@@ -25,6 +25,7 @@ constexpr uint8_t F_PINNED = 2;   // setPinnedPoints (Points/index.ts:1264-1304)
 constexpr int MAX_LEVELS = 8;     // 4^2 .. 512^2 (ForceManyBody/index.ts:19, 646-656)
 constexpr int SLOTS = 8;          // NEAR_FIELD_SLOTS (ForceManyBody/index.ts:35)
 constexpr uint32_t BIG_CELL = 64; // finest cells above this many points use a wave each
+constexpr uint32_t CELL_BLOCK = 256;  // each graph's finest cells start on a multiple of this
 
 // Link degree tiers: points with at most LANE_MAX entries run one lane each;
 // up to WAVE_MAX entries one 64-lane workgroup each; above, 1024 lanes each.
@@ -70,6 +71,7 @@ struct GDesc {
     float spring_alpha, dist, lo, span;
     float tol;              // multiple-shooting tolerance (2.5e-6 S)
     float col_strength, col_radius, col_padding, col_h;
+    float fx_scale, fx_inv;  // 2^k and 2^-k: positions as int64 multiples of 2^-k in cell sums
 };
 
 // Per-graph link CSR built once per cg_set_links, in the reference's entry
@@ -158,23 +160,30 @@ struct Ctx {
     GDesc* desc = nullptr;          // descriptor table of the tick being launched
     std::vector<GDesc> h_desc;      // placement part of every descriptor, by slot
     uint32_t* f0s = nullptr;        // [G] finest-cell bases
+    uint32_t* blk_graph = nullptr;  // [F / CELL_BLOCK] graph slot of each block of finest cells
     uint32_t* colc0s = nullptr;     // [G] collision cell bases
 
-    // many-body
-    uint32_t* keys[2] = {nullptr, nullptr};
-    uint32_t* vals[2] = {nullptr, nullptr};
-    void* sort_tmp = nullptr;
-    size_t sort_tmp_bytes = 0;
-    int key_bits = 1;
-    uint32_t* cell_begin = nullptr; // [F]
-    uint32_t* cell_end = nullptr;
-    double* sums = nullptr;         // [3 C]
+    // many-body: radix sort of present points by finest cell
+    uint32_t* rk[2] = {nullptr, nullptr};  // [NP] cell keys, ping-pong over the passes
+    uint32_t* rv[2] = {nullptr, nullptr};  // [NP] point indices, ping-pong
+    uint32_t* sorted = nullptr;     // rv[passes & 1]: points grouped by cell, next tick's visit order
+    uint32_t* hist = nullptr;       // [radix * n_rb] per-pass [digit][block] counts, scanned in place
+    uint32_t n_rb = 0;              // radix-sort blocks
+    int rb_passes = 1, rb_width = 8;
+    uint32_t* cell_begin = nullptr; // [F] first sorted position of each finest cell
+    uint32_t* cell_end = nullptr;   // [F] one past its last (0 = empty)
+    uint32_t* scan_part = nullptr;  // [n_scan_tiles] per-tile totals of the histogram scan
+    uint32_t n_scan_tiles = 0;
+    long long* sums = nullptr;      // [3 C] fixed-point sum x, sum y, count per level cell
     float4* cent = nullptr;         // [C]
     float2* sorted_pos = nullptr;   // [NP]
     int32_t* slot_idx = nullptr;    // [8 F] global point index or -1
     float2* slot_pos = nullptr;
-    uint32_t* big_list = nullptr;
-    uint32_t* big_count = nullptr;
+    uint32_t* mid_cells = nullptr;  // cells of BIG_CELL+1 .. BIG_CHUNK points
+    uint4* big_cells = nullptr;     // crowded cells {cell, first chunk, chunks, row-major cell}
+    uint2* big_chunks = nullptr;    // their chunks {cell, chunk}
+    unsigned long long* chunk_top = nullptr;  // [8 per chunk] top-8 (ticket << 32 | index)
+    uint32_t* big_count = nullptr;  // [0] crowded cells, [1] their chunks, [2] mid cells
     uint4* pyr[2] = {nullptr, nullptr};  // pyramid work lists {graph slot, level, tile, levels}
     uint32_t n_pyr[2] = {0, 0};
     double* center_part = nullptr;  // [2 chunks] per-chunk partial sums
@@ -234,13 +243,12 @@ void dfree(Ctx* c, void* p);
 
 // Launchers.  Input and output position buffers come from GDesc::inbuf; vout
 // (non-null: evaluation mode) receives the pass's velocity instead of integrating.
-int pw_run(Ctx* c, int pass, bool write_keys, float2* vout);       // k_pointwise.hip
-int keys_run(Ctx* c);
+int pw_run(Ctx* c, int pass, float2* vout);                       // k_pointwise.hip
 int center_run(Ctx* c);
 void center_chunks_host(const std::vector<uint32_t>& k0, const std::vector<uint32_t>& np,
                         std::vector<uint4>& chunks, std::vector<uint32_t>& first);
 int mb_alloc(Ctx* c);                                             // k_manybody.hip
-int mb_run(Ctx* c, bool keys_ready, float2* vout);
+int mb_run(Ctx* c, float2* vout);
 void pyramid_work_host(uint32_t slot, uint32_t gf, uint32_t L, std::vector<uint4>& w0,
                        std::vector<uint4>& w1);
 int link_build_graph(Ctx* c, Graph& g);                           // k_link.hip

@@ -1,26 +1,27 @@
 // k_manybody.hip — grid-pyramid many-body repulsion with the Monte-Carlo
 // near field, as one deterministic pipeline per tick:
 //
-//   1. finest-cell Morton key per present point (k_pointwise.hip)
-//   2. stable radix sort of (key, point index)  -> points grouped by finest
-//      cell, ascending index inside a cell; every pyramid cell at every level
-//      is then a contiguous run of the sorted order
-//   3. segment bounds per finest cell
-//   4. per finest cell: f64 sums of (x, y) and count, and the top-8 near-field
-//      slots by (ticket, index) — one lane per cell, one wave per crowded cell
-//   5. f64 2x2 merges up the pyramid, f32 centroids (cx, cy, N) per level
-//   6. per point (in sorted order): far field over every level + near field,
+//   1. LSD radix sort of the present points by finest cell, visiting them in
+//      the previous tick's cell order; cell bounds from the sorted keys
+//   2. per finest cell: fixed-point sums of (x, y), count, and the top-8
+//      near-field slots by (ticket, index) — one lane per cell, one wave per
+//      cell above 64 points, chunked blocks plus a merge above 2048 points
+//   3. 2x2 merges up the pyramid in the same fixed point, f32 centroids
+//      (cx, cy, N) per level
+//   4. per point (in cell order): far field over every level + near field,
 //      fused with the integrate pass
 //
-// No float atomics anywhere: every sum has a fixed order, so a graph gives the
-// same bits alone or packed with others.
+// Within a cell, points keep the previous tick's order; nothing depends on it
+// anyway.  Sums are integer (positions as multiples of 2^-k, k chosen per graph
+// so no sum overflows), so they are exact and order-free, and the slot choice
+// is a set function (the 8 smallest distinct tickets, the smallest index per
+// ticket).  No float atomics anywhere: a graph gives the same bits alone or
+// packed with others.
 //
 // Reference: ForceManyBody/index.ts:399-417 (run), :464-502 (drawLevels),
 // :511-559 (drawNearFieldSlots), :565-638 (drawForces), :646-694 (geometry);
 // calculate-level.vert:28-53; build-nearfield-slots.vert:46-103;
 // force-level.frag:43-98; force-nearfield.frag:64-166.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 
 #include "cg_internal.h"
@@ -29,34 +30,248 @@ namespace cg {
 namespace {
 
 constexpr uint32_t TMAX = 0xffffffffu;
-
-// ---- 3. segment bounds -----------------------------------------------------
+constexpr uint32_t SCAN_TILE = 4096;  // cells per scan tile: 256 lanes x 16
+constexpr uint32_t BIG_CHUNK = 2048;  // points per block in a crowded cell
+constexpr uint32_t MID_BLOCKS = 2048; // grid of the wave-per-cell pass (8192 waves)
 
 __global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
 }
 
-__global__ void k_mark_bounds(const uint32_t* __restrict__ sk, uint32_t np,
-                              uint32_t* __restrict__ begin, uint32_t* __restrict__ end) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+// ---- 1. counting sort by finest cell ---------------------------------------------
+
+// Finest cell of position q: calculate-level.vert:47-48 at the finest level,
+// as a Morton index inside the graph's cell range.
+__device__ inline uint32_t finest_key(const GDesc& d, float2 q) {
+    const float h = d.h[d.L - 1];
+    uint32_t cx = (uint32_t)cell_of(q.x, h, (int)d.gf);
+    uint32_t cy = (uint32_t)cell_of(q.y, h, (int)d.gf);
+    return d.f0 + morton2(cx, cy);
+}
+
+// LSD radix sort of (cell key, point index), RB_ITEMS keys per lane, digits of
+// `width` <= 8 bits.  Per pass: block digit histograms (LDS atomics only),
+// an exclusive scan of the [digit][block] table, then a stable scatter whose
+// in-block ranks come from ballot matching in position order.  No global
+// atomics: on this part they serialise at a few ns each.
+constexpr int RB_ITEMS = 8;
+constexpr int RB_TILE = 256 * RB_ITEMS;
+
+// Keys of the points in the previous tick's cell order, and the first digit's
+// block histogram.  `visit` may alias `vout` (each lane rewrites what it read).
+__global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const float2* __restrict__ pos0,
+                                                  const float2* __restrict__ pos1,
+                                                  const uint32_t* __restrict__ gid,
+                                                  const GDesc* __restrict__ desc, uint32_t np,
+                                                  uint32_t* __restrict__ kout, uint32_t* vout,
+                                                  int width, uint32_t nb, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    const int t = threadIdx.x;
+    const uint32_t radix = 1u << width, mask = radix - 1;
+    h[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RB_ITEMS; ++j) {
+        const uint32_t k = blockIdx.x * RB_TILE + j * 256 + t;
+        if (k < np) {
+            const uint32_t i = visit[k];
+            const GDesc& d = desc[gid[i]];
+            const float2 q = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1[i] : pos0[i];
+            const uint32_t key = finest_key(d, q);
+            kout[k] = key;
+            vout[k] = i;
+            atomicAdd(&h[key & mask], 1u);
+        }
+    }
+    __syncthreads();
+    if ((uint32_t)t < radix) hist[t * nb + blockIdx.x] = h[t];
+}
+
+__global__ void __launch_bounds__(256) k_radix_up(const uint32_t* __restrict__ kin, uint32_t np, int shift,
+                                                  int width, uint32_t nb, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    const int t = threadIdx.x;
+    const uint32_t radix = 1u << width, mask = radix - 1;
+    h[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RB_ITEMS; ++j) {
+        const uint32_t k = blockIdx.x * RB_TILE + j * 256 + t;
+        if (k < np) atomicAdd(&h[(kin[k] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    if ((uint32_t)t < radix) hist[t * nb + blockIdx.x] = h[t];
+}
+
+__global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__ kin,
+                                                    const uint32_t* __restrict__ vin, uint32_t np,
+                                                    int shift, int width, uint32_t nb,
+                                                    const uint32_t* __restrict__ hist,
+                                                    uint32_t* __restrict__ kout,
+                                                    uint32_t* __restrict__ vout) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wc[4][256];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t radix = 1u << width, mask = radix - 1;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    if ((uint32_t)t < radix) base[t] = hist[t * nb + blockIdx.x];
+    for (int j = 0; j < RB_ITEMS; ++j) {
+        const uint32_t k = blockIdx.x * RB_TILE + j * 256 + t;
+        const bool valid = k < np;
+        const uint32_t key = valid ? kin[k] : 0u;
+        const uint32_t val = valid ? vin[k] : 0u;
+        const uint32_t dg = (key >> shift) & mask;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wc[q][t] = 0u;
+        __syncthreads();
+        // lanes of this wave holding the same digit, in position order
+        unsigned long long m = __ballot(valid);
+        for (int bit = 0; bit < width; ++bit) {
+            const bool on_ = (dg >> bit) & 1u;
+            const unsigned long long bb = __ballot(on_);
+            m &= on_ ? bb : ~bb;
+        }
+        const uint32_t below = (uint32_t)__popcll(m & lt);
+        if (valid && below == 0) wc[wave][dg] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = base[dg] + below;
+            for (int w = 0; w < wave; ++w) off += wc[w][dg];
+            kout[off] = key;
+            vout[off] = val;
+        }
+        __syncthreads();
+        if ((uint32_t)t < radix) base[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+        __syncthreads();
+    }
+}
+
+// Cell bounds from the sorted keys: begin at each run's first position, end
+// one past its last (cell_end is cleared first; an empty cell has end 0).
+__global__ void __launch_bounds__(256) k_bin_bounds(const uint32_t* __restrict__ sk, uint32_t np,
+                                                    uint32_t* __restrict__ begin,
+                                                    uint32_t* __restrict__ end) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np) return;
-    uint32_t key = sk[k];
+    const uint32_t key = sk[k];
     if (k == 0 || sk[k - 1] != key) begin[key] = k;
     if (k == np - 1 || sk[k + 1] != key) end[key] = k + 1;
 }
 
-// ---- 4. finest cells: sums + slots ------------------------------------------
+// Exclusive scan of n counts (n a multiple of 16): per-tile totals, one block
+// scanning the totals, then each tile scanned with its offset.
+__device__ inline uint32_t block_sum_256(uint32_t v, uint32_t* sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return sh[0] + sh[1] + sh[2] + sh[3];
+}
 
-// Sorted insert of (t, i) into a top-8 list of distinct tickets.  Points arrive
-// in ascending index, so an equal ticket already in the list has the smaller
-// index and wins (the depth test's first-drawn rule, build-nearfield-slots.vert:85-101).
+__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
+                                                     uint32_t* __restrict__ part) {
+    __shared__ uint32_t sh[4];
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * 16;
+    uint32_t s = 0;
+    if (base < n) {
+        const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint4 v = p[q];
+            s += v.x + v.y + v.z + v.w;
+        }
+    }
+    s = block_sum_256(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_part(uint32_t* __restrict__ part, uint32_t nb) {
+    __shared__ uint32_t sh[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t lo = min(nb, t * per), hi = min(nb, lo + per);
+    uint32_t s = 0;
+    for (uint32_t j = lo; j < hi; ++j) s += part[j];
+    sh[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = t >= off ? sh[t - off] : 0u;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t ? sh[t - 1] : 0u;
+    for (uint32_t j = lo; j < hi; ++j) {
+        uint32_t v = part[j];
+        part[j] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scan_tiles(const uint32_t* __restrict__ in, uint32_t n,
+                                                    const uint32_t* __restrict__ part,
+                                                    uint32_t* __restrict__ out) {
+    __shared__ uint32_t sh[4];
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t v[16];
+    if (base < n) {
+        const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint4 w = p[q];
+            v[4 * q] = w.x; v[4 * q + 1] = w.y; v[4 * q + 2] = w.z; v[4 * q + 3] = w.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = 0u;
+    }
+    uint32_t tot = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tot += v[q];
+    // inclusive scan of the lane totals inside the wave, then across the 4 waves
+    uint32_t inc = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += u;
+    }
+    if (lane == 63) sh[wave] = inc;
+    __syncthreads();
+    uint32_t before = part[blockIdx.x];
+    for (int w = 0; w < wave; ++w) before += sh[w];
+    uint32_t run = before + inc - tot;
+    if (base < n) {
+        uint4* p = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint4 w;
+            w.x = run; run += v[4 * q];
+            w.y = run; run += v[4 * q + 1];
+            w.z = run; run += v[4 * q + 2];
+            w.w = run; run += v[4 * q + 3];
+            p[q] = w;
+        }
+    }
+}
+
+// ---- 2. finest cells: sums + slots ----------------------------------------------
+
+// Insert (t, i) into a top-8 list of distinct tickets, ascending.  An equal
+// ticket keeps the smaller index, whatever the visiting order: the result is
+// that of drawing points in ascending index with the depth test's first-drawn
+// rule (build-nearfield-slots.vert:85-101).
 __device__ inline void top8_insert(uint32_t (&tb)[SLOTS], uint32_t (&ti)[SLOTS], uint32_t t,
                                    uint32_t i) {
-    if (t >= tb[SLOTS - 1]) return;
+    if (t > tb[SLOTS - 1]) return;
     bool dup = false;
 #pragma unroll
-    for (int s = 0; s < SLOTS; ++s) dup |= (tb[s] == t);
+    for (int s = 0; s < SLOTS; ++s) {
+        const bool eq = tb[s] == t;
+        dup |= eq;
+        ti[s] = (eq && i < ti[s]) ? i : ti[s];
+    }
     if (dup) return;
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
@@ -69,16 +284,19 @@ __device__ inline void top8_insert(uint32_t (&tb)[SLOTS], uint32_t (&ti)[SLOTS],
     }
 }
 
-__device__ inline void write_cell(const GDesc& d, uint32_t rm, double sx, double sy, uint32_t cnt,
+// Position as an integer multiple of 2^-k (exact for |p| >= 2^(23-k)).
+__device__ inline long long fixed(float v, float scale) {
+    return __double2ll_rn((double)v * (double)scale);
+}
+
+__device__ inline void write_cell(const GDesc& d, uint32_t rm, long long sx, long long sy, uint32_t cnt,
                                   const uint32_t (&ti)[SLOTS], const float2* __restrict__ pin,
-                                  double* __restrict__ sums, int32_t* __restrict__ slot_idx,
+                                  long long* __restrict__ sums, int32_t* __restrict__ slot_idx,
                                   float2* __restrict__ slot_pos) {
     const size_t c = (size_t)d.c0 + level_off(d.L - 1) + rm;
     sums[3 * c + 0] = sx;
     sums[3 * c + 1] = sy;
-    sums[3 * c + 2] = (double)cnt;
-    // slots of an empty cell are never read (its count is 0)
-    if (cnt == 0) return;
+    sums[3 * c + 2] = (long long)cnt;
     const size_t sb = ((size_t)d.f0 + rm) * SLOTS;
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
@@ -88,84 +306,124 @@ __device__ inline void write_cell(const GDesc& d, uint32_t rm, double sx, double
     }
 }
 
-// One lane per finest cell of every graph.  Sums in index order, top-8 slots by
-// (ticket of the graph-local index, index).
-__global__ void __launch_bounds__(256) k_cells(const uint32_t* __restrict__ begin,
-                                               const uint32_t* __restrict__ end,
-                                               const uint32_t* __restrict__ sv,
-                                               const float2* __restrict__ pos0,
-                                               const float2* __restrict__ pos1,
-                                               float2* __restrict__ sorted_pos,
-                                               double* __restrict__ sums,
-                                               int32_t* __restrict__ slot_idx,
-                                               float2* __restrict__ slot_pos,
-                                               uint32_t* __restrict__ big_list,
-                                               uint32_t* __restrict__ big_count,
-                                               const uint32_t* __restrict__ f0s,
-                                               const GDesc* __restrict__ desc, uint32_t G,
-                                               uint32_t F) {
-    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F) return;
-    const GDesc& d = desc[find_graph(f0s, G, f)];
+// One lane per finest cell, one block per CELL_BLOCK cells of one graph.  Empty
+// cells write nothing: the pyramid reads their count, and their slots are
+// never read.  Cells above BIG_CELL points go to k_cells_mid (one wave each);
+// cells above BIG_CHUNK points are queued in chunks of BIG_CHUNK points for
+// k_cells_chunk and k_cells_merge.
+__global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict__ end_,
+                                                      const uint32_t* __restrict__ begin,
+                                                      const uint32_t* __restrict__ sv,
+                                                      const float2* __restrict__ pos0,
+                                                      const float2* __restrict__ pos1,
+                                                      float2* __restrict__ sorted_pos,
+                                                      long long* __restrict__ sums,
+                                                      int32_t* __restrict__ slot_idx,
+                                                      float2* __restrict__ slot_pos,
+                                                      uint32_t* __restrict__ mid_cells,
+                                                      uint4* __restrict__ big_cells,
+                                                      uint2* __restrict__ big_chunks,
+                                                      uint32_t* __restrict__ big_count,
+                                                      const uint32_t* __restrict__ blk_graph,
+                                                      const GDesc* __restrict__ desc) {
+    const uint32_t f = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    const GDesc& d = desc[blk_graph[blockIdx.x]];
     if (!(d.passes & (1u << P_MANYBODY))) return;
-    const float2* __restrict__ pin = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1 : pos0;
-    const uint32_t b = begin[f], e = end[f];
-    if (e - b > BIG_CELL) {
-        big_list[atomicAdd(big_count, 1u)] = f;
+    const uint32_t mc = f - d.f0;
+    if (mc >= d.gf * d.gf) return;  // alignment padding
+    const uint32_t e = end_[f];
+    if (e == 0) return;
+    const uint32_t b = begin[f], n = e - b;
+    const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
+    if (n > BIG_CELL && n <= BIG_CHUNK) {
+        mid_cells[atomicAdd(&big_count[2], 1u)] = f;  // one wave each (k_cells_mid)
         return;
     }
+    if (n > BIG_CHUNK) {
+        // crowded: sums are added by the chunk blocks, slots merged after them
+        const size_t c3 = 3 * ((size_t)d.c0 + level_off(d.L - 1) + rm);
+        sums[c3] = 0;
+        sums[c3 + 1] = 0;
+        sums[c3 + 2] = (long long)n;
+        const uint32_t nch = (n + BIG_CHUNK - 1) / BIG_CHUNK;
+        const uint32_t e0 = atomicAdd(&big_count[1], nch);
+        big_cells[atomicAdd(&big_count[0], 1u)] = make_uint4(f, e0, nch, rm);
+        for (uint32_t q = 0; q < nch; ++q) big_chunks[e0 + q] = make_uint2(f, q);
+        return;
+    }
+    const float2* __restrict__ pin = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1 : pos0;
     uint32_t tb[SLOTS], ti[SLOTS];
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
-    double sx = 0.0, sy = 0.0;
+    long long sx = 0, sy = 0;
     for (uint32_t k = b; k < e; ++k) {
         uint32_t i = sv[k];
         float2 p = pin[i];
         sorted_pos[k] = p;
-        sx += (double)p.x;
-        sy += (double)p.y;
+        sx += fixed(p.x, d.fx_scale);
+        sy += fixed(p.y, d.fx_scale);
         top8_insert(tb, ti, ticket_bits(i - d.p0, d.seed), i);
     }
-    const uint32_t mc = f - d.f0;
-    const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
-    write_cell(d, rm, sx, sy, e - b, ti, pin, sums, slot_idx, slot_pos);
+    write_cell(d, rm, sx, sy, n, ti, pin, sums, slot_idx, slot_pos);
 }
 
-// One wave per crowded cell: lanes stride the segment keeping private top-8
-// lists and f64 partial sums, then a fixed shuffle tree reduces the sums and
-// eight wave-wide argmin rounds merge the lists.
-__global__ void __launch_bounds__(256) k_cells_big(const uint32_t* __restrict__ begin,
-                                                   const uint32_t* __restrict__ end,
+// The 8 smallest distinct tickets over the lanes' private lists (smallest
+// index per ticket), as (ticket << 32 | index), all-ones past the end.
+__device__ inline void wave_top8(uint32_t (&tb)[SLOTS], uint32_t (&ti)[SLOTS],
+                                 unsigned long long (&out)[SLOTS]) {
+#pragma unroll
+    for (int r = 0; r < SLOTS; ++r) {
+        unsigned long long mn = ((unsigned long long)tb[0] << 32) | ti[0];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            unsigned long long other = __shfl_xor(mn, o, 64);
+            mn = other < mn ? other : mn;
+        }
+        const uint32_t wt = (uint32_t)(mn >> 32);
+        out[r] = (wt == TMAX) ? ~0ull : mn;
+        if (wt != TMAX && tb[0] == wt) {
+#pragma unroll
+            for (int s = 0; s < SLOTS - 1; ++s) { tb[s] = tb[s + 1]; ti[s] = ti[s + 1]; }
+            tb[SLOTS - 1] = TMAX;
+            ti[SLOTS - 1] = TMAX;
+        }
+    }
+}
+
+// One wave per cell of BIG_CELL+1 .. BIG_CHUNK points: lanes stride the
+// segment with private top-8 lists and integer sums, then merge.
+__global__ void __launch_bounds__(256) k_cells_mid(const uint32_t* __restrict__ end_,
+                                                   const uint32_t* __restrict__ begin,
                                                    const uint32_t* __restrict__ sv,
                                                    const float2* __restrict__ pos0,
                                                    const float2* __restrict__ pos1,
                                                    float2* __restrict__ sorted_pos,
-                                                   double* __restrict__ sums,
+                                                   long long* __restrict__ sums,
                                                    int32_t* __restrict__ slot_idx,
                                                    float2* __restrict__ slot_pos,
-                                                   const uint32_t* __restrict__ big_list,
+                                                   const uint32_t* __restrict__ mid_cells,
                                                    const uint32_t* __restrict__ big_count,
-                                                   const uint32_t* __restrict__ f0s,
-                                                   const GDesc* __restrict__ desc, uint32_t G) {
+                                                   const uint32_t* __restrict__ blk_graph,
+                                                   const GDesc* __restrict__ desc) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t nbig = *big_count;
-    for (uint32_t w = wave; w < nbig; w += nwaves) {
-        const uint32_t f = big_list[w];
-        const GDesc& d = desc[find_graph(f0s, G, f)];
+    const uint32_t nmid = big_count[2];
+    for (uint32_t w = wave; w < nmid; w += nwaves) {
+        const uint32_t f = mid_cells[w];
+        const GDesc& d = desc[blk_graph[f / CELL_BLOCK]];
         const float2* __restrict__ pin = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1 : pos0;
-        const uint32_t b = begin[f], e = end[f];
+        const uint32_t b = begin[f], e = end_[f];
         uint32_t tb[SLOTS], ti[SLOTS];
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
-        double sx = 0.0, sy = 0.0;
+        long long sx = 0, sy = 0;
         for (uint32_t k = b + lane; k < e; k += 64) {
             uint32_t i = sv[k];
             float2 p = pin[i];
             sorted_pos[k] = p;
-            sx += (double)p.x;
-            sy += (double)p.y;
+            sx += fixed(p.x, d.fx_scale);
+            sy += fixed(p.y, d.fx_scale);
             top8_insert(tb, ti, ticket_bits(i - d.p0, d.seed), i);
         }
 #pragma unroll
@@ -173,49 +431,167 @@ __global__ void __launch_bounds__(256) k_cells_big(const uint32_t* __restrict__ 
             sx += __shfl_xor(sx, o, 64);
             sy += __shfl_xor(sy, o, 64);
         }
-        uint32_t out[SLOTS];
-#pragma unroll
-        for (int r = 0; r < SLOTS; ++r) {
-            unsigned long long key = ((unsigned long long)tb[0] << 32) | ti[0];
-            unsigned long long mn = key;
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                unsigned long long other = __shfl_xor(mn, o, 64);
-                mn = other < mn ? other : mn;
-            }
-            uint32_t wt = (uint32_t)(mn >> 32), wi = (uint32_t)mn;
-            out[r] = (wt == TMAX) ? TMAX : wi;
-            if (wt != TMAX && tb[0] == wt) {
-#pragma unroll
-                for (int s = 0; s < SLOTS - 1; ++s) { tb[s] = tb[s + 1]; ti[s] = ti[s + 1]; }
-                tb[SLOTS - 1] = TMAX;
-                ti[SLOTS - 1] = TMAX;
-            }
-        }
+        unsigned long long out[SLOTS];
+        wave_top8(tb, ti, out);
+        const uint32_t mc = f - d.f0;
+        const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
+        const size_t sb = ((size_t)d.f0 + rm) * SLOTS;
         if (lane == 0) {
-            const uint32_t mc = f - d.f0;
-            const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
-            write_cell(d, rm, sx, sy, e - b, out, pin, sums, slot_idx, slot_pos);
+            const size_t c3 = 3 * ((size_t)d.c0 + level_off(d.L - 1) + rm);
+            sums[c3] = sx;
+            sums[c3 + 1] = sy;
+            sums[c3 + 2] = (long long)(e - b);
+        }
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) {
+            if (lane == s) {
+                const bool ok = out[s] != ~0ull;
+                const uint32_t i = (uint32_t)out[s];
+                slot_idx[sb + s] = ok ? (int32_t)i : -1;
+                slot_pos[sb + s] = ok ? pin[i] : make_float2(0.f, 0.f);
+            }
         }
     }
 }
 
-// ---- 5. pyramid ---------------------------------------------------------------
+// One block per chunk of BIG_CHUNK points of a crowded cell: integer sums
+// added to the cell with 64-bit atomics (exact, so the order of the chunks is
+// immaterial) and the chunk's top-8 list for the merge.
+__global__ void __launch_bounds__(256) k_cells_chunk(const uint32_t* __restrict__ end_,
+                                                     const uint32_t* __restrict__ begin,
+                                                     const uint32_t* __restrict__ sv,
+                                                     const float2* __restrict__ pos0,
+                                                     const float2* __restrict__ pos1,
+                                                     float2* __restrict__ sorted_pos,
+                                                     long long* __restrict__ sums,
+                                                     const uint2* __restrict__ big_chunks,
+                                                     const uint32_t* __restrict__ big_count,
+                                                     unsigned long long* __restrict__ chunk_top,
+                                                     const uint32_t* __restrict__ blk_graph,
+                                                     const GDesc* __restrict__ desc) {
+    __shared__ long long s_sum[2][4];
+    __shared__ unsigned long long s_top[4][SLOTS];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t nchunks = big_count[1];
+    for (uint32_t e = blockIdx.x; e < nchunks; e += gridDim.x) {
+        const uint2 ce = big_chunks[e];
+        const uint32_t f = ce.x;
+        const GDesc& d = desc[blk_graph[f / CELL_BLOCK]];
+        const float2* __restrict__ pin = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1 : pos0;
+        const uint32_t cb = begin[f], ce_ = end_[f];
+        const uint32_t b = cb + ce.y * BIG_CHUNK, end = min(ce_, b + BIG_CHUNK);
+        uint32_t tb[SLOTS], ti[SLOTS];
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
+        long long sx = 0, sy = 0;
+        for (uint32_t k = b + t; k < end; k += 256) {
+            uint32_t i = sv[k];
+            float2 p = pin[i];
+            sorted_pos[k] = p;
+            sx += fixed(p.x, d.fx_scale);
+            sy += fixed(p.y, d.fx_scale);
+            top8_insert(tb, ti, ticket_bits(i - d.p0, d.seed), i);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sx += __shfl_xor(sx, o, 64);
+            sy += __shfl_xor(sy, o, 64);
+        }
+        unsigned long long out[SLOTS];
+        wave_top8(tb, ti, out);
+        if (lane == 0) {
+            s_sum[0][wave] = sx;
+            s_sum[1][wave] = sy;
+#pragma unroll
+            for (int s = 0; s < SLOTS; ++s) s_top[wave][s] = out[s];
+        }
+        __syncthreads();
+        if (t == 0) {
+            const uint32_t mc = f - d.f0;
+            const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
+            const size_t c3 = 3 * ((size_t)d.c0 + level_off(d.L - 1) + rm);
+            const long long bx = s_sum[0][0] + s_sum[0][1] + s_sum[0][2] + s_sum[0][3];
+            const long long by = s_sum[1][0] + s_sum[1][1] + s_sum[1][2] + s_sum[1][3];
+            atomicAdd(reinterpret_cast<unsigned long long*>(&sums[c3]), (unsigned long long)bx);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&sums[c3 + 1]), (unsigned long long)by);
+            uint32_t mb[SLOTS], mi[SLOTS];
+#pragma unroll
+            for (int s = 0; s < SLOTS; ++s) { mb[s] = TMAX; mi[s] = TMAX; }
+            for (int w = 0; w < 4; ++w)
+                for (int s = 0; s < SLOTS; ++s) {
+                    const unsigned long long v = s_top[w][s];
+                    if (v != ~0ull) top8_insert(mb, mi, (uint32_t)(v >> 32), (uint32_t)v);
+                }
+#pragma unroll
+            for (int s = 0; s < SLOTS; ++s)
+                chunk_top[(size_t)e * SLOTS + s] =
+                    mb[s] == TMAX ? ~0ull : (((unsigned long long)mb[s] << 32) | mi[s]);
+        }
+        __syncthreads();
+    }
+}
 
-__device__ inline float4 centroid(double sx, double sy, double n) {
-    if (n <= 0.0) return make_float4(0.f, 0.f, 0.f, 0.f);
-    return make_float4((float)(sx / n), (float)(sy / n), (float)n, 0.f);
+// One wave per crowded cell: the top-8 over its chunks' lists, then its slots.
+__global__ void __launch_bounds__(256) k_cells_merge(const uint4* __restrict__ big_cells,
+                                                     const uint32_t* __restrict__ big_count,
+                                                     const unsigned long long* __restrict__ chunk_top,
+                                                     const float2* __restrict__ pos0,
+                                                     const float2* __restrict__ pos1,
+                                                     int32_t* __restrict__ slot_idx,
+                                                     float2* __restrict__ slot_pos,
+                                                     const uint32_t* __restrict__ blk_graph,
+                                                     const GDesc* __restrict__ desc) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t ncells = big_count[0];
+    for (uint32_t w = wave; w < ncells; w += nwaves) {
+        const uint4 bc = big_cells[w];  // {cell, first chunk, chunks, row-major cell}
+        const GDesc& d = desc[blk_graph[bc.x / CELL_BLOCK]];
+        const float2* __restrict__ pin = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1 : pos0;
+        uint32_t tb[SLOTS], ti[SLOTS];
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
+        for (uint32_t e = bc.y + lane; e < bc.y + bc.z; e += 64)
+            for (int s = 0; s < SLOTS; ++s) {
+                const unsigned long long v = chunk_top[(size_t)e * SLOTS + s];
+                if (v != ~0ull) top8_insert(tb, ti, (uint32_t)(v >> 32), (uint32_t)v);
+            }
+        unsigned long long out[SLOTS];
+        wave_top8(tb, ti, out);
+        const size_t sb = ((size_t)d.f0 + bc.w) * SLOTS;
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) {
+            if (lane == s) {
+                const bool ok = out[s] != ~0ull;
+                const uint32_t i = (uint32_t)out[s];
+                slot_idx[sb + s] = ok ? (int32_t)i : -1;
+                slot_pos[sb + s] = ok ? pin[i] : make_float2(0.f, 0.f);
+            }
+        }
+    }
+}
+
+// ---- 3. pyramid ---------------------------------------------------------------
+
+__device__ inline float4 centroid(long long sx, long long sy, long long n, float inv) {
+    if (n <= 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+    const double dn = (double)n;
+    return make_float4((float)((double)sx * (double)inv / dn), (float)((double)sy * (double)inv / dn),
+                       (float)dn, 0.f);
 }
 
 // One block per tile of one graph's level `lin` (work item {graph slot, lin,
 // T | nout << 16, tile}); produces the `nout` coarser levels of that tile down
-// to one cell per tile.  Children are summed in a fixed order.
+// to one cell per tile.  At the finest level (write_in) an empty cell (end 0)
+// has zero sums.
 __global__ void __launch_bounds__(256) k_pyramid(const uint4* __restrict__ work,
                                                  const GDesc* __restrict__ desc,
-                                                 double* __restrict__ sums,
+                                                 const uint32_t* __restrict__ cell_end,
+                                                 long long* __restrict__ sums,
                                                  float4* __restrict__ cent, int write_in) {
-    __shared__ double s0[32 * 32 * 3];
-    __shared__ double s1[16 * 16 * 3];
+    __shared__ long long s0[32 * 32 * 3];
+    __shared__ long long s1[16 * 16 * 3];
     const uint4 wk = work[blockIdx.x];
     const GDesc& d = desc[wk.x];
     if (!(d.passes & (1u << P_MANYBODY))) return;
@@ -225,17 +601,23 @@ __global__ void __launch_bounds__(256) k_pyramid(const uint4* __restrict__ work,
     const int tx = (int)wk.w % tiles_x, ty = (int)wk.w / tiles_x;
     const size_t in0 = (size_t)d.c0 + level_off(lin);
     for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
-        int x = c % T, y = c / T;
-        size_t rm = in0 + (size_t)(ty * T + y) * gin + (tx * T + x);
-        double sx = sums[3 * rm], sy = sums[3 * rm + 1], n = sums[3 * rm + 2];
+        const int x = c % T, y = c / T;
+        const int X = tx * T + x, Y = ty * T + y;
+        const size_t rm = in0 + (size_t)Y * gin + X;
+        long long sx = 0, sy = 0, n = 0;
+        if (!write_in || cell_end[d.f0 + morton2((uint32_t)X, (uint32_t)Y)] != 0) {
+            sx = sums[3 * rm];
+            sy = sums[3 * rm + 1];
+            n = sums[3 * rm + 2];
+        }
         s0[3 * c] = sx;
         s0[3 * c + 1] = sy;
         s0[3 * c + 2] = n;
-        if (write_in) cent[rm] = centroid(sx, sy, n);
+        if (write_in) cent[rm] = centroid(sx, sy, n, d.fx_inv);
     }
     __syncthreads();
-    double* cur = s0;
-    double* nxt = s1;
+    long long* cur = s0;
+    long long* nxt = s1;
     int size = T;
     for (int lev = 1; lev <= nout; ++lev) {
         int half = size / 2;
@@ -245,7 +627,7 @@ __global__ void __launch_bounds__(256) k_pyramid(const uint4* __restrict__ work,
         for (int c = threadIdx.x; c < half * half; c += blockDim.x) {
             int x = c % half, y = c / half;
             int c00 = (2 * y) * size + 2 * x, c10 = c00 + 1, c01 = c00 + size, c11 = c01 + 1;
-            double v[3];
+            long long v[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q)
                 v[q] = (cur[3 * c00 + q] + cur[3 * c10 + q]) + (cur[3 * c01 + q] + cur[3 * c11 + q]);
@@ -256,26 +638,169 @@ __global__ void __launch_bounds__(256) k_pyramid(const uint4* __restrict__ work,
             sums[3 * rm] = v[0];
             sums[3 * rm + 1] = v[1];
             sums[3 * rm + 2] = v[2];
-            cent[rm] = centroid(v[0], v[1], v[2]);
+            cent[rm] = centroid(v[0], v[1], v[2], d.fx_inv);
         }
         __syncthreads();
-        double* t = cur;
+        long long* t = cur;
         cur = nxt;
         nxt = t;
         size = half;
     }
 }
 
-// ---- 6. forces ------------------------------------------------------------------
+// ---- 4. forces ------------------------------------------------------------------
 
 // Clamped inverse-distance falloff of force-level.frag:43-56 /
-// force-nearfield.frag:64-80 without the alpha*k*N factor:
-// returns c with contribution = c * d.
+// force-nearfield.frag:64-80 without the alpha*k*N factor, for l = |d|^2 > 0:
+// contribution = c * d with c = 1/l, or l^(-3/4) below unit distance; with
+// inv = 1/|d| both are inv * min(inv, sqrt(inv)).
 __device__ inline float falloff_coef(float l) {
-    float inv = __builtin_amdgcn_rsqf(l);            // 1/|d|
-    return (l < 1.f) ? inv * __builtin_sqrtf(inv) : inv * inv;
+    const float inv = __builtin_amdgcn_rsqf(l);
+    return inv * fminf(inv, __builtin_amdgcn_sqrtf(inv));
 }
 
+// One far-field term, zero where `use` is clear (outside the ring or the grid)
+// or the cell is empty.  A ring cell lies at least one cell width from the
+// query point, so with cells of width >= 2 (FAST) l >= 1 and the falloff is 1/l.
+template <bool FAST>
+__device__ inline void far_term(float4 c, float2 p, bool use, float& fx, float& fy) {
+    const float dx = p.x - c.x, dy = p.y - c.y;
+    const float l = dx * dx + dy * dy;
+    use = use && c.z > 0.f && l > 0.f;
+    const float coef = FAST ? __builtin_amdgcn_rcpf(l) : falloff_coef(l);
+    const float w = use ? c.z * coef : 0.f;
+    fx += w * dx;
+    fy += w * dy;
+}
+
+// Bit (6 r + c) set for the cells of a 6x6 block at offsets (c, r) outside the
+// 3x3 around the query cell, which sits at offset (2 + (q & 1)) in each axis.
+__device__ inline unsigned long long ring_mask(int qx, int qy) {
+    const unsigned long long excl = (0x7ull << ((qx & 1) + 1)) * (0x1041ull << (6 * ((qy & 1) + 1)));
+    return 0xFFFFFFFFFull & ~excl;
+}
+
+// Level 0: every cell of the 4x4 grid outside the 3x3 neighbourhood
+// (force-level.frag:70-79).  `cells` is the level (global or staged).  Rows
+// are not unrolled, so only one row's loads are in flight per lane.
+template <bool FAST>
+__device__ inline void far_level0(const float4* cells, float2 p, const GDesc& d, float& fx, float& fy) {
+    const int qx = cell_of(p.x, d.h[0], 4), qy = cell_of(p.y, d.h[0], 4);
+#pragma unroll 1
+    for (int cy = 0; cy < 4; ++cy) {
+        const bool far_row = abs(cy - qy) > 1;
+#pragma unroll
+        for (int cx = 0; cx < 4; ++cx)
+            far_term<FAST>(cells[cy * 4 + cx], p, far_row || abs(cx - qx) > 1, fx, fy);
+    }
+}
+
+// Level >= 1: the aligned 6x6 child block minus the 3x3 (force-level.frag:80-95).
+// Cell (cx, cy) is read at tile[(cy - y0) * w + (cx - x0)]: the level itself
+// with x0 = y0 = 0, w = g, or a staged window of it.  Cells outside the grid
+// are masked off and read at a clamped address.
+template <bool FAST>
+__device__ inline void far_level(const float4* tile, int x0, int y0, int w, int g, int qx, int qy,
+                                 float2 p, float& fx, float& fy) {
+    const int bx = (qx >> 1) * 2 - 2, by = (qy >> 1) * 2 - 2;
+    const unsigned long long m = ring_mask(qx, qy);
+    uint32_t cmask = 0x3Fu;
+    int col[6];
+#pragma unroll
+    for (int o = 0; o < 6; ++o) {
+        const int cx = bx + o;
+        if (cx < 0 || cx >= g) cmask &= ~(1u << o);
+        col[o] = min(max(cx, 0), g - 1) - x0;
+    }
+#pragma unroll 1
+    for (int j = 0; j < 6; ++j) {
+        const int cy = by + j;
+        const uint32_t rm = (cy >= 0 && cy < g) ? ((uint32_t)(m >> (6 * j)) & cmask) : 0u;
+        const int rb = (min(max(cy, 0), g - 1) - y0) * w;
+#pragma unroll
+        for (int ii = 0; ii < 6; ++ii) far_term<FAST>(tile[rb + col[ii]], p, (rm >> ii) & 1u, fx, fy);
+    }
+}
+
+// Near field at the finest level (force-nearfield.frag:101-163): the 3x3
+// cells' sampled slots, Horvitz-Thompson weighted.  Counts come from `fin`
+// (finest centroids, global or a staged window as in far_level).
+__device__ inline float2 near_field(const float4* fin, int x0, int y0, int w, const GDesc& d, int qx,
+                                    int qy, uint32_t i, float2 p, float2 r,
+                                    const int32_t* __restrict__ slot_idx,
+                                    const float2* __restrict__ slot_pos) {
+    const int gf = (int)d.gf;
+    float wx = 0.f, wy = 0.f;
+    for (int dj = -1; dj <= 1; ++dj) {
+        const int cy = qy + dj;
+        if (cy < 0 || cy >= gf) continue;
+        for (int di = -1; di <= 1; ++di) {
+            const int cx = qx + di;
+            if (cx < 0 || cx >= gf) continue;
+            float others = fin[(cy - y0) * w + (cx - x0)].z - ((di == 0 && dj == 0) ? 1.f : 0.f);
+            if (others <= 0.f) continue;
+            const size_t sb = ((size_t)d.f0 + cy * gf + cx) * SLOTS;
+            float sxp = 0.f, syp = 0.f, sampled = 0.f;
+#pragma unroll
+            for (int s = 0; s < SLOTS; ++s) {
+                const int j = slot_idx[sb + s];
+                if (j < 0) break;
+                if ((uint32_t)j == i) continue;
+                sampled += 1.f;
+                const float2 q = slot_pos[sb + s];
+                float dx = p.x - q.x, dy = p.y - q.y;
+                float l = dx * dx + dy * dy;
+                // coincident points push along the point's jitter (force-nearfield.frag:131-136)
+                const bool same = !(l > 0.f);
+                dx = same ? r.x : dx;
+                dy = same ? r.y : dy;
+                l = same ? r.x * r.x + r.y * r.y : l;
+                const float wgt = (l > 0.f) ? falloff_coef(l) : 0.f;
+                sxp += wgt * dx;
+                syp += wgt * dy;
+            }
+            if (sampled > 0.f) {
+                float hw = others / sampled;
+                wx += hw * sxp;
+                wy += hw * syp;
+            }
+        }
+    }
+    return make_float2(wx, wy);
+}
+
+// Scaling, the near-field step clamp (force-nearfield.frag:161) and the
+// integrate pass.
+__device__ inline void mb_finish(const GDesc& d, uint32_t i, float2 p, float2 r, float fx, float fy,
+                                 float2 wn, const uint8_t* __restrict__ flags, float2* __restrict__ pos0,
+                                 float2* __restrict__ pos1, float2* __restrict__ vout) {
+    float wx = wn.x * d.ak, wy = wn.y * d.ak;
+    wx += wx * r.x;
+    wy += wy * r.y;
+    const float maxstep = 2.0f * d.h[d.L - 1];
+    float sp = sqrtf(wx * wx + wy * wy);
+    if (sp > maxstep) {
+        float s = maxstep / sp;
+        wx *= s;
+        wy *= s;
+    }
+    const float2 v = make_float2(d.ak * fx + wx, d.ak * fy + wy);
+    if (vout) {
+        vout[i] = v;
+        return;
+    }
+    float2* __restrict__ pout = ((d.inbuf >> P_MANYBODY) & 1u) ? pos0 : pos1;
+    pout[i] = (flags[i] & F_PINNED) ? p : integrate(p, v, d.friction, d.S);
+}
+
+constexpr int MB_TILE = 1024;  // staged cells per level (16 KB)
+
+// One lane per point in cell order.  A block whose points all belong to one
+// running graph stages, level by level, the union of its points' 6x6 windows
+// into LDS (256 points in cell order cover a compact region, so the window is
+// small and every lane's reads hit it); other blocks, and windows above
+// MB_TILE cells, read the level from global memory.  Either way the same
+// values are summed in the same order.
 __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ sv,
                                                   const float2* __restrict__ sorted_pos,
                                                   const float4* __restrict__ cent,
@@ -288,125 +813,107 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
                                                   float2* __restrict__ pos0,
                                                   float2* __restrict__ pos1,
                                                   float2* __restrict__ vout) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= np) return;
-    const uint32_t i = sv[k];
-    const GDesc& d = desc[gid[i]];
-    if (!(d.passes & (1u << P_MANYBODY))) return;
-    const float2 p = sorted_pos[k];
-    const int L = (int)d.L;
-    float fx = 0.f, fy = 0.f;
-
-    // far field, level 0: every cell outside the 3x3 neighbourhood (force-level.frag:70-79)
-    {
-        const float4* __restrict__ lv = cent + d.c0;
-        int qx = cell_of(p.x, d.h[0], 4), qy = cell_of(p.y, d.h[0], 4);
-#pragma unroll
-        for (int cy = 0; cy < 4; ++cy) {
-#pragma unroll
-            for (int cx = 0; cx < 4; ++cx) {
-                if (abs(cx - qx) <= 1 && abs(cy - qy) <= 1) continue;
-                float4 c = lv[cy * 4 + cx];
-                if (c.z <= 0.f) continue;
-                float dx = p.x - c.x, dy = p.y - c.y;
-                float l = dx * dx + dy * dy;
-                if (l <= 0.f) continue;
-                float w = c.z * falloff_coef(l);
-                fx += w * dx;
-                fy += w * dy;
+    __shared__ float4 s_tile[MB_TILE];
+    __shared__ int s_box[4][4];
+    __shared__ uint32_t s_g;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t k = blockIdx.x * blockDim.x + t;
+    const bool valid = k < np;
+    const uint32_t kk = valid ? k : np - 1;
+    const uint32_t i = sv[kk];
+    const uint32_t g = gid[i];
+    const bool run = (desc[g].passes >> P_MANYBODY) & 1u;
+    if (t == 0) s_g = g;
+    __syncthreads();
+    const bool uniform = __syncthreads_and(g == s_g && run);
+    if (!uniform) {
+        if (!valid || !run) return;
+        const GDesc& d = desc[g];
+        const float2 p = sorted_pos[k];
+        const int L = (int)d.L;
+        float fx = 0.f, fy = 0.f;
+        int qx = 0, qy = 0;
+        if (d.h[L - 1] >= 2.f) {
+            far_level0<true>(cent + d.c0, p, d, fx, fy);
+            for (int lvl = 1; lvl < L; ++lvl) {
+                const int gl = 4 << lvl;
+                qx = cell_of(p.x, d.h[lvl], gl);
+                qy = cell_of(p.y, d.h[lvl], gl);
+                far_level<true>(cent + d.c0 + level_off(lvl), 0, 0, gl, gl, qx, qy, p, fx, fy);
+            }
+        } else {
+            far_level0<false>(cent + d.c0, p, d, fx, fy);
+            for (int lvl = 1; lvl < L; ++lvl) {
+                const int gl = 4 << lvl;
+                qx = cell_of(p.x, d.h[lvl], gl);
+                qy = cell_of(p.y, d.h[lvl], gl);
+                far_level<false>(cent + d.c0 + level_off(lvl), 0, 0, gl, gl, qx, qy, p, fx, fy);
             }
         }
-    }
-    // far field, levels 1..L-1: the aligned 6x6 child block minus the 3x3 (force-level.frag:80-95)
-    int qx = 0, qy = 0;
-    for (int lvl = 1; lvl < L; ++lvl) {
-        const int g = 4 << lvl;
-        const float4* __restrict__ lv = cent + d.c0 + level_off(lvl);
-        qx = cell_of(p.x, d.h[lvl], g);
-        qy = cell_of(p.y, d.h[lvl], g);
-        const int bx = (qx >> 1) * 2 - 2, by = (qy >> 1) * 2 - 2;
-#pragma unroll
-        for (int j = 0; j < 6; ++j) {
-            const int cy = by + j;
-            if (cy < 0 || cy >= g) continue;
-#pragma unroll
-            for (int ii = 0; ii < 6; ++ii) {
-                const int cx = bx + ii;
-                if (cx < 0 || cx >= g) continue;
-                if (abs(cx - qx) <= 1 && abs(cy - qy) <= 1) continue;
-                float4 c = lv[cy * g + cx];
-                if (c.z <= 0.f) continue;
-                float dx = p.x - c.x, dy = p.y - c.y;
-                float l = dx * dx + dy * dy;
-                if (l <= 0.f) continue;
-                float w = c.z * falloff_coef(l);
-                fx += w * dx;
-                fy += w * dy;
-            }
-        }
-    }
-    // near field at the finest level (force-nearfield.frag:101-163)
-    const int gf = (int)d.gf;
-    const float4* __restrict__ fin = cent + d.c0 + level_off(L - 1);
-    const float2 r = jitter[i];
-    float wx = 0.f, wy = 0.f;
-    for (int dj = -1; dj <= 1; ++dj) {
-        const int cy = qy + dj;
-        if (cy < 0 || cy >= gf) continue;
-        for (int di = -1; di <= 1; ++di) {
-            const int cx = qx + di;
-            if (cx < 0 || cx >= gf) continue;
-            const int cell = cy * gf + cx;
-            float others = fin[cell].z - ((di == 0 && dj == 0) ? 1.f : 0.f);
-            if (others <= 0.f) continue;
-            const size_t sb = ((size_t)d.f0 + cell) * SLOTS;
-            float sxp = 0.f, syp = 0.f, sampled = 0.f;
-#pragma unroll
-            for (int s = 0; s < SLOTS; ++s) {
-                const int j = slot_idx[sb + s];
-                if (j < 0) break;
-                if ((uint32_t)j == i) continue;
-                sampled += 1.f;
-                const float2 q = slot_pos[sb + s];
-                float dx = p.x - q.x, dy = p.y - q.y;
-                float l = dx * dx + dy * dy;
-                if (l <= 0.f) {
-                    dx = r.x;
-                    dy = r.y;
-                    l = dx * dx + dy * dy;
-                    if (l <= 0.f) continue;
-                }
-                float w = falloff_coef(l);
-                sxp += w * dx;
-                syp += w * dy;
-            }
-            if (sampled > 0.f) {
-                float hw = others / sampled;
-                wx += hw * sxp;
-                wy += hw * syp;
-            }
-        }
-    }
-    wx *= d.ak;
-    wy *= d.ak;
-    wx += wx * r.x;
-    wy += wy * r.y;
-    {
-        const float maxstep = 2.0f * d.h[L - 1];  // force-nearfield.frag:161
-        float sp = sqrtf(wx * wx + wy * wy);
-        if (sp > maxstep) {
-            float s = maxstep / sp;
-            wx *= s;
-            wy *= s;
-        }
-    }
-    const float2 v = make_float2(d.ak * fx + wx, d.ak * fy + wy);
-    if (vout) {
-        vout[i] = v;
+        const int gf = (int)d.gf;
+        const float2 r = jitter[i];
+        const float2 wn = near_field(cent + d.c0 + level_off(L - 1), 0, 0, gf, d, qx, qy, i, p, r, slot_idx,
+                                     slot_pos);
+        mb_finish(d, i, p, r, fx, fy, wn, flags, pos0, pos1, vout);
         return;
     }
-    float2* __restrict__ pout = ((d.inbuf >> P_MANYBODY) & 1u) ? pos0 : pos1;
-    pout[i] = (flags[i] & F_PINNED) ? p : integrate(p, v, d.friction, d.S);
+    const GDesc& d = desc[__builtin_amdgcn_readfirstlane(g)];
+    const float2 p = sorted_pos[kk];
+    const int L = (int)d.L;
+    float fx = 0.f, fy = 0.f;
+    const bool fast = d.h[L - 1] >= 2.f;  // uniform: one graph
+    if (t < 16) s_tile[t] = cent[d.c0 + t];
+    __syncthreads();
+    if (fast) far_level0<true>(s_tile, p, d, fx, fy);
+    else far_level0<false>(s_tile, p, d, fx, fy);
+    int qx = 0, qy = 0, x0 = 0, y0 = 0, w = 0;
+    bool staged = false;
+    for (int lvl = 1; lvl < L; ++lvl) {
+        const int gl = 4 << lvl;
+        const float4* __restrict__ lv = cent + d.c0 + level_off(lvl);
+        qx = cell_of(p.x, d.h[lvl], gl);
+        qy = cell_of(p.y, d.h[lvl], gl);
+        // union of the block's 6x6 windows
+        int bx0 = (qx >> 1) * 2 - 2, by0 = (qy >> 1) * 2 - 2, bx1 = bx0 + 5, by1 = by0 + 5;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            bx0 = min(bx0, __shfl_xor(bx0, o, 64));
+            by0 = min(by0, __shfl_xor(by0, o, 64));
+            bx1 = max(bx1, __shfl_xor(bx1, o, 64));
+            by1 = max(by1, __shfl_xor(by1, o, 64));
+        }
+        __syncthreads();  // the previous level's tile is no longer read
+        if (lane == 0) {
+            s_box[wave][0] = bx0;
+            s_box[wave][1] = by0;
+            s_box[wave][2] = bx1;
+            s_box[wave][3] = by1;
+        }
+        __syncthreads();
+        x0 = max(0, min(min(s_box[0][0], s_box[1][0]), min(s_box[2][0], s_box[3][0])));
+        y0 = max(0, min(min(s_box[0][1], s_box[1][1]), min(s_box[2][1], s_box[3][1])));
+        const int x1 = min(gl - 1, max(max(s_box[0][2], s_box[1][2]), max(s_box[2][2], s_box[3][2])));
+        const int y1 = min(gl - 1, max(max(s_box[0][3], s_box[1][3]), max(s_box[2][3], s_box[3][3])));
+        w = x1 - x0 + 1;
+        const int area = w * (y1 - y0 + 1);
+        staged = area <= MB_TILE;
+        if (staged) {
+            for (int c = t; c < area; c += 256) s_tile[c] = lv[(y0 + c / w) * gl + x0 + c % w];
+            __syncthreads();
+            if (fast) far_level<true>(s_tile, x0, y0, w, gl, qx, qy, p, fx, fy);
+            else far_level<false>(s_tile, x0, y0, w, gl, qx, qy, p, fx, fy);
+        } else {
+            if (fast) far_level<true>(lv, 0, 0, gl, gl, qx, qy, p, fx, fy);
+            else far_level<false>(lv, 0, 0, gl, gl, qx, qy, p, fx, fy);
+        }
+    }
+    if (!valid) return;
+    const int gf = (int)d.gf;
+    const float2 r = jitter[i];
+    const float4* fin = cent + d.c0 + level_off(L - 1);
+    const float2 wn = staged ? near_field(s_tile, x0, y0, w, d, qx, qy, i, p, r, slot_idx, slot_pos)
+                             : near_field(fin, 0, 0, gf, d, qx, qy, i, p, r, slot_idx, slot_pos);
+    mb_finish(d, i, p, r, fx, fy, wn, flags, pos0, pos1, vout);
 }
 
 }  // namespace
@@ -414,72 +921,102 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
 int mb_alloc(Ctx* c) {
     int rc = CG_OK;
     const size_t np = c->NP ? c->NP : 1;
-    const size_t F = c->F ? c->F : 1, C = c->C ? c->C : 1;
-    for (int b = 0; b < 2; ++b) {
-        c->keys[b] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
-        c->vals[b] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
+    const size_t F = c->F ? c->F : CELL_BLOCK, C = c->C ? c->C : 1;
+    for (int q = 0; q < 2; ++q) {
+        c->rk[q] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
+        c->rv[q] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
     }
+    // digits: the fewest passes of at most 8 bits covering every cell key
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < (unsigned long long)F) ++bits;
+    c->rb_passes = (bits + 7) / 8;
+    c->rb_width = (bits + c->rb_passes - 1) / c->rb_passes;
+    c->sorted = c->rv[c->rb_passes & 1];
+    c->n_rb = (uint32_t)((np + RB_TILE - 1) / RB_TILE);
+    const size_t n_hist = ((size_t)1 << c->rb_width) * c->n_rb;
+    c->hist = (uint32_t*)dalloc(c, n_hist * 4, &rc); if (rc) return rc;
+    c->n_scan_tiles = (uint32_t)((n_hist + SCAN_TILE - 1) / SCAN_TILE);
+    c->scan_part = (uint32_t*)dalloc(c, (size_t)c->n_scan_tiles * 4, &rc); if (rc) return rc;
     c->sorted_pos = (float2*)dalloc(c, np * 8, &rc); if (rc) return rc;
     c->cell_begin = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
     c->cell_end = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
     c->slot_idx = (int32_t*)dalloc(c, F * SLOTS * 4, &rc); if (rc) return rc;
     c->slot_pos = (float2*)dalloc(c, F * SLOTS * 8, &rc); if (rc) return rc;
-    c->big_list = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
+    // crowded cells: at most NP / (BIG_CELL + 1) of them, and their chunks
+    const size_t ncells = np / (BIG_CELL + 1) + 1, nchunks = np / BIG_CHUNK + ncells + 1;
+    c->mid_cells = (uint32_t*)dalloc(c, ncells * 4, &rc); if (rc) return rc;
+    c->big_cells = (uint4*)dalloc(c, ncells * 16, &rc); if (rc) return rc;
+    c->big_chunks = (uint2*)dalloc(c, nchunks * 8, &rc); if (rc) return rc;
+    c->chunk_top = (unsigned long long*)dalloc(c, nchunks * SLOTS * 8, &rc); if (rc) return rc;
     c->big_count = (uint32_t*)dalloc(c, 64, &rc); if (rc) return rc;
-    c->sums = (double*)dalloc(c, C * 3 * sizeof(double), &rc); if (rc) return rc;
+    c->sums = (long long*)dalloc(c, C * 3 * sizeof(long long), &rc); if (rc) return rc;
     c->cent = (float4*)dalloc(c, C * sizeof(float4), &rc); if (rc) return rc;
-    c->key_bits = 1;
-    while (c->key_bits < 32 && (1ull << c->key_bits) < (unsigned long long)F) ++c->key_bits;
-    // sized for full 32-bit keys: the collision pass sorts its cell keys with it too
-    size_t tmp = 0;
-    CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, c->keys[0], c->keys[1], c->vals[0],
-                                                 c->vals[1], (int)np, 0, 32, c->stream));
-    c->sort_tmp_bytes = tmp;
-    c->sort_tmp = dalloc(c, tmp ? tmp : 16, &rc);
+    // the first tick visits the points in index order
+    if (c->NP)
+        CG_HIP(c, hipMemcpyAsync(c->sorted, c->present, (size_t)c->NP * 4, hipMemcpyDeviceToDevice,
+                                 c->stream));
     return rc;
 }
 
-int mb_run(Ctx* c, bool keys_ready, float2* vout) {
-    if (c->NP == 0) return CG_OK;
-    int rc = CG_OK;
-    prof_begin(c, CG_STAGE_MB_SORT);
-    if (!keys_ready) {
-        rc = keys_run(c);
-        if (rc) return rc;
+// In-place exclusive scan of n (a multiple of 16) counts.
+static void scan_counts(Ctx* c, uint32_t* a, uint32_t n) {
+    if (n <= 4096) {  // one block, a few entries per lane
+        hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(1024), 0, c->stream, a, n);
+        return;
     }
+    const uint32_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(256), 0, c->stream, a, n, c->scan_part);
+    hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(1024), 0, c->stream, c->scan_part, tiles);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(256), 0, c->stream, a, n, c->scan_part, a);
+}
+
+int mb_run(Ctx* c, float2* vout) {
+    if (c->NP == 0) return CG_OK;
     const uint32_t np = c->NP, F = c->F;
-    size_t tmp = c->sort_tmp_bytes;
-    CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, tmp, c->keys[0], c->keys[1],
-                                                 c->vals[0], c->vals[1], (int)np, 0, c->key_bits,
-                                                 c->stream));
+    prof_begin(c, CG_STAGE_MB_SORT);
+    const uint32_t nb = c->n_rb, n_hist = (1u << c->rb_width) * nb;
+    const int W = c->rb_width, P = c->rb_passes;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->cell_end, F, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->big_count, 3u, 0u);
+    hipLaunchKernelGGL(k_bin_keys, dim3(nb), dim3(256), 0, c->stream, c->sorted, c->pos[0], c->pos[1],
+                       c->gid, c->desc, np, c->rk[0], c->rv[0], W, nb, c->hist);
+    for (int p = 0; p < P; ++p) {
+        if (p)
+            hipLaunchKernelGGL(k_radix_up, dim3(nb), dim3(256), 0, c->stream, c->rk[p & 1], np, p * W, W,
+                               nb, c->hist);
+        scan_counts(c, c->hist, n_hist);
+        hipLaunchKernelGGL(k_radix_down, dim3(nb), dim3(256), 0, c->stream, c->rk[p & 1], c->rv[p & 1], np,
+                           p * W, W, nb, c->hist, c->rk[(p + 1) & 1], c->rv[(p + 1) & 1]);
+    }
+    hipLaunchKernelGGL(k_bin_bounds, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->rk[P & 1], np,
+                       c->cell_begin, c->cell_end);
     prof_end(c, CG_STAGE_MB_SORT);
     prof_begin(c, CG_STAGE_MB_CELLS);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream,
-                       c->cell_begin, F, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream,
-                       c->cell_end, F, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->big_count, 1u, 0u);
-    hipLaunchKernelGGL(k_mark_bounds, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream,
-                       c->keys[1], np, c->cell_begin, c->cell_end);
-    hipLaunchKernelGGL(k_cells, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream,
-                       c->cell_begin, c->cell_end, c->vals[1], c->pos[0], c->pos[1], c->sorted_pos,
-                       c->sums, c->slot_idx, c->slot_pos, c->big_list, c->big_count, c->f0s, c->desc,
-                       c->G, F);
-    hipLaunchKernelGGL(k_cells_big, dim3(256), dim3(256), 0, c->stream, c->cell_begin, c->cell_end,
-                       c->vals[1], c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->slot_idx,
-                       c->slot_pos, c->big_list, c->big_count, c->f0s, c->desc, c->G);
+    hipLaunchKernelGGL(k_cells, dim3(F / CELL_BLOCK), dim3(CELL_BLOCK), 0, c->stream, c->cell_end,
+                       c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums,
+                       c->slot_idx, c->slot_pos, c->mid_cells, c->big_cells, c->big_chunks,
+                       c->big_count, c->blk_graph, c->desc);
+    hipLaunchKernelGGL(k_cells_mid, dim3(MID_BLOCKS), dim3(256), 0, c->stream, c->cell_end, c->cell_begin,
+                       c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->slot_idx,
+                       c->slot_pos, c->mid_cells, c->big_count, c->blk_graph, c->desc);
+    hipLaunchKernelGGL(k_cells_chunk, dim3(1024), dim3(256), 0, c->stream, c->cell_end, c->cell_begin,
+                       c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->big_chunks,
+                       c->big_count, c->chunk_top, c->blk_graph, c->desc);
+    hipLaunchKernelGGL(k_cells_merge, dim3(64), dim3(256), 0, c->stream, c->big_cells, c->big_count,
+                       c->chunk_top, c->pos[0], c->pos[1], c->slot_idx, c->slot_pos, c->blk_graph,
+                       c->desc);
     prof_end(c, CG_STAGE_MB_CELLS);
     prof_begin(c, CG_STAGE_MB_PYRAMID);
     // pyramid: finest -> tiles of up to 32x32 -> remaining top levels
     if (c->n_pyr[0])
         hipLaunchKernelGGL(k_pyramid, dim3(c->n_pyr[0]), dim3(256), 0, c->stream, c->pyr[0], c->desc,
-                           c->sums, c->cent, 1);
+                           c->cell_end, c->sums, c->cent, 1);
     if (c->n_pyr[1])
         hipLaunchKernelGGL(k_pyramid, dim3(c->n_pyr[1]), dim3(256), 0, c->stream, c->pyr[1], c->desc,
-                           c->sums, c->cent, 0);
+                           c->cell_end, c->sums, c->cent, 0);
     prof_end(c, CG_STAGE_MB_PYRAMID);
     prof_begin(c, CG_STAGE_MB_FORCE);
-    hipLaunchKernelGGL(k_mb_force, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->vals[1],
+    hipLaunchKernelGGL(k_mb_force, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->sorted,
                        c->sorted_pos, c->cent, c->slot_idx, c->slot_pos, c->jitter, c->flags, c->gid,
                        c->desc, np, c->pos[0], c->pos[1], vout);
     prof_end(c, CG_STAGE_MB_FORCE);

@@ -1,6 +1,5 @@
 // k_pointwise.hip — per-point forces (gravity, center, mouse) fused with the
-// integrate pass, the per-graph centroid reduction, and the finest-cell key
-// pass that feeds the many-body sort.
+// integrate pass, and the per-graph centroid reduction of the center force.
 //
 // Reference: ForceGravity/force-gravity.frag:24-40, ForceCenter/
 // calculate-centermass.vert:11-30 + force-center.frag:22-38, ForceMouse/
@@ -54,15 +53,8 @@ __device__ inline float2 pw_force(float2 p, const GDesc& d, const double* __rest
     return v;
 }
 
-__device__ inline uint32_t finest_key(const GDesc& d, float2 q) {
-    const float h = d.h[d.L - 1];
-    uint32_t cx = (uint32_t)cell_of(q.x, h, (int)d.gf);
-    uint32_t cy = (uint32_t)cell_of(q.y, h, (int)d.gf);
-    return d.f0 + morton2(cx, cy);
-}
-
-// One lane per present point: force -> integrate -> (optional) finest cell key.
-// vout: evaluation mode, velocity only.
+// One lane per present point: force -> integrate.  vout: evaluation mode,
+// velocity only.
 template <int PASS>
 __global__ void __launch_bounds__(256) k_pointwise(float2* __restrict__ pos0,
                                                    float2* __restrict__ pos1,
@@ -71,9 +63,7 @@ __global__ void __launch_bounds__(256) k_pointwise(float2* __restrict__ pos0,
                                                    const uint32_t* __restrict__ gid,
                                                    const uint8_t* __restrict__ flags,
                                                    const GDesc* __restrict__ desc, uint32_t np,
-                                                   const double* __restrict__ csum,
-                                                   uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals) {
+                                                   const double* __restrict__ csum) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np) return;
     const uint32_t i = present[k];
@@ -85,34 +75,11 @@ __global__ void __launch_bounds__(256) k_pointwise(float2* __restrict__ pos0,
     float2* __restrict__ pout = b ? pos0 : pos1;
     const float2 p = pin[i];
     const float2 v = pw_force<PASS>(p, d, csum, g);
-    float2 q = p;
     if (vout) {
         vout[i] = v;
-    } else {
-        if (!(flags[i] & F_PINNED)) q = integrate(p, v, d.friction, d.S);
-        pout[i] = q;
+        return;
     }
-    if (keys) {
-        keys[k] = finest_key(d, q);
-        vals[k] = i;
-    }
-}
-
-// Key pass for graphs whose many-body input is `inbuf` of the many-body pass.
-__global__ void __launch_bounds__(256) k_keys(const float2* __restrict__ pos0,
-                                              const float2* __restrict__ pos1,
-                                              const uint32_t* __restrict__ present,
-                                              const uint32_t* __restrict__ gid,
-                                              const GDesc* __restrict__ desc, uint32_t np,
-                                              uint32_t* __restrict__ keys,
-                                              uint32_t* __restrict__ vals) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= np) return;
-    const uint32_t i = present[k];
-    const GDesc& d = desc[gid[i]];
-    const float2 q = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1[i] : pos0[i];
-    keys[k] = finest_key(d, q);
-    vals[k] = i;
+    pout[i] = (flags[i] & F_PINNED) ? p : integrate(p, v, d.friction, d.S);
 }
 
 // Centroid of each graph: one block per chunk of CENTER_CHUNK present points of
@@ -177,10 +144,8 @@ int center_run(Ctx* c) {
     return CG_OK;
 }
 
-int pw_run(Ctx* c, int pass, bool write_keys, float2* vout) {
+int pw_run(Ctx* c, int pass, float2* vout) {
     if (c->NP == 0) return CG_OK;
-    uint32_t* keys = write_keys ? c->keys[0] : nullptr;
-    uint32_t* vals = write_keys ? c->vals[0] : nullptr;
     const unsigned nb = blocks_for(c->NP, 256);
     if (pass == P_CENTER) {
         int rc = center_run(c);
@@ -188,7 +153,7 @@ int pw_run(Ctx* c, int pass, bool write_keys, float2* vout) {
     }
 #define CG_PW(P)                                                                                   \
     hipLaunchKernelGGL(k_pointwise<P>, dim3(nb), dim3(256), 0, c->stream, c->pos[0], c->pos[1], vout, \
-                       c->present, c->gid, c->flags, c->desc, c->NP, c->center_sum, keys, vals)
+                       c->present, c->gid, c->flags, c->desc, c->NP, c->center_sum)
     switch (pass) {
         case P_GRAVITY: CG_PW(P_GRAVITY); break;
         case P_CENTER: CG_PW(P_CENTER); break;
@@ -196,14 +161,6 @@ int pw_run(Ctx* c, int pass, bool write_keys, float2* vout) {
         default: return fail(c, CG_ERR_INVALID, "pointwise: unknown pass");
     }
 #undef CG_PW
-    CG_HIP(c, hipGetLastError());
-    return CG_OK;
-}
-
-int keys_run(Ctx* c) {
-    if (c->NP == 0) return CG_OK;
-    hipLaunchKernelGGL(k_keys, dim3(blocks_for(c->NP, 256)), dim3(256), 0, c->stream, c->pos[0],
-                       c->pos[1], c->present, c->gid, c->desc, c->NP, c->keys[0], c->vals[0]);
     CG_HIP(c, hipGetLastError());
     return CG_OK;
 }

@@ -78,18 +78,23 @@ def test_pointwise_forces():
         assert (got.reshape(-1, 2)[~pres] == 0).all()
 
 
-@pytest.mark.parametrize("case", ["cfg0", "clustered", "tiny"])
+@pytest.mark.parametrize("case", ["cfg0", "clustered", "crowded", "tiny"])
 def test_manybody_slots_bit_exact_and_forces(case):
     rng = np.random.default_rng(0)
     if case == "cfg0":
         S, n = 4096.0, 10_000
         xy = datagen.uniform_square(n, S, 42)
     elif case == "clustered":
-        # a dense Gaussian clump: finest cells far above 64 points (wave-per-cell path)
+        # a dense Gaussian clump: finest cells far above 64 points (crowded-cell path)
         S, n = 4096.0, 20_000
         xy = np.clip(rng.normal(2048, 40, 2 * n), 0, S).astype(np.float32)
         xy[: 2 * 500] = rng.uniform(0, S, 1000)
         xy[2 * 7: 2 * 9] = np.nan
+    elif case == "crowded":
+        # 9000 points inside one finest cell: several 2048-point chunks per cell
+        S, n = 4096.0, 12_000
+        xy = rng.uniform(0, S, 2 * n).astype(np.float32)
+        xy[: 2 * 9000] = rng.uniform(1001.0, 1003.0, 2 * 9000).astype(np.float32)
     else:
         S, n = 1024.0, 7
         xy = rng.uniform(0, S, 2 * n).astype(np.float32)
