@@ -106,6 +106,7 @@ _SIGNATURES = {
     "cg_graph_get_positions": (C.c_int, [_CTX, C.c_uint32, _F, C.c_uint32]),
     "cg_step_graphs": (C.c_int, [_CTX, _U, C.c_uint32, C.c_uint32]),
     "cg_sync": (C.c_int, [_CTX]),
+    "cg_set_capture": (C.c_int, [_CTX, C.c_int32]),
 }
 
 STAGES = ["mouse", "gravity", "center", "mb_sort", "mb_cells", "mb_pyramid", "mb_force", "link_in",
@@ -372,3 +373,6 @@ class Context:
 
     def sync(self) -> None:
         self._check(self._l.cg_sync(self._ctx))
+
+    def set_capture(self, enable: bool) -> None:
+        self._check(self._l.cg_set_capture(self._ctx, 1 if enable else 0))

@@ -90,6 +90,22 @@ void prof_end(Ctx* c, int stage) {
 
 namespace {
 
+// Copies the descriptor table of the current tick (ring entry *ctr) to `cur`
+// and advances the counter.  One block, so the counter is read before it moves.
+__global__ void __launch_bounds__(1024) k_desc_advance(const uint32_t* __restrict__ ring,
+                                                       uint32_t* __restrict__ ctr, uint32_t words,
+                                                       uint32_t* __restrict__ cur) {
+    __shared__ uint32_t s_t;
+    if (threadIdx.x == 0) s_t = *ctr;
+    __syncthreads();
+    const uint32_t* __restrict__ src = ring + (size_t)s_t * words;
+    for (uint32_t w = threadIdx.x; w < words; w += 1024) cur[w] = src[w];
+    __syncthreads();
+    if (threadIdx.x == 0) *ctr = s_t + 1;
+}
+
+__global__ void k_set_word(uint32_t* __restrict__ a, uint32_t v) { *a = v; }
+
 constexpr uint64_t GRAPH_SEED = 0x5eed5eedULL;   // every graph's generator starts here
 constexpr uint32_t DESC_ENTRIES = 1u << 16;      // descriptors per upload (ticks x graphs)
 constexpr uint32_t DESC_MAX_TICKS = 256;
@@ -178,8 +194,15 @@ void release(Ctx* c, T*& p) {
     p = nullptr;
 }
 
+void drop_graphs(Ctx* c) {
+    for (auto& e : c->graphs_exec) (void)hipGraphExecDestroy(e.second);
+    c->graphs_exec.clear();
+}
+
 // Frees the packed arrays except the position buffers and the link schedules.
 void free_packed(Ctx* c) {
+    drop_graphs(c);
+    release(c, c->desc_cur); release(c, c->tick_ctr);
     release(c, c->flags); release(c, c->present); release(c, c->gid); release(c, c->jitter);
     release(c, c->size); release(c, c->desc_base); c->desc = nullptr;
     release(c, c->f0s); release(c, c->colc0s); release(c, c->blk_graph);
@@ -190,15 +213,16 @@ void free_packed(Ctx* c) {
     }
     for (int q = 0; q < 2; ++q) { release(c, c->rk[q]); release(c, c->rv[q]); }
     c->sorted = nullptr;
-    release(c, c->hist); release(c, c->cell_begin); release(c, c->cell_end); release(c, c->scan_part);
-    c->n_scan_tiles = 0;
+    release(c, c->mb_rs.hist); release(c, c->mb_rs.part); c->mb_rs = RadixSort();
+    release(c, c->col_rs.hist); release(c, c->col_rs.part); c->col_rs = RadixSort();
+    release(c, c->cell_begin); release(c, c->cell_end);
     release(c, c->sums); release(c, c->cent); release(c, c->sorted_pos);
     release(c, c->slot_idx); release(c, c->slot_pos); release(c, c->mid_cells); release(c, c->big_cells); release(c, c->big_chunks); release(c, c->chunk_top);
     release(c, c->big_count);
     release(c, c->center_part); release(c, c->center_sum); release(c, c->center_chunks);
     release(c, c->center_first);
     c->n_center_chunks = 0;
-    release(c, c->cpresent); release(c, c->col_sort_tmp);
+    release(c, c->cpresent);
     release(c, c->col_begin); release(c, c->col_end); release(c, c->col_cell);
 }
 
@@ -368,6 +392,8 @@ int pack(Ctx* c) {
     // descriptor tables (device and pinned staging)
     c->desc_ticks = std::max<uint32_t>(1, std::min<uint32_t>(DESC_MAX_TICKS, DESC_ENTRIES / (uint32_t)gg));
     c->desc_base = (GDesc*)dalloc(c, (size_t)c->desc_ticks * gg * sizeof(GDesc), &rc);
+    c->desc_cur = (GDesc*)dalloc(c, gg * sizeof(GDesc), &rc);
+    c->tick_ctr = (uint32_t*)dalloc(c, 64, &rc);
     if (rc) return rc;
     if (c->h_desc_pinned) (void)hipHostFree(c->h_desc_pinned);
     c->h_desc_pinned = nullptr;
@@ -375,6 +401,7 @@ int pack(Ctx* c) {
     c->h_desc = st;
     c->desc = c->desc_base;
     rc = upload(c, c->desc_base, st); if (rc) return rc;
+    rc = upload(c, c->desc_cur, st); if (rc) return rc;
 
     rc = mb_alloc(c); if (rc) return rc;
     rc = col_alloc(c); if (rc) return rc;
@@ -486,6 +513,48 @@ int launch_tick(Ctx* c, const TickPlan& tp) {
     return CG_OK;
 }
 
+// One tick as launched (or captured): descriptor advance, then the passes.
+int launch_tick_seq(Ctx* c, const TickPlan& tp, uint32_t n) {
+    static_assert(sizeof(GDesc) % 4 == 0, "descriptor words");
+    const uint32_t words = c->G * (uint32_t)(sizeof(GDesc) / 4);
+    for (uint32_t q = 0; q < n; ++q) {
+        hipLaunchKernelGGL(k_desc_advance, dim3(1), dim3(1024), 0, c->stream, (const uint32_t*)c->desc_base,
+                           c->tick_ctr, words, (uint32_t*)c->desc_cur);
+        int rc = launch_tick(c, tp);
+        if (rc) return rc;
+    }
+    return CG_OK;
+}
+
+// The executable graph of n ticks of plan tp, captured on first use.  Every
+// tick kernel reads desc_cur and the device tick counter, so one graph serves
+// any tick of any chunk while the packing stands.
+int tick_graph(Ctx* c, const TickPlan& tp, uint32_t n, hipGraphExec_t* out) {
+    for (auto& e : c->graphs_exec)
+        if (e.first.first == tp.any && e.first.second == n) {
+            *out = e.second;
+            return CG_OK;
+        }
+    CG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_tick_seq(c, tp, n);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
+    c->graphs_exec.push_back({{tp.any, n}, x});
+    *out = x;
+    return CG_OK;
+}
+
+constexpr uint32_t GRAPH_TICKS = 8;  // ticks per captured graph (plus a one-tick graph for the rest)
+
 // Advances the graphs in `ids` by nticks.  The graph `sel` (or none) takes its
 // near-field seeds from `seeds` and reports alpha after each tick in `trace`.
 int run_ticks(Ctx* c, const std::vector<uint32_t>& ids, uint32_t nticks, int64_t sel,
@@ -545,13 +614,31 @@ int run_ticks(Ctx* c, const std::vector<uint32_t>& ids, uint32_t nticks, int64_t
                                  hipMemcpyHostToDevice, c->stream));
         CG_HIP(c, hipEventRecord(c->ev_desc, c->stream));
         c->desc_pending = true;
-        for (uint32_t tt = 0; tt < T; ++tt) {
-            c->desc = c->desc_base + (size_t)tt * G;
-            rc = launch_tick(c, plan[tt]);
+        hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, c->stream, c->tick_ctr, 0u);
+        c->desc = c->desc_cur;
+        // stage timing brackets single launches, so it runs uncaptured
+        const bool cap = c->capture && c->prof_mask == 0;
+        for (uint32_t tt = 0; tt < T;) {
+            uint32_t r = 1;
+            while (tt + r < T && plan[tt + r].any == plan[tt].any) ++r;
+            for (uint32_t done = 0; done < r;) {
+                const uint32_t n = (cap && r - done >= GRAPH_TICKS) ? GRAPH_TICKS : 1u;
+                if (cap) {
+                    hipGraphExec_t x = nullptr;
+                    rc = tick_graph(c, plan[tt], n, &x);
+                    if (rc) break;
+                    CG_HIP(c, hipGraphLaunch(x, c->stream));
+                } else {
+                    rc = launch_tick_seq(c, plan[tt], n);
+                    if (rc) break;
+                }
+                done += n;
+            }
             if (rc) {
                 c->desc = c->desc_base;
                 return rc;
             }
+            tt += r;
         }
     }
     c->desc = c->desc_base;
@@ -745,6 +832,7 @@ int cg_destroy(cg_ctx* x) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    drop_graphs(c);
     for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
@@ -1049,6 +1137,12 @@ int cg_debug_link_stats(cg_ctx* x, uint32_t* out8) {
     CG_HIP(c, hipMemcpyAsync(out8, c->link_stats, 32, hipMemcpyDeviceToHost, c->stream));
     CG_HIP(c, hipMemsetAsync(c->link_stats, 0, 64, c->stream));
     CG_HIP(c, hipStreamSynchronize(c->stream));
+    return CG_OK;
+}
+
+int cg_set_capture(cg_ctx* x, int32_t enable) {
+    if (!x) return CG_ERR_INVALID;
+    x->c.capture = enable != 0;
     return CG_OK;
 }
 

@@ -130,6 +130,15 @@ struct LinkDir {
     size_t warm_len = 0;
 };
 
+// A radix sort of (key, value) pairs over ping-pong buffers (k_manybody.hip).
+struct RadixSort {
+    int passes = 1, width = 8;      // digits of `width` bits cover every key
+    uint32_t n = 0, nb = 0;         // pairs, blocks of RB_TILE pairs
+    uint32_t n_hist = 0;            // (1 << width) * nb, a multiple of 16
+    uint32_t* hist = nullptr;       // [n_hist] per-pass [digit][block] counts
+    uint32_t* part = nullptr;       // per-tile totals of the histogram scan
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -167,13 +176,9 @@ struct Ctx {
     uint32_t* rk[2] = {nullptr, nullptr};  // [NP] cell keys, ping-pong over the passes
     uint32_t* rv[2] = {nullptr, nullptr};  // [NP] point indices, ping-pong
     uint32_t* sorted = nullptr;     // rv[passes & 1]: points grouped by cell, next tick's visit order
-    uint32_t* hist = nullptr;       // [radix * n_rb] per-pass [digit][block] counts, scanned in place
-    uint32_t n_rb = 0;              // radix-sort blocks
-    int rb_passes = 1, rb_width = 8;
+    RadixSort mb_rs;
     uint32_t* cell_begin = nullptr; // [F] first sorted position of each finest cell
     uint32_t* cell_end = nullptr;   // [F] one past its last (0 = empty)
-    uint32_t* scan_part = nullptr;  // [n_scan_tiles] per-tile totals of the histogram scan
-    uint32_t n_scan_tiles = 0;
     long long* sums = nullptr;      // [3 C] fixed-point sum x, sum y, count per level cell
     float4* cent = nullptr;         // [C]
     float2* sorted_pos = nullptr;   // [NP]
@@ -197,12 +202,10 @@ struct Ctx {
     uint32_t* cpresent = nullptr;   // [NPC]
     uint32_t* col_keys[2] = {nullptr, nullptr};  // [4 NPC]
     uint32_t* col_vals[2] = {nullptr, nullptr};
-    void* col_sort_tmp = nullptr;
-    size_t col_sort_bytes = 0;
+    RadixSort col_rs;
     uint32_t* col_begin = nullptr;  // [CG]
     uint32_t* col_end = nullptr;
     float4* col_cell = nullptr;     // [CG]
-    int col_bits = 1;
 
     // links
     LinkDir dir[2];
@@ -224,6 +227,14 @@ struct Ctx {
     uint32_t desc_ticks = 0;
     hipEvent_t ev_desc = nullptr;
     bool desc_pending = false;
+    // tick kernels read desc_cur, which k_desc_advance fills from the ring at
+    // desc_base (entry *tick_ctr) at the start of each tick
+    GDesc* desc_cur = nullptr;
+    uint32_t* tick_ctr = nullptr;
+
+    // captured tick sequences: {passes run, ticks} -> executable graph
+    bool capture = true;
+    std::vector<std::pair<std::pair<uint32_t, uint32_t>, hipGraphExec_t>> graphs_exec;
 };
 
 void prof_begin(Ctx* c, int stage);
@@ -247,7 +258,10 @@ int pw_run(Ctx* c, int pass, float2* vout);                       // k_pointwise
 int center_run(Ctx* c);
 void center_chunks_host(const std::vector<uint32_t>& k0, const std::vector<uint32_t>& np,
                         std::vector<uint4>& chunks, std::vector<uint32_t>& first);
-int mb_alloc(Ctx* c);                                             // k_manybody.hip
+int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range);  // k_manybody.hip
+void radix_run(Ctx* c, const RadixSort& rs, uint32_t* const (&k)[2], uint32_t* const (&v)[2],
+               bool hist0_ready);
+int mb_alloc(Ctx* c);
 int mb_run(Ctx* c, float2* vout);
 void pyramid_work_host(uint32_t slot, uint32_t gf, uint32_t L, std::vector<uint4>& w0,
                        std::vector<uint4>& w1);

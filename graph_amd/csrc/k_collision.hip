@@ -1,16 +1,15 @@
 // k_collision.hip — spatial-hash collision over four offset grids.
 //
 // Per offset o in {(0,0), (.5,0), (0,.5), (.5,.5)}: integer row-major cell key
-// per present point, stable radix sort of (key, index), segment bounds, and a
-// fixed-order per-cell reduction of (x, y, size, count).  Sums are kept
+// per present point, one radix sort of (key, index) over all four grids and all
+// graphs, segment bounds, and a fixed-order per-cell reduction of
+// (x, y, size, count).  Sums are kept
 // relative to the cell's origin so the force pass can remove the point's own
 // contribution without f32 cancellation.  The force pass sweeps the 3x3 cell
 // aggregates of all four grids per point and integrates.
 //
 // Reference: ForceCollision/index.ts:61-164 (grid geometry, sizes),
 // :264-345 (run); build-grid.vert:28-62; force-collision-spatial.frag:103-230.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <cmath>
 
@@ -71,7 +70,7 @@ __global__ void k_col_cells(const uint32_t* __restrict__ begin, const uint32_t* 
     const float2* __restrict__ pin = ((d.inbuf >> P_COLLISION) & 1u) ? pos1 : pos0;
     const uint32_t GG = d.col_G * d.col_G;
     const uint32_t local = (c - d.col_c0) % GG;
-    const uint32_t b = begin[c], e = end[c];
+    const uint32_t e = end[c], b = e ? begin[c] : 0u;  // end 0: empty (begin not written)
     float x0 = (float)(local % d.col_G) * d.col_h, y0 = (float)(local / d.col_G) * d.col_h;
     double sx = 0.0, sy = 0.0, ss = 0.0;
     for (uint32_t k = b; k < e; ++k) {
@@ -213,17 +212,13 @@ int col_run(Ctx* c, float2* vout) {
     const uint32_t nk = 4 * npc;
     hipLaunchKernelGGL(k_col_keys, dim3(blocks_for(nk, 256)), dim3(256), 0, c->stream, c->pos[0],
                        c->pos[1], c->cpresent, npc, c->gid, c->desc, c->col_keys[0], c->col_vals[0]);
-    size_t tmp = c->col_sort_bytes;
-    CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->col_sort_tmp, tmp, c->col_keys[0], c->col_keys[1],
-                                                 c->col_vals[0], c->col_vals[1], (int)nk, 0,
-                                                 c->col_bits, c->stream));
-    hipLaunchKernelGGL(k_fill32, dim3(blocks_for(CG, 256)), dim3(256), 0, c->stream, c->col_begin, CG,
-                       0u);
+    radix_run(c, c->col_rs, c->col_keys, c->col_vals, false);
+    const int fin = c->col_rs.passes & 1;
     hipLaunchKernelGGL(k_fill32, dim3(blocks_for(CG, 256)), dim3(256), 0, c->stream, c->col_end, CG, 0u);
-    hipLaunchKernelGGL(k_col_bounds, dim3(blocks_for(nk, 256)), dim3(256), 0, c->stream, c->col_keys[1],
+    hipLaunchKernelGGL(k_col_bounds, dim3(blocks_for(nk, 256)), dim3(256), 0, c->stream, c->col_keys[fin],
                        nk, c->col_begin, c->col_end);
     hipLaunchKernelGGL(k_col_cells, dim3(blocks_for(CG, 256)), dim3(256), 0, c->stream, c->col_begin,
-                       c->col_end, c->col_vals[1], c->pos[0], c->pos[1], c->size, c->colc0s, c->desc,
+                       c->col_end, c->col_vals[fin], c->pos[0], c->pos[1], c->size, c->colc0s, c->desc,
                        c->G, CG, c->col_cell);
     hipLaunchKernelGGL(k_col_force, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->col_cell,
                        c->pos[0], c->pos[1], c->cpresent, npc, c->gid, c->desc, c->size, c->flags, vout);
@@ -243,15 +238,7 @@ int col_alloc(Ctx* c) {
     c->col_begin = (uint32_t*)dalloc(c, (size_t)c->CG * 4, &rc); if (rc) return rc;
     c->col_end = (uint32_t*)dalloc(c, (size_t)c->CG * 4, &rc); if (rc) return rc;
     c->col_cell = (float4*)dalloc(c, (size_t)c->CG * 16, &rc); if (rc) return rc;
-    c->col_bits = 1;
-    while (c->col_bits < 32 && (1ull << c->col_bits) < (unsigned long long)c->CG) ++c->col_bits;
-    size_t tmp = 0;
-    CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, c->col_keys[0], c->col_keys[1],
-                                                 c->col_vals[0], c->col_vals[1], (int)nk, 0, 32,
-                                                 c->stream));
-    c->col_sort_bytes = tmp;
-    c->col_sort_tmp = dalloc(c, tmp ? tmp : 16, &rc);
-    return rc;
+    return radix_alloc(c, c->col_rs, nk, c->CG);
 }
 
 }  // namespace cg

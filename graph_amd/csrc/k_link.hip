@@ -185,9 +185,9 @@ __device__ inline float spring_term(const GDesc& d, uint4 e, float2 pp, float2 v
     float D = (r * d.span + d.lo) * d.dist;
     x = q.x - (pp.x + v.x);
     y = q.y - (pp.y + v.y);
-    float l = sqrtf(x * x + y * y);
+    float l = __builtin_amdgcn_sqrtf(x * x + y * y);
     l = fmaxf(l, D * 0.99f);
-    l = (l - D) / l;
+    l = (l - D) * __builtin_amdgcn_rcpf(l);
     l *= d.spring_alpha;
     l *= s;
     l *= b;

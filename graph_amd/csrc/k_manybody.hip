@@ -918,6 +918,23 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
 
 }  // namespace
 
+// Digits of at most 8 bits, as few passes as cover key_range, for n pairs.
+int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range) {
+    int rc = CG_OK;
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < key_range) ++bits;
+    rs.passes = (bits + 7) / 8;
+    rs.width = (bits + rs.passes - 1) / rs.passes;
+    if (rs.width < 4) rs.width = 4;  // 16+ digits: the histogram table is a multiple of 16
+    rs.n = n;
+    rs.nb = (uint32_t)(((size_t)(n ? n : 1) + RB_TILE - 1) / RB_TILE);
+    rs.n_hist = (1u << rs.width) * rs.nb;
+    rs.hist = (uint32_t*)dalloc(c, (size_t)rs.n_hist * 4, &rc);
+    if (rc) return rc;
+    rs.part = (uint32_t*)dalloc(c, (size_t)((rs.n_hist + SCAN_TILE - 1) / SCAN_TILE) * 4, &rc);
+    return rc;
+}
+
 int mb_alloc(Ctx* c) {
     int rc = CG_OK;
     const size_t np = c->NP ? c->NP : 1;
@@ -926,17 +943,9 @@ int mb_alloc(Ctx* c) {
         c->rk[q] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
         c->rv[q] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
     }
-    // digits: the fewest passes of at most 8 bits covering every cell key
-    int bits = 1;
-    while (bits < 32 && (1ull << bits) < (unsigned long long)F) ++bits;
-    c->rb_passes = (bits + 7) / 8;
-    c->rb_width = (bits + c->rb_passes - 1) / c->rb_passes;
-    c->sorted = c->rv[c->rb_passes & 1];
-    c->n_rb = (uint32_t)((np + RB_TILE - 1) / RB_TILE);
-    const size_t n_hist = ((size_t)1 << c->rb_width) * c->n_rb;
-    c->hist = (uint32_t*)dalloc(c, n_hist * 4, &rc); if (rc) return rc;
-    c->n_scan_tiles = (uint32_t)((n_hist + SCAN_TILE - 1) / SCAN_TILE);
-    c->scan_part = (uint32_t*)dalloc(c, (size_t)c->n_scan_tiles * 4, &rc); if (rc) return rc;
+    rc = radix_alloc(c, c->mb_rs, c->NP, F);
+    if (rc) return rc;
+    c->sorted = c->rv[c->mb_rs.passes & 1];
     c->sorted_pos = (float2*)dalloc(c, np * 8, &rc); if (rc) return rc;
     c->cell_begin = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
     c->cell_end = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
@@ -958,38 +967,46 @@ int mb_alloc(Ctx* c) {
     return rc;
 }
 
-// In-place exclusive scan of n (a multiple of 16) counts.
-static void scan_counts(Ctx* c, uint32_t* a, uint32_t n) {
+// In-place exclusive scan of n (a multiple of 16) counts; `part` holds one
+// word per SCAN_TILE counts.
+static void scan_counts(Ctx* c, uint32_t* a, uint32_t n, uint32_t* part) {
     if (n <= 4096) {  // one block, a few entries per lane
         hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(1024), 0, c->stream, a, n);
         return;
     }
     const uint32_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(256), 0, c->stream, a, n, c->scan_part);
-    hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(1024), 0, c->stream, c->scan_part, tiles);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(256), 0, c->stream, a, n, c->scan_part, a);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(256), 0, c->stream, a, n, part);
+    hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(1024), 0, c->stream, part, tiles);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(256), 0, c->stream, a, n, part, a);
+}
+
+// Sorts (k[0], v[0]) by key into k[passes & 1], v[passes & 1]; every pass is
+// stable.  hist0_ready: the first digit's block histogram is already in place.
+void radix_run(Ctx* c, const RadixSort& rs, uint32_t* const (&k)[2], uint32_t* const (&v)[2],
+               bool hist0_ready) {
+    const int W = rs.width;
+    for (int p = 0; p < rs.passes; ++p) {
+        if (p || !hist0_ready)
+            hipLaunchKernelGGL(k_radix_up, dim3(rs.nb), dim3(256), 0, c->stream, k[p & 1], rs.n, p * W, W,
+                               rs.nb, rs.hist);
+        scan_counts(c, rs.hist, rs.n_hist, rs.part);
+        hipLaunchKernelGGL(k_radix_down, dim3(rs.nb), dim3(256), 0, c->stream, k[p & 1], v[p & 1], rs.n,
+                           p * W, W, rs.nb, rs.hist, k[(p + 1) & 1], v[(p + 1) & 1]);
+    }
 }
 
 int mb_run(Ctx* c, float2* vout) {
     if (c->NP == 0) return CG_OK;
     const uint32_t np = c->NP, F = c->F;
     prof_begin(c, CG_STAGE_MB_SORT);
-    const uint32_t nb = c->n_rb, n_hist = (1u << c->rb_width) * nb;
-    const int W = c->rb_width, P = c->rb_passes;
+    const RadixSort& rs = c->mb_rs;
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->cell_end, F, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->big_count, 3u, 0u);
-    hipLaunchKernelGGL(k_bin_keys, dim3(nb), dim3(256), 0, c->stream, c->sorted, c->pos[0], c->pos[1],
-                       c->gid, c->desc, np, c->rk[0], c->rv[0], W, nb, c->hist);
-    for (int p = 0; p < P; ++p) {
-        if (p)
-            hipLaunchKernelGGL(k_radix_up, dim3(nb), dim3(256), 0, c->stream, c->rk[p & 1], np, p * W, W,
-                               nb, c->hist);
-        scan_counts(c, c->hist, n_hist);
-        hipLaunchKernelGGL(k_radix_down, dim3(nb), dim3(256), 0, c->stream, c->rk[p & 1], c->rv[p & 1], np,
-                           p * W, W, nb, c->hist, c->rk[(p + 1) & 1], c->rv[(p + 1) & 1]);
-    }
-    hipLaunchKernelGGL(k_bin_bounds, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->rk[P & 1], np,
-                       c->cell_begin, c->cell_end);
+    hipLaunchKernelGGL(k_bin_keys, dim3(rs.nb), dim3(256), 0, c->stream, c->sorted, c->pos[0], c->pos[1],
+                       c->gid, c->desc, np, c->rk[0], c->rv[0], rs.width, rs.nb, rs.hist);
+    radix_run(c, rs, c->rk, c->rv, true);
+    hipLaunchKernelGGL(k_bin_bounds, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream,
+                       c->rk[rs.passes & 1], np, c->cell_begin, c->cell_end);
     prof_end(c, CG_STAGE_MB_SORT);
     prof_begin(c, CG_STAGE_MB_CELLS);
     hipLaunchKernelGGL(k_cells, dim3(F / CELL_BLOCK), dim3(CELL_BLOCK), 0, c->stream, c->cell_end,
