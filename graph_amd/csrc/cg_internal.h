@@ -81,6 +81,7 @@ struct GraphLinks {
     uint4* ent_pt[2] = {nullptr, nullptr};   // {local neighbour, draw, strength, bias}
     uint32_t* off_pt[2] = {nullptr, nullptr};// [n+1]
     uint32_t* live[2] = {nullptr, nullptr};  // [n] entries toward present neighbours
+    uint32_t* lrank[2] = {nullptr, nullptr}; // [E+1] live entries before each entry
     uint32_t* degree = nullptr;              // [n]
 };
 

@@ -114,6 +114,8 @@ struct GLinkPtr {
     const uint4* ent_pt;
     const uint32_t* off_pt;
     const uint32_t* live;
+    const uint32_t* lrank;
+    uint32_t n;
 };
 
 __global__ void k_sched_keys(const uint32_t* __restrict__ present, uint32_t np,
@@ -140,25 +142,36 @@ __global__ void k_tier_bounds(const uint32_t* __restrict__ skeys, uint32_t ns,
     if (d > WAVE_MAX && (k == 0 || prev <= WAVE_MAX)) counts[1] = k;
 }
 
-// Copy each scheduled point's live entries, in link order, to its slot, with
-// global neighbour indices.
-__global__ void k_assemble(const uint32_t* __restrict__ sched, uint32_t ns,
-                           const uint32_t* __restrict__ soff, const uint32_t* __restrict__ gid,
-                           const GDesc* __restrict__ desc, const GLinkPtr* __restrict__ tab,
+__global__ void k_slot_of(const uint32_t* __restrict__ sched, uint32_t ns, uint32_t* __restrict__ slot_of) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < ns) slot_of[sched[j]] = j;
+}
+
+// One lane per CSR entry of every graph (graph s holds entries [ebase[s],
+// ebase[s+1])): a live entry (owner and neighbour present) goes to its owner's
+// schedule slot at its rank among the owner's live entries, in link order, with
+// a global neighbour index.
+__global__ void k_assemble(const uint32_t* __restrict__ ebase, uint32_t G, uint32_t total,
+                           const GLinkPtr* __restrict__ tab, const GDesc* __restrict__ desc,
+                           const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ soff,
                            const uint8_t* __restrict__ flags, uint4* __restrict__ ent) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ns) return;
-    const uint32_t p = sched[j];
-    const uint32_t g = gid[p];
-    const GLinkPtr t = tab[g];
-    if (!t.ent_pt) return;
-    const uint32_t p0 = desc[g].p0, lp = p - p0;
-    uint32_t dst = soff[j];
-    for (uint32_t e = t.off_pt[lp]; e < t.off_pt[lp + 1]; ++e) {
-        uint4 v = t.ent_pt[e];
-        v.x += p0;
-        if (!(flags[v.x] & F_ABSENT)) ent[dst++] = v;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const uint32_t s = find_graph(ebase, G, t);
+    const GLinkPtr tb = tab[s];
+    const uint32_t e = t - ebase[s];
+    // owner: the last local point whose entries start at or before e
+    uint32_t lo = 0, hi = tb.n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tb.off_pt[mid] <= e) lo = mid;
+        else hi = mid;
     }
+    const uint32_t p0 = desc[s].p0, p = p0 + lo;
+    uint4 v = tb.ent_pt[e];
+    v.x += p0;
+    if ((flags[p] | flags[v.x]) & F_ABSENT) return;
+    ent[soff[slot_of[p]] + (tb.lrank[e] - tb.lrank[tb.off_pt[lo]])] = v;
 }
 
 // Schedule slot of every chunked-tier entry: the last slot j in [first, last)
@@ -435,6 +448,7 @@ void link_free_graph(Ctx* c, Graph& g) {
         dfree(c, g.lk.ent_pt[d]);
         dfree(c, g.lk.off_pt[d]);
         dfree(c, g.lk.live[d]);
+        dfree(c, g.lk.lrank[d]);
     }
     dfree(c, g.lk.degree);
     g.lk = GraphLinks();
@@ -475,7 +489,7 @@ int link_build_graph(Ctx* c, Graph& gr) {
     uint32_t* v0 = (uint32_t*)talloc(mm * 4);
     uint32_t* v1 = (uint32_t*)talloc(mm * 4);
     uint32_t* lflag = (uint32_t*)talloc((mm + 1) * 4);
-    uint32_t* lrank = (uint32_t*)talloc((mm + 1) * 4);
+    uint32_t* lrank = (uint32_t*)talloc((mm + 1) * 4);  // scan-size probe only
     if (rc) { cleanup(); return rc; }
     CG_HIP(c, hipMemcpyAsync(d_flags, gr.flags.data(), n, hipMemcpyHostToDevice, c->stream));
     CG_HIP(c, hipMemsetAsync(outdeg, 0, ((size_t)n + 1) * 4, c->stream));
@@ -518,6 +532,7 @@ int link_build_graph(Ctx* c, Graph& gr) {
         CG_HIP(c, hipStreamSynchronize(c->stream));
         lk.E[d] = E;
         lk.ent_pt[d] = (uint4*)dalloc(c, (size_t)(E ? E : 1) * 16, &rc);
+        lk.lrank[d] = (uint32_t*)dalloc(c, ((size_t)E + 1) * 4, &rc);
         if (rc) { cleanup(); return rc; }
         CG_HIP(c, hipMemsetAsync(lflag, 0, (mm + 1) * 4, c->stream));
         if (m) {
@@ -532,9 +547,9 @@ int link_build_graph(Ctx* c, Graph& gr) {
                                    lk.ent_pt[d], lflag);
         }
         sb = tbytes;
-        CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, lflag, lrank, (int)E + 1, c->stream));
+        CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, lflag, lk.lrank[d], (int)E + 1, c->stream));
         hipLaunchKernelGGL(k_live_count, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream,
-                           lk.off_pt[d], lrank, n, lk.live[d]);
+                           lk.off_pt[d], lk.lrank[d], n, lk.live[d]);
         CG_HIP(c, hipGetLastError());
     }
     CG_HIP(c, hipStreamSynchronize(c->stream));
@@ -580,6 +595,8 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
     uint32_t* v0 = (uint32_t*)talloc(((size_t)NP + 1) * 4);
     uint32_t* counts = (uint32_t*)talloc(64);
     GLinkPtr* tab = (GLinkPtr*)talloc((size_t)G * sizeof(GLinkPtr));
+    uint32_t* ebase = (uint32_t*)talloc(((size_t)G + 1) * 4);
+    uint32_t* slot_of = (uint32_t*)talloc((size_t)(c->N ? c->N : 1) * 4);
     if (rc) { cleanup(); return rc; }
     size_t sort_bytes = 0, scan_bytes = 0;
     CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, v0, v0, (int)NP, 0, 32,
@@ -594,11 +611,17 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
     for (int d = 0; d < 2; ++d) {
         LinkDir& L = c->dir[d];
         std::vector<GLinkPtr> h_tab(G);
+        std::vector<uint32_t> h_ebase(G + 1, 0);
         for (uint32_t s = 0; s < G; ++s) {
             Graph& g = c->graphs[c->order[s]];
-            if (g.links_ready && g.m > 0) h_tab[s] = {g.lk.ent_pt[d], g.lk.off_pt[d], g.lk.live[d]};
-            else h_tab[s] = {nullptr, nullptr, nullptr};
+            if (g.links_ready && g.m > 0)
+                h_tab[s] = {g.lk.ent_pt[d], g.lk.off_pt[d], g.lk.live[d], g.lk.lrank[d], g.n};
+            else
+                h_tab[s] = {nullptr, nullptr, nullptr, nullptr, 0u};
+            h_ebase[s + 1] = h_ebase[s] + ((g.links_ready && g.m > 0) ? g.lk.E[d] : 0u);
         }
+        CG_HIP(c, hipMemcpyAsync(ebase, h_ebase.data(), (size_t)(G + 1) * 4, hipMemcpyHostToDevice,
+                                 c->stream));
         CG_HIP(c, hipMemcpyAsync(tab, h_tab.data(), (size_t)G * sizeof(GLinkPtr), hipMemcpyHostToDevice,
                                  c->stream));
         const uint32_t ns = NP;
@@ -626,8 +649,12 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         L.n_block = ns - h[1];
         L.ent = (uint4*)dalloc(c, (size_t)(L.E_live ? L.E_live : 1) * 16, &rc);
         if (rc) { cleanup(); return rc; }
-        hipLaunchKernelGGL(k_assemble, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.sched, ns,
-                           L.soff, c->gid, c->desc, tab, c->flags, L.ent);
+        hipLaunchKernelGGL(k_slot_of, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.sched, ns,
+                           slot_of);
+        const uint32_t etot = h_ebase[G];
+        if (etot)
+            hipLaunchKernelGGL(k_assemble, dim3(blocks_for(etot, 256)), dim3(256), 0, c->stream, ebase, G,
+                               etot, tab, c->desc, slot_of, L.soff, c->flags, L.ent);
         // chunked tiers: step-major offsets, entry -> slot map, warm-start offsets
         const uint32_t nch = ns - L.n_lane;
         std::vector<uint32_t> hs(nch + 1), hp(nch + 1, 0), hsched(nch ? nch : 1), hw(nch ? nch : 1, 0);

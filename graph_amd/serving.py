@@ -175,12 +175,15 @@ class BatchServer:
 
 def serve_sequential(jobs: list[Job], device: int = 0, log=None) -> tuple[list[Result], dict]:
     """The plain path: each job alone through the single-graph entry points,
-    first come first served.  Service times are measured; a job starts when it
-    has arrived and the previous one has finished."""
+    first come first served.  A job's service time is measured from context
+    creation (ingest and link build included, as the batched server's loop
+    includes them) to its readback; a job starts when it has arrived and the
+    previous one has finished."""
     results = []
     clock = 0.0
     busy = 0.0
     for job in sorted(jobs, key=lambda j: j.arrival):
+        s0 = time.perf_counter()
         with _lib.Context(device=device) as ctx:
             ctx.set_positions(job.xy)
             if job.jitter is not None:
@@ -191,10 +194,9 @@ def serve_sequential(jobs: list[Job], device: int = 0, log=None) -> tuple[list[R
                 ctx.set_point_sizes(job.sizes)
             ctx.set_alpha(1.0)
             total = alpha_ticks(job.params.decay)
-            s0 = time.perf_counter()
             ctx.step(job.params, total)
             xy = ctx.get_positions()
-            service = time.perf_counter() - s0
+        service = time.perf_counter() - s0
         start = max(clock, job.arrival)
         clock = start + service
         busy += service
