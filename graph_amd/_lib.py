@@ -28,6 +28,7 @@ FORCE_MANYBODY = 3
 FORCE_LINK_IN = 4
 FORCE_LINK_OUT = 5
 FORCE_COLLISION = 6
+FORCE_CLUSTER = 7
 
 
 class CgCreateParams(C.Structure):
@@ -54,6 +55,7 @@ class CgSimParams(C.Structure):
         ("run_forces", C.c_int32),
         ("mouse_repulsion", C.c_int32),
         ("mouse_position", C.c_float * 2),
+        ("cluster", C.c_float),
     ]
 
 
@@ -80,6 +82,12 @@ _SIGNATURES = {
     "cg_set_jitter": (C.c_int, [_CTX, _F, C.c_uint32]),
     "cg_set_point_sizes": (C.c_int, [_CTX, _F, C.c_uint32]),
     "cg_set_pinned": (C.c_int, [_CTX, _U, C.c_uint32]),
+    "cg_set_point_clusters": (C.c_int, [_CTX, _I, C.c_uint32]),
+    "cg_set_cluster_positions": (C.c_int, [_CTX, _F, C.c_uint32]),
+    "cg_set_point_cluster_strength": (C.c_int, [_CTX, _F, C.c_uint32]),
+    "cg_graph_set_point_clusters": (C.c_int, [_CTX, C.c_uint32, _I, C.c_uint32]),
+    "cg_graph_set_cluster_positions": (C.c_int, [_CTX, C.c_uint32, _F, C.c_uint32]),
+    "cg_graph_set_point_cluster_strength": (C.c_int, [_CTX, C.c_uint32, _F, C.c_uint32]),
     "cg_set_alpha": (C.c_int, [_CTX, C.c_double]),
     "cg_get_alpha": (C.c_int, [_CTX, _D]),
     "cg_step": (C.c_int, [_CTX, C.POINTER(CgSimParams), C.c_uint32, _F, _D]),
@@ -110,7 +118,7 @@ _SIGNATURES = {
 }
 
 STAGES = ["mouse", "gravity", "center", "mb_sort", "mb_cells", "mb_pyramid", "mb_force", "link_in",
-          "link_out", "collision"]
+          "link_out", "collision", "cluster"]
 
 
 def header_symbols() -> list[str]:
@@ -239,6 +247,30 @@ class Context:
 
     def set_alpha(self, alpha: float) -> None:
         self._check(self._l.cg_set_alpha(self._ctx, float(alpha)))
+
+    def set_point_clusters(self, clusters, gid: int | None = None) -> None:
+        a = None if clusters is None else np.ascontiguousarray(clusters, dtype=np.int32)
+        n = 0 if a is None else a.size
+        if gid is None:
+            self._check(self._l.cg_set_point_clusters(self._ctx, iptr(a), n))
+        else:
+            self._check(self._l.cg_graph_set_point_clusters(self._ctx, gid, iptr(a), n))
+
+    def set_cluster_positions(self, xy, gid: int | None = None) -> None:
+        a = None if xy is None else np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
+        n = 0 if a is None else a.size // 2
+        if gid is None:
+            self._check(self._l.cg_set_cluster_positions(self._ctx, fptr(a), n))
+        else:
+            self._check(self._l.cg_graph_set_cluster_positions(self._ctx, gid, fptr(a), n))
+
+    def set_point_cluster_strength(self, strength, gid: int | None = None) -> None:
+        a = None if strength is None else np.ascontiguousarray(strength, dtype=np.float32)
+        n = 0 if a is None else a.size
+        if gid is None:
+            self._check(self._l.cg_set_point_cluster_strength(self._ctx, fptr(a), n))
+        else:
+            self._check(self._l.cg_graph_set_point_cluster_strength(self._ctx, gid, fptr(a), n))
 
     def get_alpha(self) -> float:
         a = C.c_double()

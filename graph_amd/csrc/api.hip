@@ -150,6 +150,15 @@ bool same_collision(const cg_sim_params& a, const cg_sim_params& b) {
 
 bool collides(const Graph& g) { return on(g.params.collision) && g.np > 0; }
 
+bool clustered(const Graph& g) { return g.has_clusters || g.has_cluster_pos; }
+
+// Clusters/index.ts:60-64: the highest defined non-negative index plus one (at least 1).
+uint32_t cluster_count(const Graph& g) {
+    int32_t mx = 0;
+    for (uint32_t i = 0; i < g.n && i < g.clusters.size(); ++i) mx = std::max(mx, g.clusters[i]);
+    return (uint32_t)mx + 1;
+}
+
 // Finest cells a graph occupies in the packed cell arrays (rounded up to whole blocks).
 uint64_t pad_cells(uint32_t gf) {
     const uint64_t n = (uint64_t)gf * gf;
@@ -223,6 +232,10 @@ void free_packed(Ctx* c) {
     release(c, c->center_first);
     c->n_center_chunks = 0;
     release(c, c->cpresent);
+    release(c, c->pcl); release(c, c->pcs); release(c, c->cpos); release(c, c->cl_members);
+    release(c, c->cl_chunks); release(c, c->cl_first); release(c, c->cl_part); release(c, c->cl_sum);
+    c->CC = 0;
+    c->n_cl_chunks = 0;
     release(c, c->col_begin); release(c, c->col_end); release(c, c->col_cell);
 }
 
@@ -389,6 +402,60 @@ int pack(Ctx* c) {
         rc = upload(c, c->pyr[1], w1); if (rc) return rc;
     }
 
+    // clusters: packed indices, strengths, targets and member chunks
+    {
+        std::vector<int32_t> hpcl(N, -1);
+        std::vector<float> hpcs(N, 1.f);
+        std::vector<float2> hcpos;
+        std::vector<uint32_t> members, first;
+        std::vector<uint4> chunks;
+        std::vector<std::vector<uint32_t>> bucket;
+        for (uint32_t s = 0; s < G; ++s) {
+            const Graph& g = c->graphs[c->order[s]];
+            if (!clustered(g)) continue;
+            const uint32_t ncl = cluster_count(g), cl0 = (uint32_t)hcpos.size();
+            for (uint32_t q = 0; q < ncl; ++q) {
+                const bool have = 2 * (size_t)q + 1 < g.cluster_pos.size();
+                hcpos.push_back(have ? make_float2(g.cluster_pos[2 * q], g.cluster_pos[2 * q + 1])
+                                     : make_float2(-1.f, -1.f));
+            }
+            bucket.assign(ncl, {});
+            for (uint32_t i = 0; i < g.n; ++i) {
+                const int32_t q = i < g.clusters.size() ? g.clusters[i] : -1;
+                if (i < g.cluster_strength.size()) hpcs[g.p0 + i] = g.cluster_strength[i];
+                if (q < 0) continue;
+                hpcl[g.p0 + i] = (int32_t)(cl0 + (uint32_t)q);
+                if (!(g.flags[i] & F_ABSENT)) bucket[q].push_back(g.p0 + i);
+            }
+            for (uint32_t q = 0; q < ncl; ++q) {
+                first.push_back((uint32_t)chunks.size());
+                const uint32_t b = (uint32_t)members.size();
+                members.insert(members.end(), bucket[q].begin(), bucket[q].end());
+                const uint32_t e = (uint32_t)members.size();
+                for (uint32_t k = b; k < e; k += 4096)
+                    chunks.push_back(make_uint4(k, std::min(e, k + 4096), cl0 + q, s));
+            }
+        }
+        first.push_back((uint32_t)chunks.size());
+        c->CC = (uint32_t)hcpos.size();
+        c->n_cl_chunks = (uint32_t)chunks.size();
+        c->pcl = (int32_t*)dalloc(c, nn * 4, &rc);
+        c->pcs = (float*)dalloc(c, nn * 4, &rc);
+        c->cpos = (float2*)dalloc(c, (hcpos.size() + 1) * 8, &rc);
+        c->cl_members = (uint32_t*)dalloc(c, (members.size() + 1) * 4, &rc);
+        c->cl_chunks = (uint4*)dalloc(c, (chunks.size() + 1) * 16, &rc);
+        c->cl_first = (uint32_t*)dalloc(c, first.size() * 4, &rc);
+        c->cl_part = (double*)dalloc(c, (chunks.size() + 1) * 2 * sizeof(double), &rc);
+        c->cl_sum = (double*)dalloc(c, (hcpos.size() + 1) * 3 * sizeof(double), &rc);
+        if (rc) return rc;
+        rc = upload(c, c->pcl, hpcl); if (rc) return rc;
+        rc = upload(c, c->pcs, hpcs); if (rc) return rc;
+        rc = upload(c, c->cpos, hcpos); if (rc) return rc;
+        rc = upload(c, c->cl_members, members); if (rc) return rc;
+        rc = upload(c, c->cl_chunks, chunks); if (rc) return rc;
+        rc = upload(c, c->cl_first, first); if (rc) return rc;
+    }
+
     // descriptor tables (device and pinned staging)
     c->desc_ticks = std::max<uint32_t>(1, std::min<uint32_t>(DESC_MAX_TICKS, DESC_ENTRIES / (uint32_t)gg));
     c->desc_base = (GDesc*)dalloc(c, (size_t)c->desc_ticks * gg * sizeof(GDesc), &rc);
@@ -443,6 +510,7 @@ void fill_dynamic(GDesc& d, const cg_sim_params& p, float a, float seed) {
     d.col_strength = p.collision;
     d.col_radius = p.collision_radius;
     d.col_padding = p.collision_padding;
+    d.cluster_k = p.cluster;
     // fixed point of the cell sums: np * 16 S * 2^k < 2^63
     int bits_s = 0, bits_n = 0;
     while (bits_s < 128 && std::ldexp(1.0, bits_s) < (double)S) ++bits_s;
@@ -463,6 +531,7 @@ uint32_t graph_passes(const Graph& g) {
         if (on(p.center)) passes |= 1u << P_CENTER;
         passes |= 1u << P_MANYBODY;
         if (g.links_ready && g.m > 0) passes |= (1u << P_LINK_IN) | (1u << P_LINK_OUT);
+        if (clustered(g)) passes |= 1u << P_CLUSTER;
         if (collides(g)) passes |= 1u << P_COLLISION;
     }
     return passes;
@@ -502,6 +571,12 @@ int launch_tick(Ctx* c, const TickPlan& tp) {
         prof_begin(c, CG_STAGE_LINK_IN + d);
         rc = link_run(c, d, nullptr);
         prof_end(c, CG_STAGE_LINK_IN + d);
+        if (rc) return rc;
+    }
+    if (a & (1u << P_CLUSTER)) {
+        prof_begin(c, CG_STAGE_CLUSTER);
+        rc = cluster_run(c, nullptr);
+        prof_end(c, CG_STAGE_CLUSTER);
         if (rc) return rc;
     }
     if (a & (1u << P_COLLISION)) {
@@ -740,6 +815,29 @@ int graph_set_pinned(Ctx* c, Graph& g, const uint32_t* idx, uint32_t count) {
     return CG_OK;
 }
 
+int graph_set_point_clusters(Ctx* c, Graph& g, const int32_t* idx, uint32_t n) {
+    if (idx) g.clusters.assign(idx, idx + n);
+    else g.clusters.clear();
+    g.has_clusters = idx != nullptr;
+    c->dirty = true;
+    return CG_OK;
+}
+
+int graph_set_cluster_positions(Ctx* c, Graph& g, const float* xy, uint32_t count) {
+    if (xy) g.cluster_pos.assign(xy, xy + 2 * (size_t)count);
+    else g.cluster_pos.clear();
+    g.has_cluster_pos = xy != nullptr;
+    c->dirty = true;
+    return CG_OK;
+}
+
+int graph_set_point_cluster_strength(Ctx* c, Graph& g, const float* s, uint32_t n) {
+    if (s) g.cluster_strength.assign(s, s + n);
+    else g.cluster_strength.clear();
+    c->dirty = true;
+    return CG_OK;
+}
+
 int graph_get_positions(Ctx* c, Graph& g, float* xy, uint32_t n) {
     if (n != g.n || (n && !xy)) return fail(c, CG_ERR_INVALID, "get_positions: size mismatch");
     if (n == 0) return CG_OK;
@@ -761,6 +859,7 @@ int pass_of_force(int32_t force) {
         case CG_FORCE_LINK_IN: return P_LINK_IN;
         case CG_FORCE_LINK_OUT: return P_LINK_OUT;
         case CG_FORCE_COLLISION: return P_COLLISION;
+        case CG_FORCE_CLUSTER: return P_CLUSTER;
         default: return -1;
     }
 }
@@ -802,6 +901,7 @@ void cg_default_params(cg_sim_params* p) {
     p->enable_simulation = 1;
     p->run_forces = 1;
     p->mouse_repulsion = 0;
+    p->cluster = 0.1f;
 }
 
 int cg_create(cg_ctx** out, const cg_create_params* params) {
@@ -876,6 +976,21 @@ int cg_set_point_sizes(cg_ctx* x, const float* sizes, uint32_t n) {
 int cg_set_pinned(cg_ctx* x, const uint32_t* idx, uint32_t count) {
     CG_CTX(x);
     return graph_set_pinned(c, default_graph(c), idx, count);
+}
+
+int cg_set_point_clusters(cg_ctx* x, const int32_t* clusters, uint32_t n) {
+    CG_CTX(x);
+    return graph_set_point_clusters(c, default_graph(c), clusters, n);
+}
+
+int cg_set_cluster_positions(cg_ctx* x, const float* xy, uint32_t count) {
+    CG_CTX(x);
+    return graph_set_cluster_positions(c, default_graph(c), xy, count);
+}
+
+int cg_set_point_cluster_strength(cg_ctx* x, const float* strength, uint32_t n) {
+    CG_CTX(x);
+    return graph_set_point_cluster_strength(c, default_graph(c), strength, n);
 }
 
 int cg_set_alpha(cg_ctx* x, double alpha) {
@@ -982,6 +1097,24 @@ int cg_graph_set_pinned(cg_ctx* x, uint32_t id, const uint32_t* idx, uint32_t co
     return graph_set_pinned(c, *g, idx, count);
 }
 
+int cg_graph_set_point_clusters(cg_ctx* x, uint32_t id, const int32_t* clusters, uint32_t n) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    return graph_set_point_clusters(c, *g, clusters, n);
+}
+
+int cg_graph_set_cluster_positions(cg_ctx* x, uint32_t id, const float* xy, uint32_t count) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    return graph_set_cluster_positions(c, *g, xy, count);
+}
+
+int cg_graph_set_point_cluster_strength(cg_ctx* x, uint32_t id, const float* strength, uint32_t n) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    return graph_set_point_cluster_strength(c, *g, strength, n);
+}
+
 int cg_graph_set_alpha(cg_ctx* x, uint32_t id, double alpha) {
     CG_CTX(x);
     CG_GRAPH(g, id);
@@ -1041,6 +1174,7 @@ int cg_eval_force(cg_ctx* x, const cg_sim_params* p, int32_t force, float alpha,
     if (g.n == 0) return CG_OK;
     std::memset(v, 0, (size_t)g.n * 8);
     const bool runs = g.np && ((pass < P_LINK_IN) || (pass == P_COLLISION && collides(g)) ||
+                               (pass == P_CLUSTER && clustered(g)) ||
                                ((pass == P_LINK_IN || pass == P_LINK_OUT) && g.links_ready && g.m > 0));
     if (!runs) return CG_OK;
     std::vector<GDesc> table(c->G);
@@ -1062,6 +1196,7 @@ int cg_eval_force(cg_ctx* x, const cg_sim_params* p, int32_t force, float alpha,
         if (pass <= P_CENTER) rc = pw_run(c, pass, dv);
         else if (pass == P_MANYBODY) rc = mb_run(c, dv);
         else if (pass == P_COLLISION) rc = col_run(c, dv);
+        else if (pass == P_CLUSTER) rc = cluster_run(c, dv);
         else rc = link_run(c, pass - P_LINK_IN, dv);
     }
     if (rc == CG_OK) {

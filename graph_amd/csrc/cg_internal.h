@@ -42,8 +42,9 @@ enum Pass : uint32_t {
     P_MANYBODY = 3,
     P_LINK_IN = 4,
     P_LINK_OUT = 5,
-    P_COLLISION = 6,
-    P_COUNT = 7
+    P_CLUSTER = 6,
+    P_COLLISION = 7,
+    P_COUNT = 8
 };
 
 // Level l of a graph's pyramid starts this many cells after c0 (levels 4^2, 8^2, ...).
@@ -72,6 +73,7 @@ struct GDesc {
     float tol;              // multiple-shooting tolerance (2.5e-6 S)
     float col_strength, col_radius, col_padding, col_h;
     float fx_scale, fx_inv;  // 2^k and 2^-k: positions as int64 multiples of 2^-k in cell sums
+    float cluster_k;         // simulationCluster
 };
 
 // Per-graph link CSR built once per cg_set_links, in the reference's entry
@@ -101,6 +103,10 @@ struct Graph {
     std::vector<float> links, strength, rin, rout;
     bool links_set = false;
     std::vector<float> sizes;                // raw sizes (NaN = default); empty = none
+    std::vector<int32_t> clusters;           // cluster per point (< 0: none); empty = none set
+    std::vector<float> cluster_pos;          // [x, y] per cluster (< 0: centroid); empty = none set
+    std::vector<float> cluster_strength;     // per point; empty = all 1
+    bool has_clusters = false, has_cluster_pos = false;  // setPointClusters / setClusterPositions
     // derived
     uint32_t gf = 0, L = 0;
     GraphLinks lk;
@@ -208,6 +214,18 @@ struct Ctx {
     uint32_t* col_end = nullptr;
     float4* col_cell = nullptr;     // [CG]
 
+    // clusters: point -> packed cluster, per-cluster targets, present members by cluster
+    uint32_t CC = 0;                // packed clusters
+    int32_t* pcl = nullptr;         // [N] packed cluster of each point, -1 = none
+    float* pcs = nullptr;           // [N] cluster strength of each point
+    float2* cpos = nullptr;         // [CC] custom position (a negative coordinate: centroid)
+    uint32_t* cl_members = nullptr; // present clustered points grouped by cluster, index order
+    uint4* cl_chunks = nullptr;     // {member begin, member end, cluster, graph slot}
+    uint32_t* cl_first = nullptr;   // [CC + 1] first chunk of each cluster
+    uint32_t n_cl_chunks = 0;
+    double* cl_part = nullptr;      // [2 chunks] per-chunk sums
+    double* cl_sum = nullptr;       // [3 CC] sum x, sum y, count
+
     // links
     LinkDir dir[2];
     bool any_links = false;
@@ -257,6 +275,7 @@ void dfree(Ctx* c, void* p);
 // (non-null: evaluation mode) receives the pass's velocity instead of integrating.
 int pw_run(Ctx* c, int pass, float2* vout);                       // k_pointwise.hip
 int center_run(Ctx* c);
+int cluster_run(Ctx* c, float2* vout);
 void center_chunks_host(const std::vector<uint32_t>& k0, const std::vector<uint32_t>& np,
                         std::vector<uint4>& chunks, std::vector<uint32_t>& first);
 int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range);  // k_manybody.hip

@@ -1,9 +1,11 @@
-// k_pointwise.hip — per-point forces (gravity, center, mouse) fused with the
-// integrate pass, and the per-graph centroid reduction of the center force.
+// k_pointwise.hip — per-point forces (gravity, center, mouse, clusters) fused
+// with the integrate pass, and the centroid reductions of the center and
+// cluster forces.
 //
 // Reference: ForceGravity/force-gravity.frag:24-40, ForceCenter/
 // calculate-centermass.vert:11-30 + force-center.frag:22-38, ForceMouse/
-// force-mouse.frag:21-36, Points/update-position.frag:26-59.
+// force-mouse.frag:21-36, Clusters/calculate-centermass.vert:21-53 +
+// force-cluster.frag:28-53, Points/update-position.frag:26-59.
 #include <algorithm>
 
 #include "cg_internal.h"
@@ -132,7 +134,118 @@ __global__ void k_center_final(const double* __restrict__ part, const uint32_t* 
     out[3 * g + 2] = (double)desc[g].np;
 }
 
+// Cluster centroids — Clusters/calculate-centermass.vert:21-53: sums of
+// (x, y, 1) over the present points of each cluster.  One block per chunk of
+// at most CENTER_CHUNK members of one cluster (fixed f64 tree), then one lane
+// per cluster adds its chunks in order.
+__global__ void __launch_bounds__(256) k_cluster_partial(const float2* __restrict__ pos0,
+                                                         const float2* __restrict__ pos1,
+                                                         const uint32_t* __restrict__ members,
+                                                         const uint4* __restrict__ chunks,
+                                                         const GDesc* __restrict__ desc,
+                                                         double* __restrict__ part) {
+    __shared__ double sx[256], sy[256];
+    const uint4 ch = chunks[blockIdx.x];  // {member begin, member end, cluster, graph slot}
+    const GDesc& d = desc[ch.w];
+    const float2* __restrict__ pin = ((d.inbuf >> P_CLUSTER) & 1u) ? pos1 : pos0;
+    double ax = 0.0, ay = 0.0;
+    for (uint32_t k = ch.x + threadIdx.x; k < ch.y; k += 256) {
+        float2 p = pin[members[k]];
+        ax += (double)p.x;
+        ay += (double)p.y;
+    }
+    sx[threadIdx.x] = ax;
+    sy[threadIdx.x] = ay;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sx[threadIdx.x] += sx[threadIdx.x + s];
+            sy[threadIdx.x] += sy[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sx[0];
+        part[2 * blockIdx.x + 1] = sy[0];
+    }
+}
+
+__global__ void k_cluster_final(const double* __restrict__ part, const uint4* __restrict__ chunks,
+                                const uint32_t* __restrict__ first, uint32_t ncl,
+                                double* __restrict__ out) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncl) return;
+    double x = 0.0, y = 0.0, n = 0.0;
+    for (uint32_t b = first[c]; b < first[c + 1]; ++b) {
+        x += part[2 * b];
+        y += part[2 * b + 1];
+        n += (double)(chunks[b].y - chunks[b].x);
+    }
+    out[3 * c] = x;
+    out[3 * c + 1] = y;
+    out[3 * c + 2] = n;
+}
+
+// force-cluster.frag:28-53: toward the cluster's custom position, or its
+// centroid when a coordinate of the custom one is negative; no pull without a
+// cluster or at zero distance.  v = alpha * dist * simulationCluster * strength
+// along the unit vector (an empty cluster's centroid is 0/0, so no pull).
+__global__ void __launch_bounds__(256) k_cluster_force(float2* __restrict__ pos0,
+                                                       float2* __restrict__ pos1,
+                                                       float2* __restrict__ vout,
+                                                       const uint32_t* __restrict__ present,
+                                                       const uint32_t* __restrict__ gid,
+                                                       const uint8_t* __restrict__ flags,
+                                                       const GDesc* __restrict__ desc, uint32_t np,
+                                                       const int32_t* __restrict__ pcl,
+                                                       const float* __restrict__ pcs,
+                                                       const float2* __restrict__ cpos,
+                                                       const double* __restrict__ csum) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np) return;
+    const uint32_t i = present[k];
+    const GDesc& d = desc[gid[i]];
+    if (!(d.passes & (1u << P_CLUSTER))) return;
+    const bool b = (d.inbuf >> P_CLUSTER) & 1u;
+    const float2 p = (b ? pos1 : pos0)[i];
+    float2 v = make_float2(0.f, 0.f);
+    const int32_t cl = pcl[i];
+    if (cl >= 0) {
+        float2 t = cpos[cl];
+        if (t.x < 0.f || t.y < 0.f) {
+            const float n = (float)csum[3 * cl + 2];
+            t = make_float2((float)csum[3 * cl] / n, (float)csum[3 * cl + 1] / n);
+        }
+        const float dx = t.x - p.x, dy = t.y - p.y;
+        const float dist = sqrtf(dx * dx + dy * dy);
+        if (dist > 0.f) {
+            const float add = d.alpha * dist * d.cluster_k * pcs[i];
+            v.x = add * (dx / dist);
+            v.y = add * (dy / dist);
+        }
+    }
+    if (vout) {
+        vout[i] = v;
+        return;
+    }
+    (b ? pos0 : pos1)[i] = (flags[i] & F_PINNED) ? p : integrate(p, v, d.friction, d.S);
+}
+
 }  // namespace
+
+int cluster_run(Ctx* c, float2* vout) {
+    if (c->NP == 0 || c->CC == 0) return CG_OK;
+    if (c->n_cl_chunks)
+        hipLaunchKernelGGL(k_cluster_partial, dim3(c->n_cl_chunks), dim3(256), 0, c->stream, c->pos[0],
+                           c->pos[1], c->cl_members, c->cl_chunks, c->desc, c->cl_part);
+    hipLaunchKernelGGL(k_cluster_final, dim3(blocks_for(c->CC, 64)), dim3(64), 0, c->stream, c->cl_part,
+                       c->cl_chunks, c->cl_first, c->CC, c->cl_sum);
+    hipLaunchKernelGGL(k_cluster_force, dim3(blocks_for(c->NP, 256)), dim3(256), 0, c->stream, c->pos[0],
+                       c->pos[1], vout, c->present, c->gid, c->flags, c->desc, c->NP, c->pcl, c->pcs,
+                       c->cpos, c->cl_sum);
+    CG_HIP(c, hipGetLastError());
+    return CG_OK;
+}
 
 int center_run(Ctx* c) {
     if (c->NP == 0 || c->n_center_chunks == 0) return CG_OK;

@@ -206,6 +206,60 @@ static napi_value js_set_pinned(napi_env env, napi_callback_info info) {
     return NULL;
 }
 
+// setPointClusters (Int32Array, negative = none), setClusterPositions
+// (Float32Array [x, y] per cluster, negative = centroid), setPointClusterStrength
+// (Float32Array); null clears each.
+static napi_value js_set_point_clusters(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    cg_ctx* ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    void* p = NULL;
+    size_t len = 0;
+    if (argc > 1 && !get_typed(env, argv[1], napi_int32_array, &p, &len, 1)) {
+        napi_throw_type_error(env, NULL, "cosmosgl.setPointClusters: expected an Int32Array or null");
+        return NULL;
+    }
+    int rc = cg_set_point_clusters(ctx, (const int32_t*)p, (uint32_t)len);
+    if (rc != CG_OK) return throw_rc(env, ctx, rc);
+    return NULL;
+}
+
+static napi_value js_set_cluster_positions(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    cg_ctx* ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    void* p = NULL;
+    size_t len = 0;
+    if (argc > 1 && !get_typed(env, argv[1], napi_float32_array, &p, &len, 1)) {
+        napi_throw_type_error(env, NULL, "cosmosgl.setClusterPositions: expected a Float32Array or null");
+        return NULL;
+    }
+    int rc = cg_set_cluster_positions(ctx, (const float*)p, (uint32_t)(len / 2));
+    if (rc != CG_OK) return throw_rc(env, ctx, rc);
+    return NULL;
+}
+
+static napi_value js_set_point_cluster_strength(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    cg_ctx* ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    void* p = NULL;
+    size_t len = 0;
+    if (argc > 1 && !get_typed(env, argv[1], napi_float32_array, &p, &len, 1)) {
+        napi_throw_type_error(env, NULL, "cosmosgl.setPointClusterStrength: expected a Float32Array or null");
+        return NULL;
+    }
+    int rc = cg_set_point_cluster_strength(ctx, (const float*)p, (uint32_t)len);
+    if (rc != CG_OK) return throw_rc(env, ctx, rc);
+    return NULL;
+}
+
 static napi_value js_set_alpha(napi_env env, napi_callback_info info) {
     size_t argc = 2;
     napi_value argv[2];
@@ -299,6 +353,7 @@ static void read_params(napi_env env, napi_value obj, cg_sim_params* p) {
     read_bool(env, obj, "runForces", &p->run_forces);
     read_bool(env, obj, "mouseRepulsion", &p->mouse_repulsion);
     read_pair(env, obj, "mousePosition", p->mouse_position);
+    read_f32(env, obj, "simulationCluster", &p->cluster);
 }
 
 static napi_value js_step(napi_env env, napi_callback_info info) {
@@ -373,6 +428,7 @@ static napi_value js_default_params(napi_env env, napi_callback_info info) {
     PUT("simulationCollision", p.collision);
     PUT("simulationCollisionPadding", p.collision_padding);
     PUT("pointDefaultSize", p.point_default_size);
+    PUT("simulationCluster", p.cluster);
 #undef PUT
     return o;
 }
@@ -386,6 +442,9 @@ static napi_value init(napi_env env, napi_value exports) {
         {"setJitter", 0, js_set_jitter, 0, 0, 0, napi_enumerable, 0},
         {"setPointSizes", 0, js_set_point_sizes, 0, 0, 0, napi_enumerable, 0},
         {"setPinned", 0, js_set_pinned, 0, 0, 0, napi_enumerable, 0},
+        {"setPointClusters", 0, js_set_point_clusters, 0, 0, 0, napi_enumerable, 0},
+        {"setClusterPositions", 0, js_set_cluster_positions, 0, 0, 0, napi_enumerable, 0},
+        {"setPointClusterStrength", 0, js_set_point_cluster_strength, 0, 0, 0, napi_enumerable, 0},
         {"setAlpha", 0, js_set_alpha, 0, 0, 0, napi_enumerable, 0},
         {"getAlpha", 0, js_get_alpha, 0, 0, 0, napi_enumerable, 0},
         {"step", 0, js_step, 0, 0, 0, napi_enumerable, 0},

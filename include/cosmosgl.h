@@ -24,6 +24,9 @@
  *   cg_set_point_sizes Graph.setPointSizes src/index.ts:627 (collision radii,
  *                      ForceCollision/index.ts:121-148)
  *   cg_set_pinned      Graph.setPinnedPoints src/index.ts:912 -> Points.updatePinnedStatus
+ *   cg_set_point_clusters / cg_set_cluster_positions / cg_set_point_cluster_strength
+ *                      Graph.setPointClusters / setClusterPositions / setPointClusterStrength
+ *                      src/index.ts:854, 871, 888 -> Clusters.create
  *                      Points/index.ts:1264-1304
  *   cg_set_alpha       Graph.start(alpha) / render(alpha) src/index.ts:1457, 936
  *   cg_step            Graph.runSimulationStep src/index.ts:2006-2099 (nticks times)
@@ -76,6 +79,7 @@ typedef struct cg_sim_params {
     int32_t run_forces;                 /* forceExecution || (running && !zoom gate) */
     int32_t mouse_repulsion;            /* right button held && enableRightClickRepulsion */
     float mouse_position[2];            /* Store.mousePosition, space coordinates */
+    float cluster;                      /* simulationCluster (cluster pull coefficient) */
 } cg_sim_params;
 
 /* Fills the reference defaults (src/variables.ts:10-19, 68-83: spaceSize 4096). */
@@ -109,6 +113,17 @@ int cg_set_point_sizes(cg_ctx* ctx, const float* sizes, uint32_t n);
 /* Pinned point indices (indices outside [0, n) are ignored); count 0 clears. */
 int cg_set_pinned(cg_ctx* ctx, const uint32_t* indices, uint32_t count);
 
+/* Clusters — Graph.setPointClusters src/index.ts:854, setClusterPositions :871,
+ * setPointClusterStrength :888 -> Clusters.create (Clusters/index.ts:54-160).
+ * clusters: one index per point, negative = no cluster (the reference's
+ * undefined); NULL clears.  positions: [x, y] per cluster, a negative
+ * coordinate = pull toward the cluster's centroid; NULL clears.  strength: one
+ * coefficient per point (default 1); NULL clears.  The cluster pass runs while
+ * clusters or cluster positions are set (src/index.ts:2057-2061). */
+int cg_set_point_clusters(cg_ctx* ctx, const int32_t* clusters, uint32_t n);
+int cg_set_cluster_positions(cg_ctx* ctx, const float* xy, uint32_t count);
+int cg_set_point_cluster_strength(cg_ctx* ctx, const float* strength, uint32_t n);
+
 int cg_set_alpha(cg_ctx* ctx, double alpha);
 int cg_get_alpha(cg_ctx* ctx, double* alpha);
 
@@ -138,6 +153,7 @@ const char* cg_last_error(const cg_ctx* ctx);
 #define CG_FORCE_LINK_IN 4
 #define CG_FORCE_LINK_OUT 5
 #define CG_FORCE_COLLISION 6
+#define CG_FORCE_CLUSTER 7
 
 /* Velocity that one force pass writes from the current positions (nothing is
  * integrated).  v: 2n floats; absent points get 0. */
@@ -180,7 +196,8 @@ uint64_t cg_device_bytes(const cg_ctx* ctx);
 #define CG_STAGE_LINK_IN 7
 #define CG_STAGE_LINK_OUT 8
 #define CG_STAGE_COLLISION 9
-#define CG_STAGES 10
+#define CG_STAGE_CLUSTER 10
+#define CG_STAGES 11
 #define CG_PROFILE_LINK_STATS (1u << 31) /* also count link sweeps (cg_debug_link_stats) */
 int cg_set_profiling(cg_ctx* ctx, uint32_t stage_mask);
 /* Tick sequences are captured once into HIP graphs and replayed (on by
@@ -209,6 +226,9 @@ int cg_graph_set_links(cg_ctx* ctx, uint32_t id, const float* links, uint32_t m,
 int cg_graph_set_jitter(cg_ctx* ctx, uint32_t id, const float* xy, uint32_t n);
 int cg_graph_set_point_sizes(cg_ctx* ctx, uint32_t id, const float* sizes, uint32_t n);
 int cg_graph_set_pinned(cg_ctx* ctx, uint32_t id, const uint32_t* indices, uint32_t count);
+int cg_graph_set_point_clusters(cg_ctx* ctx, uint32_t id, const int32_t* clusters, uint32_t n);
+int cg_graph_set_cluster_positions(cg_ctx* ctx, uint32_t id, const float* xy, uint32_t count);
+int cg_graph_set_point_cluster_strength(cg_ctx* ctx, uint32_t id, const float* strength, uint32_t n);
 int cg_graph_set_alpha(cg_ctx* ctx, uint32_t id, double alpha);
 int cg_graph_get_alpha(cg_ctx* ctx, uint32_t id, double* alpha);
 int cg_graph_get_positions(cg_ctx* ctx, uint32_t id, float* xy, uint32_t n);

@@ -252,6 +252,44 @@ void or_mouse(uint32_t n, const double* pos, double mx, double my, double k, dou
     }
 }
 
+/* clusters — Clusters/calculate-centermass.vert:21-53 (sum of (x, y, 1) over
+ * the present points of each cluster; unclustered points, cl < 0, contribute
+ * nothing) and force-cluster.frag:28-53: the target is the custom position when
+ * both of its coordinates are >= 0, else the centroid; when the distance d to it
+ * is > 0, v = alpha * d * coef * strength along the unit vector.  An empty
+ * cluster's centroid is 0/0: no pull. */
+void or_cluster(uint32_t n, const double* pos, const uint8_t* absent, const int32_t* cl, uint32_t ncl,
+                const double* cpos, const float* strength, double alpha, double coef, double* v) {
+    double* sum = (double*)calloc(3 * (size_t)(ncl ? ncl : 1), sizeof(double));
+    for (uint32_t i = 0; i < n; ++i) {
+        if ((absent && absent[i]) || cl[i] < 0) continue;
+        sum[3 * cl[i]] += pos[2 * i];
+        sum[3 * cl[i] + 1] += pos[2 * i + 1];
+        sum[3 * cl[i] + 2] += 1.0;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        v[2 * i] = 0.0;
+        v[2 * i + 1] = 0.0;
+        const int32_t c = cl[i];
+        if (c < 0) continue;
+        double tx, ty;
+        if (cpos && cpos[2 * c] >= 0.0 && cpos[2 * c + 1] >= 0.0) {
+            tx = cpos[2 * c];
+            ty = cpos[2 * c + 1];
+        } else {
+            tx = sum[3 * c] / sum[3 * c + 2];
+            ty = sum[3 * c + 1] / sum[3 * c + 2];
+        }
+        const double dx = tx - pos[2 * i], dy = ty - pos[2 * i + 1];
+        const double d = sqrt(dx * dx + dy * dy);
+        if (!(d > 0.0)) continue;
+        const double k = alpha * d * coef * (strength ? (double)strength[i] : 1.0);
+        v[2 * i] = k * dx / d;
+        v[2 * i + 1] = k * dy / d;
+    }
+    free(sum);
+}
+
 /* clamped inverse-distance falloff shared by force-level.frag:43-56 (with the
  * cell count) and force-nearfield.frag:64-80 (count 1). */
 static void falloff(double dx, double dy, double w, double* vx, double* vy) {

@@ -60,6 +60,7 @@ def lib():
         l.or_collision.argtypes = [C.c_uint32, D, B, F, C.c_float, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.c_double, D]
         l.or_integrate.argtypes = [C.c_uint32, D, D, B, B, C.c_double, C.c_double]
+        l.or_cluster.argtypes = [C.c_uint32, D, B, I, C.c_uint32, D, F, C.c_double, C.c_double, D]
         l.or_alpha_step.argtypes = [C.c_double, C.c_double]
         l.or_alpha_step.restype = C.c_double
         _lib = l
@@ -155,6 +156,25 @@ def center(xy, alpha, c, absent=None):
     return v
 
 
+def cluster(xy, clusters, alpha, coef, cluster_pos=None, strength=None, absent=None):
+    """Cluster pull (or_cluster); clusters: per-point index, negative = none;
+    cluster_pos: [x, y] per cluster (a negative coordinate = centroid) or None."""
+    pos = _f64(xy).reshape(-1)
+    n = pos.size // 2
+    cl = np.ascontiguousarray(clusters, dtype=np.int32)
+    ncl = int(max(0, cl.max(initial=0))) + 1
+    cp = None
+    if cluster_pos is not None:
+        cp = np.full(2 * ncl, -1.0)
+        src = np.asarray(cluster_pos, dtype=np.float64).reshape(-1)[: 2 * ncl]
+        cp[: src.size] = src
+    s = None if strength is None else _f32(strength)
+    v = np.zeros(2 * n)
+    lib().or_cluster(n, _p(pos, D), _p(_u8(absent), B), _p(cl, I), ncl, _p(cp, D), _p(s, F), alpha, coef,
+                     _p(v, D))
+    return v
+
+
 def mouse(xy, mx, my, k):
     pos = _f64(xy).reshape(-1)
     n = pos.size // 2
@@ -244,7 +264,8 @@ class Links:
         return v
 
 
-def tick(pos, alpha, seed, p, links=None, jitter=None, absent=None, pinned=None, sizes=None):
+def tick(pos, alpha, seed, p, links=None, jitter=None, absent=None, pinned=None, sizes=None,
+         clusters=None):
     """One FP64 tick in the order of runSimulationStep (src/index.ts:2031-2087);
     p is a dict of the simulation* keys (space_size, gravity, center, repulsion,
     link_spring, link_distance, lo, hi, friction, decay, collision, collision_radius,
@@ -261,6 +282,10 @@ def tick(pos, alpha, seed, p, links=None, jitter=None, absent=None, pinned=None,
         for d in (0, 1):
             v = links.force(pos, d, alpha, p["link_spring"], p["link_distance"], p["lo"], p["hi"], absent)
             pos = integrate(pos, v, fr, S, pinned, absent)
+    if clusters is not None:  # {"index": per point, "positions": or None, "strength": or None}
+        v = cluster(pos, clusters["index"], alpha, p.get("cluster", 0.1), clusters.get("positions"),
+                    clusters.get("strength"), absent)
+        pos = integrate(pos, v, fr, S, pinned, absent)
     if p.get("collision", 0):
         v = collision(pos, sizes, S, alpha, p["collision"], p.get("collision_radius", 0.0),
                       p.get("collision_padding", 0.0), absent=absent)

@@ -36,10 +36,10 @@ def spec(kind: str, seed: int) -> dict:
         links = datagen.barabasi_albert(n, 1, seed)
         p = workloads.sim_params(S, collision=1.0)
         sizes = (2.0 * datagen.lognormal(n, np.log(2.0), 0.5, seed)).astype(np.float32)
-    elif kind == "nolinks":
+    elif kind == "nolinks":  # and clusters (centroid and custom targets)
         n, S = 1500, 1024.0
         links = None
-        p = workloads.sim_params(S, gravity=0.0)
+        p = workloads.sim_params(S, gravity=0.0, cluster=0.2)
         sizes = None
     else:
         raise ValueError(kind)
@@ -49,8 +49,12 @@ def spec(kind: str, seed: int) -> dict:
         xy[::97] = np.nan  # absent points
         xy[5::97, 1] = np.nan  # half-NaN: absent too (Points/position-utils.ts:17-24)
     r_in, r_out = (None, None) if links is None else workloads._draws(links.size // 2, seed)
+    clusters = None
+    if kind == "nolinks":
+        clusters = dict(index=(np.arange(n) % 7 - 1).astype(np.int32),
+                        positions=np.array([100.0, 900.0, -1.0, 0.0], np.float32))
     return dict(kind=kind, n=n, xy=xy, links=links, r_in=r_in, r_out=r_out, params=p, sizes=sizes,
-                pinned=np.arange(0, n, 211, dtype=np.uint32))
+                pinned=np.arange(0, n, 211, dtype=np.uint32), clusters=clusters)
 
 
 def load(ctx: _lib.Context, s: dict) -> int:
@@ -62,6 +66,9 @@ def load(ctx: _lib.Context, s: dict) -> int:
     if s["sizes"] is not None:
         ctx.graph_set_point_sizes(gid, s["sizes"])
     ctx.graph_set_pinned(gid, s["pinned"])
+    if s["clusters"] is not None:
+        ctx.set_point_clusters(s["clusters"]["index"], gid=gid)
+        ctx.set_cluster_positions(s["clusters"]["positions"], gid=gid)
     return gid
 
 

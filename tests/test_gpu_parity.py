@@ -290,3 +290,26 @@ def test_napi_addon_matches_ctypes(tmp_path):
     want = c.get_positions()
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     assert np.array_equal(np.fromfile(tmp_path / "alpha.f64", np.float64), tr)
+
+
+def test_cluster_force_vs_oracle():
+    # Clusters/force-cluster.frag: custom targets, centroid targets (unset or a negative
+    # coordinate), unclustered points, absent points out of the centroids, per-point strength
+    rng = np.random.default_rng(17)
+    S, n = 4096.0, 6000
+    xy = rng.uniform(0, S, 2 * n).astype(np.float32)
+    xy[2 * 50: 2 * 60] = np.nan
+    cl = rng.integers(-1, 10, n).astype(np.int32)
+    cpos = np.array([1000.0, 3000.0, -1.0, 200.0], np.float32)  # cluster 0 custom, 1 centroid
+    strength = rng.uniform(0.2, 2.0, n).astype(np.float32)
+    absent = O.absent_mask(xy)
+    c = ctx_for(xy)
+    c.set_point_clusters(cl)
+    c.set_cluster_positions(cpos)
+    c.set_point_cluster_strength(strength)
+    p = params(S, cluster=0.3)
+    got = c.eval_force(p, _lib.FORCE_CLUSTER, 0.8)
+    want = O.cluster(xy, cl, 0.8, 0.3, cluster_pos=cpos, strength=strength, absent=absent)
+    check(got, want, S, "cluster", idx=absent == 0)
+    # unclustered points get no pull
+    assert np.all(got.reshape(-1, 2)[cl < 0] == 0)

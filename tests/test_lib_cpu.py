@@ -115,9 +115,11 @@ def test_napi_addon_exports():
     out = subprocess.run([node, "-e", code, str(addon)], capture_output=True, text=True, check=True).stdout
     names, params = out.strip().split("\n")
     assert names.split(",") == sorted(["create", "destroy", "setPositions", "setLinks", "setJitter",
-                                        "setPointSizes", "setPinned", "setAlpha", "getAlpha", "step",
-                                        "getPositions", "defaultParams"])
+                                        "setPointSizes", "setPinned", "setPointClusters",
+                                        "setClusterPositions", "setPointClusterStrength", "setAlpha",
+                                        "getAlpha", "step", "getPositions", "defaultParams"])
     import json
 
     d = json.loads(params)
     assert d["spaceSize"] == 4096 and d["simulationDecay"] == 5000
+    assert abs(d["simulationCluster"] - 0.1) < 1e-7

@@ -408,3 +408,38 @@ def test_integrate(oracle):
                            absent=np.array([0, 0, 0, 1], np.uint8))
     np.testing.assert_allclose(out[:6], [10.85, 8.3, 4096.0, 0.0, 1.0, 1.0])
     assert np.isnan(out[6:]).all()
+
+
+# ---- clusters (force-cluster.frag:28-53, calculate-centermass.vert:21-53) ----
+
+def test_cluster_custom_target_closed_form(oracle):
+    rng = np.random.default_rng(3)
+    xy = rng.uniform(0, 100, (50, 2))
+    cl = np.zeros(50, np.int32)
+    s = rng.uniform(0.5, 2.0, 50).astype(np.float32)
+    v = oracle.cluster(xy, cl, 0.7, 0.1, cluster_pos=[40.0, 60.0], strength=s).reshape(-1, 2)
+    want = 0.7 * 0.1 * s.astype(np.float64)[:, None] * (np.array([40.0, 60.0]) - xy)
+    assert np.allclose(v, want, rtol=1e-12, atol=1e-12)
+
+
+def test_cluster_centroid_symmetric_and_absent_excluded(oracle):
+    # four points symmetric about (10, 20): centroid (10, 20); a far absent point changes nothing
+    xy = np.array([[9, 20], [11, 20], [10, 19], [10, 21], [1000, 1000]], np.float64)
+    cl = np.zeros(5, np.int32)
+    absent = np.array([0, 0, 0, 0, 1], np.uint8)
+    v = oracle.cluster(xy, cl, 1.0, 0.5, absent=absent).reshape(-1, 2)
+    assert np.allclose(v[:4], 0.5 * (np.array([10.0, 20.0]) - xy[:4]), atol=1e-12)
+    # a negative coordinate of the custom position means "use the centroid"
+    v2 = oracle.cluster(xy, cl, 1.0, 0.5, cluster_pos=[-1.0, 500.0], absent=absent).reshape(-1, 2)
+    assert np.array_equal(v, v2)
+
+
+def test_cluster_unclustered_and_empty_clusters_get_no_pull(oracle):
+    xy = np.array([[1.0, 1.0], [5.0, 5.0], [9.0, 3.0]])
+    cl = np.array([-1, 2, -5], np.int32)  # cluster 2 has one member; clusters 0, 1 are empty
+    v = oracle.cluster(xy, cl, 1.0, 1.0).reshape(-1, 2)
+    assert np.array_equal(v[[0, 2]], np.zeros((2, 2)))
+    assert np.array_equal(v[1], np.zeros(2))  # its own centroid: zero distance
+    # a point whose cluster has no present member and no custom position: 0/0, no pull
+    v3 = oracle.cluster(xy, np.array([0, 0, 0], np.int32), 1.0, 1.0, absent=np.array([1, 1, 1], np.uint8))
+    assert np.array_equal(v3, np.zeros(6))
