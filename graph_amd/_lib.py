@@ -115,6 +115,10 @@ _SIGNATURES = {
     "cg_step_graphs": (C.c_int, [_CTX, _U, C.c_uint32, C.c_uint32]),
     "cg_sync": (C.c_int, [_CTX]),
     "cg_set_capture": (C.c_int, [_CTX, C.c_int32]),
+    "cg_drag_points": (C.c_int, [_CTX, _U, _F, C.c_uint32]),
+    "cg_get_positions_at": (C.c_int, [_CTX, _U, C.c_uint32, _F]),
+    "cg_graph_drag_points": (C.c_int, [_CTX, C.c_uint32, _U, _F, C.c_uint32]),
+    "cg_graph_get_positions_at": (C.c_int, [_CTX, C.c_uint32, _U, C.c_uint32, _F]),
 }
 
 STAGES = ["mouse", "gravity", "center", "mb_sort", "mb_cells", "mb_pyramid", "mb_force", "link_in",
@@ -287,6 +291,24 @@ class Context:
     def get_positions(self) -> np.ndarray:
         out = np.empty(2 * self.n, dtype=np.float32)
         self._check(self._l.cg_get_positions(self._ctx, fptr(out), self.n))
+        return out
+
+    def drag_points(self, indices, xy, gid: int | None = None) -> None:
+        idx = np.ascontiguousarray(indices, dtype=np.uint32)
+        p = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
+        assert p.size == 2 * idx.size
+        if gid is None:
+            self._check(self._l.cg_drag_points(self._ctx, uptr(idx), fptr(p), idx.size))
+        else:
+            self._check(self._l.cg_graph_drag_points(self._ctx, gid, uptr(idx), fptr(p), idx.size))
+
+    def get_positions_at(self, indices, gid: int | None = None) -> np.ndarray:
+        idx = np.ascontiguousarray(indices, dtype=np.uint32)
+        out = np.empty(2 * idx.size, dtype=np.float32)
+        if gid is None:
+            self._check(self._l.cg_get_positions_at(self._ctx, uptr(idx), idx.size, fptr(out)))
+        else:
+            self._check(self._l.cg_graph_get_positions_at(self._ctx, gid, uptr(idx), idx.size, fptr(out)))
         return out
 
     # --- inspection -----------------------------------------------------------

@@ -106,6 +106,23 @@ __global__ void __launch_bounds__(1024) k_desc_advance(const uint32_t* __restric
 
 __global__ void k_set_word(uint32_t* __restrict__ a, uint32_t v) { *a = v; }
 
+// drag-point.frag:28-31: listed points (graph-local indices below n) take new positions.
+__global__ void k_drag(float2* __restrict__ pos, uint32_t p0, uint32_t n, const uint8_t* __restrict__ flags,
+                       const uint32_t* __restrict__ idx, const float2* __restrict__ xy, uint32_t count) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count || idx[k] >= n) return;
+    const uint32_t i = p0 + idx[k];
+    if (!(flags[i] & F_ABSENT)) pos[i] = xy[k];
+}
+
+// Tracked positions: NaN past the point count (absent points hold NaN already).
+__global__ void k_gather(const float2* __restrict__ pos, uint32_t p0, uint32_t n,
+                         const uint32_t* __restrict__ idx, uint32_t count, float2* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    out[k] = idx[k] < n ? pos[p0 + idx[k]] : make_float2(__int_as_float(0x7fc00000), __int_as_float(0x7fc00000));
+}
+
 constexpr uint64_t GRAPH_SEED = 0x5eed5eedULL;   // every graph's generator starts here
 constexpr uint32_t DESC_ENTRIES = 1u << 16;      // descriptors per upload (ticks x graphs)
 constexpr uint32_t DESC_MAX_TICKS = 256;
@@ -850,6 +867,65 @@ int graph_get_positions(Ctx* c, Graph& g, float* xy, uint32_t n) {
     return CG_OK;
 }
 
+int io_reserve(Ctx* c, uint32_t count) {
+    if (count <= c->io_cap) return CG_OK;
+    dfree(c, c->io_idx);
+    dfree(c, c->io_xy);
+    c->io_idx = nullptr;
+    c->io_xy = nullptr;
+    c->io_cap = 0;
+    int rc = CG_OK;
+    const uint32_t cap = std::max<uint32_t>(count, 1024);
+    c->io_idx = (uint32_t*)dalloc(c, (size_t)cap * 4, &rc);
+    c->io_xy = (float2*)dalloc(c, (size_t)cap * 8, &rc);
+    if (rc) return rc;
+    c->io_cap = cap;
+    return CG_OK;
+}
+
+int graph_drag_points(Ctx* c, Graph& g, const uint32_t* idx, const float* xy, uint32_t count) {
+    if (count && (!idx || !xy)) return fail(c, CG_ERR_INVALID, "drag_points: null input");
+    if (count == 0) return CG_OK;
+    if (g.xy_pending) {  // not on the device yet: edit the pending copy
+        for (uint32_t k = 0; k < count; ++k)
+            if (idx[k] < g.n && !(g.flags[idx[k]] & F_ABSENT)) {
+                g.xy[2 * (size_t)idx[k]] = xy[2 * k];
+                g.xy[2 * (size_t)idx[k] + 1] = xy[2 * k + 1];
+            }
+        return CG_OK;
+    }
+    int rc = io_reserve(c, count);
+    if (rc) return rc;
+    CG_HIP(c, hipMemcpyAsync(c->io_idx, idx, (size_t)count * 4, hipMemcpyHostToDevice, c->stream));
+    CG_HIP(c, hipMemcpyAsync(c->io_xy, xy, (size_t)count * 8, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_drag, dim3(blocks_for(count, 256)), dim3(256), 0, c->stream, c->pos[g.cur], g.p0, g.n,
+                       c->flags, c->io_idx, c->io_xy, count);
+    CG_HIP(c, hipGetLastError());
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    return CG_OK;
+}
+
+int graph_get_positions_at(Ctx* c, Graph& g, const uint32_t* idx, uint32_t count, float* xy) {
+    if (count && (!idx || !xy)) return fail(c, CG_ERR_INVALID, "get_positions_at: null input");
+    if (count == 0) return CG_OK;
+    if (g.xy_pending) {
+        for (uint32_t k = 0; k < count; ++k) {
+            const bool ok = idx[k] < g.n;
+            xy[2 * k] = ok ? g.xy[2 * (size_t)idx[k]] : NAN;
+            xy[2 * k + 1] = ok ? g.xy[2 * (size_t)idx[k] + 1] : NAN;
+        }
+        return CG_OK;
+    }
+    int rc = io_reserve(c, count);
+    if (rc) return rc;
+    CG_HIP(c, hipMemcpyAsync(c->io_idx, idx, (size_t)count * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(count, 256)), dim3(256), 0, c->stream, c->pos[g.cur], g.p0,
+                       g.n, c->io_idx, count, c->io_xy);
+    CG_HIP(c, hipMemcpyAsync(xy, c->io_xy, (size_t)count * 8, hipMemcpyDeviceToHost, c->stream));
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    return CG_OK;
+}
+
 int pass_of_force(int32_t force) {
     switch (force) {
         case CG_FORCE_GRAVITY: return P_GRAVITY;
@@ -1025,6 +1101,21 @@ int cg_get_positions(cg_ctx* x, float* xy, uint32_t n) {
     return graph_get_positions(c, c->graphs[0], xy, n);
 }
 
+int cg_drag_points(cg_ctx* x, const uint32_t* indices, const float* xy, uint32_t count) {
+    CG_CTX(x);
+    if (c->graphs.empty() || !c->graphs[0].live) return fail(c, CG_ERR_STATE, "cg_drag_points: no graph");
+    return graph_drag_points(c, c->graphs[0], indices, xy, count);
+}
+
+int cg_get_positions_at(cg_ctx* x, const uint32_t* indices, uint32_t count, float* xy) {
+    CG_CTX(x);
+    if (c->graphs.empty() || !c->graphs[0].live) {
+        for (uint32_t k = 0; xy && k < 2 * count; ++k) xy[k] = NAN;
+        return CG_OK;
+    }
+    return graph_get_positions_at(c, c->graphs[0], indices, count, xy);
+}
+
 // ---- packed graph set ------------------------------------------------------
 
 int cg_graph_add(cg_ctx* x, uint32_t* id) {
@@ -1134,6 +1225,19 @@ int cg_graph_get_positions(cg_ctx* x, uint32_t id, float* xy, uint32_t n) {
     CG_CTX(x);
     CG_GRAPH(g, id);
     return graph_get_positions(c, *g, xy, n);
+}
+
+int cg_graph_drag_points(cg_ctx* x, uint32_t id, const uint32_t* indices, const float* xy,
+                         uint32_t count) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    return graph_drag_points(c, *g, indices, xy, count);
+}
+
+int cg_graph_get_positions_at(cg_ctx* x, uint32_t id, const uint32_t* indices, uint32_t count, float* xy) {
+    CG_CTX(x);
+    CG_GRAPH(g, id);
+    return graph_get_positions_at(c, *g, indices, count, xy);
 }
 
 int cg_step_graphs(cg_ctx* x, const uint32_t* ids, uint32_t count, uint32_t nticks) {

@@ -251,6 +251,11 @@ struct Ctx {
     GDesc* desc_cur = nullptr;
     uint32_t* tick_ctr = nullptr;
 
+    // staging of cg_drag_points / cg_get_positions_at (grown on demand)
+    uint32_t* io_idx = nullptr;
+    float2* io_xy = nullptr;
+    uint32_t io_cap = 0;
+
     // captured tick sequences: {passes run, ticks} -> executable graph
     bool capture = true;
     std::vector<std::pair<std::pair<uint32_t, uint32_t>, hipGraphExec_t>> graphs_exec;

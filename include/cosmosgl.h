@@ -31,6 +31,8 @@
  *   cg_set_alpha       Graph.start(alpha) / render(alpha) src/index.ts:1457, 936
  *   cg_step            Graph.runSimulationStep src/index.ts:2006-2099 (nticks times)
  *   cg_get_positions   Graph.getPointPositions src/index.ts:1106-1130 (readPixels)
+ *   cg_drag_points     Points.drag Points/index.ts:1961-1988 (called from src/index.ts:2303-2313)
+ *   cg_get_positions_at  getTrackedPointPositionsArray src/index.ts:1346-1388
  *   cg_get_alpha       Store.alpha, reported through onSimulationTick src/index.ts:2090
  */
 #ifndef COSMOSGL_H
@@ -138,6 +140,16 @@ int cg_step(cg_ctx* ctx, const cg_sim_params* params, uint32_t nticks,
 /* Copies n positions to xy (2n floats); absent points read back NaN. */
 int cg_get_positions(cg_ctx* ctx, float* xy, uint32_t n);
 
+/* Points.drag (Points/index.ts:1961-1988, drag-point.frag:21-33): the listed
+ * points take the given positions (the dragged point: the mouse position, not
+ * clamped) before the next tick.  Absent points stay absent. */
+int cg_drag_points(cg_ctx* ctx, const uint32_t* indices, const float* xy, uint32_t count);
+
+/* getTrackedPointPositionsArray (src/index.ts:1346-1388, Points/index.ts:2267-2337):
+ * positions of the listed points in list order; an absent point, or an index
+ * at or past the point count, reads NaN. */
+int cg_get_positions_at(cg_ctx* ctx, const uint32_t* indices, uint32_t count, float* xy);
+
 const char* cg_last_error(const cg_ctx* ctx);
 
 /* ---------------------------------------------------------------------- */
@@ -232,6 +244,10 @@ int cg_graph_set_point_cluster_strength(cg_ctx* ctx, uint32_t id, const float* s
 int cg_graph_set_alpha(cg_ctx* ctx, uint32_t id, double alpha);
 int cg_graph_get_alpha(cg_ctx* ctx, uint32_t id, double* alpha);
 int cg_graph_get_positions(cg_ctx* ctx, uint32_t id, float* xy, uint32_t n);
+int cg_graph_drag_points(cg_ctx* ctx, uint32_t id, const uint32_t* indices, const float* xy,
+                         uint32_t count);
+int cg_graph_get_positions_at(cg_ctx* ctx, uint32_t id, const uint32_t* indices, uint32_t count,
+                              float* xy);
 
 /* Advances the listed graphs by nticks ticks each with their stored params
  * (Graph.runSimulationStep per graph); the other graphs of the set keep

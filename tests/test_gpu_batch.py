@@ -159,3 +159,24 @@ def test_captured_ticks_match_direct_launches():
         same = np.array_equal(bits(xa), bits(ya)) and np.array_equal(bits(xb), bits(yb))
         say(f"call {k}: captured == direct: {same}")
         assert same
+
+
+def test_drag_and_tracked_positions():
+    s = spec("ba", 21)  # has absent points (every 97th)
+    with _lib.Context() as ctx:
+        ctx.set_positions(s["xy"])
+        ctx.set_links(s["links"], None, s["r_in"], s["r_out"])
+        ctx.step(s["params"], 3)
+        # Points.drag: a dragged point takes the mouse position, unclamped; absent stays absent
+        idx = np.array([6, 97, 1234], np.uint32)  # 97 is absent
+        new = np.array([[-10.0, 3000.0], [1.0, 2.0], [500.5, 600.25]], np.float32)
+        ctx.drag_points(idx, new)
+        got = ctx.get_positions_at(np.array([6, 97, 1234, s["n"] + 3], np.uint32)).reshape(-1, 2)
+        assert np.array_equal(got[0], new[0]) and np.array_equal(got[2], new[2])
+        assert np.isnan(got[1]).all() and np.isnan(got[3]).all()
+        # the next tick starts from the dragged positions; tracking agrees with the full readback
+        ctx.step(s["params"], 2)
+        full = ctx.get_positions().reshape(-1, 2)
+        track = ctx.get_positions_at(np.array([6, 1234, 7], np.uint32)).reshape(-1, 2)
+        assert np.array_equal(bits(track), bits(full[[6, 1234, 7]]))
+        assert 0.0 <= full[6, 0] <= s["params"].space_size  # integrated and clamped after the drag
