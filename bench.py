@@ -71,6 +71,50 @@ def cpu_baseline(wl: dict, seconds: float) -> dict:
                       f"oracle (oracle/cosmos_oracle.c, 1 thread) in {el:.1f} s"}
 
 
+def force_error_log(wl: dict, ticks: int, every: int, sample: int, log) -> dict:
+    """The 1M/5M run with its per-pass force error: every `every` ticks, each
+    pass (gravity, many-body, link IN, link OUT) is evaluated on the GPU from the
+    live state and by the FP64 oracle from the same state on a fixed sample of
+    points; the error is max |v_gpu - v_oracle| / spaceSize over the sample."""
+    from graph_amd import _lib
+    from oracle import oracle as O
+
+    n, S, p = wl["n"], wl["S"], wl["params"]
+    ctx = _lib.Context(device=0)
+    ctx.set_positions(wl["xy"])
+    ctx.set_jitter(wl["jitter"])
+    ctx.set_links(wl["links"], None, wl["r_in"], wl["r_out"])
+    ctx.set_alpha(1.0)
+    Lk = O.Links(n, wl["links"], wl["r_in"], wl["r_out"])
+    idx = np.sort(np.random.default_rng(5).choice(n, sample, replace=False)).astype(np.uint32)
+    lo, hi = p.link_dist_random_variation_range[0], p.link_dist_random_variation_range[1]
+    rows, worst = [], 0.0
+    for t in range(0, ticks + 1, every):
+        if t:
+            ctx.step(p, every)
+        xy = ctx.get_positions()
+        a = float(np.float32(ctx.get_alpha()))
+        seed = 0.25 + (t % 97) / 200.0
+
+        def err(got, want):
+            d = got.reshape(-1, 2)[idx].astype(np.float64) - want.reshape(-1, 2)[idx]
+            return float(np.hypot(d[:, 0], d[:, 1]).max() / S)
+
+        e = {"gravity": err(ctx.eval_force(p, _lib.FORCE_GRAVITY, a), O.gravity(xy, S, a, p.gravity)),
+             "manybody": err(ctx.eval_force(p, _lib.FORCE_MANYBODY, a, seed),
+                             O.manybody(xy, S, a, p.repulsion, seed, jitter=wl["jitter"], eval_idx=idx))}
+        for d, name in ((0, "link_in"), (1, "link_out")):
+            e[name] = err(ctx.eval_force(p, _lib.FORCE_LINK_IN + d, a),
+                          Lk.force(xy, d, a, p.link_spring, p.link_distance, lo, hi))
+        worst = max(worst, max(e.values()))
+        rows.append({"tick": t, "alpha": a, **{k: float(f"{v:.3e}") for k, v in e.items()}})
+        if t % 250 == 0:
+            log(f"force error at tick {t}: " + ", ".join(f"{k} {v:.2e}" for k, v in e.items()))
+    ctx.close()
+    return {"ticks": ticks, "every": every, "sample_points": sample, "tolerance": 1e-5,
+            "max_error_over_S": worst, "within_tolerance": worst <= 1e-5, "log": rows}
+
+
 def bench_batched(count: int, ticks: int, sample: int, log) -> dict:
     """Batched layouts: `count` independent BA graphs packed into one graph set,
     `ticks` ticks each, against the plain one-graph-at-a-time loop (the
@@ -178,7 +222,8 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
     ap.add_argument("--serve-segment", type=int, default=100)
-    ap.add_argument("--skip", default="", help="comma list of legs to skip: batched,serving")
+    ap.add_argument("--error-ticks", type=int, default=1000)
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: errors,batched,serving")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -268,6 +313,8 @@ def main() -> int:
     ctx.close()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     skip = set(filter(None, args.skip.split(",")))
+    if "errors" not in skip:
+        out["force_error"] = force_error_log(wl, args.error_ticks, 50, 4096, log)
     if "batched" not in skip:
         out["batched"] = bench_batched(args.batch_graphs, args.batch_ticks, args.batch_sample, log)
     if "serving" not in skip:
