@@ -223,6 +223,9 @@ def main() -> int:
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
     ap.add_argument("--serve-segment", type=int, default=100)
     ap.add_argument("--error-ticks", type=int, default=1000)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for rocprofv3 --pmc: calibration kernels of known traffic (256 MiB), then 5 "
+                         "direct-launch ticks of the 1M/5M workload; prints nothing else")
     ap.add_argument("--skip", default="", help="comma list of legs to skip: errors,batched,serving")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -237,6 +240,22 @@ def main() -> int:
 
     wl = workloads.cfg1()
     n = wl["n"]
+    if args.traffic_probe:
+        # calibration kernels of known traffic: a 256 MiB fill (writes 256 MiB), then an
+        # elementwise scale of it into a second buffer (reads 256 MiB, writes 256 MiB)
+        src = torch.ones(64 * 2**20, dtype=torch.float32, device="cuda")
+        dst = src * 2.0
+        torch.cuda.synchronize()
+        del dst
+        ctx = _lib.Context(device=0)
+        ctx.set_positions(wl["xy"])
+        ctx.set_jitter(wl["jitter"])
+        ctx.set_links(wl["links"], None, wl["r_in"], wl["r_out"])
+        ctx.set_capture(False)
+        ctx.step(wl["params"], 5)
+        ctx.sync()
+        ctx.close()
+        return 0
     ctx = _lib.Context(device=0)
     ctx.set_positions(wl["xy"])
     ctx.set_jitter(wl["jitter"])

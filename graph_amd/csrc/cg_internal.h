@@ -371,4 +371,14 @@ __device__ inline uint32_t find_graph(const uint32_t* __restrict__ base, uint32_
 
 inline unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
+// workgroup i runs on XCD i % 8.  This maps i to a logical block so that XCD x
+// runs one contiguous x-th share of [0, nb): blocks of nearby data then share
+// an L2 instead of every L2 fetching the whole working set.
+constexpr uint32_t XCDS = 8;
+__device__ inline uint32_t xcd_block(uint32_t i, uint32_t nb) {
+    const uint32_t q = nb / XCDS, r = nb % XCDS, x = i % XCDS, j = i / XCDS;
+    return x * q + min(x, r) + j;
+}
+
 }  // namespace cg

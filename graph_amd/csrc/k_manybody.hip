@@ -817,7 +817,7 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
     __shared__ int s_box[4][4];
     __shared__ uint32_t s_g;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const uint32_t k = blockIdx.x * blockDim.x + t;
+    const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + t;
     const bool valid = k < np;
     const uint32_t kk = valid ? k : np - 1;
     const uint32_t i = sv[kk];

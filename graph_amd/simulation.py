@@ -2,9 +2,12 @@
 
 The reference's public simulation API (src/index.ts) — ``setPointPositions``,
 ``setLinks``, ``setLinkStrength``, ``setPinnedPoints``, ``setPointSizes``,
+``setPointClusters``, ``setClusterPositions``, ``setPointClusterStrength``,
 ``render``, ``start``, ``stop``, ``pause``, ``unpause``, ``step``,
-``getPointPositions`` and the ``simulation*`` config keys with their
-callbacks — over the C ABI of ``include/cosmosgl.h``.  Rendering, zoom and
+``getPointPositions``, ``trackPointPositionsByIndices`` with
+``getTrackedPointPositionsArray`` / ``Map``, point dragging and the
+``simulation*`` config keys with their callbacks — over the C ABI of
+``include/cosmosgl.h``.  Rendering, zoom and
 picking are not part of this path.
 
 Random draws follow the reference's order: the many-body jitter (two draws per
@@ -43,6 +46,7 @@ DEFAULT_CONFIG: dict[str, Any] = {
     "simulationCollision": 0,
     "simulationCollisionRadius": None,
     "simulationCollisionPadding": 0,
+    "simulationCluster": 0.1,
     "enableRightClickRepulsion": False,
     "enableSimulationDuringZoom": False,
     "randomSeed": None,
@@ -83,7 +87,12 @@ class Graph:
         self._input_strength: np.ndarray | None = None
         self._input_sizes: np.ndarray | None = None
         self._pinned: list[int] | None = None
-        self._dirty = {"positions": False, "links": False, "sizes": False, "pinned": False}
+        self._clusters: np.ndarray | None = None
+        self._cluster_positions: np.ndarray | None = None
+        self._cluster_strength: np.ndarray | None = None
+        self._tracked: np.ndarray = np.zeros(0, np.uint32)
+        self._dirty = {"positions": False, "links": False, "sizes": False, "pinned": False,
+                       "clusters": False}
         self.alpha = 1.0
         self.is_simulation_running = False
         self.simulation_progress = 0.0
@@ -134,6 +143,7 @@ class Graph:
         p.run_forces = 1 if run_forces else 0
         p.mouse_repulsion = 1 if (self.is_right_click_mouse and c["enableRightClickRepulsion"]) else 0
         p.mouse_position[0], p.mouse_position[1] = self.mouse_position
+        p.cluster = c["simulationCluster"] if c["simulationCluster"] is not None else 0.1
         return p
 
     # ---- data setters (src/index.ts:548, 627, 723, 836, 912) -------------------------
@@ -163,6 +173,29 @@ class Graph:
         self._pinned = None if indices is None else [int(i) for i in indices if float(i).is_integer()]
         self._dirty["pinned"] = True
 
+    def setPointClusters(self, clusters) -> None:
+        """src/index.ts:854: one cluster index per point; None (undefined) = no cluster."""
+        if clusters is None:
+            self._clusters = None
+        else:
+            self._clusters = np.array([-1 if c is None or not (c == c) else int(c) for c in clusters],
+                                      dtype=np.int32)
+        self._dirty["clusters"] = True
+
+    def setClusterPositions(self, positions) -> None:
+        """src/index.ts:871: [x, y] per cluster; None entries pull toward the centroid."""
+        if positions is None:
+            self._cluster_positions = None
+        else:
+            self._cluster_positions = np.array([-1.0 if v is None else float(v) for v in positions],
+                                               dtype=np.float32)
+        self._dirty["clusters"] = True
+
+    def setPointClusterStrength(self, strength) -> None:
+        """src/index.ts:888: per-point cluster force coefficient (default 1)."""
+        self._cluster_strength = None if strength is None else np.asarray(strength, dtype=np.float32)
+        self._dirty["clusters"] = True
+
     # ---- lifecycle ---------------------------------------------------------------
     def render(self, simulationAlpha: float | None = None) -> None:
         """GraphData.update + create (src/index.ts:936-1003, 1635-1688)."""
@@ -176,6 +209,10 @@ class Graph:
         if self._dirty["pinned"] or self._dirty["positions"]:
             n = self.points_number or 0
             self._ctx.set_pinned([i for i in (self._pinned or []) if 0 <= i < n])
+        if self._dirty["clusters"]:
+            self._ctx.set_point_clusters(self._clusters)
+            self._ctx.set_cluster_positions(self._cluster_positions)
+            self._ctx.set_point_cluster_strength(self._cluster_strength)
         for k in self._dirty:
             self._dirty[k] = False
         if simulationAlpha is not None:
@@ -274,6 +311,28 @@ class Graph:
                 for a in trace:
                     cb(float(a), None, None)
         return trace
+
+    # ---- tracking and dragging (src/index.ts:1346-1388; Points/index.ts:1961-1988) ----
+    def trackPointPositionsByIndices(self, indices) -> None:
+        self._tracked = np.asarray(list(indices), dtype=np.uint32)
+
+    def getTrackedPointPositionsArray(self) -> list[float]:
+        """Tracked positions in tracking order; absent (or past the count) read NaN."""
+        if self._tracked.size == 0:
+            return []
+        return self._ctx.get_positions_at(self._tracked).astype(np.float64).tolist()
+
+    def getTrackedPointPositionsMap(self) -> dict[int, tuple[float, float]]:
+        """Tracked positions by index; absent points and indices past the count are omitted."""
+        if self._tracked.size == 0:
+            return {}
+        xy = self._ctx.get_positions_at(self._tracked).reshape(-1, 2)
+        return {int(i): (float(x), float(y)) for i, (x, y) in zip(self._tracked, xy)
+                if x == x and y == y}
+
+    def drag_point(self, index: int, position) -> None:
+        """Points.drag: the dragged (hovered) point takes the mouse position."""
+        self._ctx.drag_points([index], np.asarray(position, dtype=np.float32).reshape(1, 2))
 
     def getPointPositions(self) -> list[float]:
         """src/index.ts:1106-1130 — absent points read back as NaN."""
