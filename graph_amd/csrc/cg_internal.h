@@ -130,6 +130,7 @@ struct LinkDir {
     uint32_t n_sched = 0;
     uint32_t n_lane = 0, n_wave = 0, n_block = 0;
     uint32_t e_chunk0 = 0;       // first entry of the chunked tiers
+    uint32_t e_hub0 = 0;         // first entry of the 1024-lane tier
     uint32_t* poff = nullptr;    // [n_wave+n_block+1] step-major offsets into link_pre
     uint32_t* eslot = nullptr;   // schedule slot of each chunked-tier entry
     uint32_t* woff = nullptr;    // [n_wave+n_block] warm-start offset of each chunked slot

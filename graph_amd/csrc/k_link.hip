@@ -665,6 +665,7 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
                                      hipMemcpyDeviceToHost, c->stream));
         CG_HIP(c, hipStreamSynchronize(c->stream));
         L.e_chunk0 = hs[0];
+        L.e_hub0 = hs[L.n_wave];
         for (uint32_t k = 0; k < nch; ++k) {
             uint32_t dd = hs[k + 1] - hs[k];
             uint32_t B = (k < L.n_wave) ? 64u : 1024u;
@@ -733,10 +734,7 @@ int link_run(Ctx* c, int dir, float2* vout) {
     const uint32_t first = L.n_lane, block_slot = L.n_lane + L.n_wave;
     // sweep counters only while requested (same-address atomics per block)
     uint32_t* stats = (c->prof_mask & CG_PROFILE_LINK_STATS) ? c->link_stats : nullptr;
-    if (e1 > e0)
-        hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(e1 - e0, 256)), dim3(256), 0, c->stream, pass,
-                           L.ent, e0, e1, L.eslot, L.soff, L.poff, L.sched, first, block_slot, c->gid,
-                           c->desc, c->pos[0], c->pos[1], c->link_pre);
+    const uint32_t eh = L.e_hub0;  // entries [e0, eh) feed the 64-lane tier, [eh, e1) the hubs
     const bool fork = L.n_block && c->side;
     if (L.n_block) {
         hipStream_t s = c->stream;
@@ -745,11 +743,19 @@ int link_run(Ctx* c, int dir, float2* vout) {
             CG_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
             s = c->side;
         }
+        if (e1 > eh)
+            hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(e1 - eh, 256)), dim3(256), 0, s, pass, L.ent, eh,
+                               e1, L.eslot + (eh - e0), L.soff, L.poff, L.sched, first, block_slot, c->gid,
+                               c->desc, c->pos[0], c->pos[1], c->link_pre);
         hipLaunchKernelGGL(k_link_chunk<1024>, dim3(L.n_block), dim3(1024), 0, s, pass, L.sched, L.soff,
                            L.poff, L.woff, c->link_pre, first, block_slot, c->gid, c->desc, c->pos[0],
                            c->pos[1], c->flags, L.warm, stats, vout);
         if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
+    if (eh > e0)
+        hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(eh - e0, 256)), dim3(256), 0, c->stream, pass, L.ent, e0,
+                           eh, L.eslot, L.soff, L.poff, L.sched, first, block_slot, c->gid, c->desc,
+                           c->pos[0], c->pos[1], c->link_pre);
     if (L.n_wave)
         hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_wave), dim3(64), 0, c->stream, pass, L.sched, L.soff,
                            L.poff, L.woff, c->link_pre, first, first, c->gid, c->desc, c->pos[0],

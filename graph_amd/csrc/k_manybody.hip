@@ -673,13 +673,6 @@ __device__ inline void far_term(float4 c, float2 p, bool use, float& fx, float& 
     fy += w * dy;
 }
 
-// Bit (6 r + c) set for the cells of a 6x6 block at offsets (c, r) outside the
-// 3x3 around the query cell, which sits at offset (2 + (q & 1)) in each axis.
-__device__ inline unsigned long long ring_mask(int qx, int qy) {
-    const unsigned long long excl = (0x7ull << ((qx & 1) + 1)) * (0x1041ull << (6 * ((qy & 1) + 1)));
-    return 0xFFFFFFFFFull & ~excl;
-}
-
 // Level 0: every cell of the 4x4 grid outside the 3x3 neighbourhood
 // (force-level.frag:70-79).  `cells` is the level (global or staged).  Rows
 // are not unrolled, so only one row's loads are in flight per lane.
@@ -696,29 +689,44 @@ __device__ inline void far_level0(const float4* cells, float2 p, const GDesc& d,
 }
 
 // Level >= 1: the aligned 6x6 child block minus the 3x3 (force-level.frag:80-95).
-// Cell (cx, cy) is read at tile[(cy - y0) * w + (cx - x0)]: the level itself
-// with x0 = y0 = 0, w = g, or a staged window of it.  Cells outside the grid
-// are masked off and read at a clamped address.
+// The 3x3 sits at offset 1 + (q & 1) in each axis of the block, so the ring is
+// exactly: rows {0, 4 - 3 py, 5} in full, and rows 1 + py .. 3 + py at columns
+// {0, 4 - 3 px, 5} — 27 cells, visited without a mask.  Cell (cx, cy) is read
+// at tile[(cy - y0) * w + (cx - x0)]: the level itself with x0 = y0 = 0, w = g,
+// or a staged window of it.  Cells outside the grid are read at a clamped
+// address and weighted 0.
 template <bool FAST>
 __device__ inline void far_level(const float4* tile, int x0, int y0, int w, int g, int qx, int qy,
                                  float2 p, float& fx, float& fy) {
     const int bx = (qx >> 1) * 2 - 2, by = (qy >> 1) * 2 - 2;
-    const unsigned long long m = ring_mask(qx, qy);
-    uint32_t cmask = 0x3Fu;
+    const int px = qx & 1, py = qy & 1;
     int col[6];
+    bool cok[6];
 #pragma unroll
     for (int o = 0; o < 6; ++o) {
         const int cx = bx + o;
-        if (cx < 0 || cx >= g) cmask &= ~(1u << o);
+        cok[o] = cx >= 0 && cx < g;
         col[o] = min(max(cx, 0), g - 1) - x0;
     }
+    const int hcol = px ? col[1] : col[4];  // the hole rows' middle column, 4 - 3 px
+    const bool hok = px ? cok[1] : cok[4];
 #pragma unroll 1
-    for (int j = 0; j < 6; ++j) {
-        const int cy = by + j;
-        const uint32_t rm = (cy >= 0 && cy < g) ? ((uint32_t)(m >> (6 * j)) & cmask) : 0u;
+    for (int k = 0; k < 3; ++k) {  // full rows
+        const int r = k == 0 ? 0 : (k == 1 ? 4 - 3 * py : 5);
+        const int cy = by + r;
+        const bool rok = cy >= 0 && cy < g;
         const int rb = (min(max(cy, 0), g - 1) - y0) * w;
 #pragma unroll
-        for (int ii = 0; ii < 6; ++ii) far_term<FAST>(tile[rb + col[ii]], p, (rm >> ii) & 1u, fx, fy);
+        for (int ii = 0; ii < 6; ++ii) far_term<FAST>(tile[rb + col[ii]], p, rok && cok[ii], fx, fy);
+    }
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) {  // hole rows: the three columns outside the hole
+        const int cy = by + 1 + py + k;
+        const bool rok = cy >= 0 && cy < g;
+        const int rb = (min(max(cy, 0), g - 1) - y0) * w;
+        far_term<FAST>(tile[rb + col[0]], p, rok && cok[0], fx, fy);
+        far_term<FAST>(tile[rb + hcol], p, rok && hok, fx, fy);
+        far_term<FAST>(tile[rb + col[5]], p, rok && cok[5], fx, fy);
     }
 }
 
