@@ -30,6 +30,7 @@ constexpr uint32_t CELL_BLOCK = 256;  // each graph's finest cells start on a mu
 // Link degree tiers: points with at most LANE_MAX entries run one lane each;
 // up to WAVE_MAX entries one 64-lane workgroup each; above, 1024 lanes each.
 constexpr uint32_t LANE_MAX = 32;
+constexpr uint32_t LANE_SPLIT = 8;  // lane-tier schedule classes: <= LANE_SPLIT entries, then the rest
 constexpr uint32_t WAVE_MAX = 4096;
 
 // Passes of one tick, in order (src/index.ts:2012-2078).  Bit p of GDesc::passes
@@ -125,7 +126,7 @@ struct Graph {
 struct LinkDir {
     uint32_t E_live = 0;         // live entries over all graphs
     uint4* ent = nullptr;        // schedule-ordered live entries (global neighbour indices)
-    uint32_t* sched = nullptr;   // present points by ascending live degree
+    uint32_t* sched = nullptr;   // present points: lane tier in two degree classes (index order), then by degree
     uint32_t* soff = nullptr;    // [n_sched+1]
     uint32_t n_sched = 0;
     uint32_t n_lane = 0, n_wave = 0, n_block = 0;

@@ -118,6 +118,17 @@ struct GLinkPtr {
     uint32_t n;
 };
 
+__device__ inline uint32_t live_degree(uint32_t p, const uint32_t* __restrict__ gid,
+                                       const GDesc* __restrict__ desc, const GLinkPtr* __restrict__ tab) {
+    const uint32_t g = gid[p];
+    const GLinkPtr t = tab[g];
+    return t.live ? t.live[p - desc[g].p0] : 0u;
+}
+
+// Schedule sort keys.  Chunked-tier points sort by live degree.  Lane-tier
+// points sort into two classes only (at most LANE_SPLIT entries, then the
+// rest), so within a class the stable sort keeps ascending point index and a
+// wave's own-point loads and stores fall on neighbouring addresses.
 __global__ void k_sched_keys(const uint32_t* __restrict__ present, uint32_t np,
                              const uint32_t* __restrict__ gid, const GDesc* __restrict__ desc,
                              const GLinkPtr* __restrict__ tab, uint32_t* __restrict__ keys,
@@ -125,14 +136,22 @@ __global__ void k_sched_keys(const uint32_t* __restrict__ present, uint32_t np,
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np) return;
     const uint32_t p = present[k];
-    const uint32_t g = gid[p];
-    const GLinkPtr t = tab[g];
-    keys[k] = t.live ? t.live[p - desc[g].p0] : 0u;
+    const uint32_t deg = live_degree(p, gid, desc, tab);
+    keys[k] = deg > LANE_MAX ? deg : (deg > LANE_SPLIT ? 1u : 0u);
     vals[k] = p;
 }
 
-// Tier boundaries in the degree-sorted schedule: first position above LANE_MAX
-// and first above WAVE_MAX (counts pre-filled with ns).  One writer each.
+// Live degree of every schedule slot, in schedule order.
+__global__ void k_sched_degree(const uint32_t* __restrict__ sched, uint32_t ns,
+                               const uint32_t* __restrict__ gid, const GDesc* __restrict__ desc,
+                               const GLinkPtr* __restrict__ tab, uint32_t* __restrict__ deg) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < ns) deg[j] = live_degree(sched[j], gid, desc, tab);
+}
+
+// Tier boundaries over the schedule's degrees (lane-tier degrees first, then
+// the chunked tiers ascending): first position above LANE_MAX and first above
+// WAVE_MAX (counts pre-filled with ns).  One writer each.
 __global__ void k_tier_bounds(const uint32_t* __restrict__ skeys, uint32_t ns,
                               uint32_t* __restrict__ counts) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -216,8 +235,10 @@ __global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __r
                                                    float2* __restrict__ pos1,
                                                    const uint8_t* __restrict__ flags,
                                                    float2* __restrict__ vout) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= count) return;
+    // the longer-chain class starts first
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count) return;
+    const uint32_t j = count - 1 - r;
     const uint32_t p = sched[j];
     const GDesc& d = desc[gid[p]];
     if (!(d.passes & (1u << pass))) return;
@@ -331,7 +352,8 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint32_t* __re
     __shared__ float s_delta[B / 64];
     __shared__ float2 s_final;
     const int t = threadIdx.x;
-    const uint32_t j = first + blockIdx.x;
+    // the schedule ascends by degree: the longest chains start first
+    const uint32_t j = first + (gridDim.x - 1 - blockIdx.x);
     const uint32_t p = sched[j];
     const GDesc& d = desc[gid[p]];
     if (!(d.passes & (1u << pass))) return;  // uniform over the block
@@ -634,6 +656,8 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         size_t sb = tbytes;
         CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(d_tmp, sb, k0, k1, v0, L.sched, (int)ns, 0, 32,
                                                      c->stream));
+        hipLaunchKernelGGL(k_sched_degree, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.sched, ns,
+                           c->gid, c->desc, tab, k1);
         CG_HIP(c, hipMemsetAsync(k1 + ns, 0, 4, c->stream));
         sb = tbytes;
         CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, k1, L.soff, (int)ns + 1, c->stream));

@@ -306,6 +306,38 @@ __device__ inline void write_cell(const GDesc& d, uint32_t rm, long long sx, lon
     }
 }
 
+// Accumulates the points at sorted positions k0, k0 + STRIDE, ... below e into
+// one lane's integer sums and top-8 list, copying their positions to
+// sorted_pos.  Loads go out CELL_BATCH at a time (indices, then positions) so
+// a lane waits on two memory latencies per batch rather than per point; the
+// sums are exact and the top-8 rule is order-free, so the batching changes no
+// result.
+constexpr int CELL_BATCH = 8;
+template <uint32_t STRIDE>
+__device__ inline void cell_points(uint32_t k0, uint32_t e, const uint32_t* __restrict__ sv,
+                                   const float2* __restrict__ pin, float2* __restrict__ sorted_pos,
+                                   const GDesc& d, long long& sx, long long& sy, uint32_t (&tb)[SLOTS],
+                                   uint32_t (&ti)[SLOTS]) {
+    for (uint32_t k = k0; k < e; k += CELL_BATCH * STRIDE) {
+        uint32_t ii[CELL_BATCH];
+        float2 pp[CELL_BATCH];
+#pragma unroll
+        for (int u = 0; u < CELL_BATCH; ++u) ii[u] = sv[min(k + u * STRIDE, e - 1)];
+#pragma unroll
+        for (int u = 0; u < CELL_BATCH; ++u) pp[u] = pin[ii[u]];
+#pragma unroll
+        for (int u = 0; u < CELL_BATCH; ++u) {
+            const uint32_t kk = k + u * STRIDE;
+            if (kk < e) {
+                sorted_pos[kk] = pp[u];
+                sx += fixed(pp[u].x, d.fx_scale);
+                sy += fixed(pp[u].y, d.fx_scale);
+                top8_insert(tb, ti, ticket_bits(ii[u] - d.p0, d.seed), ii[u]);
+            }
+        }
+    }
+}
+
 // One lane per finest cell, one block per CELL_BLOCK cells of one graph.  Empty
 // cells write nothing: the pyramid reads their count, and their slots are
 // never read.  Cells above BIG_CELL points go to k_cells_mid (one wave each);
@@ -356,14 +388,7 @@ __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
     long long sx = 0, sy = 0;
-    for (uint32_t k = b; k < e; ++k) {
-        uint32_t i = sv[k];
-        float2 p = pin[i];
-        sorted_pos[k] = p;
-        sx += fixed(p.x, d.fx_scale);
-        sy += fixed(p.y, d.fx_scale);
-        top8_insert(tb, ti, ticket_bits(i - d.p0, d.seed), i);
-    }
+    cell_points<1>(b, e, sv, pin, sorted_pos, d, sx, sy, tb, ti);
     write_cell(d, rm, sx, sy, n, ti, pin, sums, slot_idx, slot_pos);
 }
 
@@ -418,14 +443,7 @@ __global__ void __launch_bounds__(256) k_cells_mid(const uint32_t* __restrict__ 
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
         long long sx = 0, sy = 0;
-        for (uint32_t k = b + lane; k < e; k += 64) {
-            uint32_t i = sv[k];
-            float2 p = pin[i];
-            sorted_pos[k] = p;
-            sx += fixed(p.x, d.fx_scale);
-            sy += fixed(p.y, d.fx_scale);
-            top8_insert(tb, ti, ticket_bits(i - d.p0, d.seed), i);
-        }
+        cell_points<64>(b + lane, e, sv, pin, sorted_pos, d, sx, sy, tb, ti);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             sx += __shfl_xor(sx, o, 64);
@@ -484,14 +502,7 @@ __global__ void __launch_bounds__(256) k_cells_chunk(const uint32_t* __restrict_
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
         long long sx = 0, sy = 0;
-        for (uint32_t k = b + t; k < end; k += 256) {
-            uint32_t i = sv[k];
-            float2 p = pin[i];
-            sorted_pos[k] = p;
-            sx += fixed(p.x, d.fx_scale);
-            sy += fixed(p.y, d.fx_scale);
-            top8_insert(tb, ti, ticket_bits(i - d.p0, d.seed), i);
-        }
+        cell_points<256>(b + t, end, sv, pin, sorted_pos, d, sx, sy, tb, ti);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             sx += __shfl_xor(sx, o, 64);
