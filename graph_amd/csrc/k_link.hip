@@ -333,6 +333,21 @@ __device__ inline void spring_step(float4 e, float2 pp, float2& v, float (&J)[4]
     J[0] = j0; J[1] = j1; J[2] = j2; J[3] = j3;
 }
 
+// Runs one lane's chunk (steps entries, stride B) from start velocity v,
+// accumulating the chunk's Jacobian; PF loads in flight (index clamped into
+// the chunk, so all are issued together).
+template <int PF, int B>
+__device__ inline void run_chunk(const float4* __restrict__ mine, uint32_t steps, float2 pp, float2& v,
+                                 float (&J)[4]) {
+    for (uint32_t k = 0; k < steps; k += PF) {
+        float4 buf[PF];
+#pragma unroll
+        for (int u = 0; u < PF; ++u) buf[u] = mine[(size_t)min(k + u, steps - 1) * B];
+#pragma unroll
+        for (int u = 0; u < PF; ++u) spring_step(buf[u], pp, v, J, k + u < steps);
+    }
+}
+
 template <int B>
 __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint32_t* __restrict__ sched,
                                                   const uint32_t* __restrict__ soff,
@@ -373,14 +388,9 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint32_t* __re
     for (int sweep = 0;; ++sweep) {
         float2 v = g;
         float J[4] = {1.f, 0.f, 0.f, 1.f};
-        for (uint32_t k = 0; k < steps; k += PREFETCH) {
-            // unconditional loads (index clamped into the chunk) so all are in flight together
-            float4 buf[PREFETCH];
-#pragma unroll
-            for (int u = 0; u < PREFETCH; ++u) buf[u] = mine[(size_t)min(k + u, steps - 1) * B];
-#pragma unroll
-            for (int u = 0; u < PREFETCH; ++u) spring_step(buf[u], pp, v, J, k + u < steps);
-        }
+        // short chunks (most 64-lane points) take a short batch: no idle steps
+        if (c <= 4) run_chunk<4, B>(mine, steps, pp, v, J);
+        else run_chunk<PREFETCH, B>(mine, steps, pp, v, J);
         // chunk map  s -> J (s - g) + v  =  J s + b;  inclusive scan of the composition
         float cJ[4] = {J[0], J[1], J[2], J[3]};
         float cx = v.x - (J[0] * g.x + J[2] * g.y), cy = v.y - (J[1] * g.x + J[3] * g.y);
