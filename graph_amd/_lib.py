@@ -352,12 +352,14 @@ class Context:
         return {s: (float(ms[i]), int(cnt[i])) for i, s in enumerate(STAGES) if cnt[i]}
 
     def debug_link_stats(self) -> dict:
-        out = np.zeros(8, np.uint32)
+        out = np.zeros(12, np.uint32)
         self._check(self._l.cg_debug_link_stats(self._ctx, uptr(out)))
         t = {}
         for name, o in (("wave", 0), ("block", 4)):
             s, n, mx, cap = (int(v) for v in out[o:o + 4])
-            t[name] = {"mean_sweeps": s / n if n else 0.0, "points": n, "max_sweeps": mx, "at_cap": cap}
+            t[name] = {"mean_sweeps": s / n if n else 0.0, "units": n, "max_sweeps": mx, "at_cap": cap}
+        s, n, mx, cap = (int(v) for v in out[8:12])
+        t["split"] = {"mean_rounds": s / n if n else 0.0, "points": n, "max_rounds": mx, "at_cap": cap}
         return t
 
     def device_bytes(self) -> int:

@@ -32,6 +32,8 @@ constexpr uint32_t CELL_BLOCK = 256;  // each graph's finest cells start on a mu
 constexpr uint32_t LANE_MAX = 32;
 constexpr uint32_t LANE_SPLIT = 8;  // lane-tier schedule classes: <= LANE_SPLIT entries, then the rest
 constexpr uint32_t WAVE_MAX = 4096;
+constexpr uint32_t HUB_SEG = 16384;  // entries per segment of a split 1024-lane point
+constexpr int HUB_ROUNDS = 3;        // segment/fix-up rounds per pass for split points
 
 // Passes of one tick, in order (src/index.ts:2012-2078).  Bit p of GDesc::passes
 // says the graph runs pass p this tick; bit p of GDesc::inbuf names the position
@@ -132,9 +134,17 @@ struct LinkDir {
     uint32_t n_lane = 0, n_wave = 0, n_block = 0;
     uint32_t e_chunk0 = 0;       // first entry of the chunked tiers
     uint32_t e_hub0 = 0;         // first entry of the 1024-lane tier
-    uint32_t* poff = nullptr;    // [n_wave+n_block+1] step-major offsets into link_pre
-    uint32_t* eslot = nullptr;   // schedule slot of each chunked-tier entry
-    uint32_t* woff = nullptr;    // [n_wave+n_block] warm-start offset of each chunked slot
+    // chunked-tier work units: one per 64-lane point, then one per segment of
+    // each 1024-lane point (hubs above HUB_SEG entries are split)
+    uint32_t n_units = 0, n_wave_units = 0;
+    uint32_t first_split_unit = 0;  // split hubs come last (degree order)
+    uint4* unit_a = nullptr;     // {first entry, entries, offset into link_pre, graph}
+    uint4* unit_b = nullptr;     // {schedule slot, warm offset, split hub or NO_HUB, segment}
+    uint32_t* eslot = nullptr;   // work unit of each chunked-tier entry
+    uint2* hubs = nullptr;       // split hubs: {first unit, segments}
+    uint32_t n_hubs = 0;
+    float4* seg_out = nullptr;   // [2 n_units] segment end, start used, end Jacobian
+    uint32_t* hub_done = nullptr;  // [2 n_hubs] fix-up converged this pass; last correction (f32)
     float2* warm = nullptr;      // last converged chunk starts, per graph regions
     size_t warm_len = 0;
 };

@@ -17,6 +17,12 @@
 //     moves more than the tolerance; the last chunk's end, moved to its
 //     corrected start, is the point's velocity.  Converged starts are kept per
 //     point as the next call's first guesses.
+//   - a 1024-lane point above HUB_SEG entries is cut into segments, one
+//     workgroup each (one CU no longer carries a whole hub): each segment is
+//     solved as above from a guessed start and reports its end and end
+//     Jacobian; a fix-up composes the segments from the exact start 0 and
+//     corrects the guesses (the same Newton step, one level up), for up to
+//     HUB_ROUNDS rounds.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -193,16 +199,16 @@ __global__ void k_assemble(const uint32_t* __restrict__ ebase, uint32_t G, uint3
     ent[soff[slot_of[p]] + (tb.lrank[e] - tb.lrank[tb.off_pt[lo]])] = v;
 }
 
-// Schedule slot of every chunked-tier entry: the last slot j in [first, last)
-// with soff[j] <= e, by binary search.
-__global__ void k_eslot(const uint32_t* __restrict__ soff, uint32_t first, uint32_t last,
-                        uint32_t e0, uint32_t e1, uint32_t* __restrict__ eslot) {
+// Work unit of every chunked-tier entry: the last unit u with ubeg[u] <= e,
+// by binary search (ubeg holds n_units + 1 ascending entry offsets).
+__global__ void k_eslot(const uint32_t* __restrict__ ubeg, uint32_t n_units, uint32_t e0, uint32_t e1,
+                        uint32_t* __restrict__ eslot) {
     uint32_t e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= e1) return;
-    uint32_t lo = first, hi = last;  // invariant: soff[lo] <= e < soff[hi]
+    uint32_t lo = 0, hi = n_units;  // invariant: ubeg[lo] <= e < ubeg[hi]
     while (hi - lo > 1) {
         uint32_t mid = lo + (hi - lo) / 2;
-        if (soff[mid] <= e) lo = mid;
+        if (ubeg[mid] <= e) lo = mid;
         else hi = mid;
     }
     eslot[e - e0] = lo;
@@ -277,32 +283,31 @@ constexpr int PREFETCH = 16;  // entries in flight per lane in the chunk loop
 
 // Per-entry terms that do not depend on the running velocity, gathered in one
 // coalesced pass for the chunked tiers: (q.x, q.y, D, K) with D the rest length
-// and K = linkSpring * alpha * strength * bias.  Stored step-major per point:
-// entry k of a point with d entries run by B lanes (chunk c = ceil(d/B)) sits at
-// poff[slot] + (k % c) * B + k / c, so lane t reads its step-s term at s*B + t
-// and a wave's loads are one contiguous segment.
+// and K = linkSpring * alpha * strength * bias.  A work unit (a 64-lane point,
+// or one segment of a hub's chain run by 1024 lanes) stores its len entries
+// step-major: with c = ceil(len/B) entries per lane, entry k of the unit sits at
+// poff + (k % c) * B + k / c, so lane t reads its step-s term at s*B + t and a
+// wave's loads are one contiguous segment.  unit_a = {first entry, len, poff,
+// graph}.
 __global__ void __launch_bounds__(256) k_link_pre(int pass, const uint4* __restrict__ ent,
                                                   uint32_t e0, uint32_t e1,
                                                   const uint32_t* __restrict__ eslot,
-                                                  const uint32_t* __restrict__ soff,
-                                                  const uint32_t* __restrict__ poff,
-                                                  const uint32_t* __restrict__ sched,
-                                                  uint32_t first_slot, uint32_t block_slot,
-                                                  const uint32_t* __restrict__ gid,
+                                                  const uint4* __restrict__ unit_a, uint32_t wave_units,
                                                   const GDesc* __restrict__ desc,
                                                   const float2* __restrict__ pos0,
                                                   const float2* __restrict__ pos1,
                                                   float4* __restrict__ pre) {
     uint32_t e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= e1) return;
-    const uint32_t j = eslot[e - e0];
-    const GDesc& d = desc[gid[sched[j]]];
+    const uint32_t u = eslot[e - e0];
+    const uint4 U = unit_a[u];
+    const GDesc& d = desc[U.w];
     if (!(d.passes & (1u << pass))) return;
     const float2* __restrict__ pin = ((d.inbuf >> pass) & 1u) ? pos1 : pos0;
-    const uint32_t local = e - soff[j], deg = soff[j + 1] - soff[j];
-    const uint32_t B = j < block_slot ? 64u : 1024u;
-    const uint32_t c = (deg + B - 1) / B;
-    const uint32_t dst = poff[j - first_slot] + (local % c) * B + local / c;
+    const uint32_t local = e - U.x;
+    const uint32_t B = u < wave_units ? 64u : 1024u;
+    const uint32_t c = (U.y + B - 1) / B;
+    const uint32_t dst = U.z + (local % c) * B + local / c;
     uint4 en = ent[e];
     float2 q = pin[en.x];
     float r = __uint_as_float(en.y), s = __uint_as_float(en.z), b = __uint_as_float(en.w);
@@ -348,102 +353,150 @@ __device__ inline void run_chunk(const float4* __restrict__ mine, uint32_t steps
     }
 }
 
+// Affine map s -> A s + b of a velocity (A column-major 2x2).
+struct Aff {
+    float a0, a1, a2, a3, bx, by;
+};
+
+__device__ inline Aff aff_identity() { return Aff{1.f, 0.f, 0.f, 1.f, 0.f, 0.f}; }
+
+// f then g:  s -> g(f(s))
+__device__ inline Aff aff_then(const Aff& f, const Aff& g) {
+    return Aff{g.a0 * f.a0 + g.a2 * f.a1, g.a1 * f.a0 + g.a3 * f.a1,
+               g.a0 * f.a2 + g.a2 * f.a3, g.a1 * f.a2 + g.a3 * f.a3,
+               g.a0 * f.bx + g.a2 * f.by + g.bx, g.a1 * f.bx + g.a3 * f.by + g.by};
+}
+
+__device__ inline Aff aff_shfl_up(const Aff& m, int o) {
+    return Aff{__shfl_up(m.a0, o, 64), __shfl_up(m.a1, o, 64), __shfl_up(m.a2, o, 64),
+               __shfl_up(m.a3, o, 64), __shfl_up(m.bx, o, 64), __shfl_up(m.by, o, 64)};
+}
+
+// Scan of the lanes' chunk maps in lane order over a block of B lanes: returns
+// lane t's exclusive prefix (maps 0..t-1 composed; identity for lane 0) and
+// sets `total` to all B maps composed.  Shuffles inside each wave, then wave 0
+// scans the wave totals: two barriers per call for B = 1024, none for 64.
 template <int B>
-__global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint32_t* __restrict__ sched,
-                                                  const uint32_t* __restrict__ soff,
-                                                  const uint32_t* __restrict__ poff,
-                                                  const uint32_t* __restrict__ woff,
-                                                  const float4* __restrict__ pre,
-                                                  uint32_t first_slot, uint32_t first,
-                                                  const uint32_t* __restrict__ gid,
+__device__ inline Aff block_scan(Aff m, Aff* s_wave, Aff* s_wave_inc, Aff& total) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const Aff p = aff_shfl_up(m, o);
+        if (lane >= o) m = aff_then(p, m);
+    }
+    Aff ex = aff_shfl_up(m, 1);
+    if (lane == 0) ex = aff_identity();
+    if (B == 64) {
+        total = Aff{__shfl(m.a0, 63, 64), __shfl(m.a1, 63, 64), __shfl(m.a2, 63, 64),
+                    __shfl(m.a3, 63, 64), __shfl(m.bx, 63, 64), __shfl(m.by, 63, 64)};
+        return ex;
+    }
+    constexpr int NW = B / 64;
+    if (lane == 63) s_wave[w] = m;
+    __syncthreads();
+    if (w == 0) {
+        Aff a = lane < NW ? s_wave[lane] : aff_identity();
+#pragma unroll
+        for (int o = 1; o < NW; o <<= 1) {
+            const Aff p = aff_shfl_up(a, o);
+            if (lane >= o) a = aff_then(p, a);
+        }
+        if (lane < NW) s_wave_inc[lane] = a;
+    }
+    __syncthreads();
+    total = s_wave_inc[NW - 1];
+    return w == 0 ? ex : aff_then(s_wave_inc[w - 1], ex);
+}
+
+constexpr uint32_t NO_HUB = 0xffffffffu;
+
+// Chunked tiers: one workgroup of B lanes per work unit.  unit_a = {first
+// entry, len, poff, graph}; unit_b = {slot, warm offset, split hub or NO_HUB,
+// segment}.  Each lane runs its chunk of the unit's chain from a guessed start
+// (warm: the start converged on the previous call), the block scans the chunk
+// maps, and sweeps repeat until no start moves more than the tolerance; the
+// last chunk's end, moved to its corrected start, is the unit's end velocity.
+// The chain of a unit starts from the velocity in lane 0's warm slot: 0 for a
+// point's first unit, the fix-up's current guess for a later segment of a
+// split hub.  A whole point writes its integrated position; a segment writes
+// {end, start used} and its end's Jacobian in its start for the fix-up.
+// Round r > 0 reruns only the segments after the first of split hubs whose
+// fix-up has not converged.
+template <int B>
+__global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restrict__ unit_a,
+                                                  const uint4* __restrict__ unit_b, uint32_t u_first,
+                                                  int round, const uint32_t* __restrict__ sched,
                                                   const GDesc* __restrict__ desc,
                                                   float2* __restrict__ pos0,
                                                   float2* __restrict__ pos1,
                                                   const uint8_t* __restrict__ flags,
+                                                  const float4* __restrict__ pre,
                                                   float2* __restrict__ warm,
+                                                  float4* __restrict__ seg_out,
+                                                  uint32_t* __restrict__ hub_done,
                                                   uint32_t* __restrict__ stats,
                                                   float2* __restrict__ vout) {
-    __shared__ float sJ[4][B], sX[B], sY[B];
+    __shared__ Aff s_wave[B / 64], s_wave_inc[B / 64];
     __shared__ float s_delta[B / 64];
     __shared__ float2 s_final;
     const int t = threadIdx.x;
-    // the schedule ascends by degree: the longest chains start first
-    const uint32_t j = first + (gridDim.x - 1 - blockIdx.x);
-    const uint32_t p = sched[j];
-    const GDesc& d = desc[gid[p]];
+    // units ascend by chain length: the longest start first
+    const uint32_t u = u_first + (gridDim.x - 1 - blockIdx.x);
+    const uint4 U = unit_a[u], V = unit_b[u];
+    const GDesc& d = desc[U.w];
     if (!(d.passes & (1u << pass))) return;  // uniform over the block
+    const bool split = V.z != NO_HUB;
+    if (round > 0 && (!split || V.w == 0 || hub_done[V.z])) return;
+    if (split && V.w == 0 && t == 0) hub_done[V.z] = 0u;  // read by the fix-up after this launch
+    const uint32_t p = sched[V.x];
     const bool bsel = (d.inbuf >> pass) & 1u;
     const float2* __restrict__ pin = bsel ? pos1 : pos0;
-    // chunk starts converged on the previous call: the first guesses
-    float2* __restrict__ wslot = warm + woff[j - first_slot] + t;
-    const uint32_t deg = soff[j + 1] - soff[j];
-    const uint32_t c = (deg + B - 1) / B;
-    const uint32_t lo = min(deg, (uint32_t)t * c), hi = min(deg, lo + c);
+    float2* __restrict__ wslot = warm + V.y + t;
+    const uint32_t len = U.y;
+    const uint32_t c = (len + B - 1) / B;
+    const uint32_t lo = min(len, (uint32_t)t * c), hi = min(len, lo + c);
     const uint32_t steps = hi - lo;
-    const int last = (int)((deg - 1) / c);
+    const int last = (int)((len - 1) / c);
     const float2 pp = pin[p];
-    const float4* __restrict__ mine = pre + poff[j - first_slot] + t;
+    const float4* __restrict__ mine = pre + U.z + t;
     float2 g = *wslot;
+    const float2 G = warm[V.y];  // lane 0's slot: the unit's start
     float prev = INFINITY;
+    float J[4];
+    float2 v;
+    Aff tot;
     for (int sweep = 0;; ++sweep) {
-        float2 v = g;
-        float J[4] = {1.f, 0.f, 0.f, 1.f};
+        v = g;
+        J[0] = 1.f; J[1] = 0.f; J[2] = 0.f; J[3] = 1.f;
         // short chunks (most 64-lane points) take a short batch: no idle steps
         if (c <= 4) run_chunk<4, B>(mine, steps, pp, v, J);
         else run_chunk<PREFETCH, B>(mine, steps, pp, v, J);
-        // chunk map  s -> J (s - g) + v  =  J s + b;  inclusive scan of the composition
-        float cJ[4] = {J[0], J[1], J[2], J[3]};
-        float cx = v.x - (J[0] * g.x + J[2] * g.y), cy = v.y - (J[1] * g.x + J[3] * g.y);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) sJ[q][t] = cJ[q];
-        sX[t] = cx;
-        sY[t] = cy;
-        __syncthreads();
-        for (int off = 1; off < B; off <<= 1) {
-            float pJ[4] = {1.f, 0.f, 0.f, 1.f}, px = 0.f, py = 0.f;
-            if (t >= off) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) pJ[q] = sJ[q][t - off];
-                px = sX[t - off];
-                py = sY[t - off];
-            }
-            __syncthreads();
-            if (t >= off) {
-                // mine after theirs: s -> cJ (pJ s + p) + c
-                float nx = cJ[0] * px + cJ[2] * py + cx;
-                float ny = cJ[1] * px + cJ[3] * py + cy;
-                float n0 = cJ[0] * pJ[0] + cJ[2] * pJ[1], n1 = cJ[1] * pJ[0] + cJ[3] * pJ[1];
-                float n2 = cJ[0] * pJ[2] + cJ[2] * pJ[3], n3 = cJ[1] * pJ[2] + cJ[3] * pJ[3];
-                cJ[0] = n0; cJ[1] = n1; cJ[2] = n2; cJ[3] = n3;
-                cx = nx;
-                cy = ny;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) sJ[q][t] = cJ[q];
-                sX[t] = cx;
-                sY[t] = cy;
-            }
-            __syncthreads();
-        }
-        float2 start = (t == 0) ? make_float2(0.f, 0.f) : make_float2(sX[t - 1], sY[t - 1]);
+        // chunk map  s -> J (s - g) + v;  exclusive prefix applied to the unit's start
+        const Aff m{J[0], J[1], J[2], J[3], v.x - (J[0] * g.x + J[2] * g.y), v.y - (J[1] * g.x + J[3] * g.y)};
+        const Aff ex = block_scan<B>(m, s_wave, s_wave_inc, tot);
+        const float2 start = make_float2(ex.a0 * G.x + ex.a2 * G.y + ex.bx, ex.a1 * G.x + ex.a3 * G.y + ex.by);
         float dlt = fmaxf(fabsf(start.x - g.x), fabsf(start.y - g.y));
         if (steps == 0) dlt = 0.f;
         // block max of the start corrections
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) dlt = fmaxf(dlt, __shfl_xor(dlt, o, 64));
-        if ((t & 63) == 0) s_delta[t >> 6] = dlt;
-        __syncthreads();
-        float mx = s_delta[0];
-        for (int w = 1; w < B / 64; ++w) mx = fmaxf(mx, s_delta[w]);
+        float mx = dlt;
+        if (B > 64) {
+            if ((t & 63) == 0) s_delta[t >> 6] = dlt;
+            __syncthreads();
+            mx = s_delta[0];
+            for (int w = 1; w < B / 64; ++w) mx = fmaxf(mx, s_delta[w]);
+        }
         // converged, at the f32 noise floor (no longer halving), or at the cap
         if (mx <= d.tol || (sweep >= 2 && mx > 0.5f * prev) || sweep == MAX_SWEEPS) {
             // the last chunk's end, moved to its corrected start
             if (t == last) {
-                float ex = start.x - g.x, ey = start.y - g.y;
-                s_final = make_float2(v.x + J[0] * ex + J[2] * ey, v.y + J[1] * ex + J[3] * ey);
+                float ex_ = start.x - g.x, ey = start.y - g.y;
+                s_final = make_float2(v.x + J[0] * ex_ + J[2] * ey, v.y + J[1] * ex_ + J[3] * ey);
             }
             *wslot = start;
             if (stats && t == 0) {
-                // sweeps used, points, max sweeps, points not within tolerance (per tier)
+                // sweeps used, units, max sweeps, units not within tolerance (per tier)
                 const int o = B == 64 ? 0 : 4;
                 atomicAdd(&stats[o + 0], (uint32_t)(sweep + 1));
                 atomicAdd(&stats[o + 1], 1u);
@@ -454,17 +507,77 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint32_t* __re
         }
         prev = mx;
         g = start;
-        __syncthreads();  // s_delta is rewritten next sweep
+        if (B > 64) __syncthreads();  // s_delta is rewritten next sweep
     }
     __syncthreads();
     if (t == 0) {
-        float2 v = s_final;
-        if (vout) {
-            vout[p] = v;
+        const float2 vf = s_final;
+        if (split) {
+            seg_out[2 * (size_t)u] = make_float4(vf.x, vf.y, G.x, G.y);
+            seg_out[2 * (size_t)u + 1] = make_float4(tot.a0, tot.a1, tot.a2, tot.a3);
+        } else if (vout) {
+            vout[p] = vf;
         } else {
             float2* __restrict__ pout = bsel ? pos0 : pos1;
-            pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, v, d.friction, d.S);
+            pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, vf, d.friction, d.S);
         }
+    }
+}
+
+// Fix-up of the split hubs, one lane per hub (hubs[h] = {first unit, segments}):
+// composes the segments' linearised end maps from the exact start 0,
+//   true_{s+1} = E_s + A_s (true_s - G_s),
+// stores each true start as the segment's next start guess, and when every
+// guess was already within the tolerance, the corrections stopped halving
+// (the f32 noise floor of long segments), or on the final round, writes the
+// point's velocity (the last segment's end from its true start).
+// hub_done holds [n_hubs] flags, then [n_hubs] previous largest corrections.
+__global__ void k_hub_fix(int pass, const uint2* __restrict__ hubs, uint32_t n_hubs,
+                          const uint4* __restrict__ unit_a, const uint4* __restrict__ unit_b,
+                          int round, int final_round, const uint32_t* __restrict__ sched,
+                          const GDesc* __restrict__ desc, float2* __restrict__ pos0,
+                          float2* __restrict__ pos1, const uint8_t* __restrict__ flags,
+                          const float4* __restrict__ seg_out, float2* __restrict__ warm,
+                          uint32_t* __restrict__ hub_done, uint32_t* __restrict__ stats,
+                          float2* __restrict__ vout) {
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= n_hubs) return;
+    const uint2 hb = hubs[h];
+    const GDesc& d = desc[unit_a[hb.x].w];
+    if (!(d.passes & (1u << pass))) return;
+    if (hub_done[h]) return;
+    float2 tru = make_float2(0.f, 0.f);
+    float dmax = 0.f;
+    for (uint32_t s = 0; s < hb.y; ++s) {
+        const uint32_t u = hb.x + s;
+        const float4 o0 = seg_out[2 * (size_t)u], A = seg_out[2 * (size_t)u + 1];
+        if (s > 0) {
+            dmax = fmaxf(dmax, fmaxf(fabsf(tru.x - o0.z), fabsf(tru.y - o0.w)));
+            warm[unit_b[u].y] = tru;  // lane 0 of the segment: its next start
+        }
+        const float dx = tru.x - o0.z, dy = tru.y - o0.w;
+        tru = make_float2(o0.x + A.x * dx + A.z * dy, o0.y + A.y * dx + A.w * dy);
+    }
+    float* __restrict__ dprev = reinterpret_cast<float*>(hub_done + n_hubs);
+    const bool ok = dmax <= d.tol || (round > 0 && dmax > 0.5f * dprev[h]);
+    dprev[h] = dmax;
+    if (!ok && !final_round) return;
+    hub_done[h] = 1u;
+    const uint32_t p = sched[unit_b[hb.x].x];
+    const bool bsel = (d.inbuf >> pass) & 1u;
+    if (vout) {
+        vout[p] = tru;
+    } else {
+        const float2 pp = (bsel ? pos1 : pos0)[p];
+        float2* __restrict__ pout = bsel ? pos0 : pos1;
+        pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, tru, d.friction, d.S);
+    }
+    if (stats) {
+        // rounds used, split hubs, max rounds, hubs stopped at the round cap
+        atomicAdd(&stats[8], (uint32_t)(round + 1));
+        atomicAdd(&stats[9], 1u);
+        atomicMax(&stats[10], (uint32_t)(round + 1));
+        if (!ok) atomicAdd(&stats[11], 1u);
     }
 }
 
@@ -597,8 +710,8 @@ int link_build_graph(Ctx* c, Graph& gr) {
 // not change carries its region over from the previous packing (old).
 int link_pack(Ctx* c, LinkDir (&old)[2]) {
     auto free_dir = [&](LinkDir& L) {
-        dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff); dfree(c, L.poff);
-        dfree(c, L.eslot); dfree(c, L.woff); dfree(c, L.warm);
+        dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff); dfree(c, L.unit_a); dfree(c, L.unit_b);
+        dfree(c, L.eslot); dfree(c, L.hubs); dfree(c, L.seg_out); dfree(c, L.hub_done); dfree(c, L.warm);
         L = LinkDir();
     };
     for (int d = 0; d < 2; ++d) c->dir[d] = LinkDir();
@@ -689,9 +802,9 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         if (etot)
             hipLaunchKernelGGL(k_assemble, dim3(blocks_for(etot, 256)), dim3(256), 0, c->stream, ebase, G,
                                etot, tab, c->desc, slot_of, L.soff, c->flags, L.ent);
-        // chunked tiers: step-major offsets, entry -> slot map, warm-start offsets
+        // chunked tiers: work units, step-major offsets, entry -> unit map, warm-start offsets
         const uint32_t nch = ns - L.n_lane;
-        std::vector<uint32_t> hs(nch + 1), hp(nch + 1, 0), hsched(nch ? nch : 1), hw(nch ? nch : 1, 0);
+        std::vector<uint32_t> hs(nch + 1), hsched(nch ? nch : 1);
         CG_HIP(c, hipMemcpyAsync(hs.data(), L.soff + L.n_lane, (size_t)(nch + 1) * 4,
                                  hipMemcpyDeviceToHost, c->stream));
         if (nch)
@@ -700,38 +813,81 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         CG_HIP(c, hipStreamSynchronize(c->stream));
         L.e_chunk0 = hs[0];
         L.e_hub0 = hs[L.n_wave];
+        // A 64-lane point is one unit; a 1024-lane point of d entries is
+        // ceil(d / HUB_SEG) near-equal segments, one unit each, in chain order.
+        std::vector<uint4> ua, ub;
+        std::vector<uint32_t> ubeg, slot_u0(nch + 1, 0), slot_graph(nch ? nch : 1, 0);
+        std::vector<uint2> hh;
+        uint32_t pos = 0;
+        L.first_split_unit = 0;
         for (uint32_t k = 0; k < nch; ++k) {
-            uint32_t dd = hs[k + 1] - hs[k];
-            uint32_t B = (k < L.n_wave) ? 64u : 1024u;
-            hp[k + 1] = hp[k] + ((dd + B - 1) / B) * B;
+            const uint32_t dd = hs[k + 1] - hs[k];
+            const bool hub = k >= L.n_wave;
+            const uint32_t B = hub ? 1024u : 64u;
+            const uint32_t S = hub ? (dd + HUB_SEG - 1) / HUB_SEG : 1u;
+            const uint32_t p = hsched[k];
+            const uint32_t gs = (uint32_t)(std::upper_bound(h_p0.begin(), h_p0.end(), p) - h_p0.begin()) - 1;
+            slot_graph[k] = gs;
+            slot_u0[k] = (uint32_t)ua.size();
+            uint32_t hub_id = NO_HUB;
+            if (S > 1) {
+                if (hh.empty()) L.first_split_unit = (uint32_t)ua.size();
+                hub_id = (uint32_t)hh.size();
+                hh.push_back(make_uint2((uint32_t)ua.size(), S));
+            }
+            uint32_t eb = hs[k];
+            for (uint32_t q = 0; q < S; ++q) {
+                const uint32_t len = dd / S + (q < dd % S ? 1u : 0u);
+                ua.push_back(make_uint4(eb, len, pos, gs));
+                ub.push_back(make_uint4(L.n_lane + k, 0u, hub_id, q));
+                ubeg.push_back(eb);
+                eb += len;
+                pos += ((len + B - 1) / B) * B;
+            }
         }
-        pre_need = std::max<size_t>(pre_need, (size_t)hp[nch]);
+        slot_u0[nch] = (uint32_t)ua.size();
+        ubeg.push_back(hs[nch]);
+        L.n_units = (uint32_t)ua.size();
+        L.n_wave_units = L.n_wave;
+        L.n_hubs = (uint32_t)hh.size();
+        if (hh.empty()) L.first_split_unit = L.n_units;
+        pre_need = std::max<size_t>(pre_need, (size_t)pos);
+        // warm-start regions: per graph, its chunked points in local index order,
+        // each point's units in chain order, B lanes each
         std::vector<std::vector<std::pair<uint32_t, uint32_t>>> per(G);  // (local point, chunk slot)
-        for (uint32_t k = 0; k < nch; ++k) {
-            uint32_t p = hsched[k];
-            uint32_t s = (uint32_t)(std::upper_bound(h_p0.begin(), h_p0.end(), p) - h_p0.begin()) - 1;
-            per[s].push_back({p - h_p0[s], k});
-        }
+        for (uint32_t k = 0; k < nch; ++k) per[slot_graph[k]].push_back({hsched[k] - h_p0[slot_graph[k]], k});
         size_t wpos = 0;
         std::vector<std::pair<size_t, size_t>> region(G);  // (offset, length) per graph slot
         for (uint32_t s = 0; s < G; ++s) {
             std::sort(per[s].begin(), per[s].end());
             size_t start = wpos;
             for (auto& pk : per[s]) {
-                hw[pk.second] = (uint32_t)wpos;
-                wpos += (pk.second < L.n_wave) ? 64 : 1024;
+                for (uint32_t u = slot_u0[pk.second]; u < slot_u0[pk.second + 1]; ++u) {
+                    ub[u].y = (uint32_t)wpos;
+                    wpos += (pk.second < L.n_wave) ? 64 : 1024;
+                }
             }
             region[s] = {start, wpos - start};
         }
         L.warm_len = wpos;
-        L.poff = (uint32_t*)dalloc(c, (size_t)(nch + 1) * 4, &rc);
-        L.woff = (uint32_t*)dalloc(c, (size_t)(nch ? nch : 1) * 4, &rc);
+        const size_t nu = ua.size() ? ua.size() : 1;
+        L.unit_a = (uint4*)dalloc(c, nu * 16, &rc);
+        L.unit_b = (uint4*)dalloc(c, nu * 16, &rc);
+        L.seg_out = (float4*)dalloc(c, nu * 32, &rc);
+        L.hubs = (uint2*)dalloc(c, (hh.size() ? hh.size() : 1) * 8, &rc);
+        L.hub_done = (uint32_t*)dalloc(c, (hh.size() ? hh.size() : 1) * 8, &rc);
         L.eslot = (uint32_t*)dalloc(c, (size_t)(L.E_live - L.e_chunk0 + 1) * 4, &rc);
         L.warm = (float2*)dalloc(c, (wpos ? wpos : 1) * 8, &rc);
+        uint32_t* d_ubeg = (uint32_t*)talloc(ubeg.size() * 4);
         if (rc) { cleanup(); return rc; }
-        CG_HIP(c, hipMemcpyAsync(L.poff, hp.data(), (size_t)(nch + 1) * 4, hipMemcpyHostToDevice, c->stream));
-        CG_HIP(c, hipMemcpyAsync(L.woff, hw.data(), (size_t)(nch ? nch : 1) * 4, hipMemcpyHostToDevice,
-                                 c->stream));
+        if (!ua.empty()) {
+            CG_HIP(c, hipMemcpyAsync(L.unit_a, ua.data(), ua.size() * 16, hipMemcpyHostToDevice, c->stream));
+            CG_HIP(c, hipMemcpyAsync(L.unit_b, ub.data(), ub.size() * 16, hipMemcpyHostToDevice, c->stream));
+        }
+        if (!hh.empty())
+            CG_HIP(c, hipMemcpyAsync(L.hubs, hh.data(), hh.size() * 8, hipMemcpyHostToDevice, c->stream));
+        CG_HIP(c, hipMemcpyAsync(d_ubeg, ubeg.data(), ubeg.size() * 4, hipMemcpyHostToDevice, c->stream));
+        CG_HIP(c, hipMemsetAsync(L.hub_done, 0, (hh.size() ? hh.size() : 1) * 8, c->stream));
         CG_HIP(c, hipMemsetAsync(L.warm, 0, (wpos ? wpos : 1) * 8, c->stream));
         for (uint32_t s = 0; s < G; ++s) {
             Graph& g = c->graphs[c->order[s]];
@@ -743,7 +899,7 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         }
         if (L.E_live > L.e_chunk0)
             hipLaunchKernelGGL(k_eslot, dim3(blocks_for(L.E_live - L.e_chunk0, 256)), dim3(256), 0,
-                               c->stream, L.soff, L.n_lane, ns, L.e_chunk0, L.E_live, L.eslot);
+                               c->stream, d_ubeg, L.n_units, L.e_chunk0, L.E_live, L.eslot);
         CG_HIP(c, hipGetLastError());
         // keep the staging vectors alive until the copies above have run
         CG_HIP(c, hipStreamSynchronize(c->stream));
@@ -762,15 +918,16 @@ int link_run(Ctx* c, int dir, float2* vout) {
     if (!c->any_links) return CG_OK;
     const LinkDir& L = c->dir[dir];
     const int pass = P_LINK_IN + dir;
-    // The hub tier (one 1024-lane block per hub) runs on the side stream while the
-    // lane and 64-lane tiers fill the other CUs on the main stream.
-    const uint32_t e0 = L.e_chunk0, e1 = L.E_live;
-    const uint32_t first = L.n_lane, block_slot = L.n_lane + L.n_wave;
+    // The 1024-lane tier runs on the side stream while the lane and 64-lane
+    // tiers fill the other CUs on the main stream.  Its split points take
+    // HUB_ROUNDS rounds of segment solves and fix-ups; rounds after
+    // convergence return at once.
+    const uint32_t e0 = L.e_chunk0, e1 = L.E_live, eh = L.e_hub0;
+    const uint32_t n_hub_units = L.n_units - L.n_wave_units;
     // sweep counters only while requested (same-address atomics per block)
     uint32_t* stats = (c->prof_mask & CG_PROFILE_LINK_STATS) ? c->link_stats : nullptr;
-    const uint32_t eh = L.e_hub0;  // entries [e0, eh) feed the 64-lane tier, [eh, e1) the hubs
-    const bool fork = L.n_block && c->side;
-    if (L.n_block) {
+    const bool fork = n_hub_units && c->side;
+    if (n_hub_units) {
         hipStream_t s = c->stream;
         if (fork) {
             CG_HIP(c, hipEventRecord(c->ev_fork, c->stream));
@@ -779,21 +936,29 @@ int link_run(Ctx* c, int dir, float2* vout) {
         }
         if (e1 > eh)
             hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(e1 - eh, 256)), dim3(256), 0, s, pass, L.ent, eh,
-                               e1, L.eslot + (eh - e0), L.soff, L.poff, L.sched, first, block_slot, c->gid,
-                               c->desc, c->pos[0], c->pos[1], c->link_pre);
-        hipLaunchKernelGGL(k_link_chunk<1024>, dim3(L.n_block), dim3(1024), 0, s, pass, L.sched, L.soff,
-                           L.poff, L.woff, c->link_pre, first, block_slot, c->gid, c->desc, c->pos[0],
-                           c->pos[1], c->flags, L.warm, stats, vout);
+                               e1, L.eslot + (eh - e0), L.unit_a, L.n_wave_units, c->desc, c->pos[0],
+                               c->pos[1], c->link_pre);
+        for (int r = 0; r < HUB_ROUNDS; ++r) {
+            const uint32_t u0 = r == 0 ? L.n_wave_units : L.first_split_unit;
+            if (r > 0 && !L.n_hubs) break;
+            hipLaunchKernelGGL(k_link_chunk<1024>, dim3(L.n_units - u0), dim3(1024), 0, s, pass, L.unit_a,
+                               L.unit_b, u0, r, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, c->link_pre,
+                               L.warm, L.seg_out, L.hub_done, stats, vout);
+            if (L.n_hubs)
+                hipLaunchKernelGGL(k_hub_fix, dim3(blocks_for(L.n_hubs, 64)), dim3(64), 0, s, pass, L.hubs,
+                                   L.n_hubs, L.unit_a, L.unit_b, r, (int)(r == HUB_ROUNDS - 1), L.sched,
+                                   c->desc, c->pos[0], c->pos[1], c->flags, L.seg_out, L.warm, L.hub_done,
+                                   stats, vout);
+        }
         if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
     if (eh > e0)
         hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(eh - e0, 256)), dim3(256), 0, c->stream, pass, L.ent, e0,
-                           eh, L.eslot, L.soff, L.poff, L.sched, first, block_slot, c->gid, c->desc,
-                           c->pos[0], c->pos[1], c->link_pre);
-    if (L.n_wave)
-        hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_wave), dim3(64), 0, c->stream, pass, L.sched, L.soff,
-                           L.poff, L.woff, c->link_pre, first, first, c->gid, c->desc, c->pos[0],
-                           c->pos[1], c->flags, L.warm, stats, vout);
+                           eh, L.eslot, L.unit_a, L.n_wave_units, c->desc, c->pos[0], c->pos[1], c->link_pre);
+    if (L.n_wave_units)
+        hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_wave_units), dim3(64), 0, c->stream, pass, L.unit_a,
+                           L.unit_b, 0u, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, c->link_pre,
+                           L.warm, L.seg_out, L.hub_done, stats, vout);
     if (L.n_lane)
         hipLaunchKernelGGL(k_link_lane, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, pass,
                            L.sched, L.soff, L.ent, L.n_lane, c->gid, c->desc, c->pos[0], c->pos[1],
