@@ -146,6 +146,7 @@ struct LinkDir {
     float4* seg_out = nullptr;   // [2 n_units] segment end, start used, end Jacobian
     uint32_t* hub_done = nullptr;  // [2 n_hubs] fix-up converged this pass; last correction (f32)
     float2* warm = nullptr;      // last converged chunk starts, per graph regions
+    float4* wjac = nullptr;      // split segments: each lane's prefix Jacobian at that start
     size_t warm_len = 0;
 };
 
@@ -368,6 +369,20 @@ __device__ inline float2 integrate(float2 p, float2 v, float friction, float S) 
     x = fminf(fmaxf(x, 0.f), S);
     y = fminf(fmaxf(y, 0.f), S);
     return make_float2(x, y);
+}
+
+// Gravity velocity at p — force-gravity.frag:31-38: alpha*g*|d|*0.1 along d
+// toward the centre of the space; the atan/cos/sin round trip is d/|d|.
+__device__ inline float2 gravity_velocity(float2 p, const GDesc& d) {
+    float2 v = make_float2(0.f, 0.f);
+    float dx = d.S * 0.5f - p.x, dy = d.S * 0.5f - p.y;
+    float dist = sqrtf(dx * dx + dy * dy);
+    if (dist > 0.f) {
+        float add = d.alpha * d.gravity * dist * 0.1f;
+        v.x = add * (dx / dist);
+        v.y = add * (dy / dist);
+    }
+    return v;
 }
 
 // Largest j in [0, G) with base[j] <= x (base ascending, base[0] == 0).

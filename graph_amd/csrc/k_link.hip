@@ -432,6 +432,7 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
                                                   const uint8_t* __restrict__ flags,
                                                   const float4* __restrict__ pre,
                                                   float2* __restrict__ warm,
+                                                  float4* __restrict__ wjac,
                                                   float4* __restrict__ seg_out,
                                                   uint32_t* __restrict__ hub_done,
                                                   uint32_t* __restrict__ stats,
@@ -461,6 +462,15 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
     const float4* __restrict__ mine = pre + U.z + t;
     float2 g = *wslot;
     const float2 G = warm[V.y];  // lane 0's slot: the unit's start
+    if (split && t > 0) {
+        // the lane's start converged from the previous start G0, moved to the
+        // current G along its prefix Jacobian (zero before the first run)
+        const float4 P = wjac[V.y + t];
+        const float4 o0 = seg_out[2 * (size_t)u];
+        const float dx = G.x - o0.z, dy = G.y - o0.w;
+        g.x += P.x * dx + P.z * dy;
+        g.y += P.y * dx + P.w * dy;
+    }
     float prev = INFINITY;
     float J[4];
     float2 v;
@@ -495,6 +505,7 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
                 s_final = make_float2(v.x + J[0] * ex_ + J[2] * ey, v.y + J[1] * ex_ + J[3] * ey);
             }
             *wslot = start;
+            if (split) wjac[V.y + t] = make_float4(ex.a0, ex.a1, ex.a2, ex.a3);
             if (stats && t == 0) {
                 // sweeps used, units, max sweeps, units not within tolerance (per tier)
                 const int o = B == 64 ? 0 : 4;
@@ -524,23 +535,25 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
     }
 }
 
-// Fix-up of the split hubs, one lane per hub (hubs[h] = {first unit, segments}):
+// Fix-up of the split hubs, one wave per hub (hubs[h] = {first unit, segments}):
 // composes the segments' linearised end maps from the exact start 0,
 //   true_{s+1} = E_s + A_s (true_s - G_s),
 // stores each true start as the segment's next start guess, and when every
 // guess was already within the tolerance, the corrections stopped halving
 // (the f32 noise floor of long segments), or on the final round, writes the
-// point's velocity (the last segment's end from its true start).
+// point's velocity (the last segment's end from its true start).  Lanes load
+// 64 segments at a time; the chain runs on broadcasts, the same in every lane.
 // hub_done holds [n_hubs] flags, then [n_hubs] previous largest corrections.
-__global__ void k_hub_fix(int pass, const uint2* __restrict__ hubs, uint32_t n_hubs,
-                          const uint4* __restrict__ unit_a, const uint4* __restrict__ unit_b,
-                          int round, int final_round, const uint32_t* __restrict__ sched,
-                          const GDesc* __restrict__ desc, float2* __restrict__ pos0,
-                          float2* __restrict__ pos1, const uint8_t* __restrict__ flags,
-                          const float4* __restrict__ seg_out, float2* __restrict__ warm,
-                          uint32_t* __restrict__ hub_done, uint32_t* __restrict__ stats,
-                          float2* __restrict__ vout) {
-    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64) k_hub_fix(int pass, const uint2* __restrict__ hubs, uint32_t n_hubs,
+                                                const uint4* __restrict__ unit_a,
+                                                const uint4* __restrict__ unit_b, int round, int final_round,
+                                                const uint32_t* __restrict__ sched,
+                                                const GDesc* __restrict__ desc, float2* __restrict__ pos0,
+                                                float2* __restrict__ pos1, const uint8_t* __restrict__ flags,
+                                                const float4* __restrict__ seg_out, float2* __restrict__ warm,
+                                                uint32_t* __restrict__ hub_done, uint32_t* __restrict__ stats,
+                                                float2* __restrict__ vout) {
+    const uint32_t h = blockIdx.x, lane = threadIdx.x;
     if (h >= n_hubs) return;
     const uint2 hb = hubs[h];
     const GDesc& d = desc[unit_a[hb.x].w];
@@ -548,16 +561,27 @@ __global__ void k_hub_fix(int pass, const uint2* __restrict__ hubs, uint32_t n_h
     if (hub_done[h]) return;
     float2 tru = make_float2(0.f, 0.f);
     float dmax = 0.f;
-    for (uint32_t s = 0; s < hb.y; ++s) {
-        const uint32_t u = hb.x + s;
-        const float4 o0 = seg_out[2 * (size_t)u], A = seg_out[2 * (size_t)u + 1];
-        if (s > 0) {
-            dmax = fmaxf(dmax, fmaxf(fabsf(tru.x - o0.z), fabsf(tru.y - o0.w)));
-            warm[unit_b[u].y] = tru;  // lane 0 of the segment: its next start
+    for (uint32_t s0 = 0; s0 < hb.y; s0 += 64) {
+        const uint32_t s = s0 + lane;
+        const bool have = s < hb.y;
+        const size_t u = (size_t)hb.x + (have ? s : s0);
+        const float4 o0 = seg_out[2 * u], A = seg_out[2 * u + 1];
+        const uint32_t wo = unit_b[u].y;
+        float2 mine = make_float2(0.f, 0.f);
+        const uint32_t cnt = min(64u, hb.y - s0);
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const float ex = __shfl(o0.x, k, 64), ey = __shfl(o0.y, k, 64);
+            const float gx = __shfl(o0.z, k, 64), gy = __shfl(o0.w, k, 64);
+            const float a0 = __shfl(A.x, k, 64), a1 = __shfl(A.y, k, 64);
+            const float a2 = __shfl(A.z, k, 64), a3 = __shfl(A.w, k, 64);
+            if (lane == k) mine = tru;
+            if (s0 + k > 0) dmax = fmaxf(dmax, fmaxf(fabsf(tru.x - gx), fabsf(tru.y - gy)));
+            const float dx = tru.x - gx, dy = tru.y - gy;
+            tru = make_float2(ex + a0 * dx + a2 * dy, ey + a1 * dx + a3 * dy);
         }
-        const float dx = tru.x - o0.z, dy = tru.y - o0.w;
-        tru = make_float2(o0.x + A.x * dx + A.z * dy, o0.y + A.y * dx + A.w * dy);
+        if (have && s > 0) warm[wo] = mine;  // lane 0 of the segment: its next start
     }
+    if (lane != 0) return;
     float* __restrict__ dprev = reinterpret_cast<float*>(hub_done + n_hubs);
     const bool ok = dmax <= d.tol || (round > 0 && dmax > 0.5f * dprev[h]);
     dprev[h] = dmax;
@@ -712,6 +736,7 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
     auto free_dir = [&](LinkDir& L) {
         dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff); dfree(c, L.unit_a); dfree(c, L.unit_b);
         dfree(c, L.eslot); dfree(c, L.hubs); dfree(c, L.seg_out); dfree(c, L.hub_done); dfree(c, L.warm);
+        dfree(c, L.wjac);
         L = LinkDir();
     };
     for (int d = 0; d < 2; ++d) c->dir[d] = LinkDir();
@@ -878,6 +903,7 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         L.hub_done = (uint32_t*)dalloc(c, (hh.size() ? hh.size() : 1) * 8, &rc);
         L.eslot = (uint32_t*)dalloc(c, (size_t)(L.E_live - L.e_chunk0 + 1) * 4, &rc);
         L.warm = (float2*)dalloc(c, (wpos ? wpos : 1) * 8, &rc);
+        L.wjac = (float4*)dalloc(c, (wpos ? wpos : 1) * 16, &rc);
         uint32_t* d_ubeg = (uint32_t*)talloc(ubeg.size() * 4);
         if (rc) { cleanup(); return rc; }
         if (!ua.empty()) {
@@ -889,6 +915,8 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         CG_HIP(c, hipMemcpyAsync(d_ubeg, ubeg.data(), ubeg.size() * 4, hipMemcpyHostToDevice, c->stream));
         CG_HIP(c, hipMemsetAsync(L.hub_done, 0, (hh.size() ? hh.size() : 1) * 8, c->stream));
         CG_HIP(c, hipMemsetAsync(L.warm, 0, (wpos ? wpos : 1) * 8, c->stream));
+        CG_HIP(c, hipMemsetAsync(L.wjac, 0, (wpos ? wpos : 1) * 16, c->stream));
+        CG_HIP(c, hipMemsetAsync(L.seg_out, 0, nu * 32, c->stream));
         for (uint32_t s = 0; s < G; ++s) {
             Graph& g = c->graphs[c->order[s]];
             if (old[d].warm && g.wn[d] && g.wn[d] == region[s].second)
@@ -943,9 +971,9 @@ int link_run(Ctx* c, int dir, float2* vout) {
             if (r > 0 && !L.n_hubs) break;
             hipLaunchKernelGGL(k_link_chunk<1024>, dim3(L.n_units - u0), dim3(1024), 0, s, pass, L.unit_a,
                                L.unit_b, u0, r, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, c->link_pre,
-                               L.warm, L.seg_out, L.hub_done, stats, vout);
+                               L.warm, L.wjac, L.seg_out, L.hub_done, stats, vout);
             if (L.n_hubs)
-                hipLaunchKernelGGL(k_hub_fix, dim3(blocks_for(L.n_hubs, 64)), dim3(64), 0, s, pass, L.hubs,
+                hipLaunchKernelGGL(k_hub_fix, dim3(L.n_hubs), dim3(64), 0, s, pass, L.hubs,
                                    L.n_hubs, L.unit_a, L.unit_b, r, (int)(r == HUB_ROUNDS - 1), L.sched,
                                    c->desc, c->pos[0], c->pos[1], c->flags, L.seg_out, L.warm, L.hub_done,
                                    stats, vout);
@@ -958,7 +986,7 @@ int link_run(Ctx* c, int dir, float2* vout) {
     if (L.n_wave_units)
         hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_wave_units), dim3(64), 0, c->stream, pass, L.unit_a,
                            L.unit_b, 0u, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, c->link_pre,
-                           L.warm, L.seg_out, L.hub_done, stats, vout);
+                           L.warm, L.wjac, L.seg_out, L.hub_done, stats, vout);
     if (L.n_lane)
         hipLaunchKernelGGL(k_link_lane, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, pass,
                            L.sched, L.soff, L.ent, L.n_lane, c->gid, c->desc, c->pos[0], c->pos[1],

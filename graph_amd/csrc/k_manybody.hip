@@ -34,11 +34,6 @@ constexpr uint32_t SCAN_TILE = 4096;  // cells per scan tile: 256 lanes x 16
 constexpr uint32_t BIG_CHUNK = 2048;  // points per block in a crowded cell
 constexpr uint32_t MID_BLOCKS = 2048; // grid of the wave-per-cell pass (8192 waves)
 
-__global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
-}
-
 // ---- 1. counting sort by finest cell ---------------------------------------------
 
 // Finest cell of position q: calculate-level.vert:47-48 at the finest level,
@@ -60,15 +55,21 @@ constexpr int RB_TILE = 256 * RB_ITEMS;
 
 // Keys of the points in the previous tick's cell order, and the first digit's
 // block histogram.  `visit` may alias `vout` (each lane rewrites what it read).
+// Also clears cell_end (n_cells words) and the three crowded-cell counters for
+// this tick.
 __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const float2* __restrict__ pos0,
                                                   const float2* __restrict__ pos1,
                                                   const uint32_t* __restrict__ gid,
                                                   const GDesc* __restrict__ desc, uint32_t np,
                                                   uint32_t* __restrict__ kout, uint32_t* vout,
-                                                  int width, uint32_t nb, uint32_t* __restrict__ hist) {
+                                                  int width, uint32_t nb, uint32_t* __restrict__ hist,
+                                                  uint32_t* __restrict__ cell_end, uint32_t n_cells,
+                                                  uint32_t* __restrict__ big_count) {
     __shared__ uint32_t h[256];
     const int t = threadIdx.x;
     const uint32_t radix = 1u << width, mask = radix - 1;
+    for (uint32_t f = blockIdx.x * 256 + t; f < n_cells; f += nb * 256) cell_end[f] = 0u;
+    if (blockIdx.x == 0 && t < 3) big_count[t] = 0u;
     h[t] = 0;
     __syncthreads();
 #pragma unroll
@@ -186,26 +187,52 @@ __global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict_
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__global__ void __launch_bounds__(1024) k_scan_part(uint32_t* __restrict__ part, uint32_t nb) {
-    __shared__ uint32_t sh[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (nb + 1023) / 1024;
-    const uint32_t lo = min(nb, t * per), hi = min(nb, lo + per);
-    uint32_t s = 0;
-    for (uint32_t j = lo; j < hi; ++j) s += part[j];
-    sh[t] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = t >= off ? sh[t - off] : 0u;
+// In-place exclusive scan of n counts by one block of 1024 lanes, in tiles of
+// SCAN_ONE_TILE counts staged through LDS (coalesced in and out); each lane
+// scans 8 consecutive counts, the lanes' totals are scanned with wave shuffles
+// and the tile's total carries into the next tile.  LDS rows are padded by one
+// word per 8 so a lane's run starts in its own bank.
+constexpr uint32_t SCAN_ONE_TILE = 8192;
+constexpr uint32_t SCAN_ONE_MAX = SCAN_ONE_TILE;  // larger tables: three launches spread the work
+__global__ void __launch_bounds__(1024) k_scan_one(uint32_t* __restrict__ a, uint32_t n) {
+    __shared__ uint32_t tile[SCAN_ONE_TILE + SCAN_ONE_TILE / 8];
+    __shared__ uint32_t wsum[16];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += SCAN_ONE_TILE) {
+        const uint32_t cnt = min(SCAN_ONE_TILE, n - base);
+        for (uint32_t i = t; i < SCAN_ONE_TILE; i += 1024) tile[i + i / 8] = i < cnt ? a[base + i] : 0u;
         __syncthreads();
-        sh[t] += v;
+        uint32_t v[8], s = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            v[q] = tile[t * 9 + q];
+            s += v[q];
+        }
+        uint32_t inc = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(inc, o, 64);
+            if (lane >= (uint32_t)o) inc += u;
+        }
+        if (lane == 63) wsum[w] = inc;
         __syncthreads();
-    }
-    uint32_t run = t ? sh[t - 1] : 0u;
-    for (uint32_t j = lo; j < hi; ++j) {
-        uint32_t v = part[j];
-        part[j] = run;
-        run += v;
+        uint32_t before = carry, total = 0;
+        for (uint32_t k = 0; k < 16; ++k) {
+            const uint32_t x = wsum[k];
+            before += k < w ? x : 0u;
+            total += x;
+        }
+        uint32_t run = before + inc - s;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            tile[t * 9 + q] = run;
+            run += v[q];
+        }
+        __syncthreads();
+        for (uint32_t i = t; i < cnt; i += 1024) a[base + i] = tile[i + i / 8];
+        carry += total;
+        __syncthreads();  // tile and wsum are rewritten by the next tile
     }
 }
 
@@ -989,13 +1016,13 @@ int mb_alloc(Ctx* c) {
 // In-place exclusive scan of n (a multiple of 16) counts; `part` holds one
 // word per SCAN_TILE counts.
 static void scan_counts(Ctx* c, uint32_t* a, uint32_t n, uint32_t* part) {
-    if (n <= 4096) {  // one block, a few entries per lane
-        hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(1024), 0, c->stream, a, n);
+    if (n <= SCAN_ONE_MAX) {  // one block: one launch instead of three
+        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, c->stream, a, n);
         return;
     }
     const uint32_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(256), 0, c->stream, a, n, part);
-    hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(1024), 0, c->stream, part, tiles);
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, c->stream, part, tiles);
     hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(256), 0, c->stream, a, n, part, a);
 }
 
@@ -1019,10 +1046,9 @@ int mb_run(Ctx* c, float2* vout) {
     const uint32_t np = c->NP, F = c->F;
     prof_begin(c, CG_STAGE_MB_SORT);
     const RadixSort& rs = c->mb_rs;
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->cell_end, F, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->big_count, 3u, 0u);
     hipLaunchKernelGGL(k_bin_keys, dim3(rs.nb), dim3(256), 0, c->stream, c->sorted, c->pos[0], c->pos[1],
-                       c->gid, c->desc, np, c->rk[0], c->rv[0], rs.width, rs.nb, rs.hist);
+                       c->gid, c->desc, np, c->rk[0], c->rv[0], rs.width, rs.nb, rs.hist,
+                       c->cell_end, F, c->big_count);
     radix_run(c, rs, c->rk, c->rv, true);
     hipLaunchKernelGGL(k_bin_bounds, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream,
                        c->rk[rs.passes & 1], np, c->cell_begin, c->cell_end);

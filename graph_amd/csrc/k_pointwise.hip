@@ -21,15 +21,7 @@ __device__ inline float2 pw_force(float2 p, const GDesc& d, const double* __rest
                                   uint32_t g) {
     float2 v = make_float2(0.f, 0.f);
     if (PASS == P_GRAVITY) {
-        // force-gravity.frag:31-38: alpha*g*|d|*0.1 along d; the atan/cos/sin
-        // round trip is the unit vector d/|d|.
-        float dx = d.S * 0.5f - p.x, dy = d.S * 0.5f - p.y;
-        float dist = sqrtf(dx * dx + dy * dy);
-        if (dist > 0.f) {
-            float add = d.alpha * d.gravity * dist * 0.1f;
-            v.x = add * (dx / dist);
-            v.y = add * (dy / dist);
-        }
+        v = gravity_velocity(p, d);
     } else if (PASS == P_CENTER) {
         // force-center.frag:27-35, centermass (sum x, sum y, count) summed in f64
         float cmx = (float)csum[3 * g] / (float)csum[3 * g + 2];

@@ -313,3 +313,50 @@ def test_cluster_force_vs_oracle():
     check(got, want, S, "cluster", idx=absent == 0)
     # unclustered points get no pull
     assert np.all(got.reshape(-1, 2)[cl < 0] == 0)
+
+
+@pytest.mark.parametrize("center", [0.0, 0.5])
+def test_ticks_vs_oracle_pass_order(center):
+    """Whole ticks (gravity, centre, many-body, both link directions, each
+    integrated) against the oracle's passes composed with positions stored in
+    f32 between passes, as the reference's position textures hold them.
+    Repulsion is 0 so the many-body pass integrates a zero velocity: the check
+    then covers the pass order and buffer hand-offs (with centre off, gravity
+    runs inside the many-body binning) without the cell-boundary
+    discontinuities of the quadtree.  Per point |dp| <= 1e-5 * spaceSize per
+    tick."""
+    S, n, ticks = 4096.0, 10_000, 3
+    xy = datagen.uniform_square(n, S, 5)
+    xy[2 * 7: 2 * 9] = np.nan
+    links = datagen.barabasi_albert(n, 2, 5)
+    m = links.size // 2
+    r_in, r_out = datagen.uniform(m, 0, 1, 6), datagen.uniform(m, 0, 1, 7)
+    pinned = [0, 11, 12]
+    c = ctx_for(xy, links=links, r_in=r_in, r_out=r_out)
+    c.set_pinned(pinned)
+    c.set_alpha(1.0)
+    p = params(S, repulsion=0.0, gravity=0.25, center=center, friction=0.85, decay=5000, link_spring=1.0,
+               link_distance=10.0, range=(1.0, 1.2))
+    c.step(p, ticks)
+    gpu = c.get_positions().reshape(-1, 2).astype(np.float64)
+    absent = O.absent_mask(xy)
+    pin = np.zeros(n, np.uint8)
+    pin[pinned] = 1
+    Lk = O.Links(n, links, r_in, r_out)
+    pos, a = xy.astype(np.float32).astype(np.float64), 1.0
+    f32 = lambda q: q.astype(np.float32).astype(np.float64)
+    for _ in range(ticks):
+        pos = f32(O.integrate(pos, O.gravity(pos, S, a, 0.25), 0.85, S, pin, absent))
+        if center:
+            pos = f32(O.integrate(pos, O.center(pos, a, center, absent), 0.85, S, pin, absent))
+        for d in (0, 1):
+            v = Lk.force(pos, d, a, 1.0, 10.0, 1.0, 1.2, absent)
+            pos = f32(O.integrate(pos, v, 0.85, S, pin, absent))
+        a = O.alpha_step(a, 5000)
+    e = np.hypot(*(gpu - pos.reshape(-1, 2)).T)
+    live = absent == 0
+    assert np.isnan(gpu[~live]).all()
+    assert np.array_equal(gpu[pinned], xy.reshape(-1, 2)[pinned].astype(np.float64))
+    worst = float(e[live].max())
+    say(f"{ticks} ticks, centre {center}: max |dp| = {worst:.3e} ({worst / S:.2e} S)")
+    assert worst <= TOL * S * ticks
