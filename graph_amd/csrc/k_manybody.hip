@@ -785,23 +785,32 @@ __device__ inline float2 near_field(const float4* fin, int x0, int y0, int w, co
             if (cx < 0 || cx >= gf) continue;
             float others = fin[(cy - y0) * w + (cx - x0)].z - ((di == 0 && dj == 0) ? 1.f : 0.f);
             if (others <= 0.f) continue;
+            // the cell's 8 slots in 6 vector loads; filled slots lead, -1 after
+            // them (slots past the first -1 and the point itself weigh 0)
             const size_t sb = ((size_t)d.f0 + cy * gf + cx) * SLOTS;
+            const int4* __restrict__ si = reinterpret_cast<const int4*>(slot_idx + sb);
+            const float4* __restrict__ sp = reinterpret_cast<const float4*>(slot_pos + sb);
+            const int4 ia = si[0], ib = si[1];
+            const float4 q01 = sp[0], q23 = sp[1], q45 = sp[2], q67 = sp[3];
+            const int js[SLOTS] = {ia.x, ia.y, ia.z, ia.w, ib.x, ib.y, ib.z, ib.w};
+            const float qs[2 * SLOTS] = {q01.x, q01.y, q01.z, q01.w, q23.x, q23.y, q23.z, q23.w,
+                                         q45.x, q45.y, q45.z, q45.w, q67.x, q67.y, q67.z, q67.w};
             float sxp = 0.f, syp = 0.f, sampled = 0.f;
+            bool open = true;
 #pragma unroll
             for (int s = 0; s < SLOTS; ++s) {
-                const int j = slot_idx[sb + s];
-                if (j < 0) break;
-                if ((uint32_t)j == i) continue;
-                sampled += 1.f;
-                const float2 q = slot_pos[sb + s];
-                float dx = p.x - q.x, dy = p.y - q.y;
+                const int j = js[s];
+                open = open && j >= 0;
+                const bool use = open && (uint32_t)j != i;
+                sampled += use ? 1.f : 0.f;
+                float dx = p.x - qs[2 * s], dy = p.y - qs[2 * s + 1];
                 float l = dx * dx + dy * dy;
                 // coincident points push along the point's jitter (force-nearfield.frag:131-136)
                 const bool same = !(l > 0.f);
                 dx = same ? r.x : dx;
                 dy = same ? r.y : dy;
                 l = same ? r.x * r.x + r.y * r.y : l;
-                const float wgt = (l > 0.f) ? falloff_coef(l) : 0.f;
+                const float wgt = (use && l > 0.f) ? falloff_coef(l) : 0.f;
                 sxp += wgt * dx;
                 syp += wgt * dy;
             }
