@@ -31,7 +31,7 @@ SINGLE_KERNEL = ("gravity", "mb_force", "mouse")
 
 
 def stage_work(stage: str, wl: dict, ctx_info: dict) -> tuple[str, float]:
-    """Algorithmic bytes (or flops) of one launch of a stage; see DESIGN.md."""
+    """Algorithmic bytes (or flops) of one launch of a stage."""
     n, np_, L, gf = ctx_info["n"], ctx_info["np"], ctx_info["L"], ctx_info["gf"]
     if stage in ("gravity", "center", "mouse"):
         # present index 4, position in 8, flags 1, position out 8, key 4, value 4
@@ -47,6 +47,20 @@ def stage_work(stage: str, wl: dict, ctx_info: dict) -> tuple[str, float]:
         # entry 16 + gathered neighbour position 8 per entry; per point 25
         return "hbm", 24.0 * E + 25.0 * np_
     return "hbm", 0.0
+
+
+def profiled_traffic(kernel: str) -> tuple[float | None, str | None]:
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE runs of `bench.py --traffic-probe`, scaled by the
+    calibration kernels there), or None when it holds no figure for the kernel."""
+    path = ROOT / "profiles" / "traffic_cfg1.md"
+    if not path.exists():
+        return None, None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kernel:
+            return float(parts[2]), "profiles/traffic_cfg1.md (rocprofv3 --pmc FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"
+    return None, None
 
 
 def cpu_baseline(wl: dict, seconds: float) -> dict:
@@ -306,6 +320,10 @@ def main() -> int:
         roof = {"bound": "valu", "achieved": achieved, "peak": VALU_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / VALU_FP32_PEAK_TFLOPS, "traffic": None}
     roof["kernel"] = {"gravity": "k_pointwise<GRAVITY>", "mb_force": "k_mb_force"}.get(dominant, dominant)
+    roof["traffic"], src = profiled_traffic(roof["kernel"])
+    if src:
+        roof["traffic_unit"] = "bytes per launch"
+        roof["traffic_source"] = src
     roof["avg_ms"] = k_avg_s * 1e3
     roof["timing"] = (f"HIP events around each of {k_cnt} launches over {args.steps} direct-launch ticks "
                       f"run after the timed region ({direct_ms:.3f} ms/tick there)")
