@@ -30,7 +30,7 @@ constexpr uint32_t CELL_BLOCK = 256;  // each graph's finest cells start on a mu
 // Link degree tiers: points with at most LANE_MAX entries run one lane each;
 // up to WAVE_MAX entries one 64-lane workgroup each; above, 1024 lanes each.
 constexpr uint32_t LANE_MAX = 32;
-constexpr uint32_t LANE_SPLIT = 8;  // lane-tier schedule classes: <= LANE_SPLIT entries, then the rest
+constexpr uint32_t LANE_SPLIT = 8;  // lane-tier schedule classes: <= 2, <= LANE_SPLIT entries, then the rest
 constexpr uint32_t WAVE_MAX = 4096;
 constexpr uint32_t HUB_SEG = 16384;  // entries per segment of a split 1024-lane point
 constexpr int HUB_ROUNDS = 3;        // segment/fix-up rounds per pass for split points
@@ -128,7 +128,7 @@ struct Graph {
 struct LinkDir {
     uint32_t E_live = 0;         // live entries over all graphs
     uint4* ent = nullptr;        // schedule-ordered live entries (global neighbour indices)
-    uint32_t* sched = nullptr;   // present points: lane tier in two degree classes (index order), then by degree
+    uint32_t* sched = nullptr;   // present points: lane tier in three degree classes (index order), then by degree
     uint32_t* soff = nullptr;    // [n_sched+1]
     uint32_t n_sched = 0;
     uint32_t n_lane = 0, n_wave = 0, n_block = 0;

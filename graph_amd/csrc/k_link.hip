@@ -132,9 +132,10 @@ __device__ inline uint32_t live_degree(uint32_t p, const uint32_t* __restrict__ 
 }
 
 // Schedule sort keys.  Chunked-tier points sort by live degree.  Lane-tier
-// points sort into two classes only (at most LANE_SPLIT entries, then the
-// rest), so within a class the stable sort keeps ascending point index and a
-// wave's own-point loads and stores fall on neighbouring addresses.
+// points sort into three classes only (at most 2 entries, at most LANE_SPLIT,
+// then the rest), so within a class the stable sort keeps ascending point
+// index and a wave's own-point loads and stores fall on neighbouring
+// addresses, while a wave's chains stay of similar length.
 __global__ void k_sched_keys(const uint32_t* __restrict__ present, uint32_t np,
                              const uint32_t* __restrict__ gid, const GDesc* __restrict__ desc,
                              const GLinkPtr* __restrict__ tab, uint32_t* __restrict__ keys,
@@ -143,7 +144,7 @@ __global__ void k_sched_keys(const uint32_t* __restrict__ present, uint32_t np,
     if (k >= np) return;
     const uint32_t p = present[k];
     const uint32_t deg = live_degree(p, gid, desc, tab);
-    keys[k] = deg > LANE_MAX ? deg : (deg > LANE_SPLIT ? 1u : 0u);
+    keys[k] = deg > LANE_MAX ? deg : (deg > LANE_SPLIT ? 2u : (deg > 2 ? 1u : 0u));
     vals[k] = p;
 }
 
@@ -232,28 +233,12 @@ __device__ inline float spring_term(const GDesc& d, uint4 e, float2 pp, float2 v
     return l;
 }
 
-__global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __restrict__ sched,
-                                                   const uint32_t* __restrict__ soff,
-                                                   const uint4* __restrict__ ent, uint32_t count,
-                                                   const uint32_t* __restrict__ gid,
-                                                   const GDesc* __restrict__ desc,
-                                                   float2* __restrict__ pos0,
-                                                   float2* __restrict__ pos1,
-                                                   const uint8_t* __restrict__ flags,
-                                                   float2* __restrict__ vout) {
-    // the longer-chain class starts first
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= count) return;
-    const uint32_t j = count - 1 - r;
-    const uint32_t p = sched[j];
-    const GDesc& d = desc[gid[p]];
-    if (!(d.passes & (1u << pass))) return;
-    const bool bsel = (d.inbuf >> pass) & 1u;
-    const float2* __restrict__ pin = bsel ? pos1 : pos0;
-    const uint32_t b = soff[j], e = soff[j + 1];
-    const float2 pp = pin[p];
+// One lane's chain of entries [b, e), G entries fetched together (G entry
+// loads, then G neighbour gathers).
+template <int G>
+__device__ inline float2 lane_chain(const GDesc& d, const uint4* __restrict__ ent, uint32_t b, uint32_t e,
+                                    const float2* __restrict__ pin, float2 pp) {
     float2 v = make_float2(0.f, 0.f);
-    constexpr int G = 8;  // entries fetched together: G entry loads, then G neighbour gathers
     for (uint32_t k = b; k < e; k += G) {
         uint4 en[G];
         float2 q[G];
@@ -270,6 +255,36 @@ __global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __r
             v.y += y * t;
         }
     }
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __restrict__ sched,
+                                                   const uint32_t* __restrict__ soff,
+                                                   const uint4* __restrict__ ent, uint32_t count,
+                                                   const uint32_t* __restrict__ gid,
+                                                   const GDesc* __restrict__ desc,
+                                                   float2* __restrict__ pos0,
+                                                   float2* __restrict__ pos1,
+                                                   const uint8_t* __restrict__ flags,
+                                                   float2* __restrict__ vout) {
+    // the longer-chain classes start first
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = r < count;
+    const uint32_t j = in ? count - 1 - r : 0u;
+    const uint32_t p = sched[j];
+    const GDesc& d = desc[gid[p]];
+    const bool active = in && (d.passes & (1u << pass));
+    const uint32_t b = soff[j], e = active ? soff[j + 1] : b;
+    // batch depth from the wave's longest chain (every lane is still here)
+    uint32_t mx = e - b;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
+    if (!active) return;
+    const bool bsel = (d.inbuf >> pass) & 1u;
+    const float2* __restrict__ pin = bsel ? pos1 : pos0;
+    const float2 pp = pin[p];
+    const float2 v = mx <= 2 ? lane_chain<2>(d, ent, b, e, pin, pp)
+                             : (mx <= 4 ? lane_chain<4>(d, ent, b, e, pin, pp) : lane_chain<8>(d, ent, b, e, pin, pp));
     if (vout) {
         vout[p] = v;
         return;
@@ -947,9 +962,9 @@ int link_run(Ctx* c, int dir, float2* vout) {
     const LinkDir& L = c->dir[dir];
     const int pass = P_LINK_IN + dir;
     // The 1024-lane tier runs on the side stream while the lane and 64-lane
-    // tiers fill the other CUs on the main stream.  Its split points take
-    // HUB_ROUNDS rounds of segment solves and fix-ups; rounds after
-    // convergence return at once.
+    // tiers fill the other CUs on the main stream (a third stream for the lane
+    // tier measured slower).  Split points take HUB_ROUNDS rounds of segment
+    // solves and fix-ups; rounds after convergence return at once.
     const uint32_t e0 = L.e_chunk0, e1 = L.E_live, eh = L.e_hub0;
     const uint32_t n_hub_units = L.n_units - L.n_wave_units;
     // sweep counters only while requested (same-address atomics per block)
