@@ -134,6 +134,7 @@ struct LinkDir {
     uint32_t n_lane = 0, n_wave = 0, n_block = 0;
     uint32_t e_chunk0 = 0;       // first entry of the chunked tiers
     uint32_t e_hub0 = 0;         // first entry of the 1024-lane tier
+    uint32_t e_wave_pre0 = 0;    // first 64-lane entry that goes through link_pre
     // chunked-tier work units: one per 64-lane point, then one per segment of
     // each 1024-lane point (hubs above HUB_SEG entries are split)
     uint32_t n_units = 0, n_wave_units = 0;

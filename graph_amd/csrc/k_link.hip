@@ -437,6 +437,10 @@ constexpr uint32_t NO_HUB = 0xffffffffu;
 // {end, start used} and its end's Jacobian in its start for the fix-up.
 // Round r > 0 reruns only the segments after the first of split hubs whose
 // fix-up has not converged.
+// A 64-lane unit of at most REG_MAX entries (at most 4 per lane) reads its
+// entries and neighbours itself, once, and keeps the terms in registers for
+// every sweep; the pre-gather pass skips those entries.
+constexpr uint32_t REG_MAX = 256;
 template <int B>
 __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restrict__ unit_a,
                                                   const uint4* __restrict__ unit_b, uint32_t u_first,
@@ -445,6 +449,7 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
                                                   float2* __restrict__ pos0,
                                                   float2* __restrict__ pos1,
                                                   const uint8_t* __restrict__ flags,
+                                                  const uint4* __restrict__ ent,
                                                   const float4* __restrict__ pre,
                                                   float2* __restrict__ warm,
                                                   float4* __restrict__ wjac,
@@ -490,12 +495,31 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
     float J[4];
     float2 v;
     Aff tot;
+    const bool in_regs = B == 64 && len <= REG_MAX;  // uniform: c <= 4
+    float4 reg[4];
+    if (in_regs) {
+        uint4 en[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) en[s] = ent[U.x + min(lo + s, len - 1)];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const float2 q = pin[en[s].x];
+            const float r = __uint_as_float(en[s].y), sv = __uint_as_float(en[s].z),
+                        bv = __uint_as_float(en[s].w);
+            reg[s] = make_float4(q.x, q.y, (r * d.span + d.lo) * d.dist, d.spring_alpha * sv * bv);
+        }
+    }
     for (int sweep = 0;; ++sweep) {
         v = g;
         J[0] = 1.f; J[1] = 0.f; J[2] = 0.f; J[3] = 1.f;
-        // short chunks (most 64-lane points) take a short batch: no idle steps
-        if (c <= 4) run_chunk<4, B>(mine, steps, pp, v, J);
-        else run_chunk<PREFETCH, B>(mine, steps, pp, v, J);
+        if (in_regs) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) spring_step(reg[s], pp, v, J, (uint32_t)s < steps);
+        } else if (c <= 4) {  // short chunks take a short batch: no idle steps
+            run_chunk<4, B>(mine, steps, pp, v, J);
+        } else {
+            run_chunk<PREFETCH, B>(mine, steps, pp, v, J);
+        }
         // chunk map  s -> J (s - g) + v;  exclusive prefix applied to the unit's start
         const Aff m{J[0], J[1], J[2], J[3], v.x - (J[0] * g.x + J[2] * g.y), v.y - (J[1] * g.x + J[3] * g.y)};
         const Aff ex = block_scan<B>(m, s_wave, s_wave_inc, tot);
@@ -887,6 +911,13 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         }
         slot_u0[nch] = (uint32_t)ua.size();
         ubeg.push_back(hs[nch]);
+        // 64-lane units ascend by length: the register-resident ones come first
+        L.e_wave_pre0 = L.e_hub0;
+        for (uint32_t k = 0; k < L.n_wave; ++k)
+            if (hs[k + 1] - hs[k] > REG_MAX) {
+                L.e_wave_pre0 = hs[k];
+                break;
+            }
         L.n_units = (uint32_t)ua.size();
         L.n_wave_units = L.n_wave;
         L.n_hubs = (uint32_t)hh.size();
@@ -985,7 +1016,8 @@ int link_run(Ctx* c, int dir, float2* vout) {
             const uint32_t u0 = r == 0 ? L.n_wave_units : L.first_split_unit;
             if (r > 0 && !L.n_hubs) break;
             hipLaunchKernelGGL(k_link_chunk<1024>, dim3(L.n_units - u0), dim3(1024), 0, s, pass, L.unit_a,
-                               L.unit_b, u0, r, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, c->link_pre,
+                               L.unit_b, u0, r, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent,
+                               c->link_pre,
                                L.warm, L.wjac, L.seg_out, L.hub_done, stats, vout);
             if (L.n_hubs)
                 hipLaunchKernelGGL(k_hub_fix, dim3(L.n_hubs), dim3(64), 0, s, pass, L.hubs,
@@ -995,12 +1027,15 @@ int link_run(Ctx* c, int dir, float2* vout) {
         }
         if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
-    if (eh > e0)
-        hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(eh - e0, 256)), dim3(256), 0, c->stream, pass, L.ent, e0,
-                           eh, L.eslot, L.unit_a, L.n_wave_units, c->desc, c->pos[0], c->pos[1], c->link_pre);
+    const uint32_t ew = L.e_wave_pre0;  // 64-lane entries [e0, ew) stay in registers
+    if (eh > ew)
+        hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(eh - ew, 256)), dim3(256), 0, c->stream, pass, L.ent, ew,
+                           eh, L.eslot + (ew - e0), L.unit_a, L.n_wave_units, c->desc, c->pos[0], c->pos[1],
+                           c->link_pre);
     if (L.n_wave_units)
         hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_wave_units), dim3(64), 0, c->stream, pass, L.unit_a,
-                           L.unit_b, 0u, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, c->link_pre,
+                           L.unit_b, 0u, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent,
+                           c->link_pre,
                            L.warm, L.wjac, L.seg_out, L.hub_done, stats, vout);
     if (L.n_lane)
         hipLaunchKernelGGL(k_link_lane, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, pass,
