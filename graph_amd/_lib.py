@@ -352,10 +352,10 @@ class Context:
         return {s: (float(ms[i]), int(cnt[i])) for i, s in enumerate(STAGES) if cnt[i]}
 
     def debug_link_stats(self) -> dict:
-        out = np.zeros(12, np.uint32)
+        out = np.zeros(16, np.uint32)
         self._check(self._l.cg_debug_link_stats(self._ctx, uptr(out)))
         t = {}
-        for name, o in (("wave", 0), ("block", 4)):
+        for name, o in (("t64", 0), ("t256", 4)):
             s, n, mx, cap = (int(v) for v in out[o:o + 4])
             t[name] = {"mean_sweeps": s / n if n else 0.0, "units": n, "max_sweeps": mx, "at_cap": cap}
         s, n, mx, cap = (int(v) for v in out[8:12])

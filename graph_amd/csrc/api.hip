@@ -1367,13 +1367,13 @@ int cg_debug_link_csr(cg_ctx* x, int32_t dir, uint32_t* offsets, uint32_t* nbr, 
     return CG_OK;
 }
 
-int cg_debug_link_stats(cg_ctx* x, uint32_t* out12) {
-    if (!x || !out12) return CG_ERR_INVALID;
+int cg_debug_link_stats(cg_ctx* x, uint32_t* out16) {
+    if (!x || !out16) return CG_ERR_INVALID;
     Ctx* c = &x->c;
-    std::memset(out12, 0, 12 * sizeof(uint32_t));
+    std::memset(out16, 0, 16 * sizeof(uint32_t));
     if (!c->link_stats) return CG_OK;
     CG_HIP(c, hipSetDevice(c->device));
-    CG_HIP(c, hipMemcpyAsync(out12, c->link_stats, 48, hipMemcpyDeviceToHost, c->stream));
+    CG_HIP(c, hipMemcpyAsync(out16, c->link_stats, 64, hipMemcpyDeviceToHost, c->stream));
     CG_HIP(c, hipMemsetAsync(c->link_stats, 0, 64, c->stream));
     CG_HIP(c, hipStreamSynchronize(c->stream));
     return CG_OK;

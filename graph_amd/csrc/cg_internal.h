@@ -28,11 +28,13 @@ constexpr uint32_t BIG_CELL = 64; // finest cells above this many points use a w
 constexpr uint32_t CELL_BLOCK = 256;  // each graph's finest cells start on a multiple of this
 
 // Link degree tiers: points with at most LANE_MAX entries run one lane each;
-// up to WAVE_MAX entries one 64-lane workgroup each; above, 1024 lanes each.
+// up to T64_MAX entries one 64-lane workgroup each; above that 256-lane
+// workgroups, at most 4 entries per lane: a point above HUB_SEG entries is
+// cut into segments of at most HUB_SEG.
 constexpr uint32_t LANE_MAX = 32;
 constexpr uint32_t LANE_SPLIT = 8;  // lane-tier schedule classes: <= 2, <= LANE_SPLIT entries, then the rest
-constexpr uint32_t WAVE_MAX = 4096;
-constexpr uint32_t HUB_SEG = 16384;  // entries per segment of a split 1024-lane point
+constexpr uint32_t T64_MAX = 256;
+constexpr uint32_t HUB_SEG = 1024;   // entries per segment of a split 256-lane point
 constexpr int HUB_ROUNDS = 3;        // segment/fix-up rounds per pass for split points
 
 // Passes of one tick, in order (src/index.ts:2012-2078).  Bit p of GDesc::passes
@@ -131,23 +133,20 @@ struct LinkDir {
     uint32_t* sched = nullptr;   // present points: lane tier in three degree classes (index order), then by degree
     uint32_t* soff = nullptr;    // [n_sched+1]
     uint32_t n_sched = 0;
-    uint32_t n_lane = 0, n_wave = 0, n_block = 0;
-    uint32_t e_chunk0 = 0;       // first entry of the chunked tiers
-    uint32_t e_hub0 = 0;         // first entry of the 1024-lane tier
-    uint32_t e_wave_pre0 = 0;    // first 64-lane entry that goes through link_pre
-    // chunked-tier work units: one per 64-lane point, then one per segment of
-    // each 1024-lane point (hubs above HUB_SEG entries are split)
-    uint32_t n_units = 0, n_wave_units = 0;
-    uint32_t first_split_unit = 0;  // split hubs come last (degree order)
-    uint4* unit_a = nullptr;     // {first entry, entries, offset into link_pre, graph}
-    uint4* unit_b = nullptr;     // {schedule slot, warm offset, split hub or NO_HUB, segment}
-    uint32_t* eslot = nullptr;   // work unit of each chunked-tier entry
-    uint2* hubs = nullptr;       // split hubs: {first unit, segments}
+    uint32_t n_lane = 0;         // lane-tier points (first in the schedule)
+    // chunked-tier work units, in schedule order: one per 64-lane point, then
+    // one per segment of each 256-lane point (one segment up to HUB_SEG entries)
+    uint32_t n_units = 0, n_u64 = 0;
+    uint32_t first_split_unit = 0;  // split points come last (degree order)
+    uint4* unit_a = nullptr;     // {first entry, entries, -, graph}
+    uint4* unit_b = nullptr;     // {schedule slot, warm offset, split point or NO_HUB, segment}
+    uint2* hubs = nullptr;       // split points: {first unit, segments}
     uint32_t n_hubs = 0;
     float4* seg_out = nullptr;   // [2 n_units] segment end, start used, end Jacobian
     uint32_t* hub_done = nullptr;  // [2 n_hubs] fix-up converged this pass; last correction (f32)
     float2* warm = nullptr;      // last converged chunk starts, per graph regions
     float4* wjac = nullptr;      // split segments: each lane's prefix Jacobian at that start
+                                 // (lane 0: the segment start it converged from)
     size_t warm_len = 0;
 };
 
@@ -243,8 +242,7 @@ struct Ctx {
     // links
     LinkDir dir[2];
     bool any_links = false;
-    float4* link_pre = nullptr;
-    uint32_t* link_stats = nullptr;
+    uint32_t* link_stats = nullptr;  // [16] sweep counters (cg_debug_link_stats)
 
     // stage timing (cg_set_profiling)
     uint32_t prof_mask = 0;

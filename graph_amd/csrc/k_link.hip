@@ -9,15 +9,17 @@
 // The reference evaluates each point's links serially, every term seeing the
 // velocity accumulated so far.  Points are scheduled by live degree:
 //   - up to LANE_MAX entries: one lane per point, the serial loop as written;
-//   - more: one workgroup per point (64 or 1024 lanes).  The chain is cut into
-//     one chunk per lane; each sweep runs every chunk serially from a guessed
-//     start velocity, carrying the chunk's end velocity and 2x2 Jacobian, and
-//     an affine prefix scan over the chunks yields the next guesses (multiple
-//     shooting with Newton-accurate chunk maps).  Sweeps repeat until no start
-//     moves more than the tolerance; the last chunk's end, moved to its
-//     corrected start, is the point's velocity.  Converged starts are kept per
-//     point as the next call's first guesses.
-//   - a 1024-lane point above HUB_SEG entries is cut into segments, one
+//   - up to T64_MAX entries: one 64-lane workgroup per point; above: 256
+//     lanes.  The chain is cut into one chunk per lane, at most 4 entries,
+//     whose terms the lane gathers once into registers; each
+//     sweep runs every chunk from a guessed start velocity, carrying the
+//     chunk's end velocity and 2x2 Jacobian, and an affine prefix scan over
+//     the chunks yields the next guesses (multiple shooting with
+//     Newton-accurate chunk maps).  Sweeps repeat until no start moves more
+//     than the tolerance; the last chunk's end, moved to its corrected start,
+//     is the point's velocity.  Converged starts are kept per point as the
+//     next call's first guesses.
+//   - a point above HUB_SEG entries is cut into segments, one 256-lane
 //     workgroup each (one CU no longer carries a whole hub): each segment is
 //     solved as above from a guessed start and reports its end and end
 //     Jacobian; a fix-up composes the segments from the exact start 0 and
@@ -157,15 +159,15 @@ __global__ void k_sched_degree(const uint32_t* __restrict__ sched, uint32_t ns,
 }
 
 // Tier boundaries over the schedule's degrees (lane-tier degrees first, then
-// the chunked tiers ascending): first position above LANE_MAX and first above
-// WAVE_MAX (counts pre-filled with ns).  One writer each.
+// the chunked tiers ascending): first position above LANE_MAX and above
+// T64_MAX (counts pre-filled with ns).  One writer each.
 __global__ void k_tier_bounds(const uint32_t* __restrict__ skeys, uint32_t ns,
                               uint32_t* __restrict__ counts) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
     uint32_t d = skeys[k], prev = k ? skeys[k - 1] : 0u;
     if (d > LANE_MAX && (k == 0 || prev <= LANE_MAX)) counts[0] = k;
-    if (d > WAVE_MAX && (k == 0 || prev <= WAVE_MAX)) counts[1] = k;
+    if (d > T64_MAX && (k == 0 || prev <= T64_MAX)) counts[1] = k;
 }
 
 __global__ void k_slot_of(const uint32_t* __restrict__ sched, uint32_t ns, uint32_t* __restrict__ slot_of) {
@@ -198,21 +200,6 @@ __global__ void k_assemble(const uint32_t* __restrict__ ebase, uint32_t G, uint3
     v.x += p0;
     if ((flags[p] | flags[v.x]) & F_ABSENT) return;
     ent[soff[slot_of[p]] + (tb.lrank[e] - tb.lrank[tb.off_pt[lo]])] = v;
-}
-
-// Work unit of every chunked-tier entry: the last unit u with ubeg[u] <= e,
-// by binary search (ubeg holds n_units + 1 ascending entry offsets).
-__global__ void k_eslot(const uint32_t* __restrict__ ubeg, uint32_t n_units, uint32_t e0, uint32_t e1,
-                        uint32_t* __restrict__ eslot) {
-    uint32_t e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= e1) return;
-    uint32_t lo = 0, hi = n_units;  // invariant: ubeg[lo] <= e < ubeg[hi]
-    while (hi - lo > 1) {
-        uint32_t mid = lo + (hi - lo) / 2;
-        if (ubeg[mid] <= e) lo = mid;
-        else hi = mid;
-    }
-    eslot[e - e0] = lo;
 }
 
 // ---- spring pass ------------------------------------------------------------------
@@ -294,43 +281,7 @@ __global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __r
 }
 
 constexpr int MAX_SWEEPS = 8;
-constexpr int PREFETCH = 16;  // entries in flight per lane in the chunk loop
-
-// Per-entry terms that do not depend on the running velocity, gathered in one
-// coalesced pass for the chunked tiers: (q.x, q.y, D, K) with D the rest length
-// and K = linkSpring * alpha * strength * bias.  A work unit (a 64-lane point,
-// or one segment of a hub's chain run by 1024 lanes) stores its len entries
-// step-major: with c = ceil(len/B) entries per lane, entry k of the unit sits at
-// poff + (k % c) * B + k / c, so lane t reads its step-s term at s*B + t and a
-// wave's loads are one contiguous segment.  unit_a = {first entry, len, poff,
-// graph}.
-__global__ void __launch_bounds__(256) k_link_pre(int pass, const uint4* __restrict__ ent,
-                                                  uint32_t e0, uint32_t e1,
-                                                  const uint32_t* __restrict__ eslot,
-                                                  const uint4* __restrict__ unit_a, uint32_t wave_units,
-                                                  const GDesc* __restrict__ desc,
-                                                  const float2* __restrict__ pos0,
-                                                  const float2* __restrict__ pos1,
-                                                  float4* __restrict__ pre) {
-    uint32_t e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= e1) return;
-    const uint32_t u = eslot[e - e0];
-    const uint4 U = unit_a[u];
-    const GDesc& d = desc[U.w];
-    if (!(d.passes & (1u << pass))) return;
-    const float2* __restrict__ pin = ((d.inbuf >> pass) & 1u) ? pos1 : pos0;
-    const uint32_t local = e - U.x;
-    const uint32_t B = u < wave_units ? 64u : 1024u;
-    const uint32_t c = (U.y + B - 1) / B;
-    const uint32_t dst = U.z + (local % c) * B + local / c;
-    uint4 en = ent[e];
-    float2 q = pin[en.x];
-    float r = __uint_as_float(en.y), s = __uint_as_float(en.z), b = __uint_as_float(en.w);
-    float D = (r * d.span + d.lo) * d.dist;
-    pre[dst] = make_float4(q.x, q.y, D, d.spring_alpha * s * b);
-}
-
-// One serial spring step from a pre-gathered entry (no change when !live),
+// One serial spring step from an entry's terms (no change when !live),
 // also applying the step's Jacobian to J (column-major 2x2, J <- M J):
 //   u' = u + x t(u),  x = q - u,  t = K (1 - D / max(|x|, 0.99 D))
 //   M = du'/du = (1 - t) I - (K D / |x|^3) x x^T   (the last term is 0 when clamped)
@@ -351,21 +302,6 @@ __device__ inline void spring_step(float4 e, float2 pp, float2& v, float (&J)[4]
     float j0 = m00 * J[0] + m01 * J[1], j1 = m01 * J[0] + m11 * J[1];
     float j2 = m00 * J[2] + m01 * J[3], j3 = m01 * J[2] + m11 * J[3];
     J[0] = j0; J[1] = j1; J[2] = j2; J[3] = j3;
-}
-
-// Runs one lane's chunk (steps entries, stride B) from start velocity v,
-// accumulating the chunk's Jacobian; PF loads in flight (index clamped into
-// the chunk, so all are issued together).
-template <int PF, int B>
-__device__ inline void run_chunk(const float4* __restrict__ mine, uint32_t steps, float2 pp, float2& v,
-                                 float (&J)[4]) {
-    for (uint32_t k = 0; k < steps; k += PF) {
-        float4 buf[PF];
-#pragma unroll
-        for (int u = 0; u < PF; ++u) buf[u] = mine[(size_t)min(k + u, steps - 1) * B];
-#pragma unroll
-        for (int u = 0; u < PF; ++u) spring_step(buf[u], pp, v, J, k + u < steps);
-    }
 }
 
 // Affine map s -> A s + b of a velocity (A column-major 2x2).
@@ -437,10 +373,20 @@ constexpr uint32_t NO_HUB = 0xffffffffu;
 // {end, start used} and its end's Jacobian in its start for the fix-up.
 // Round r > 0 reruns only the segments after the first of split hubs whose
 // fix-up has not converged.
-// A 64-lane unit of at most REG_MAX entries (at most 4 per lane) reads its
-// entries and neighbours itself, once, and keeps the terms in registers for
-// every sweep; the pre-gather pass skips those entries.
-constexpr uint32_t REG_MAX = 256;
+// Chunked tiers: one workgroup of B lanes per work unit of at most 4 B
+// entries.  unit_a = {first entry, len, -, graph}; unit_b = {slot, warm offset,
+// split point or NO_HUB, segment}.  Each lane gathers its chunk's terms
+// (q.x, q.y, D, K) once into registers (D the rest length, K = linkSpring *
+// alpha * strength * bias), runs the chunk from a guessed start (warm: the
+// start converged on the previous call), the block scans the chunk maps, and
+// sweeps repeat until no start moves more than the tolerance; the last
+// chunk's end, moved to its corrected start, is the unit's end velocity.  The
+// chain of a unit starts from the velocity in lane 0's warm slot: 0 for a
+// point's first unit, the fix-up's current guess for a later segment of a
+// split point.  A whole point writes its integrated position; a segment writes
+// {end, start used} and its end's Jacobian for the fix-up.  Round r > 0
+// reruns only the segments after the first of split points whose fix-up has
+// not converged.
 template <int B>
 __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restrict__ unit_a,
                                                   const uint4* __restrict__ unit_b, uint32_t u_first,
@@ -450,7 +396,6 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
                                                   float2* __restrict__ pos1,
                                                   const uint8_t* __restrict__ flags,
                                                   const uint4* __restrict__ ent,
-                                                  const float4* __restrict__ pre,
                                                   float2* __restrict__ warm,
                                                   float4* __restrict__ wjac,
                                                   float4* __restrict__ seg_out,
@@ -474,30 +419,14 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
     const float2* __restrict__ pin = bsel ? pos1 : pos0;
     float2* __restrict__ wslot = warm + V.y + t;
     const uint32_t len = U.y;
-    const uint32_t c = (len + B - 1) / B;
+    const uint32_t c = (len + B - 1) / B;  // <= 4 by construction
     const uint32_t lo = min(len, (uint32_t)t * c), hi = min(len, lo + c);
     const uint32_t steps = hi - lo;
     const int last = (int)((len - 1) / c);
     const float2 pp = pin[p];
-    const float4* __restrict__ mine = pre + U.z + t;
-    float2 g = *wslot;
-    const float2 G = warm[V.y];  // lane 0's slot: the unit's start
-    if (split && t > 0) {
-        // the lane's start converged from the previous start G0, moved to the
-        // current G along its prefix Jacobian (zero before the first run)
-        const float4 P = wjac[V.y + t];
-        const float4 o0 = seg_out[2 * (size_t)u];
-        const float dx = G.x - o0.z, dy = G.y - o0.w;
-        g.x += P.x * dx + P.z * dy;
-        g.y += P.y * dx + P.w * dy;
-    }
-    float prev = INFINITY;
-    float J[4];
-    float2 v;
-    Aff tot;
-    const bool in_regs = B == 64 && len <= REG_MAX;  // uniform: c <= 4
+    // the lane's entries [lo, hi) of the unit, gathered once
     float4 reg[4];
-    if (in_regs) {
+    {
         uint4 en[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) en[s] = ent[U.x + min(lo + s, len - 1)];
@@ -509,17 +438,27 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
             reg[s] = make_float4(q.x, q.y, (r * d.span + d.lo) * d.dist, d.spring_alpha * sv * bv);
         }
     }
+    float2 g = *wslot;
+    const float2 G = warm[V.y];  // lane 0's slot: the unit's start
+    if (split && t > 0) {
+        // the lane's start converged from the previous start G0 (kept in lane
+        // 0's slot of wjac), moved to the current G along its prefix Jacobian
+        // (both zero before the first run)
+        const float4 P = wjac[V.y + t];
+        const float4 G0 = wjac[V.y];
+        const float dx = G.x - G0.x, dy = G.y - G0.y;
+        g.x += P.x * dx + P.z * dy;
+        g.y += P.y * dx + P.w * dy;
+    }
+    float prev = INFINITY;
+    float J[4];
+    float2 v;
+    Aff tot;
     for (int sweep = 0;; ++sweep) {
         v = g;
         J[0] = 1.f; J[1] = 0.f; J[2] = 0.f; J[3] = 1.f;
-        if (in_regs) {
 #pragma unroll
-            for (int s = 0; s < 4; ++s) spring_step(reg[s], pp, v, J, (uint32_t)s < steps);
-        } else if (c <= 4) {  // short chunks take a short batch: no idle steps
-            run_chunk<4, B>(mine, steps, pp, v, J);
-        } else {
-            run_chunk<PREFETCH, B>(mine, steps, pp, v, J);
-        }
+        for (int s = 0; s < 4; ++s) spring_step(reg[s], pp, v, J, (uint32_t)s < steps);
         // chunk map  s -> J (s - g) + v;  exclusive prefix applied to the unit's start
         const Aff m{J[0], J[1], J[2], J[3], v.x - (J[0] * g.x + J[2] * g.y), v.y - (J[1] * g.x + J[3] * g.y)};
         const Aff ex = block_scan<B>(m, s_wave, s_wave_inc, tot);
@@ -544,7 +483,8 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
                 s_final = make_float2(v.x + J[0] * ex_ + J[2] * ey, v.y + J[1] * ex_ + J[3] * ey);
             }
             *wslot = start;
-            if (split) wjac[V.y + t] = make_float4(ex.a0, ex.a1, ex.a2, ex.a3);
+            if (split)
+                wjac[V.y + t] = t == 0 ? make_float4(G.x, G.y, 0.f, 0.f) : make_float4(ex.a0, ex.a1, ex.a2, ex.a3);
             if (stats && t == 0) {
                 // sweeps used, units, max sweeps, units not within tolerance (per tier)
                 const int o = B == 64 ? 0 : 4;
@@ -574,15 +514,16 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
     }
 }
 
-// Fix-up of the split hubs, one wave per hub (hubs[h] = {first unit, segments}):
-// composes the segments' linearised end maps from the exact start 0,
+// Fix-up of the split points, one wave per point (hubs[h] = {first unit,
+// segments}): the segments' linearised end maps x -> E_s + A_s (x - G_s),
+// composed from the exact start 0, give each segment's true start,
 //   true_{s+1} = E_s + A_s (true_s - G_s),
-// stores each true start as the segment's next start guess, and when every
-// guess was already within the tolerance, the corrections stopped halving
-// (the f32 noise floor of long segments), or on the final round, writes the
-// point's velocity (the last segment's end from its true start).  Lanes load
-// 64 segments at a time; the chain runs on broadcasts, the same in every lane.
-// hub_done holds [n_hubs] flags, then [n_hubs] previous largest corrections.
+// stored as the segment's next start guess; when every guess was already
+// within the tolerance, the corrections stopped halving (the f32 noise floor
+// of long chains), or on the final round, the point's velocity (the last
+// segment's end from its true start) is written.  The composition is a wave
+// scan of affine maps over 64 segments at a time.  hub_done holds [n_hubs]
+// flags, then [n_hubs] previous largest corrections.
 __global__ void __launch_bounds__(64) k_hub_fix(int pass, const uint2* __restrict__ hubs, uint32_t n_hubs,
                                                 const uint4* __restrict__ unit_a,
                                                 const uint4* __restrict__ unit_b, int round, int final_round,
@@ -598,7 +539,7 @@ __global__ void __launch_bounds__(64) k_hub_fix(int pass, const uint2* __restric
     const GDesc& d = desc[unit_a[hb.x].w];
     if (!(d.passes & (1u << pass))) return;
     if (hub_done[h]) return;
-    float2 tru = make_float2(0.f, 0.f);
+    float2 x0 = make_float2(0.f, 0.f);  // true start of the chunk's first segment
     float dmax = 0.f;
     for (uint32_t s0 = 0; s0 < hb.y; s0 += 64) {
         const uint32_t s = s0 + lane;
@@ -606,21 +547,28 @@ __global__ void __launch_bounds__(64) k_hub_fix(int pass, const uint2* __restric
         const size_t u = (size_t)hb.x + (have ? s : s0);
         const float4 o0 = seg_out[2 * u], A = seg_out[2 * u + 1];
         const uint32_t wo = unit_b[u].y;
-        float2 mine = make_float2(0.f, 0.f);
-        const uint32_t cnt = min(64u, hb.y - s0);
-        for (uint32_t k = 0; k < cnt; ++k) {
-            const float ex = __shfl(o0.x, k, 64), ey = __shfl(o0.y, k, 64);
-            const float gx = __shfl(o0.z, k, 64), gy = __shfl(o0.w, k, 64);
-            const float a0 = __shfl(A.x, k, 64), a1 = __shfl(A.y, k, 64);
-            const float a2 = __shfl(A.z, k, 64), a3 = __shfl(A.w, k, 64);
-            if (lane == k) mine = tru;
-            if (s0 + k > 0) dmax = fmaxf(dmax, fmaxf(fabsf(tru.x - gx), fabsf(tru.y - gy)));
-            const float dx = tru.x - gx, dy = tru.y - gy;
-            tru = make_float2(ex + a0 * dx + a2 * dy, ey + a1 * dx + a3 * dy);
+        Aff m = have ? Aff{A.x, A.y, A.z, A.w, o0.x - (A.x * o0.z + A.z * o0.w), o0.y - (A.y * o0.z + A.w * o0.w)}
+                     : aff_identity();
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const Aff q = aff_shfl_up(m, o);
+            if ((int)lane >= o) m = aff_then(q, m);
         }
-        if (have && s > 0) warm[wo] = mine;  // lane 0 of the segment: its next start
+        Aff ex = aff_shfl_up(m, 1);
+        if (lane == 0) ex = aff_identity();
+        const float2 tru = make_float2(ex.a0 * x0.x + ex.a2 * x0.y + ex.bx, ex.a1 * x0.x + ex.a3 * x0.y + ex.by);
+        if (have && s > 0) {
+            dmax = fmaxf(dmax, fmaxf(fabsf(tru.x - o0.z), fabsf(tru.y - o0.w)));
+            warm[wo] = tru;  // lane 0 of the segment: its next start
+        }
+        const Aff all{__shfl(m.a0, 63, 64), __shfl(m.a1, 63, 64), __shfl(m.a2, 63, 64),
+                      __shfl(m.a3, 63, 64), __shfl(m.bx, 63, 64), __shfl(m.by, 63, 64)};
+        x0 = make_float2(all.a0 * x0.x + all.a2 * x0.y + all.bx, all.a1 * x0.x + all.a3 * x0.y + all.by);
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, o, 64));
     if (lane != 0) return;
+    const float2 tru = x0;  // the last segment's end from its true start
     float* __restrict__ dprev = reinterpret_cast<float*>(hub_done + n_hubs);
     const bool ok = dmax <= d.tol || (round > 0 && dmax > 0.5f * dprev[h]);
     dprev[h] = dmax;
@@ -636,7 +584,7 @@ __global__ void __launch_bounds__(64) k_hub_fix(int pass, const uint2* __restric
         pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, tru, d.friction, d.S);
     }
     if (stats) {
-        // rounds used, split hubs, max rounds, hubs stopped at the round cap
+        // rounds used, split points, max rounds, points stopped at the round cap
         atomicAdd(&stats[8], (uint32_t)(round + 1));
         atomicAdd(&stats[9], 1u);
         atomicMax(&stats[10], (uint32_t)(round + 1));
@@ -645,6 +593,11 @@ __global__ void __launch_bounds__(64) k_hub_fix(int pass, const uint2* __restric
 }
 
 __global__ void k_copy_f2(const float2* __restrict__ src, float2* __restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_copy_f4(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[i];
 }
@@ -774,13 +727,10 @@ int link_build_graph(Ctx* c, Graph& gr) {
 int link_pack(Ctx* c, LinkDir (&old)[2]) {
     auto free_dir = [&](LinkDir& L) {
         dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff); dfree(c, L.unit_a); dfree(c, L.unit_b);
-        dfree(c, L.eslot); dfree(c, L.hubs); dfree(c, L.seg_out); dfree(c, L.hub_done); dfree(c, L.warm);
-        dfree(c, L.wjac);
+        dfree(c, L.hubs); dfree(c, L.seg_out); dfree(c, L.hub_done); dfree(c, L.warm); dfree(c, L.wjac);
         L = LinkDir();
     };
     for (int d = 0; d < 2; ++d) c->dir[d] = LinkDir();
-    dfree(c, c->link_pre);
-    c->link_pre = nullptr;
     c->any_links = false;
     const uint32_t G = c->G, NP = c->NP;
     for (uint32_t s = 0; s < G; ++s) {
@@ -816,7 +766,6 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
     if (rc) { cleanup(); return rc; }
     std::vector<uint32_t> h_p0(G);
     for (uint32_t s = 0; s < G; ++s) h_p0[s] = c->graphs[c->order[s]].p0;
-    size_t pre_need = 0;
     for (int d = 0; d < 2; ++d) {
         LinkDir& L = c->dir[d];
         std::vector<GLinkPtr> h_tab(G);
@@ -856,8 +805,7 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         CG_HIP(c, hipMemcpyAsync(&L.E_live, L.soff + ns, 4, hipMemcpyDeviceToHost, c->stream));
         CG_HIP(c, hipStreamSynchronize(c->stream));
         L.n_lane = h[0];
-        L.n_wave = h[1] - h[0];
-        L.n_block = ns - h[1];
+        const uint32_t n64 = h[1] - h[0];
         L.ent = (uint4*)dalloc(c, (size_t)(L.E_live ? L.E_live : 1) * 16, &rc);
         if (rc) { cleanup(); return rc; }
         hipLaunchKernelGGL(k_slot_of, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.sched, ns,
@@ -866,7 +814,7 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         if (etot)
             hipLaunchKernelGGL(k_assemble, dim3(blocks_for(etot, 256)), dim3(256), 0, c->stream, ebase, G,
                                etot, tab, c->desc, slot_of, L.soff, c->flags, L.ent);
-        // chunked tiers: work units, step-major offsets, entry -> unit map, warm-start offsets
+        // chunked tiers: work units, warm-start offsets
         const uint32_t nch = ns - L.n_lane;
         std::vector<uint32_t> hs(nch + 1), hsched(nch ? nch : 1);
         CG_HIP(c, hipMemcpyAsync(hs.data(), L.soff + L.n_lane, (size_t)(nch + 1) * 4,
@@ -875,23 +823,21 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
             CG_HIP(c, hipMemcpyAsync(hsched.data(), L.sched + L.n_lane, (size_t)nch * 4,
                                      hipMemcpyDeviceToHost, c->stream));
         CG_HIP(c, hipStreamSynchronize(c->stream));
-        L.e_chunk0 = hs[0];
-        L.e_hub0 = hs[L.n_wave];
-        // A 64-lane point is one unit; a 1024-lane point of d entries is
+        // A 64-lane point is one unit; a 256-lane point of d entries is
         // ceil(d / HUB_SEG) near-equal segments, one unit each, in chain order.
+        // Every unit holds at most 4 entries per lane.
         std::vector<uint4> ua, ub;
-        std::vector<uint32_t> ubeg, slot_u0(nch + 1, 0), slot_graph(nch ? nch : 1, 0);
+        std::vector<uint32_t> slot_u0(nch + 1, 0), slot_graph(nch ? nch : 1, 0), slot_lanes(nch ? nch : 1, 0);
         std::vector<uint2> hh;
-        uint32_t pos = 0;
         L.first_split_unit = 0;
         for (uint32_t k = 0; k < nch; ++k) {
             const uint32_t dd = hs[k + 1] - hs[k];
-            const bool hub = k >= L.n_wave;
-            const uint32_t B = hub ? 1024u : 64u;
-            const uint32_t S = hub ? (dd + HUB_SEG - 1) / HUB_SEG : 1u;
+            const uint32_t B = k < n64 ? 64u : 256u;
+            const uint32_t S = B == 256u ? (dd + HUB_SEG - 1) / HUB_SEG : 1u;
             const uint32_t p = hsched[k];
             const uint32_t gs = (uint32_t)(std::upper_bound(h_p0.begin(), h_p0.end(), p) - h_p0.begin()) - 1;
             slot_graph[k] = gs;
+            slot_lanes[k] = B;
             slot_u0[k] = (uint32_t)ua.size();
             uint32_t hub_id = NO_HUB;
             if (S > 1) {
@@ -902,27 +848,20 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
             uint32_t eb = hs[k];
             for (uint32_t q = 0; q < S; ++q) {
                 const uint32_t len = dd / S + (q < dd % S ? 1u : 0u);
-                ua.push_back(make_uint4(eb, len, pos, gs));
+                if (len == 0 || len > 4 * B) {
+                    cleanup();
+                    return fail(c, CG_ERR_INVALID, "link schedule: a work unit outside 1..4 entries per lane");
+                }
+                ua.push_back(make_uint4(eb, len, 0u, gs));
                 ub.push_back(make_uint4(L.n_lane + k, 0u, hub_id, q));
-                ubeg.push_back(eb);
                 eb += len;
-                pos += ((len + B - 1) / B) * B;
             }
         }
         slot_u0[nch] = (uint32_t)ua.size();
-        ubeg.push_back(hs[nch]);
-        // 64-lane units ascend by length: the register-resident ones come first
-        L.e_wave_pre0 = L.e_hub0;
-        for (uint32_t k = 0; k < L.n_wave; ++k)
-            if (hs[k + 1] - hs[k] > REG_MAX) {
-                L.e_wave_pre0 = hs[k];
-                break;
-            }
         L.n_units = (uint32_t)ua.size();
-        L.n_wave_units = L.n_wave;
+        L.n_u64 = n64;
         L.n_hubs = (uint32_t)hh.size();
         if (hh.empty()) L.first_split_unit = L.n_units;
-        pre_need = std::max<size_t>(pre_need, (size_t)pos);
         // warm-start regions: per graph, its chunked points in local index order,
         // each point's units in chain order, B lanes each
         std::vector<std::vector<std::pair<uint32_t, uint32_t>>> per(G);  // (local point, chunk slot)
@@ -935,7 +874,7 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
             for (auto& pk : per[s]) {
                 for (uint32_t u = slot_u0[pk.second]; u < slot_u0[pk.second + 1]; ++u) {
                     ub[u].y = (uint32_t)wpos;
-                    wpos += (pk.second < L.n_wave) ? 64 : 1024;
+                    wpos += slot_lanes[pk.second];
                 }
             }
             region[s] = {start, wpos - start};
@@ -947,10 +886,8 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         L.seg_out = (float4*)dalloc(c, nu * 32, &rc);
         L.hubs = (uint2*)dalloc(c, (hh.size() ? hh.size() : 1) * 8, &rc);
         L.hub_done = (uint32_t*)dalloc(c, (hh.size() ? hh.size() : 1) * 8, &rc);
-        L.eslot = (uint32_t*)dalloc(c, (size_t)(L.E_live - L.e_chunk0 + 1) * 4, &rc);
         L.warm = (float2*)dalloc(c, (wpos ? wpos : 1) * 8, &rc);
         L.wjac = (float4*)dalloc(c, (wpos ? wpos : 1) * 16, &rc);
-        uint32_t* d_ubeg = (uint32_t*)talloc(ubeg.size() * 4);
         if (rc) { cleanup(); return rc; }
         if (!ua.empty()) {
             CG_HIP(c, hipMemcpyAsync(L.unit_a, ua.data(), ua.size() * 16, hipMemcpyHostToDevice, c->stream));
@@ -958,28 +895,26 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         }
         if (!hh.empty())
             CG_HIP(c, hipMemcpyAsync(L.hubs, hh.data(), hh.size() * 8, hipMemcpyHostToDevice, c->stream));
-        CG_HIP(c, hipMemcpyAsync(d_ubeg, ubeg.data(), ubeg.size() * 4, hipMemcpyHostToDevice, c->stream));
         CG_HIP(c, hipMemsetAsync(L.hub_done, 0, (hh.size() ? hh.size() : 1) * 8, c->stream));
         CG_HIP(c, hipMemsetAsync(L.warm, 0, (wpos ? wpos : 1) * 8, c->stream));
         CG_HIP(c, hipMemsetAsync(L.wjac, 0, (wpos ? wpos : 1) * 16, c->stream));
         CG_HIP(c, hipMemsetAsync(L.seg_out, 0, nu * 32, c->stream));
         for (uint32_t s = 0; s < G; ++s) {
             Graph& g = c->graphs[c->order[s]];
-            if (old[d].warm && g.wn[d] && g.wn[d] == region[s].second)
+            if (old[d].warm && g.wn[d] && g.wn[d] == region[s].second) {
                 hipLaunchKernelGGL(k_copy_f2, dim3(blocks_for(g.wn[d], 256)), dim3(256), 0, c->stream,
                                    old[d].warm + g.w0[d], L.warm + region[s].first, (size_t)g.wn[d]);
+                hipLaunchKernelGGL(k_copy_f4, dim3(blocks_for(g.wn[d], 256)), dim3(256), 0, c->stream,
+                                   old[d].wjac + g.w0[d], L.wjac + region[s].first, (size_t)g.wn[d]);
+            }
             g.w0[d] = (uint32_t)region[s].first;
             g.wn[d] = (uint32_t)region[s].second;
         }
-        if (L.E_live > L.e_chunk0)
-            hipLaunchKernelGGL(k_eslot, dim3(blocks_for(L.E_live - L.e_chunk0, 256)), dim3(256), 0,
-                               c->stream, d_ubeg, L.n_units, L.e_chunk0, L.E_live, L.eslot);
         CG_HIP(c, hipGetLastError());
         // keep the staging vectors alive until the copies above have run
         CG_HIP(c, hipStreamSynchronize(c->stream));
     }
     for (int d = 0; d < 2; ++d) free_dir(old[d]);
-    c->link_pre = (float4*)dalloc(c, (pre_need ? pre_need : 1) * 16, &rc);
     if (!c->link_stats && !rc) {
         c->link_stats = (uint32_t*)dalloc(c, 64, &rc);
         if (!rc) CG_HIP(c, hipMemsetAsync(c->link_stats, 0, 64, c->stream));
@@ -992,51 +927,42 @@ int link_run(Ctx* c, int dir, float2* vout) {
     if (!c->any_links) return CG_OK;
     const LinkDir& L = c->dir[dir];
     const int pass = P_LINK_IN + dir;
-    // The 1024-lane tier runs on the side stream while the lane and 64-lane
-    // tiers fill the other CUs on the main stream (a third stream for the lane
-    // tier measured slower).  Split points take HUB_ROUNDS rounds of segment
-    // solves and fix-ups; rounds after convergence return at once.
-    const uint32_t e0 = L.e_chunk0, e1 = L.E_live, eh = L.e_hub0;
-    const uint32_t n_hub_units = L.n_units - L.n_wave_units;
+    // The 256-lane tier's first round runs alone on the main stream; then the
+    // split points' fix-ups and later rounds (a chain of short launches) run
+    // on the side stream while the 64-lane and lane tiers fill the GPU on the
+    // main stream, which joins the side stream at the end.  Rounds after
+    // convergence return at once.
+    const uint32_t u256 = L.n_u64;
     // sweep counters only while requested (same-address atomics per block)
     uint32_t* stats = (c->prof_mask & CG_PROFILE_LINK_STATS) ? c->link_stats : nullptr;
-    const bool fork = n_hub_units && c->side;
-    if (n_hub_units) {
+    if (L.n_units > u256)
+        hipLaunchKernelGGL(k_link_chunk<256>, dim3(L.n_units - u256), dim3(256), 0, c->stream, pass, L.unit_a,
+                           L.unit_b, u256, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm,
+                           L.wjac, L.seg_out, L.hub_done, stats, vout);
+    const bool fork = L.n_hubs && c->side;
+    if (L.n_hubs) {
         hipStream_t s = c->stream;
         if (fork) {
             CG_HIP(c, hipEventRecord(c->ev_fork, c->stream));
             CG_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
             s = c->side;
         }
-        if (e1 > eh)
-            hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(e1 - eh, 256)), dim3(256), 0, s, pass, L.ent, eh,
-                               e1, L.eslot + (eh - e0), L.unit_a, L.n_wave_units, c->desc, c->pos[0],
-                               c->pos[1], c->link_pre);
         for (int r = 0; r < HUB_ROUNDS; ++r) {
-            const uint32_t u0 = r == 0 ? L.n_wave_units : L.first_split_unit;
-            if (r > 0 && !L.n_hubs) break;
-            hipLaunchKernelGGL(k_link_chunk<1024>, dim3(L.n_units - u0), dim3(1024), 0, s, pass, L.unit_a,
-                               L.unit_b, u0, r, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent,
-                               c->link_pre,
-                               L.warm, L.wjac, L.seg_out, L.hub_done, stats, vout);
-            if (L.n_hubs)
-                hipLaunchKernelGGL(k_hub_fix, dim3(L.n_hubs), dim3(64), 0, s, pass, L.hubs,
-                                   L.n_hubs, L.unit_a, L.unit_b, r, (int)(r == HUB_ROUNDS - 1), L.sched,
-                                   c->desc, c->pos[0], c->pos[1], c->flags, L.seg_out, L.warm, L.hub_done,
-                                   stats, vout);
+            if (r > 0)
+                hipLaunchKernelGGL(k_link_chunk<256>, dim3(L.n_units - L.first_split_unit), dim3(256), 0, s, pass,
+                                   L.unit_a, L.unit_b, L.first_split_unit, r, L.sched, c->desc, c->pos[0],
+                                   c->pos[1], c->flags, L.ent, L.warm, L.wjac, L.seg_out, L.hub_done, stats,
+                                   vout);
+            hipLaunchKernelGGL(k_hub_fix, dim3(L.n_hubs), dim3(64), 0, s, pass, L.hubs, L.n_hubs, L.unit_a,
+                               L.unit_b, r, (int)(r == HUB_ROUNDS - 1), L.sched, c->desc, c->pos[0], c->pos[1],
+                               c->flags, L.seg_out, L.warm, L.hub_done, stats, vout);
         }
         if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
-    const uint32_t ew = L.e_wave_pre0;  // 64-lane entries [e0, ew) stay in registers
-    if (eh > ew)
-        hipLaunchKernelGGL(k_link_pre, dim3(blocks_for(eh - ew, 256)), dim3(256), 0, c->stream, pass, L.ent, ew,
-                           eh, L.eslot + (ew - e0), L.unit_a, L.n_wave_units, c->desc, c->pos[0], c->pos[1],
-                           c->link_pre);
-    if (L.n_wave_units)
-        hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_wave_units), dim3(64), 0, c->stream, pass, L.unit_a,
-                           L.unit_b, 0u, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent,
-                           c->link_pre,
-                           L.warm, L.wjac, L.seg_out, L.hub_done, stats, vout);
+    if (L.n_u64)
+        hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_u64), dim3(64), 0, c->stream, pass, L.unit_a, L.unit_b,
+                           0u, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm, L.wjac,
+                           L.seg_out, L.hub_done, stats, vout);
     if (L.n_lane)
         hipLaunchKernelGGL(k_link_lane, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, pass,
                            L.sched, L.soff, L.ent, L.n_lane, c->gid, c->desc, c->pos[0], c->pos[1],

@@ -186,11 +186,11 @@ int cg_debug_link_csr(cg_ctx* ctx, int32_t dir, uint32_t* offsets, uint32_t* nbr
 
 /* Multiple-shooting counters of the chunked link tiers since the last call
  * (then reset), collected while cg_set_profiling has CG_PROFILE_LINK_STATS:
- * out[0..3] for the 64-lane tier and out[4..7] for the 1024-lane tier's work
+ * out[0..3] for the 64-lane tier and out[4..7] for the 256-lane tier's work
  * units = {sweeps summed over units, units, max sweeps, units stopped at the
  * sweep cap}; out[8..11] for the points split into segments = {rounds summed,
- * points, max rounds, points stopped at the round cap}. */
-int cg_debug_link_stats(cg_ctx* ctx, uint32_t* out12);
+ * points, max rounds, points stopped at the round cap}; out[12..15] zero. */
+int cg_debug_link_stats(cg_ctx* ctx, uint32_t* out16);
 
 /* Bytes of device memory the context holds. */
 uint64_t cg_device_bytes(const cg_ctx* ctx);
