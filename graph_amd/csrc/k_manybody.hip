@@ -46,12 +46,14 @@ __device__ inline uint32_t finest_key(const GDesc& d, float2 q) {
 }
 
 // LSD radix sort of (cell key, point index), RB_ITEMS keys per lane, digits of
-// `width` <= 8 bits.  Per pass: block digit histograms (LDS atomics only),
+// `width` <= RB_MAXW bits.  Per pass: block digit histograms (LDS atomics only),
 // an exclusive scan of the [digit][block] table, then a stable scatter whose
 // in-block ranks come from ballot matching in position order.  No global
 // atomics: on this part they serialise at a few ns each.
 constexpr int RB_ITEMS = 8;
 constexpr int RB_TILE = 256 * RB_ITEMS;
+constexpr int RB_MAXW = 9;               // digits of up to 9 bits: 18-bit cell keys in two passes
+constexpr int RB_RADIX = 1 << RB_MAXW;
 
 // Keys of the points in the previous tick's cell order, and the first digit's
 // block histogram.  `visit` may alias `vout` (each lane rewrites what it read).
@@ -65,12 +67,12 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
                                                   int width, uint32_t nb, uint32_t* __restrict__ hist,
                                                   uint32_t* __restrict__ cell_end, uint32_t n_cells,
                                                   uint32_t* __restrict__ big_count) {
-    __shared__ uint32_t h[256];
+    __shared__ uint32_t h[RB_RADIX];
     const int t = threadIdx.x;
     const uint32_t radix = 1u << width, mask = radix - 1;
     for (uint32_t f = blockIdx.x * 256 + t; f < n_cells; f += nb * 256) cell_end[f] = 0u;
     if (blockIdx.x == 0 && t < 3) big_count[t] = 0u;
-    h[t] = 0;
+    for (uint32_t q = t; q < radix; q += 256) h[q] = 0;
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
@@ -86,15 +88,15 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
         }
     }
     __syncthreads();
-    if ((uint32_t)t < radix) hist[t * nb + blockIdx.x] = h[t];
+    for (uint32_t q = t; q < radix; q += 256) hist[q * nb + blockIdx.x] = h[q];
 }
 
 __global__ void __launch_bounds__(256) k_radix_up(const uint32_t* __restrict__ kin, uint32_t np, int shift,
                                                   int width, uint32_t nb, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[256];
+    __shared__ uint32_t h[RB_RADIX];
     const int t = threadIdx.x;
     const uint32_t radix = 1u << width, mask = radix - 1;
-    h[t] = 0;
+    for (uint32_t q = t; q < radix; q += 256) h[q] = 0;
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
@@ -102,7 +104,7 @@ __global__ void __launch_bounds__(256) k_radix_up(const uint32_t* __restrict__ k
         if (k < np) atomicAdd(&h[(kin[k] >> shift) & mask], 1u);
     }
     __syncthreads();
-    if ((uint32_t)t < radix) hist[t * nb + blockIdx.x] = h[t];
+    for (uint32_t q = t; q < radix; q += 256) hist[q * nb + blockIdx.x] = h[q];
 }
 
 __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__ kin,
@@ -111,20 +113,22 @@ __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ hist,
                                                     uint32_t* __restrict__ kout,
                                                     uint32_t* __restrict__ vout) {
-    __shared__ uint32_t base[256];
-    __shared__ uint32_t wc[4][256];
+    __shared__ uint32_t base[RB_RADIX];
+    __shared__ uint32_t wc[4][RB_RADIX];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t radix = 1u << width, mask = radix - 1;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    if ((uint32_t)t < radix) base[t] = hist[t * nb + blockIdx.x];
+    for (uint32_t q = t; q < radix; q += 256) base[q] = hist[q * nb + blockIdx.x];
     for (int j = 0; j < RB_ITEMS; ++j) {
         const uint32_t k = blockIdx.x * RB_TILE + j * 256 + t;
         const bool valid = k < np;
         const uint32_t key = valid ? kin[k] : 0u;
         const uint32_t val = valid ? vin[k] : 0u;
         const uint32_t dg = (key >> shift) & mask;
+        for (uint32_t q = t; q < radix; q += 256) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) wc[q][t] = 0u;
+            for (int w = 0; w < 4; ++w) wc[w][q] = 0u;
+        }
         __syncthreads();
         // lanes of this wave holding the same digit, in position order
         unsigned long long m = __ballot(valid);
@@ -143,7 +147,7 @@ __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__
             vout[off] = val;
         }
         __syncthreads();
-        if ((uint32_t)t < radix) base[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+        for (uint32_t q = t; q < radix; q += 256) base[q] += wc[0][q] + wc[1][q] + wc[2][q] + wc[3][q];
         __syncthreads();
     }
 }
@@ -973,12 +977,12 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
 
 }  // namespace
 
-// Digits of at most 8 bits, as few passes as cover key_range, for n pairs.
+// Digits of at most RB_MAXW bits, as few passes as cover key_range, for n pairs.
 int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range) {
     int rc = CG_OK;
     int bits = 1;
     while (bits < 32 && (1ull << bits) < key_range) ++bits;
-    rs.passes = (bits + 7) / 8;
+    rs.passes = (bits + RB_MAXW - 1) / RB_MAXW;
     rs.width = (bits + rs.passes - 1) / rs.passes;
     if (rs.width < 4) rs.width = 4;  // 16+ digits: the histogram table is a multiple of 16
     rs.n = n;
