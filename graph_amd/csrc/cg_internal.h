@@ -154,9 +154,9 @@ struct LinkDir {
 struct RadixSort {
     int passes = 1, width = 8;      // digits of `width` bits cover every key
     uint32_t n = 0, nb = 0;         // pairs, blocks of RB_TILE pairs
-    uint32_t n_hist = 0;            // (1 << width) * nb, a multiple of 16
+    uint32_t n_hist = 0;            // (1 << width) * nb
     uint32_t* hist = nullptr;       // [n_hist] per-pass [digit][block] counts
-    uint32_t* part = nullptr;       // per-tile totals of the histogram scan
+    uint32_t* tot = nullptr;        // [1 << width] digit totals of the current pass
 };
 
 struct Ctx {

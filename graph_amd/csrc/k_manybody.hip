@@ -30,7 +30,6 @@ namespace cg {
 namespace {
 
 constexpr uint32_t TMAX = 0xffffffffu;
-constexpr uint32_t SCAN_TILE = 4096;  // cells per scan tile: 256 lanes x 16
 constexpr uint32_t BIG_CHUNK = 2048;  // points per block in a crowded cell
 constexpr uint32_t MID_BLOCKS = 2048; // grid of the wave-per-cell pass (8192 waves)
 
@@ -111,14 +110,33 @@ __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ vin, uint32_t np,
                                                     int shift, int width, uint32_t nb,
                                                     const uint32_t* __restrict__ hist,
+                                                    const uint32_t* __restrict__ totals,
                                                     uint32_t* __restrict__ kout,
                                                     uint32_t* __restrict__ vout) {
     __shared__ uint32_t base[RB_RADIX];
     __shared__ uint32_t wc[4][RB_RADIX];
+    __shared__ uint32_t sw[4];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t radix = 1u << width, mask = radix - 1;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    for (uint32_t q = t; q < radix; q += 256) base[q] = hist[q * nb + blockIdx.x];
+    {
+        // the digits' offsets: exclusive scan of the totals (2 digits per lane),
+        // plus this block's offset within each digit's row
+        const uint32_t d0 = 2 * t, d1 = 2 * t + 1;
+        const uint32_t v0 = d0 < radix ? totals[d0] : 0u, v1 = d1 < radix ? totals[d1] : 0u;
+        uint32_t inc = v0 + v1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += u;
+        }
+        if (lane == 63) sw[wave] = inc;
+        __syncthreads();
+        const uint32_t before = (wave > 0 ? sw[0] : 0u) + (wave > 1 ? sw[1] : 0u) + (wave > 2 ? sw[2] : 0u);
+        const uint32_t ex = before + inc - v0 - v1;
+        if (d0 < radix) base[d0] = ex + hist[d0 * nb + blockIdx.x];
+        if (d1 < radix) base[d1] = ex + v0 + hist[d1 * nb + blockIdx.x];
+    }
     for (int j = 0; j < RB_ITEMS; ++j) {
         const uint32_t k = blockIdx.x * RB_TILE + j * 256 + t;
         const bool valid = k < np;
@@ -164,53 +182,22 @@ __global__ void __launch_bounds__(256) k_bin_bounds(const uint32_t* __restrict__
     if (k == np - 1 || sk[k + 1] != key) end[key] = k + 1;
 }
 
-// Exclusive scan of n counts (n a multiple of 16): per-tile totals, one block
-// scanning the totals, then each tile scanned with its offset.
-__device__ inline uint32_t block_sum_256(uint32_t v, uint32_t* sh) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
-    return sh[0] + sh[1] + sh[2] + sh[3];
-}
-
-__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
-                                                     uint32_t* __restrict__ part) {
-    __shared__ uint32_t sh[4];
-    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * 16;
-    uint32_t s = 0;
-    if (base < n) {
-        const uint4* p = reinterpret_cast<const uint4*>(in + base);
+// Exclusive scan of each digit's row of the [digit][block] histogram table in
+// place, one block per digit (rows are scanned 1024 counts at a time with the
+// carry), and the digit's total.  The digits' own offsets (an exclusive scan
+// of the totals) are added by the scatter kernel, so one launch serves a pass.
+__global__ void __launch_bounds__(256) k_scan_rows(uint32_t* __restrict__ hist, uint32_t nb,
+                                                   uint32_t* __restrict__ totals) {
+    __shared__ uint32_t sw[4];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t* __restrict__ row = hist + (size_t)blockIdx.x * nb;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += 1024) {
+        const uint32_t i0 = base + 4 * t;
+        uint32_t v[4], s = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            uint4 v = p[q];
-            s += v.x + v.y + v.z + v.w;
-        }
-    }
-    s = block_sum_256(s, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-// In-place exclusive scan of n counts by one block of 1024 lanes, in tiles of
-// SCAN_ONE_TILE counts staged through LDS (coalesced in and out); each lane
-// scans 8 consecutive counts, the lanes' totals are scanned with wave shuffles
-// and the tile's total carries into the next tile.  LDS rows are padded by one
-// word per 8 so a lane's run starts in its own bank.
-constexpr uint32_t SCAN_ONE_TILE = 8192;
-constexpr uint32_t SCAN_ONE_MAX = SCAN_ONE_TILE;  // larger tables: three launches spread the work
-__global__ void __launch_bounds__(1024) k_scan_one(uint32_t* __restrict__ a, uint32_t n) {
-    __shared__ uint32_t tile[SCAN_ONE_TILE + SCAN_ONE_TILE / 8];
-    __shared__ uint32_t wsum[16];
-    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < n; base += SCAN_ONE_TILE) {
-        const uint32_t cnt = min(SCAN_ONE_TILE, n - base);
-        for (uint32_t i = t; i < SCAN_ONE_TILE; i += 1024) tile[i + i / 8] = i < cnt ? a[base + i] : 0u;
-        __syncthreads();
-        uint32_t v[8], s = 0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            v[q] = tile[t * 9 + q];
+            v[q] = i0 + q < nb ? row[i0 + q] : 0u;
             s += v[q];
         }
         uint32_t inc = s;
@@ -219,72 +206,20 @@ __global__ void __launch_bounds__(1024) k_scan_one(uint32_t* __restrict__ a, uin
             const uint32_t u = __shfl_up(inc, o, 64);
             if (lane >= (uint32_t)o) inc += u;
         }
-        if (lane == 63) wsum[w] = inc;
+        if (lane == 63) sw[w] = inc;
         __syncthreads();
-        uint32_t before = carry, total = 0;
-        for (uint32_t k = 0; k < 16; ++k) {
-            const uint32_t x = wsum[k];
-            before += k < w ? x : 0u;
-            total += x;
-        }
-        uint32_t run = before + inc - s;
+        const uint32_t before = (w > 0 ? sw[0] : 0u) + (w > 1 ? sw[1] : 0u) + (w > 2 ? sw[2] : 0u);
+        const uint32_t total = sw[0] + sw[1] + sw[2] + sw[3];
+        uint32_t run = carry + before + inc - s;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            tile[t * 9 + q] = run;
+        for (int q = 0; q < 4; ++q) {
+            if (i0 + q < nb) row[i0 + q] = run;
             run += v[q];
         }
-        __syncthreads();
-        for (uint32_t i = t; i < cnt; i += 1024) a[base + i] = tile[i + i / 8];
         carry += total;
-        __syncthreads();  // tile and wsum are rewritten by the next tile
+        __syncthreads();  // sw is rewritten by the next 1024 counts
     }
-}
-
-__global__ void __launch_bounds__(256) k_scan_tiles(const uint32_t* __restrict__ in, uint32_t n,
-                                                    const uint32_t* __restrict__ part,
-                                                    uint32_t* __restrict__ out) {
-    __shared__ uint32_t sh[4];
-    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * 16;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t v[16];
-    if (base < n) {
-        const uint4* p = reinterpret_cast<const uint4*>(in + base);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            uint4 w = p[q];
-            v[4 * q] = w.x; v[4 * q + 1] = w.y; v[4 * q + 2] = w.z; v[4 * q + 3] = w.w;
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = 0u;
-    }
-    uint32_t tot = 0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) tot += v[q];
-    // inclusive scan of the lane totals inside the wave, then across the 4 waves
-    uint32_t inc = tot;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t u = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += u;
-    }
-    if (lane == 63) sh[wave] = inc;
-    __syncthreads();
-    uint32_t before = part[blockIdx.x];
-    for (int w = 0; w < wave; ++w) before += sh[w];
-    uint32_t run = before + inc - tot;
-    if (base < n) {
-        uint4* p = reinterpret_cast<uint4*>(out + base);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            uint4 w;
-            w.x = run; run += v[4 * q];
-            w.y = run; run += v[4 * q + 1];
-            w.z = run; run += v[4 * q + 2];
-            w.w = run; run += v[4 * q + 3];
-            p[q] = w;
-        }
-    }
+    if (t == 0) totals[blockIdx.x] = carry;
 }
 
 // ---- 2. finest cells: sums + slots ----------------------------------------------
@@ -984,13 +919,12 @@ int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range) {
     while (bits < 32 && (1ull << bits) < key_range) ++bits;
     rs.passes = (bits + RB_MAXW - 1) / RB_MAXW;
     rs.width = (bits + rs.passes - 1) / rs.passes;
-    if (rs.width < 4) rs.width = 4;  // 16+ digits: the histogram table is a multiple of 16
     rs.n = n;
     rs.nb = (uint32_t)(((size_t)(n ? n : 1) + RB_TILE - 1) / RB_TILE);
     rs.n_hist = (1u << rs.width) * rs.nb;
     rs.hist = (uint32_t*)dalloc(c, (size_t)rs.n_hist * 4, &rc);
     if (rc) return rc;
-    rs.part = (uint32_t*)dalloc(c, (size_t)((rs.n_hist + SCAN_TILE - 1) / SCAN_TILE) * 4, &rc);
+    rs.tot = (uint32_t*)dalloc(c, (size_t)(1u << rs.width) * 4, &rc);
     return rc;
 }
 
@@ -1026,19 +960,6 @@ int mb_alloc(Ctx* c) {
     return rc;
 }
 
-// In-place exclusive scan of n (a multiple of 16) counts; `part` holds one
-// word per SCAN_TILE counts.
-static void scan_counts(Ctx* c, uint32_t* a, uint32_t n, uint32_t* part) {
-    if (n <= SCAN_ONE_MAX) {  // one block: one launch instead of three
-        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, c->stream, a, n);
-        return;
-    }
-    const uint32_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(256), 0, c->stream, a, n, part);
-    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, c->stream, part, tiles);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(256), 0, c->stream, a, n, part, a);
-}
-
 // Sorts (k[0], v[0]) by key into k[passes & 1], v[passes & 1]; every pass is
 // stable.  hist0_ready: the first digit's block histogram is already in place.
 void radix_run(Ctx* c, const RadixSort& rs, uint32_t* const (&k)[2], uint32_t* const (&v)[2],
@@ -1048,9 +969,9 @@ void radix_run(Ctx* c, const RadixSort& rs, uint32_t* const (&k)[2], uint32_t* c
         if (p || !hist0_ready)
             hipLaunchKernelGGL(k_radix_up, dim3(rs.nb), dim3(256), 0, c->stream, k[p & 1], rs.n, p * W, W,
                                rs.nb, rs.hist);
-        scan_counts(c, rs.hist, rs.n_hist, rs.part);
+        hipLaunchKernelGGL(k_scan_rows, dim3(1u << W), dim3(256), 0, c->stream, rs.hist, rs.nb, rs.tot);
         hipLaunchKernelGGL(k_radix_down, dim3(rs.nb), dim3(256), 0, c->stream, k[p & 1], v[p & 1], rs.n,
-                           p * W, W, rs.nb, rs.hist, k[(p + 1) & 1], v[(p + 1) & 1]);
+                           p * W, W, rs.nb, rs.hist, rs.tot, k[(p + 1) & 1], v[(p + 1) & 1]);
     }
 }
 
