@@ -24,7 +24,7 @@ constexpr uint8_t F_PINNED = 2;   // setPinnedPoints (Points/index.ts:1264-1304)
 
 constexpr int MAX_LEVELS = 8;     // 4^2 .. 512^2 (ForceManyBody/index.ts:19, 646-656)
 constexpr int SLOTS = 8;          // NEAR_FIELD_SLOTS (ForceManyBody/index.ts:35)
-constexpr uint32_t BIG_CELL = 64; // finest cells above this many points use a wave each
+constexpr uint32_t BIG_CELL = 16; // finest cells above this many points use a wave each
 constexpr uint32_t CELL_BLOCK = 256;  // each graph's finest cells start on a multiple of this
 
 // Link degree tiers: points with at most LANE_MAX entries run one lane each;

@@ -5,7 +5,8 @@
 //      the previous tick's cell order; cell bounds from the sorted keys
 //   2. per finest cell: fixed-point sums of (x, y), count, and the top-8
 //      near-field slots by (ticket, index) — one lane per cell, one wave per
-//      cell above 64 points, chunked blocks plus a merge above 2048 points
+//      cell above BIG_CELL points (beside the crowded tier, on the side
+//      stream), chunked blocks plus a merge above BIG_CHUNK points
 //   3. 2x2 merges up the pyramid in the same fixed point, f32 centroids
 //      (cx, cy, N) per level
 //   4. per point (in cell order): far field over every level + near field,
@@ -992,15 +993,24 @@ int mb_run(Ctx* c, float2* vout) {
                        c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums,
                        c->slot_idx, c->slot_pos, c->mid_cells, c->big_cells, c->big_chunks,
                        c->big_count, c->blk_graph, c->desc);
-    hipLaunchKernelGGL(k_cells_mid, dim3(MID_BLOCKS), dim3(256), 0, c->stream, c->cell_end, c->cell_begin,
-                       c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->slot_idx,
+    // the wave-per-cell tier and the crowded-cell chunks touch disjoint cells:
+    // the first runs on the side stream beside the other two
+    const bool fork = c->side != nullptr;
+    if (fork) {
+        CG_HIP(c, hipEventRecord(c->ev_fork, c->stream));
+        CG_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    }
+    hipLaunchKernelGGL(k_cells_mid, dim3(MID_BLOCKS), dim3(256), 0, fork ? c->side : c->stream, c->cell_end,
+                       c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->slot_idx,
                        c->slot_pos, c->mid_cells, c->big_count, c->blk_graph, c->desc);
+    if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
     hipLaunchKernelGGL(k_cells_chunk, dim3(1024), dim3(256), 0, c->stream, c->cell_end, c->cell_begin,
                        c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->big_chunks,
                        c->big_count, c->chunk_top, c->blk_graph, c->desc);
     hipLaunchKernelGGL(k_cells_merge, dim3(64), dim3(256), 0, c->stream, c->big_cells, c->big_count,
                        c->chunk_top, c->pos[0], c->pos[1], c->slot_idx, c->slot_pos, c->blk_graph,
                        c->desc);
+    if (fork) CG_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     prof_end(c, CG_STAGE_MB_CELLS);
     prof_begin(c, CG_STAGE_MB_PYRAMID);
     // pyramid: finest -> tiles of up to 32x32 -> remaining top levels
