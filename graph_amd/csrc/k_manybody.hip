@@ -863,13 +863,13 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
     else far_level0<false>(s_tile, p, d, fx, fy);
     int qx = 0, qy = 0, x0 = 0, y0 = 0, w = 0;
     bool staged = false;
-    for (int lvl = 1; lvl < L; ++lvl) {
-        const int gl = 4 << lvl;
-        const float4* __restrict__ lv = cent + d.c0 + level_off(lvl);
-        qx = cell_of(p.x, d.h[lvl], gl);
-        qy = cell_of(p.y, d.h[lvl], gl);
-        // union of the block's 6x6 windows
-        int bx0 = (qx >> 1) * 2 - 2, by0 = (qy >> 1) * 2 - 2, bx1 = bx0 + 5, by1 = by0 + 5;
+    // The block's box of finest cells, once: a coarser level's cell is the
+    // finest cell >> k within one cell of rounding, so every level's window is
+    // the shifted box padded by one cell, widened to the aligned 6x6 blocks.
+    {
+        const int gfine = 4 << (L - 1);
+        int bx0 = cell_of(p.x, d.h[L - 1], gfine), by0 = cell_of(p.y, d.h[L - 1], gfine);
+        int bx1 = bx0, by1 = by0;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             bx0 = min(bx0, __shfl_xor(bx0, o, 64));
@@ -877,23 +877,38 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
             bx1 = max(bx1, __shfl_xor(bx1, o, 64));
             by1 = max(by1, __shfl_xor(by1, o, 64));
         }
-        __syncthreads();  // the previous level's tile is no longer read
         if (lane == 0) {
             s_box[wave][0] = bx0;
             s_box[wave][1] = by0;
             s_box[wave][2] = bx1;
             s_box[wave][3] = by1;
         }
-        __syncthreads();
-        x0 = max(0, min(min(s_box[0][0], s_box[1][0]), min(s_box[2][0], s_box[3][0])));
-        y0 = max(0, min(min(s_box[0][1], s_box[1][1]), min(s_box[2][1], s_box[3][1])));
-        const int x1 = min(gl - 1, max(max(s_box[0][2], s_box[1][2]), max(s_box[2][2], s_box[3][2])));
-        const int y1 = min(gl - 1, max(max(s_box[0][3], s_box[1][3]), max(s_box[2][3], s_box[3][3])));
+    }
+    __syncthreads();
+    const int fx0 = min(min(s_box[0][0], s_box[1][0]), min(s_box[2][0], s_box[3][0]));
+    const int fy0 = min(min(s_box[0][1], s_box[1][1]), min(s_box[2][1], s_box[3][1]));
+    const int fx1 = max(max(s_box[0][2], s_box[1][2]), max(s_box[2][2], s_box[3][2]));
+    const int fy1 = max(max(s_box[0][3], s_box[1][3]), max(s_box[2][3], s_box[3][3]));
+    for (int lvl = 1; lvl < L; ++lvl) {
+        const int gl = 4 << lvl, k = L - 1 - lvl;
+        const float4* __restrict__ lv = cent + d.c0 + level_off(lvl);
+        qx = cell_of(p.x, d.h[lvl], gl);
+        qy = cell_of(p.y, d.h[lvl], gl);
+        x0 = max(0, (((fx0 >> k) - 1) >> 1) * 2 - 2);
+        y0 = max(0, (((fy0 >> k) - 1) >> 1) * 2 - 2);
+        const int x1 = min(gl - 1, (((fx1 >> k) + 1) >> 1) * 2 + 3);
+        const int y1 = min(gl - 1, (((fy1 >> k) + 1) >> 1) * 2 + 3);
         w = x1 - x0 + 1;
         const int area = w * (y1 - y0 + 1);
         staged = area <= MB_TILE;
+        __syncthreads();  // the previous level's tile is no longer read
         if (staged) {
-            for (int c = t; c < area; c += 256) s_tile[c] = lv[(y0 + c / w) * gl + x0 + c % w];
+            // row = floor((c + 1/2) / w) through a float reciprocal: exact for c, w < 2^10
+            const float iw = __builtin_amdgcn_rcpf((float)w);
+            for (int c = t; c < area; c += 256) {
+                const int row = (int)(((float)c + 0.5f) * iw);
+                s_tile[c] = lv[(y0 + row) * gl + x0 + (c - row * w)];
+            }
             __syncthreads();
             if (fast) far_level<true>(s_tile, x0, y0, w, gl, qx, qy, p, fx, fy);
             else far_level<false>(s_tile, x0, y0, w, gl, qx, qy, p, fx, fy);
