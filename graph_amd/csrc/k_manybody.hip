@@ -736,7 +736,9 @@ __device__ inline float2 near_field(const float4* fin, int x0, int y0, int w, co
             const float qs[2 * SLOTS] = {q01.x, q01.y, q01.z, q01.w, q23.x, q23.y, q23.z, q23.w,
                                          q45.x, q45.y, q45.z, q45.w, q67.x, q67.y, q67.z, q67.w};
             float sxp = 0.f, syp = 0.f, sampled = 0.f;
-            bool open = true;
+            float dxs[SLOTS], dys[SLOTS], ls[SLOTS];
+            bool uses[SLOTS];
+            bool open = true, small = false;
 #pragma unroll
             for (int s = 0; s < SLOTS; ++s) {
                 const int j = js[s];
@@ -750,9 +752,29 @@ __device__ inline float2 near_field(const float4* fin, int x0, int y0, int w, co
                 dx = same ? r.x : dx;
                 dy = same ? r.y : dy;
                 l = same ? r.x * r.x + r.y * r.y : l;
-                const float wgt = (use && l > 0.f) ? falloff_coef(l) : 0.f;
-                sxp += wgt * dx;
-                syp += wgt * dy;
+                dxs[s] = dx;
+                dys[s] = dy;
+                ls[s] = l;
+                uses[s] = use && l > 0.f;
+                small = small || (uses[s] && l < 1.f);
+            }
+            // below unit distance the falloff is l^-3/4 (a square root); when no
+            // lane of the wave has such a pair, 1/l is inv * inv, the same value
+            if (__ballot(small) != 0ull) {
+#pragma unroll
+                for (int s = 0; s < SLOTS; ++s) {
+                    const float wgt = uses[s] ? falloff_coef(ls[s]) : 0.f;
+                    sxp += wgt * dxs[s];
+                    syp += wgt * dys[s];
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < SLOTS; ++s) {
+                    const float inv = __builtin_amdgcn_rsqf(ls[s]);
+                    const float wgt = uses[s] ? inv * inv : 0.f;
+                    sxp += wgt * dxs[s];
+                    syp += wgt * dys[s];
+                }
             }
             if (sampled > 0.f) {
                 float hw = others / sampled;
