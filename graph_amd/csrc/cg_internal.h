@@ -35,7 +35,7 @@ constexpr uint32_t LANE_MAX = 32;
 constexpr uint32_t LANE_SPLIT = 8;  // lane-tier schedule classes: <= 2, <= LANE_SPLIT entries, then the rest
 constexpr uint32_t T64_MAX = 256;
 constexpr uint32_t HUB_SEG = 1024;   // entries per segment of a split 256-lane point
-constexpr int HUB_ROUNDS = 3;        // segment/fix-up rounds per pass for split points
+constexpr int HUB_ROUNDS = 5;        // segment/fix-up rounds per pass for split points
 
 // Passes of one tick, in order (src/index.ts:2012-2078).  Bit p of GDesc::passes
 // says the graph runs pass p this tick; bit p of GDesc::inbuf names the position
