@@ -282,6 +282,7 @@ __global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __r
 
 constexpr int MAX_SWEEPS = 8;
 constexpr float NOISE_FLOOR = 16.f;  // stalls below this many tolerances end the iteration
+constexpr float FIX_ACCEPT = 4.f;    // split points: largest accepted start correction, in tolerances
 // One serial spring step from an entry's terms (no change when !live),
 // also applying the step's Jacobian to J (column-major 2x2, J <- M J):
 //   u' = u + x t(u),  x = q - u,  t = K (1 - D / max(|x|, 0.99 D))
@@ -521,9 +522,9 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
 // composed from the exact start 0, give each segment's true start,
 //   true_{s+1} = E_s + A_s (true_s - G_s),
 // stored as the segment's next start guess; when every guess was already
-// within the tolerance, the corrections stopped halving within NOISE_FLOOR
-// tolerances (the f32 noise floor), or on the final round, the point's
-// velocity (the last segment's end from its true start) is written.  The composition is a wave
+// within FIX_ACCEPT tolerances, the corrections stopped halving within
+// NOISE_FLOOR tolerances (the f32 noise floor), or on the final round, the
+// point's velocity (the last segment's end from its true start) is written.  The composition is a wave
 // scan of affine maps over 64 segments at a time.  hub_done holds [n_hubs]
 // flags, then [n_hubs] previous largest corrections.
 __global__ void __launch_bounds__(64) k_hub_fix(int pass, const uint2* __restrict__ hubs, uint32_t n_hubs,
@@ -572,10 +573,13 @@ __global__ void __launch_bounds__(64) k_hub_fix(int pass, const uint2* __restric
     if (lane != 0) return;
     const float2 tru = x0;  // the last segment's end from its true start
     float* __restrict__ dprev = reinterpret_cast<float*>(hub_done + n_hubs);
-    // a stall counts as the noise floor only close to the tolerance: a
-    // correction that stops halving far above it is a slow Newton step (a
-    // spring crossing its clamp) and takes another round
-    const bool ok = dmax <= d.tol || (round > 0 && dmax > 0.5f * dprev[h] && dmax <= NOISE_FLOOR * d.tol);
+    // The output applies the last correction through the segment Jacobians, so
+    // its error is second order in dmax: corrections up to FIX_ACCEPT
+    // tolerances are accepted.  A stall counts as the noise floor only close
+    // to the tolerance: a correction that stops halving far above it is a slow
+    // Newton step (a spring crossing its clamp) and takes another round.
+    const bool ok = dmax <= FIX_ACCEPT * d.tol ||
+                    (round > 0 && dmax > 0.5f * dprev[h] && dmax <= NOISE_FLOOR * d.tol);
     dprev[h] = dmax;
     if (!ok && !final_round) return;
     hub_done[h] = 1u;

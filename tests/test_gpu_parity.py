@@ -178,6 +178,29 @@ def test_link_forces_vs_serial_oracle(case):
             check(got, want, S, f"link {case} dir {d} alpha {alpha}", idx=absent == 0)
 
 
+def test_link_forces_after_ticks_vs_oracle():
+    """Link forces from a live state, 40 ticks into a layout (warm starts in
+    play; the points above 1024 entries per direction, up to 41,950, are split
+    into segments and fixed up), against the serial oracle for every point."""
+    S, n = 8192.0, 200_000
+    links = datagen.chung_lu(n, 2.1, 1.0, 1_000_000, 7)
+    xy = datagen.uniform_disk(n, 4096, 4096, 2048, 3)
+    m = links.size // 2
+    r_in, r_out = datagen.uniform(m, 0, 1, 11), datagen.uniform(m, 0, 1, 12)
+    c = ctx_for(xy, links=links, r_in=r_in, r_out=r_out, jitter=datagen.jitter(n, 4))
+    p = params(S, link_spring=1.0, link_distance=10.0, range=(1.0, 1.2), gravity=0.25, repulsion=0.5)
+    c.set_alpha(1.0)
+    c.step(p, 40)
+    live = c.get_positions()
+    a = float(np.float32(c.get_alpha()))
+    L = O.Links(n, links, r_in, r_out)
+    absent = O.absent_mask(live)
+    for d in (0, 1):
+        got = c.eval_force(p, _lib.FORCE_LINK_IN + d, a)
+        want = L.force(live, d, a, 1.0, 10.0, 1.0, 1.2, absent)
+        check(got, want, S, f"link dir {d} after 40 ticks", idx=absent == 0)
+
+
 def test_collision_vs_oracle():
     S, n = 4096.0, 20_000
     rng = np.random.default_rng(4)
