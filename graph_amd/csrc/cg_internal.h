@@ -142,8 +142,8 @@ struct LinkDir {
     uint4* unit_b = nullptr;     // {schedule slot, warm offset, split point or NO_HUB, segment}
     uint2* hubs = nullptr;       // split points: {first unit, segments}
     uint32_t n_hubs = 0;
-    float4* seg_out = nullptr;   // [2 n_units] segment end, start used, end Jacobian
-    uint32_t* hub_done = nullptr;  // [2 n_hubs] fix-up converged this pass; last correction (f32)
+    float4* seg_out = nullptr;   // [2][2 n_units] by round parity: segment end, start used; end Jacobian
+    uint32_t* hub_done = nullptr;  // [3 n_hubs] round that found it holding (0: open); largest corrections by parity (f32)
     float2* warm = nullptr;      // last converged chunk starts, per graph regions
     float4* wjac = nullptr;      // split segments: each lane's prefix Jacobian at that start
                                  // (lane 0: the segment start it converged from)

@@ -375,6 +375,118 @@ constexpr uint32_t NO_HUB = 0xffffffffu;
 // {end, start used} and its end's Jacobian in its start for the fix-up.
 // Round r > 0 reruns only the segments after the first of split hubs whose
 // fix-up has not converged.
+// Composition of split point hb's segments (hb = {first unit, segments}) by
+// one full wave: each segment's linearised end map x -> E_s + A_s (x - G_s)
+// (seg_out: {E, G used}, {A}), composed from the exact start 0, gives the true
+// start of every segment,
+//   true_{s+1} = E_s + A_s (true_s - G_s),
+// and the point's velocity, the last segment's end from its true start.  A
+// wave scan of affine maps over 64 segments at a time.  Returns the true start
+// of segment `want`, the end, and the largest correction |true_s - G_s|;
+// `store` also writes every true start into its segment's lane-0 warm slot.
+struct Composed {
+    float2 want_start, end;
+    float dmax;
+};
+
+__device__ inline Composed hub_compose(uint2 hb, uint32_t lane, uint32_t want,
+                                       const float4* __restrict__ seg_out, const uint4* __restrict__ unit_b,
+                                       float2* __restrict__ warm, bool store) {
+    float2 x0 = make_float2(0.f, 0.f);  // true start of the chunk's first segment
+    float dmax = 0.f;
+    float2 wanted = make_float2(0.f, 0.f);
+    for (uint32_t s0 = 0; s0 < hb.y; s0 += 64) {
+        const uint32_t s = s0 + lane;
+        const bool have = s < hb.y;
+        const size_t u = (size_t)hb.x + (have ? s : s0);
+        const float4 o0 = seg_out[2 * u], A = seg_out[2 * u + 1];
+        Aff m = have ? Aff{A.x, A.y, A.z, A.w, o0.x - (A.x * o0.z + A.z * o0.w), o0.y - (A.y * o0.z + A.w * o0.w)}
+                     : aff_identity();
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const Aff q = aff_shfl_up(m, o);
+            if ((int)lane >= o) m = aff_then(q, m);
+        }
+        Aff ex = aff_shfl_up(m, 1);
+        if (lane == 0) ex = aff_identity();
+        const float2 tru = make_float2(ex.a0 * x0.x + ex.a2 * x0.y + ex.bx, ex.a1 * x0.x + ex.a3 * x0.y + ex.by);
+        if (have && s > 0) {
+            dmax = fmaxf(dmax, fmaxf(fabsf(tru.x - o0.z), fabsf(tru.y - o0.w)));
+            if (store) warm[unit_b[u].y] = tru;
+        }
+        if (want >= s0 && want < s0 + 64) {
+            wanted.x = __shfl(tru.x, (int)(want - s0), 64);
+            wanted.y = __shfl(tru.y, (int)(want - s0), 64);
+        }
+        const Aff all{__shfl(m.a0, 63, 64), __shfl(m.a1, 63, 64), __shfl(m.a2, 63, 64),
+                      __shfl(m.a3, 63, 64), __shfl(m.bx, 63, 64), __shfl(m.by, 63, 64)};
+        x0 = make_float2(all.a0 * x0.x + all.a2 * x0.y + all.bx, all.a1 * x0.x + all.a3 * x0.y + all.by);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, o, 64));
+    return Composed{wanted, x0, dmax};
+}
+
+// Whether split point h's segments hold after the round just evaluated: every
+// start correction within FIX_ACCEPT tolerances (the output applies the last
+// correction through the segment Jacobians, so its error is second order), or
+// a stall within NOISE_FLOOR tolerances (the f32 noise floor; a stall far
+// above is a slow Newton step, a spring crossing its clamp, and takes another
+// round).  hub_done holds [n_hubs] marks (the round that found the point
+// holding, 0 while open), then the largest correction of the evaluation of
+// round e at [e & 1][h] ([2][n_hubs] f32).
+__device__ inline bool hub_holds(float dmax, int evaluated, uint32_t h, uint32_t n_hubs,
+                                 const uint32_t* __restrict__ hub_done, float tol) {
+    const float* __restrict__ dp = reinterpret_cast<const float*>(hub_done + n_hubs);
+    const float prev = evaluated > 0 ? dp[((evaluated - 1) & 1) * n_hubs + h] : INFINITY;
+    return dmax <= FIX_ACCEPT * tol || (evaluated > 0 && dmax > 0.5f * prev && dmax <= NOISE_FLOOR * tol);
+}
+
+// The split point's velocity, integrated (or into vout), and the round counters.
+__device__ inline void hub_finish(uint32_t p, float2 v, const GDesc& d, int pass, float2* __restrict__ pos0,
+                                  float2* __restrict__ pos1, const uint8_t* __restrict__ flags,
+                                  float2* __restrict__ vout, uint32_t* __restrict__ stats, int rounds, bool ok) {
+    const bool bsel = (d.inbuf >> pass) & 1u;
+    if (vout) {
+        vout[p] = v;
+    } else {
+        const float2 pp = (bsel ? pos1 : pos0)[p];
+        float2* __restrict__ pout = bsel ? pos0 : pos1;
+        pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, v, d.friction, d.S);
+    }
+    if (stats) {
+        // rounds used, split points, max rounds, points stopped at the round cap
+        atomicAdd(&stats[8], (uint32_t)rounds);
+        atomicAdd(&stats[9], 1u);
+        atomicMax(&stats[10], (uint32_t)rounds);
+        if (!ok) atomicAdd(&stats[11], 1u);
+    }
+}
+
+// Final fix-up after the last round, one wave per split point still open:
+// composes its segments, stores their true starts as the next call's guesses
+// and writes the point's velocity.
+__global__ void __launch_bounds__(64) k_hub_final(int pass, const uint2* __restrict__ hubs, uint32_t n_hubs,
+                                                  const uint4* __restrict__ unit_a,
+                                                  const uint4* __restrict__ unit_b,
+                                                  const uint32_t* __restrict__ sched,
+                                                  const GDesc* __restrict__ desc, float2* __restrict__ pos0,
+                                                  float2* __restrict__ pos1, const uint8_t* __restrict__ flags,
+                                                  const float4* __restrict__ seg_out, float2* __restrict__ warm,
+                                                  const uint32_t* __restrict__ hub_done,
+                                                  uint32_t* __restrict__ stats, float2* __restrict__ vout) {
+    const uint32_t h = blockIdx.x, lane = threadIdx.x;
+    if (h >= n_hubs) return;
+    const uint2 hb = hubs[h];
+    const GDesc& d = desc[unit_a[hb.x].w];
+    if (!(d.passes & (1u << pass))) return;
+    if (hub_done[h]) return;
+    const Composed f = hub_compose(hb, lane, 0u, seg_out, unit_b, warm, true);
+    if (lane != 0) return;
+    const bool ok = hub_holds(f.dmax, HUB_ROUNDS - 1, h, n_hubs, hub_done, d.tol);
+    hub_finish(sched[unit_b[hb.x].x], f.end, d, pass, pos0, pos1, flags, vout, stats, HUB_ROUNDS, ok);
+}
+
 // Chunked tiers: one workgroup of B lanes per work unit of at most 4 B
 // entries.  unit_a = {first entry, len, -, graph}; unit_b = {slot, warm offset,
 // split point or NO_HUB, segment}.  Each lane gathers its chunk's terms
@@ -400,13 +512,15 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
                                                   const uint4* __restrict__ ent,
                                                   float2* __restrict__ warm,
                                                   float4* __restrict__ wjac,
-                                                  float4* __restrict__ seg_out,
+                                                  float4* __restrict__ seg_out, uint32_t n_units,
+                                                  const uint2* __restrict__ hubs, uint32_t n_hubs,
                                                   uint32_t* __restrict__ hub_done,
                                                   uint32_t* __restrict__ stats,
                                                   float2* __restrict__ vout) {
     __shared__ Aff s_wave[B / 64], s_wave_inc[B / 64];
     __shared__ float s_delta[B / 64];
     __shared__ float2 s_final;
+    __shared__ float4 s_fix;
     const int t = threadIdx.x;
     // units ascend by chain length: the longest start first
     const uint32_t u = u_first + (gridDim.x - 1 - blockIdx.x);
@@ -414,9 +528,37 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
     const GDesc& d = desc[U.w];
     if (!(d.passes & (1u << pass))) return;  // uniform over the block
     const bool split = V.z != NO_HUB;
-    if (round > 0 && (!split || V.w == 0 || hub_done[V.z])) return;
-    if (split && V.w == 0 && t == 0) hub_done[V.z] = 0u;  // read by the fix-up after this launch
+    // a point that held in an earlier round is done (the round that finds it
+    // holding marks it with its number, and every block of that round still
+    // stores its segment's start for the next call)
+    if (round > 0 && (!split || V.w == 0 || (hub_done[V.z] != 0u && hub_done[V.z] != (uint32_t)round))) return;
+    if (split && V.w == 0 && t == 0) hub_done[V.z] = 0u;  // read by the rounds after this launch
     const uint32_t p = sched[V.x];
+    if (round > 0) {
+        // the fix-up of the previous round, by every open segment block for
+        // itself (the same composition in each): its own true start, and
+        // whether the point already holds; segment 1's block then writes it
+        if (t < 64) {
+            // segment results by round parity: this round reads the last round's
+            const float4* __restrict__ seg_rd = seg_out + (size_t)((round - 1) & 1) * 2 * n_units;
+            const Composed f = hub_compose(hubs[V.z], (uint32_t)t, V.w, seg_rd, unit_b, warm, false);
+            const bool ok = hub_holds(f.dmax, round - 1, V.z, n_hubs, hub_done, d.tol);
+            if (t == 0) {
+                s_fix = make_float4(f.want_start.x, f.want_start.y, ok ? 1.f : 0.f, 0.f);
+                if (V.w == 1) {
+                    reinterpret_cast<float*>(hub_done + n_hubs)[((round - 1) & 1) * n_hubs + V.z] = f.dmax;
+                    if (ok) {
+                        hub_done[V.z] = (uint32_t)round;
+                        hub_finish(p, f.end, d, pass, pos0, pos1, flags, vout, stats, round, true);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const float4 fx = s_fix;
+        if (t == 0) warm[V.y] = make_float2(fx.x, fx.y);  // the segment's start: this round's, or the next call's
+        if (fx.z != 0.f) return;
+    }
     const bool bsel = (d.inbuf >> pass) & 1u;
     const float2* __restrict__ pin = bsel ? pos1 : pos0;
     float2* __restrict__ wslot = warm + V.y + t;
@@ -441,7 +583,8 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
         }
     }
     float2 g = *wslot;
-    const float2 G = warm[V.y];  // lane 0's slot: the unit's start
+    // the unit's start: lane 0's slot (for a later round, the fix-up just made)
+    const float2 G = round > 0 ? make_float2(s_fix.x, s_fix.y) : warm[V.y];
     if (split && t > 0) {
         // the lane's start converged from the previous start G0 (kept in lane
         // 0's slot of wjac), moved to the current G along its prefix Jacobian
@@ -506,98 +649,19 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
     if (t == 0) {
         const float2 vf = s_final;
         if (split) {
-            seg_out[2 * (size_t)u] = make_float4(vf.x, vf.y, G.x, G.y);
-            seg_out[2 * (size_t)u + 1] = make_float4(tot.a0, tot.a1, tot.a2, tot.a3);
+            // this round's buffer; a first segment (run in round 0 only) fills both
+            for (int b = 0; b < 2; ++b) {
+                if (b != (round & 1) && !(round == 0 && V.w == 0)) continue;
+                float4* __restrict__ so = seg_out + (size_t)b * 2 * n_units;
+                so[2 * (size_t)u] = make_float4(vf.x, vf.y, G.x, G.y);
+                so[2 * (size_t)u + 1] = make_float4(tot.a0, tot.a1, tot.a2, tot.a3);
+            }
         } else if (vout) {
             vout[p] = vf;
         } else {
             float2* __restrict__ pout = bsel ? pos0 : pos1;
             pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, vf, d.friction, d.S);
         }
-    }
-}
-
-// Fix-up of the split points, one wave per point (hubs[h] = {first unit,
-// segments}): the segments' linearised end maps x -> E_s + A_s (x - G_s),
-// composed from the exact start 0, give each segment's true start,
-//   true_{s+1} = E_s + A_s (true_s - G_s),
-// stored as the segment's next start guess; when every guess was already
-// within FIX_ACCEPT tolerances, the corrections stopped halving within
-// NOISE_FLOOR tolerances (the f32 noise floor), or on the final round, the
-// point's velocity (the last segment's end from its true start) is written.  The composition is a wave
-// scan of affine maps over 64 segments at a time.  hub_done holds [n_hubs]
-// flags, then [n_hubs] previous largest corrections.
-__global__ void __launch_bounds__(64) k_hub_fix(int pass, const uint2* __restrict__ hubs, uint32_t n_hubs,
-                                                const uint4* __restrict__ unit_a,
-                                                const uint4* __restrict__ unit_b, int round, int final_round,
-                                                const uint32_t* __restrict__ sched,
-                                                const GDesc* __restrict__ desc, float2* __restrict__ pos0,
-                                                float2* __restrict__ pos1, const uint8_t* __restrict__ flags,
-                                                const float4* __restrict__ seg_out, float2* __restrict__ warm,
-                                                uint32_t* __restrict__ hub_done, uint32_t* __restrict__ stats,
-                                                float2* __restrict__ vout) {
-    const uint32_t h = blockIdx.x, lane = threadIdx.x;
-    if (h >= n_hubs) return;
-    const uint2 hb = hubs[h];
-    const GDesc& d = desc[unit_a[hb.x].w];
-    if (!(d.passes & (1u << pass))) return;
-    if (hub_done[h]) return;
-    float2 x0 = make_float2(0.f, 0.f);  // true start of the chunk's first segment
-    float dmax = 0.f;
-    for (uint32_t s0 = 0; s0 < hb.y; s0 += 64) {
-        const uint32_t s = s0 + lane;
-        const bool have = s < hb.y;
-        const size_t u = (size_t)hb.x + (have ? s : s0);
-        const float4 o0 = seg_out[2 * u], A = seg_out[2 * u + 1];
-        const uint32_t wo = unit_b[u].y;
-        Aff m = have ? Aff{A.x, A.y, A.z, A.w, o0.x - (A.x * o0.z + A.z * o0.w), o0.y - (A.y * o0.z + A.w * o0.w)}
-                     : aff_identity();
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const Aff q = aff_shfl_up(m, o);
-            if ((int)lane >= o) m = aff_then(q, m);
-        }
-        Aff ex = aff_shfl_up(m, 1);
-        if (lane == 0) ex = aff_identity();
-        const float2 tru = make_float2(ex.a0 * x0.x + ex.a2 * x0.y + ex.bx, ex.a1 * x0.x + ex.a3 * x0.y + ex.by);
-        if (have && s > 0) {
-            dmax = fmaxf(dmax, fmaxf(fabsf(tru.x - o0.z), fabsf(tru.y - o0.w)));
-            warm[wo] = tru;  // lane 0 of the segment: its next start
-        }
-        const Aff all{__shfl(m.a0, 63, 64), __shfl(m.a1, 63, 64), __shfl(m.a2, 63, 64),
-                      __shfl(m.a3, 63, 64), __shfl(m.bx, 63, 64), __shfl(m.by, 63, 64)};
-        x0 = make_float2(all.a0 * x0.x + all.a2 * x0.y + all.bx, all.a1 * x0.x + all.a3 * x0.y + all.by);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, o, 64));
-    if (lane != 0) return;
-    const float2 tru = x0;  // the last segment's end from its true start
-    float* __restrict__ dprev = reinterpret_cast<float*>(hub_done + n_hubs);
-    // The output applies the last correction through the segment Jacobians, so
-    // its error is second order in dmax: corrections up to FIX_ACCEPT
-    // tolerances are accepted.  A stall counts as the noise floor only close
-    // to the tolerance: a correction that stops halving far above it is a slow
-    // Newton step (a spring crossing its clamp) and takes another round.
-    const bool ok = dmax <= FIX_ACCEPT * d.tol ||
-                    (round > 0 && dmax > 0.5f * dprev[h] && dmax <= NOISE_FLOOR * d.tol);
-    dprev[h] = dmax;
-    if (!ok && !final_round) return;
-    hub_done[h] = 1u;
-    const uint32_t p = sched[unit_b[hb.x].x];
-    const bool bsel = (d.inbuf >> pass) & 1u;
-    if (vout) {
-        vout[p] = tru;
-    } else {
-        const float2 pp = (bsel ? pos1 : pos0)[p];
-        float2* __restrict__ pout = bsel ? pos0 : pos1;
-        pout[p] = (flags[p] & F_PINNED) ? pp : integrate(pp, tru, d.friction, d.S);
-    }
-    if (stats) {
-        // rounds used, split points, max rounds, points stopped at the round cap
-        atomicAdd(&stats[8], (uint32_t)(round + 1));
-        atomicAdd(&stats[9], 1u);
-        atomicMax(&stats[10], (uint32_t)(round + 1));
-        if (!ok) atomicAdd(&stats[11], 1u);
     }
 }
 
@@ -892,9 +956,9 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         const size_t nu = ua.size() ? ua.size() : 1;
         L.unit_a = (uint4*)dalloc(c, nu * 16, &rc);
         L.unit_b = (uint4*)dalloc(c, nu * 16, &rc);
-        L.seg_out = (float4*)dalloc(c, nu * 32, &rc);
+        L.seg_out = (float4*)dalloc(c, nu * 64, &rc);  // two buffers by round parity
         L.hubs = (uint2*)dalloc(c, (hh.size() ? hh.size() : 1) * 8, &rc);
-        L.hub_done = (uint32_t*)dalloc(c, (hh.size() ? hh.size() : 1) * 8, &rc);
+        L.hub_done = (uint32_t*)dalloc(c, (hh.size() ? hh.size() : 1) * 12, &rc);
         L.warm = (float2*)dalloc(c, (wpos ? wpos : 1) * 8, &rc);
         L.wjac = (float4*)dalloc(c, (wpos ? wpos : 1) * 16, &rc);
         if (rc) { cleanup(); return rc; }
@@ -904,10 +968,10 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         }
         if (!hh.empty())
             CG_HIP(c, hipMemcpyAsync(L.hubs, hh.data(), hh.size() * 8, hipMemcpyHostToDevice, c->stream));
-        CG_HIP(c, hipMemsetAsync(L.hub_done, 0, (hh.size() ? hh.size() : 1) * 8, c->stream));
+        CG_HIP(c, hipMemsetAsync(L.hub_done, 0, (hh.size() ? hh.size() : 1) * 12, c->stream));
         CG_HIP(c, hipMemsetAsync(L.warm, 0, (wpos ? wpos : 1) * 8, c->stream));
         CG_HIP(c, hipMemsetAsync(L.wjac, 0, (wpos ? wpos : 1) * 16, c->stream));
-        CG_HIP(c, hipMemsetAsync(L.seg_out, 0, nu * 32, c->stream));
+        CG_HIP(c, hipMemsetAsync(L.seg_out, 0, nu * 64, c->stream));
         for (uint32_t s = 0; s < G; ++s) {
             Graph& g = c->graphs[c->order[s]];
             if (old[d].warm && g.wn[d] && g.wn[d] == region[s].second) {
@@ -936,18 +1000,18 @@ int link_run(Ctx* c, int dir, float2* vout) {
     if (!c->any_links) return CG_OK;
     const LinkDir& L = c->dir[dir];
     const int pass = P_LINK_IN + dir;
-    // The 256-lane tier's first round runs alone on the main stream; then the
-    // split points' fix-ups and later rounds (a chain of short launches) run
-    // on the side stream while the 64-lane and lane tiers fill the GPU on the
-    // main stream, which joins the side stream at the end.  Rounds after
-    // convergence return at once.
+    // The 256-lane tier's first round runs alone on the main stream; the split
+    // points' later rounds (each opening with the fix-up of the one before)
+    // and the final fix-up run on the side stream while the 64-lane and lane
+    // tiers fill the GPU on the main stream, which joins the side stream at the
+    // end.  Rounds after a point holds return at once.
     const uint32_t u256 = L.n_u64;
     // sweep counters only while requested (same-address atomics per block)
     uint32_t* stats = (c->prof_mask & CG_PROFILE_LINK_STATS) ? c->link_stats : nullptr;
     if (L.n_units > u256)
         hipLaunchKernelGGL(k_link_chunk<256>, dim3(L.n_units - u256), dim3(256), 0, c->stream, pass, L.unit_a,
                            L.unit_b, u256, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm,
-                           L.wjac, L.seg_out, L.hub_done, stats, vout);
+                           L.wjac, L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
     const bool fork = L.n_hubs && c->side;
     if (L.n_hubs) {
         hipStream_t s = c->stream;
@@ -956,22 +1020,21 @@ int link_run(Ctx* c, int dir, float2* vout) {
             CG_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
             s = c->side;
         }
-        for (int r = 0; r < HUB_ROUNDS; ++r) {
-            if (r > 0)
-                hipLaunchKernelGGL(k_link_chunk<256>, dim3(L.n_units - L.first_split_unit), dim3(256), 0, s, pass,
-                                   L.unit_a, L.unit_b, L.first_split_unit, r, L.sched, c->desc, c->pos[0],
-                                   c->pos[1], c->flags, L.ent, L.warm, L.wjac, L.seg_out, L.hub_done, stats,
-                                   vout);
-            hipLaunchKernelGGL(k_hub_fix, dim3(L.n_hubs), dim3(64), 0, s, pass, L.hubs, L.n_hubs, L.unit_a,
-                               L.unit_b, r, (int)(r == HUB_ROUNDS - 1), L.sched, c->desc, c->pos[0], c->pos[1],
-                               c->flags, L.seg_out, L.warm, L.hub_done, stats, vout);
-        }
+        for (int r = 1; r < HUB_ROUNDS; ++r)
+            hipLaunchKernelGGL(k_link_chunk<256>, dim3(L.n_units - L.first_split_unit), dim3(256), 0, s, pass,
+                               L.unit_a, L.unit_b, L.first_split_unit, r, L.sched, c->desc, c->pos[0],
+                               c->pos[1], c->flags, L.ent, L.warm, L.wjac, L.seg_out, L.n_units, L.hubs,
+                               L.n_hubs, L.hub_done, stats, vout);
+        hipLaunchKernelGGL(k_hub_final, dim3(L.n_hubs), dim3(64), 0, s, pass, L.hubs, L.n_hubs, L.unit_a,
+                           L.unit_b, L.sched, c->desc, c->pos[0], c->pos[1], c->flags,
+                           L.seg_out + (size_t)((HUB_ROUNDS - 1) & 1) * 2 * L.n_units, L.warm, L.hub_done, stats,
+                           vout);
         if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
     if (L.n_u64)
         hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_u64), dim3(64), 0, c->stream, pass, L.unit_a, L.unit_b,
                            0u, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm, L.wjac,
-                           L.seg_out, L.hub_done, stats, vout);
+                           L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
     if (L.n_lane)
         hipLaunchKernelGGL(k_link_lane, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, pass,
                            L.sched, L.soff, L.ent, L.n_lane, c->gid, c->desc, c->pos[0], c->pos[1],
