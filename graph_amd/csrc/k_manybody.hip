@@ -552,8 +552,11 @@ __global__ void __launch_bounds__(256) k_cells_merge(const uint4* __restrict__ b
 
 // ---- 3. pyramid ---------------------------------------------------------------
 
+// An empty cell's centroid sits at EMPTY_AT in both coordinates with count 0:
+// a far-field term then weighs exactly 0 with no test (0 * 1/l, l finite).
+constexpr float EMPTY_AT = -1.0e18f;
 __device__ inline float4 centroid(long long sx, long long sy, long long n, float inv) {
-    if (n <= 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+    if (n <= 0) return make_float4(EMPTY_AT, EMPTY_AT, 0.f, 0.f);
     const double dn = (double)n;
     return make_float4((float)((double)sx * (double)inv / dn), (float)((double)sy * (double)inv / dn),
                        (float)dn, 0.f);
@@ -637,14 +640,14 @@ __device__ inline float falloff_coef(float l) {
     return inv * fminf(inv, __builtin_amdgcn_sqrtf(inv));
 }
 
-// One far-field term, zero where `use` is clear (outside the ring or the grid)
-// or the cell is empty.  A ring cell lies at least one cell width from the
-// query point, so with cells of width >= 2 (FAST) l >= 1 and the falloff is 1/l.
+// One far-field term, zero where `use` is clear (outside the ring or the grid;
+// an empty cell weighs 0 through its count and far-away centroid).  A ring
+// cell lies at least one cell width from the query point (so l > 0), and with
+// cells of width >= 2 (FAST) l >= 1 and the falloff is 1/l.
 template <bool FAST>
 __device__ inline void far_term(float4 c, float2 p, bool use, float& fx, float& fy) {
     const float dx = p.x - c.x, dy = p.y - c.y;
     const float l = dx * dx + dy * dy;
-    use = use && c.z > 0.f && l > 0.f;
     const float coef = FAST ? __builtin_amdgcn_rcpf(l) : falloff_coef(l);
     const float w = use ? c.z * coef : 0.f;
     fx += w * dx;
