@@ -711,6 +711,31 @@ __device__ inline void far_level(const float4* tile, int x0, int y0, int w, int 
     }
 }
 
+// The same 27 ring cells read from a staged window padded with empty cells
+// wherever the 6x6 block leaves the grid, so no term needs a mask: the window
+// starts at (x0, y0) and is w cells wide.
+template <bool FAST>
+__device__ inline void far_level_padded(const float4* tile, int x0, int y0, int w, int qx, int qy, float2 p,
+                                        float& fx, float& fy) {
+    const int bx = (qx >> 1) * 2 - 2 - x0, by = (qy >> 1) * 2 - 2 - y0;
+    const int px = qx & 1, py = qy & 1;
+    const int hc = 4 - 3 * px;  // the hole rows' middle column
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) {  // full rows
+        const int r = k == 0 ? 0 : (k == 1 ? 4 - 3 * py : 5);
+        const float4* row = tile + (by + r) * w + bx;
+#pragma unroll
+        for (int ii = 0; ii < 6; ++ii) far_term<FAST>(row[ii], p, true, fx, fy);
+    }
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) {  // hole rows: the three columns outside the hole
+        const float4* row = tile + (by + 1 + py + k) * w + bx;
+        far_term<FAST>(row[0], p, true, fx, fy);
+        far_term<FAST>(row[hc], p, true, fx, fy);
+        far_term<FAST>(row[5], p, true, fx, fy);
+    }
+}
+
 // Near field at the finest level (force-nearfield.frag:101-163): the 3x3
 // cells' sampled slots, Horvitz-Thompson weighted.  Counts come from `fin`
 // (finest centroids, global or a staged window as in far_level).
@@ -919,10 +944,11 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
         const float4* __restrict__ lv = cent + d.c0 + level_off(lvl);
         qx = cell_of(p.x, d.h[lvl], gl);
         qy = cell_of(p.y, d.h[lvl], gl);
-        x0 = max(0, (((fx0 >> k) - 1) >> 1) * 2 - 2);
-        y0 = max(0, (((fy0 >> k) - 1) >> 1) * 2 - 2);
-        const int x1 = min(gl - 1, (((fx1 >> k) + 1) >> 1) * 2 + 3);
-        const int y1 = min(gl - 1, (((fy1 >> k) + 1) >> 1) * 2 + 3);
+        // the window is not clipped to the grid: cells outside it stage as empty
+        x0 = (((fx0 >> k) - 1) >> 1) * 2 - 2;
+        y0 = (((fy0 >> k) - 1) >> 1) * 2 - 2;
+        const int x1 = (((fx1 >> k) + 1) >> 1) * 2 + 3;
+        const int y1 = (((fy1 >> k) + 1) >> 1) * 2 + 3;
         w = x1 - x0 + 1;
         const int area = w * (y1 - y0 + 1);
         staged = area <= MB_TILE;
@@ -932,11 +958,13 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
             const float iw = __builtin_amdgcn_rcpf((float)w);
             for (int c = t; c < area; c += 256) {
                 const int row = (int)(((float)c + 0.5f) * iw);
-                s_tile[c] = lv[(y0 + row) * gl + x0 + (c - row * w)];
+                const int gx = x0 + (c - row * w), gy = y0 + row;
+                s_tile[c] = (gx >= 0 && gx < gl && gy >= 0 && gy < gl) ? lv[gy * gl + gx]
+                                                                        : make_float4(EMPTY_AT, EMPTY_AT, 0.f, 0.f);
             }
             __syncthreads();
-            if (fast) far_level<true>(s_tile, x0, y0, w, gl, qx, qy, p, fx, fy);
-            else far_level<false>(s_tile, x0, y0, w, gl, qx, qy, p, fx, fy);
+            if (fast) far_level_padded<true>(s_tile, x0, y0, w, qx, qy, p, fx, fy);
+            else far_level_padded<false>(s_tile, x0, y0, w, qx, qy, p, fx, fy);
         } else {
             if (fast) far_level<true>(lv, 0, 0, gl, gl, qx, qy, p, fx, fy);
             else far_level<false>(lv, 0, 0, gl, gl, qx, qy, p, fx, fy);
