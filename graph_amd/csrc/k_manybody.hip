@@ -767,6 +767,8 @@ __device__ inline float2 near_field(const float4* fin, int x0, int y0, int w, co
             float dxs[SLOTS], dys[SLOTS], ls[SLOTS];
             bool uses[SLOTS];
             bool open = true, small = false;
+            // coincident points share a cell: only the point's own cell can hold one
+            const bool own = di == 0 && dj == 0;  // uniform over the wave
 #pragma unroll
             for (int s = 0; s < SLOTS; ++s) {
                 const int j = js[s];
@@ -775,11 +777,13 @@ __device__ inline float2 near_field(const float4* fin, int x0, int y0, int w, co
                 sampled += use ? 1.f : 0.f;
                 float dx = p.x - qs[2 * s], dy = p.y - qs[2 * s + 1];
                 float l = dx * dx + dy * dy;
-                // coincident points push along the point's jitter (force-nearfield.frag:131-136)
-                const bool same = !(l > 0.f);
-                dx = same ? r.x : dx;
-                dy = same ? r.y : dy;
-                l = same ? r.x * r.x + r.y * r.y : l;
+                if (own) {
+                    // coincident points push along the point's jitter (force-nearfield.frag:131-136)
+                    const bool same = !(l > 0.f);
+                    dx = same ? r.x : dx;
+                    dy = same ? r.y : dy;
+                    l = same ? r.x * r.x + r.y * r.y : l;
+                }
                 dxs[s] = dx;
                 dys[s] = dy;
                 ls[s] = l;
