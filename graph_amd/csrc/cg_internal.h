@@ -201,7 +201,8 @@ struct Ctx {
     uint32_t* cell_end = nullptr;   // [F] one past its last (0 = empty)
     long long* sums = nullptr;      // [3 C] fixed-point sum x, sum y, count per level cell
     float4* cent = nullptr;         // [C]
-    float2* sorted_pos = nullptr;   // [NP]
+    float2* rp[2] = {nullptr, nullptr};  // [NP] position payload of the radix sort
+    float2* sorted_pos = nullptr;   // [NP] positions in cell order (one of rp)
     int32_t* slot_idx = nullptr;    // [8 F] global point index or -1
     float2* slot_pos = nullptr;
     uint32_t* mid_cells = nullptr;  // cells of BIG_CELL+1 .. BIG_CHUNK points
@@ -297,7 +298,7 @@ void center_chunks_host(const std::vector<uint32_t>& k0, const std::vector<uint3
                         std::vector<uint4>& chunks, std::vector<uint32_t>& first);
 int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range);  // k_manybody.hip
 void radix_run(Ctx* c, const RadixSort& rs, uint32_t* const (&k)[2], uint32_t* const (&v)[2],
-               bool hist0_ready);
+               bool hist0_ready, float2* const* pay = nullptr);
 int mb_alloc(Ctx* c);
 int mb_run(Ctx* c, float2* vout);
 void pyramid_work_host(uint32_t slot, uint32_t gf, uint32_t L, std::vector<uint4>& w0,

@@ -66,7 +66,8 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
                                                   uint32_t* __restrict__ kout, uint32_t* vout,
                                                   int width, uint32_t nb, uint32_t* __restrict__ hist,
                                                   uint32_t* __restrict__ cell_end, uint32_t n_cells,
-                                                  uint32_t* __restrict__ big_count) {
+                                                  uint32_t* __restrict__ big_count,
+                                                  float2* __restrict__ pout) {
     __shared__ uint32_t h[RB_RADIX];
     const int t = threadIdx.x;
     const uint32_t radix = 1u << width, mask = radix - 1;
@@ -84,6 +85,7 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
             const uint32_t key = finest_key(d, q);
             kout[k] = key;
             vout[k] = i;
+            pout[k] = q;  // the position rides the sort: cell order without a second gather
             atomicAdd(&h[key & mask], 1u);
         }
     }
@@ -113,7 +115,9 @@ __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ hist,
                                                     const uint32_t* __restrict__ totals,
                                                     uint32_t* __restrict__ kout,
-                                                    uint32_t* __restrict__ vout) {
+                                                    uint32_t* __restrict__ vout,
+                                                    const float2* __restrict__ pin,
+                                                    float2* __restrict__ pout) {
     __shared__ uint32_t base[RB_RADIX];
     __shared__ uint32_t wc[4][RB_RADIX];
     __shared__ uint32_t sw[4];
@@ -143,6 +147,7 @@ __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__
         const bool valid = k < np;
         const uint32_t key = valid ? kin[k] : 0u;
         const uint32_t val = valid ? vin[k] : 0u;
+        const float2 pay = (valid && pin) ? pin[k] : make_float2(0.f, 0.f);
         const uint32_t dg = (key >> shift) & mask;
         for (uint32_t q = t; q < radix; q += 256) {
 #pragma unroll
@@ -164,6 +169,7 @@ __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__
             for (int w = 0; w < wave; ++w) off += wc[w][dg];
             kout[off] = key;
             vout[off] = val;
+            if (pout) pout[off] = pay;
         }
         __syncthreads();
         for (uint32_t q = t; q < radix; q += 256) base[q] += wc[0][q] + wc[1][q] + wc[2][q] + wc[3][q];
@@ -274,29 +280,28 @@ __device__ inline void write_cell(const GDesc& d, uint32_t rm, long long sx, lon
 }
 
 // Accumulates the points at sorted positions k0, k0 + STRIDE, ... below e into
-// one lane's integer sums and top-8 list, copying their positions to
-// sorted_pos.  Loads go out CELL_BATCH at a time (indices, then positions) so
-// a lane waits on two memory latencies per batch rather than per point; the
-// sums are exact and the top-8 rule is order-free, so the batching changes no
-// result.
+// one lane's integer sums and top-8 list (positions from the sort's payload,
+// in cell order).  Loads go out CELL_BATCH at a time so a lane waits on one
+// memory latency per batch rather than per point; the sums are exact and the
+// top-8 rule is order-free, so the batching changes no result.
 constexpr int CELL_BATCH = 8;
 template <uint32_t STRIDE>
 __device__ inline void cell_points(uint32_t k0, uint32_t e, const uint32_t* __restrict__ sv,
-                                   const float2* __restrict__ pin, float2* __restrict__ sorted_pos,
-                                   const GDesc& d, long long& sx, long long& sy, uint32_t (&tb)[SLOTS],
-                                   uint32_t (&ti)[SLOTS]) {
+                                   const float2* __restrict__ sorted_pos, const GDesc& d, long long& sx,
+                                   long long& sy, uint32_t (&tb)[SLOTS], uint32_t (&ti)[SLOTS]) {
     for (uint32_t k = k0; k < e; k += CELL_BATCH * STRIDE) {
         uint32_t ii[CELL_BATCH];
         float2 pp[CELL_BATCH];
 #pragma unroll
-        for (int u = 0; u < CELL_BATCH; ++u) ii[u] = sv[min(k + u * STRIDE, e - 1)];
-#pragma unroll
-        for (int u = 0; u < CELL_BATCH; ++u) pp[u] = pin[ii[u]];
+        for (int u = 0; u < CELL_BATCH; ++u) {
+            const uint32_t kk = min(k + u * STRIDE, e - 1);
+            ii[u] = sv[kk];
+            pp[u] = sorted_pos[kk];
+        }
 #pragma unroll
         for (int u = 0; u < CELL_BATCH; ++u) {
             const uint32_t kk = k + u * STRIDE;
             if (kk < e) {
-                sorted_pos[kk] = pp[u];
                 sx += fixed(pp[u].x, d.fx_scale);
                 sy += fixed(pp[u].y, d.fx_scale);
                 top8_insert(tb, ti, ticket_bits(ii[u] - d.p0, d.seed), ii[u]);
@@ -315,7 +320,7 @@ __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ sv,
                                                       const float2* __restrict__ pos0,
                                                       const float2* __restrict__ pos1,
-                                                      float2* __restrict__ sorted_pos,
+                                                      const float2* __restrict__ sorted_pos,
                                                       long long* __restrict__ sums,
                                                       int32_t* __restrict__ slot_idx,
                                                       float2* __restrict__ slot_pos,
@@ -355,7 +360,7 @@ __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
     long long sx = 0, sy = 0;
-    cell_points<1>(b, e, sv, pin, sorted_pos, d, sx, sy, tb, ti);
+    cell_points<1>(b, e, sv, sorted_pos, d, sx, sy, tb, ti);
     write_cell(d, rm, sx, sy, n, ti, pin, sums, slot_idx, slot_pos);
 }
 
@@ -389,7 +394,7 @@ __global__ void __launch_bounds__(256) k_cells_mid(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ sv,
                                                    const float2* __restrict__ pos0,
                                                    const float2* __restrict__ pos1,
-                                                   float2* __restrict__ sorted_pos,
+                                                   const float2* __restrict__ sorted_pos,
                                                    long long* __restrict__ sums,
                                                    int32_t* __restrict__ slot_idx,
                                                    float2* __restrict__ slot_pos,
@@ -410,7 +415,7 @@ __global__ void __launch_bounds__(256) k_cells_mid(const uint32_t* __restrict__ 
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
         long long sx = 0, sy = 0;
-        cell_points<64>(b + lane, e, sv, pin, sorted_pos, d, sx, sy, tb, ti);
+        cell_points<64>(b + lane, e, sv, sorted_pos, d, sx, sy, tb, ti);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             sx += __shfl_xor(sx, o, 64);
@@ -447,7 +452,7 @@ __global__ void __launch_bounds__(256) k_cells_chunk(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ sv,
                                                      const float2* __restrict__ pos0,
                                                      const float2* __restrict__ pos1,
-                                                     float2* __restrict__ sorted_pos,
+                                                     const float2* __restrict__ sorted_pos,
                                                      long long* __restrict__ sums,
                                                      const uint2* __restrict__ big_chunks,
                                                      const uint32_t* __restrict__ big_count,
@@ -469,7 +474,7 @@ __global__ void __launch_bounds__(256) k_cells_chunk(const uint32_t* __restrict_
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
         long long sx = 0, sy = 0;
-        cell_points<256>(b + t, end, sv, pin, sorted_pos, d, sx, sy, tb, ti);
+        cell_points<256>(b + t, end, sv, sorted_pos, d, sx, sy, tb, ti);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             sx += __shfl_xor(sx, o, 64);
@@ -1012,7 +1017,11 @@ int mb_alloc(Ctx* c) {
     rc = radix_alloc(c, c->mb_rs, c->NP, F);
     if (rc) return rc;
     c->sorted = c->rv[c->mb_rs.passes & 1];
-    c->sorted_pos = (float2*)dalloc(c, np * 8, &rc); if (rc) return rc;
+    for (int q = 0; q < 2; ++q) {
+        c->rp[q] = (float2*)dalloc(c, np * 8, &rc);
+        if (rc) return rc;
+    }
+    c->sorted_pos = c->rp[c->mb_rs.passes & 1];  // the sort's position payload
     c->cell_begin = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
     c->cell_end = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
     c->slot_idx = (int32_t*)dalloc(c, F * SLOTS * 4, &rc); if (rc) return rc;
@@ -1036,7 +1045,7 @@ int mb_alloc(Ctx* c) {
 // Sorts (k[0], v[0]) by key into k[passes & 1], v[passes & 1]; every pass is
 // stable.  hist0_ready: the first digit's block histogram is already in place.
 void radix_run(Ctx* c, const RadixSort& rs, uint32_t* const (&k)[2], uint32_t* const (&v)[2],
-               bool hist0_ready) {
+               bool hist0_ready, float2* const* pay) {
     const int W = rs.width;
     for (int p = 0; p < rs.passes; ++p) {
         if (p || !hist0_ready)
@@ -1044,7 +1053,8 @@ void radix_run(Ctx* c, const RadixSort& rs, uint32_t* const (&k)[2], uint32_t* c
                                rs.nb, rs.hist);
         hipLaunchKernelGGL(k_scan_rows, dim3(1u << W), dim3(256), 0, c->stream, rs.hist, rs.nb, rs.tot);
         hipLaunchKernelGGL(k_radix_down, dim3(rs.nb), dim3(256), 0, c->stream, k[p & 1], v[p & 1], rs.n,
-                           p * W, W, rs.nb, rs.hist, rs.tot, k[(p + 1) & 1], v[(p + 1) & 1]);
+                           p * W, W, rs.nb, rs.hist, rs.tot, k[(p + 1) & 1], v[(p + 1) & 1],
+                           pay ? pay[p & 1] : nullptr, pay ? pay[(p + 1) & 1] : nullptr);
     }
 }
 
@@ -1055,8 +1065,8 @@ int mb_run(Ctx* c, float2* vout) {
     const RadixSort& rs = c->mb_rs;
     hipLaunchKernelGGL(k_bin_keys, dim3(rs.nb), dim3(256), 0, c->stream, c->sorted, c->pos[0], c->pos[1],
                        c->gid, c->desc, np, c->rk[0], c->rv[0], rs.width, rs.nb, rs.hist,
-                       c->cell_end, F, c->big_count);
-    radix_run(c, rs, c->rk, c->rv, true);
+                       c->cell_end, F, c->big_count, c->rp[0]);
+    radix_run(c, rs, c->rk, c->rv, true, c->rp);
     hipLaunchKernelGGL(k_bin_bounds, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream,
                        c->rk[rs.passes & 1], np, c->cell_begin, c->cell_end);
     prof_end(c, CG_STAGE_MB_SORT);
