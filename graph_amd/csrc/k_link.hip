@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __r
 
 constexpr int MAX_SWEEPS = 8;
 constexpr float NOISE_FLOOR = 16.f;  // stalls below this many tolerances end the iteration
-constexpr float FIX_ACCEPT = 4.f;    // split points: largest accepted start correction, in tolerances
+constexpr float FIX_ACCEPT = 4.f;    // largest accepted start correction, in tolerances
 // One serial spring step from an entry's terms (no change when !live),
 // also applying the step's Jacobian to J (column-major 2x2, J <- M J):
 //   u' = u + x t(u),  x = q - u,  t = K (1 - D / max(|x|, 0.99 D))
@@ -620,9 +620,12 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
             mx = s_delta[0];
             for (int w = 1; w < B / 64; ++w) mx = fmaxf(mx, s_delta[w]);
         }
-        // converged, at the f32 noise floor (no longer halving, near the
+        // converged (the output moves the last chunk's end through its
+        // Jacobian, so corrections up to FIX_ACCEPT tolerances leave a second
+        // order error), at the f32 noise floor (no longer halving, near the
         // tolerance), or at the cap
-        if (mx <= d.tol || (sweep >= 2 && mx > 0.5f * prev && mx <= NOISE_FLOOR * d.tol) || sweep == MAX_SWEEPS) {
+        if (mx <= FIX_ACCEPT * d.tol || (sweep >= 2 && mx > 0.5f * prev && mx <= NOISE_FLOOR * d.tol) ||
+            sweep == MAX_SWEEPS) {
             // the last chunk's end, moved to its corrected start
             if (t == last) {
                 float ex_ = start.x - g.x, ey = start.y - g.y;
@@ -632,7 +635,7 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
             if (split)
                 wjac[V.y + t] = t == 0 ? make_float4(G.x, G.y, 0.f, 0.f) : make_float4(ex.a0, ex.a1, ex.a2, ex.a3);
             if (stats && t == 0) {
-                // sweeps used, units, max sweeps, units not within tolerance (per tier)
+                // sweeps used, units, max sweeps, units stopped above the tolerance (per tier)
                 const int o = B == 64 ? 0 : 4;
                 atomicAdd(&stats[o + 0], (uint32_t)(sweep + 1));
                 atomicAdd(&stats[o + 1], 1u);
