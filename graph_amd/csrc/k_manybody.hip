@@ -75,17 +75,30 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
     if (blockIdx.x == 0 && t < 3) big_count[t] = 0u;
     for (uint32_t q = t; q < radix; q += 256) h[q] = 0;
     __syncthreads();
+    // the lane's RB_ITEMS points go through each dependent load together
+    // (index, graph, position), so a wave waits three latencies, not 3 x RB_ITEMS
+    uint32_t ii[RB_ITEMS], gg[RB_ITEMS];
+    float2 qq[RB_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RB_ITEMS; ++j) {
+        const uint32_t k = min(blockIdx.x * RB_TILE + j * 256 + t, np - 1);
+        ii[j] = visit[k];
+    }
+#pragma unroll
+    for (int j = 0; j < RB_ITEMS; ++j) gg[j] = gid[ii[j]];
+#pragma unroll
+    for (int j = 0; j < RB_ITEMS; ++j) {
+        const GDesc& d = desc[gg[j]];
+        qq[j] = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1[ii[j]] : pos0[ii[j]];
+    }
 #pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
         const uint32_t k = blockIdx.x * RB_TILE + j * 256 + t;
         if (k < np) {
-            const uint32_t i = visit[k];
-            const GDesc& d = desc[gid[i]];
-            const float2 q = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1[i] : pos0[i];
-            const uint32_t key = finest_key(d, q);
+            const uint32_t key = finest_key(desc[gg[j]], qq[j]);
             kout[k] = key;
-            vout[k] = i;
-            pout[k] = q;  // the position rides the sort: cell order without a second gather
+            vout[k] = ii[j];
+            pout[k] = qq[j];  // the position rides the sort: cell order without a second gather
             atomicAdd(&h[key & mask], 1u);
         }
     }
@@ -100,10 +113,13 @@ __global__ void __launch_bounds__(256) k_radix_up(const uint32_t* __restrict__ k
     const uint32_t radix = 1u << width, mask = radix - 1;
     for (uint32_t q = t; q < radix; q += 256) h[q] = 0;
     __syncthreads();
+    uint32_t keys[RB_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RB_ITEMS; ++j) keys[j] = kin[min(blockIdx.x * RB_TILE + j * 256 + t, np - 1)];
 #pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
         const uint32_t k = blockIdx.x * RB_TILE + j * 256 + t;
-        if (k < np) atomicAdd(&h[(kin[k] >> shift) & mask], 1u);
+        if (k < np) atomicAdd(&h[(keys[j] >> shift) & mask], 1u);
     }
     __syncthreads();
     for (uint32_t q = t; q < radix; q += 256) hist[q * nb + blockIdx.x] = h[q];
@@ -142,12 +158,23 @@ __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__
         if (d0 < radix) base[d0] = ex + hist[d0 * nb + blockIdx.x];
         if (d1 < radix) base[d1] = ex + v0 + hist[d1 * nb + blockIdx.x];
     }
+    // all of the lane's items in flight before the ranking rounds
+    uint32_t keys[RB_ITEMS], vals[RB_ITEMS];
+    float2 pays[RB_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RB_ITEMS; ++j) {
+        const uint32_t k = min(blockIdx.x * RB_TILE + j * 256 + t, np - 1);
+        keys[j] = kin[k];
+        vals[j] = vin[k];
+        pays[j] = pin ? pin[k] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
         const uint32_t k = blockIdx.x * RB_TILE + j * 256 + t;
         const bool valid = k < np;
-        const uint32_t key = valid ? kin[k] : 0u;
-        const uint32_t val = valid ? vin[k] : 0u;
-        const float2 pay = (valid && pin) ? pin[k] : make_float2(0.f, 0.f);
+        const uint32_t key = valid ? keys[j] : 0u;
+        const uint32_t val = vals[j];
+        const float2 pay = pays[j];
         const uint32_t dg = (key >> shift) & mask;
         for (uint32_t q = t; q < radix; q += 256) {
 #pragma unroll
