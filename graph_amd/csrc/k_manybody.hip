@@ -690,8 +690,7 @@ __device__ inline void far_term(float4 c, float2 p, bool use, float& fx, float& 
 // (force-level.frag:70-79).  `cells` is the level (global or staged).  Rows
 // are not unrolled, so only one row's loads are in flight per lane.
 template <bool FAST>
-__device__ inline void far_level0(const float4* cells, float2 p, const GDesc& d, float& fx, float& fy) {
-    const int qx = cell_of(p.x, d.h[0], 4), qy = cell_of(p.y, d.h[0], 4);
+__device__ inline void far_level0(const float4* cells, float2 p, int qx, int qy, float& fx, float& fy) {
 #pragma unroll 1
     for (int cy = 0; cy < 4; ++cy) {
         const bool far_row = abs(cy - qy) > 1;
@@ -913,25 +912,21 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
         const float2 p = sorted_pos[k];
         const int L = (int)d.L;
         float fx = 0.f, fy = 0.f;
-        int qx = 0, qy = 0;
+        const int gf = (int)d.gf;
+        const int qx = cell_of(p.x, d.h[L - 1], gf), qy = cell_of(p.y, d.h[L - 1], gf);
         if (d.h[L - 1] >= 2.f) {
-            far_level0<true>(cent + d.c0, p, d, fx, fy);
+            far_level0<true>(cent + d.c0, p, qx >> (L - 1), qy >> (L - 1), fx, fy);
             for (int lvl = 1; lvl < L; ++lvl) {
-                const int gl = 4 << lvl;
-                qx = cell_of(p.x, d.h[lvl], gl);
-                qy = cell_of(p.y, d.h[lvl], gl);
-                far_level<true>(cent + d.c0 + level_off(lvl), 0, 0, gl, gl, qx, qy, p, fx, fy);
+                const int gl = 4 << lvl, k = L - 1 - lvl;
+                far_level<true>(cent + d.c0 + level_off(lvl), 0, 0, gl, gl, qx >> k, qy >> k, p, fx, fy);
             }
         } else {
-            far_level0<false>(cent + d.c0, p, d, fx, fy);
+            far_level0<false>(cent + d.c0, p, qx >> (L - 1), qy >> (L - 1), fx, fy);
             for (int lvl = 1; lvl < L; ++lvl) {
-                const int gl = 4 << lvl;
-                qx = cell_of(p.x, d.h[lvl], gl);
-                qy = cell_of(p.y, d.h[lvl], gl);
-                far_level<false>(cent + d.c0 + level_off(lvl), 0, 0, gl, gl, qx, qy, p, fx, fy);
+                const int gl = 4 << lvl, k = L - 1 - lvl;
+                far_level<false>(cent + d.c0 + level_off(lvl), 0, 0, gl, gl, qx >> k, qy >> k, p, fx, fy);
             }
         }
-        const int gf = (int)d.gf;
         const float2 r = jitter[i];
         const float2 wn = near_field(cent + d.c0 + level_off(L - 1), 0, 0, gf, d, qx, qy, i, p, r, slot_idx,
                                      slot_pos);
@@ -943,18 +938,20 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
     const int L = (int)d.L;
     float fx = 0.f, fy = 0.f;
     const bool fast = d.h[L - 1] >= 2.f;  // uniform: one graph
+    // The point's finest cell; its cell at a coarser level is this >> k
+    // exactly (h[lvl] = h[L-1] 2^k, and floor(x / 2^k) = floor(x) >> k).
+    const int gf = (int)d.gf;
+    const int fqx = cell_of(p.x, d.h[L - 1], gf), fqy = cell_of(p.y, d.h[L - 1], gf);
     if (t < 16) s_tile[t] = cent[d.c0 + t];
     __syncthreads();
-    if (fast) far_level0<true>(s_tile, p, d, fx, fy);
-    else far_level0<false>(s_tile, p, d, fx, fy);
-    int qx = 0, qy = 0, x0 = 0, y0 = 0, w = 0;
+    if (fast) far_level0<true>(s_tile, p, fqx >> (L - 1), fqy >> (L - 1), fx, fy);
+    else far_level0<false>(s_tile, p, fqx >> (L - 1), fqy >> (L - 1), fx, fy);
+    int x0 = 0, y0 = 0, w = 0;
     bool staged = false;
-    // The block's box of finest cells, once: a coarser level's cell is the
-    // finest cell >> k within one cell of rounding, so every level's window is
-    // the shifted box padded by one cell, widened to the aligned 6x6 blocks.
+    // The block's box of finest cells, once: every level's window is the
+    // shifted box padded by one cell, widened to the aligned 6x6 blocks.
     {
-        const int gfine = 4 << (L - 1);
-        int bx0 = cell_of(p.x, d.h[L - 1], gfine), by0 = cell_of(p.y, d.h[L - 1], gfine);
+        int bx0 = fqx, by0 = fqy;
         int bx1 = bx0, by1 = by0;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
@@ -978,8 +975,7 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
     for (int lvl = 1; lvl < L; ++lvl) {
         const int gl = 4 << lvl, k = L - 1 - lvl;
         const float4* __restrict__ lv = cent + d.c0 + level_off(lvl);
-        qx = cell_of(p.x, d.h[lvl], gl);
-        qy = cell_of(p.y, d.h[lvl], gl);
+        const int qx = fqx >> k, qy = fqy >> k;
         // the window is not clipped to the grid: cells outside it stage as empty
         x0 = (((fx0 >> k) - 1) >> 1) * 2 - 2;
         y0 = (((fy0 >> k) - 1) >> 1) * 2 - 2;
@@ -1007,11 +1003,10 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
         }
     }
     if (!valid) return;
-    const int gf = (int)d.gf;
     const float2 r = jitter[i];
     const float4* fin = cent + d.c0 + level_off(L - 1);
-    const float2 wn = staged ? near_field(s_tile, x0, y0, w, d, qx, qy, i, p, r, slot_idx, slot_pos)
-                             : near_field(fin, 0, 0, gf, d, qx, qy, i, p, r, slot_idx, slot_pos);
+    const float2 wn = staged ? near_field(s_tile, x0, y0, w, d, fqx, fqy, i, p, r, slot_idx, slot_pos)
+                             : near_field(fin, 0, 0, gf, d, fqx, fqy, i, p, r, slot_idx, slot_pos);
     mb_finish(d, i, p, r, fx, fy, wn, flags, pos0, pos1, vout);
 }
 
