@@ -770,7 +770,7 @@ __device__ inline void far_level_padded(const float4* tile, int x0, int y0, int 
 // Near field at the finest level (force-nearfield.frag:101-163): the 3x3
 // cells' sampled slots, Horvitz-Thompson weighted.  Counts come from `fin`
 // (finest centroids, global or a staged window as in far_level).
-__device__ inline float2 near_field(const float4* fin, int x0, int y0, int w, const GDesc& d, int qx,
+__device__ __forceinline__ float2 near_field(const float4* fin, int x0, int y0, int w, const GDesc& d, int qx,
                                     int qy, uint32_t i, float2 p, float2 r,
                                     const int32_t* __restrict__ slot_idx,
                                     const float2* __restrict__ slot_pos) {
