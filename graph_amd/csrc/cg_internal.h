@@ -205,7 +205,8 @@ struct Ctx {
     float2* sorted_pos = nullptr;   // [NP] positions in cell order (one of rp)
     int32_t* slot_idx = nullptr;    // [8 F] global point index or -1
     float2* slot_pos = nullptr;
-    uint32_t* mid_cells = nullptr;  // cells of BIG_CELL+1 .. BIG_CHUNK points
+    uint32_t* mid_cells = nullptr;  // cells of BIG_CELL+1 .. BIG_CHUNK points: two lists of mid_cap
+    uint32_t mid_cap = 0;
     uint4* big_cells = nullptr;     // crowded cells {cell, first chunk, chunks, row-major cell}
     uint2* big_chunks = nullptr;    // their chunks {cell, chunk}
     unsigned long long* chunk_top = nullptr;  // [8 per chunk] top-8 (ticket << 32 | index)

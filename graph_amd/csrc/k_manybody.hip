@@ -32,7 +32,9 @@ namespace {
 
 constexpr uint32_t TMAX = 0xffffffffu;
 constexpr uint32_t BIG_CHUNK = 2048;  // points per block in a crowded cell
-constexpr uint32_t MID_BLOCKS = 2048; // grid of the wave-per-cell pass (8192 waves)
+constexpr uint32_t MID_BLOCKS = 2048; // grid of each mid-cell pass (8192 waves)
+constexpr uint32_t MID_SMALL = 64;    // mid cells up to this many points take MID_GROUP lanes
+constexpr int MID_GROUP = 16;
 
 // ---- 1. counting sort by finest cell ---------------------------------------------
 
@@ -57,7 +59,7 @@ constexpr int RB_RADIX = 1 << RB_MAXW;
 
 // Keys of the points in the previous tick's cell order, and the first digit's
 // block histogram.  `visit` may alias `vout` (each lane rewrites what it read).
-// Also clears cell_end (n_cells words) and the three crowded-cell counters for
+// Also clears cell_end (n_cells words) and the four crowded-cell counters for
 // this tick.
 __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const float2* __restrict__ pos0,
                                                   const float2* __restrict__ pos1,
@@ -72,7 +74,7 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
     const int t = threadIdx.x;
     const uint32_t radix = 1u << width, mask = radix - 1;
     for (uint32_t f = blockIdx.x * 256 + t; f < n_cells; f += nb * 256) cell_end[f] = 0u;
-    if (blockIdx.x == 0 && t < 3) big_count[t] = 0u;
+    if (blockIdx.x == 0 && t < 4) big_count[t] = 0u;
     for (uint32_t q = t; q < radix; q += 256) h[q] = 0;
     __syncthreads();
     // the lane's RB_ITEMS points go through each dependent load together
@@ -339,9 +341,10 @@ __device__ inline void cell_points(uint32_t k0, uint32_t e, const uint32_t* __re
 
 // One lane per finest cell, one block per CELL_BLOCK cells of one graph.  Empty
 // cells write nothing: the pyramid reads their count, and their slots are
-// never read.  Cells above BIG_CELL points go to k_cells_mid (one wave each);
-// cells above BIG_CHUNK points are queued in chunks of BIG_CHUNK points for
-// k_cells_chunk and k_cells_merge.
+// never read.  Cells above BIG_CELL points go to k_cells_mid: up to MID_SMALL
+// points MID_GROUP lanes each (listed from mid_cells[0]), above it one wave
+// each (listed from mid_cells[mid_cap]); cells above BIG_CHUNK points are
+// queued in chunks of BIG_CHUNK points for k_cells_chunk and k_cells_merge.
 __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict__ end_,
                                                       const uint32_t* __restrict__ begin,
                                                       const uint32_t* __restrict__ sv,
@@ -351,7 +354,7 @@ __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict
                                                       long long* __restrict__ sums,
                                                       int32_t* __restrict__ slot_idx,
                                                       float2* __restrict__ slot_pos,
-                                                      uint32_t* __restrict__ mid_cells,
+                                                      uint32_t* __restrict__ mid_cells, uint32_t mid_cap,
                                                       uint4* __restrict__ big_cells,
                                                       uint2* __restrict__ big_chunks,
                                                       uint32_t* __restrict__ big_count,
@@ -366,8 +369,12 @@ __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict
     if (e == 0) return;
     const uint32_t b = begin[f], n = e - b;
     const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
-    if (n > BIG_CELL && n <= BIG_CHUNK) {
-        mid_cells[atomicAdd(&big_count[2], 1u)] = f;  // one wave each (k_cells_mid)
+    if (n > BIG_CELL && n <= MID_SMALL) {
+        mid_cells[atomicAdd(&big_count[2], 1u)] = f;  // MID_GROUP lanes each (k_cells_mid)
+        return;
+    }
+    if (n > MID_SMALL && n <= BIG_CHUNK) {
+        mid_cells[mid_cap + atomicAdd(&big_count[3], 1u)] = f;  // one wave each (k_cells_mid)
         return;
     }
     if (n > BIG_CHUNK) {
@@ -391,15 +398,17 @@ __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict
     write_cell(d, rm, sx, sy, n, ti, pin, sums, slot_idx, slot_pos);
 }
 
-// The 8 smallest distinct tickets over the lanes' private lists (smallest
-// index per ticket), as (ticket << 32 | index), all-ones past the end.
+// The 8 smallest distinct tickets over the private lists of each aligned
+// group of G lanes (smallest index per ticket), as (ticket << 32 | index),
+// all-ones past the end.
+template <int G>
 __device__ inline void wave_top8(uint32_t (&tb)[SLOTS], uint32_t (&ti)[SLOTS],
                                  unsigned long long (&out)[SLOTS]) {
 #pragma unroll
     for (int r = 0; r < SLOTS; ++r) {
         unsigned long long mn = ((unsigned long long)tb[0] << 32) | ti[0];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
+        for (int o = G / 2; o > 0; o >>= 1) {
             unsigned long long other = __shfl_xor(mn, o, 64);
             mn = other < mn ? other : mn;
         }
@@ -414,8 +423,12 @@ __device__ inline void wave_top8(uint32_t (&tb)[SLOTS], uint32_t (&ti)[SLOTS],
     }
 }
 
-// One wave per cell of BIG_CELL+1 .. BIG_CHUNK points: lanes stride the
-// segment with private top-8 lists and integer sums, then merge.
+// G lanes per listed cell (MID_GROUP for cells of BIG_CELL+1 .. MID_SMALL
+// points, a wave for MID_SMALL+1 .. BIG_CHUNK): the group's lanes stride the
+// cell's segment with private top-8 lists and integer sums, then merge inside
+// the group.  Every lane of a group runs the same iterations, so the group
+// shuffles never read a lane that has left the loop.
+template <int G>
 __global__ void __launch_bounds__(256) k_cells_mid(const uint32_t* __restrict__ end_,
                                                    const uint32_t* __restrict__ begin,
                                                    const uint32_t* __restrict__ sv,
@@ -425,16 +438,16 @@ __global__ void __launch_bounds__(256) k_cells_mid(const uint32_t* __restrict__ 
                                                    long long* __restrict__ sums,
                                                    int32_t* __restrict__ slot_idx,
                                                    float2* __restrict__ slot_pos,
-                                                   const uint32_t* __restrict__ mid_cells,
-                                                   const uint32_t* __restrict__ big_count,
+                                                   const uint32_t* __restrict__ cells,
+                                                   const uint32_t* __restrict__ count,
                                                    const uint32_t* __restrict__ blk_graph,
                                                    const GDesc* __restrict__ desc) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t nmid = big_count[2];
-    for (uint32_t w = wave; w < nmid; w += nwaves) {
-        const uint32_t f = mid_cells[w];
+    const int gl = threadIdx.x & (G - 1);
+    const uint32_t group = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const uint32_t ngroups = (gridDim.x * blockDim.x) / G;
+    const uint32_t nmid = *count;
+    for (uint32_t w = group; w < nmid; w += ngroups) {
+        const uint32_t f = cells[w];
         const GDesc& d = desc[blk_graph[f / CELL_BLOCK]];
         const float2* __restrict__ pin = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1 : pos0;
         const uint32_t b = begin[f], e = end_[f];
@@ -442,18 +455,18 @@ __global__ void __launch_bounds__(256) k_cells_mid(const uint32_t* __restrict__ 
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
         long long sx = 0, sy = 0;
-        cell_points<64>(b + lane, e, sv, sorted_pos, d, sx, sy, tb, ti);
+        cell_points<G>(b + gl, e, sv, sorted_pos, d, sx, sy, tb, ti);
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
+        for (int o = G / 2; o > 0; o >>= 1) {
             sx += __shfl_xor(sx, o, 64);
             sy += __shfl_xor(sy, o, 64);
         }
         unsigned long long out[SLOTS];
-        wave_top8(tb, ti, out);
+        wave_top8<G>(tb, ti, out);
         const uint32_t mc = f - d.f0;
         const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
         const size_t sb = ((size_t)d.f0 + rm) * SLOTS;
-        if (lane == 0) {
+        if (gl == 0) {
             const size_t c3 = 3 * ((size_t)d.c0 + level_off(d.L - 1) + rm);
             sums[c3] = sx;
             sums[c3 + 1] = sy;
@@ -461,7 +474,7 @@ __global__ void __launch_bounds__(256) k_cells_mid(const uint32_t* __restrict__ 
         }
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) {
-            if (lane == s) {
+            if (gl == s) {
                 const bool ok = out[s] != ~0ull;
                 const uint32_t i = (uint32_t)out[s];
                 slot_idx[sb + s] = ok ? (int32_t)i : -1;
@@ -508,7 +521,7 @@ __global__ void __launch_bounds__(256) k_cells_chunk(const uint32_t* __restrict_
             sy += __shfl_xor(sy, o, 64);
         }
         unsigned long long out[SLOTS];
-        wave_top8(tb, ti, out);
+        wave_top8<64>(tb, ti, out);
         if (lane == 0) {
             s_sum[0][wave] = sx;
             s_sum[1][wave] = sy;
@@ -562,13 +575,20 @@ __global__ void __launch_bounds__(256) k_cells_merge(const uint4* __restrict__ b
         uint32_t tb[SLOTS], ti[SLOTS];
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
-        for (uint32_t e = bc.y + lane; e < bc.y + bc.z; e += 64)
-            for (int s = 0; s < SLOTS; ++s) {
-                const unsigned long long v = chunk_top[(size_t)e * SLOTS + s];
-                if (v != ~0ull) top8_insert(tb, ti, (uint32_t)(v >> 32), (uint32_t)v);
+        for (uint32_t e = bc.y + lane; e < bc.y + bc.z; e += 64) {
+            // the chunk's whole list in flight at once (4 vector loads)
+            const ulonglong2* __restrict__ ct = reinterpret_cast<const ulonglong2*>(chunk_top + (size_t)e * SLOTS);
+            ulonglong2 v2[SLOTS / 2];
+#pragma unroll
+            for (int q = 0; q < SLOTS / 2; ++q) v2[q] = ct[q];
+#pragma unroll
+            for (int q = 0; q < SLOTS / 2; ++q) {
+                if (v2[q].x != ~0ull) top8_insert(tb, ti, (uint32_t)(v2[q].x >> 32), (uint32_t)v2[q].x);
+                if (v2[q].y != ~0ull) top8_insert(tb, ti, (uint32_t)(v2[q].y >> 32), (uint32_t)v2[q].y);
             }
+        }
         unsigned long long out[SLOTS];
-        wave_top8(tb, ti, out);
+        wave_top8<64>(tb, ti, out);
         const size_t sb = ((size_t)d.f0 + bc.w) * SLOTS;
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) {
@@ -1050,7 +1070,8 @@ int mb_alloc(Ctx* c) {
     c->slot_pos = (float2*)dalloc(c, F * SLOTS * 8, &rc); if (rc) return rc;
     // crowded cells: at most NP / (BIG_CELL + 1) of them, and their chunks
     const size_t ncells = np / (BIG_CELL + 1) + 1, nchunks = np / BIG_CHUNK + ncells + 1;
-    c->mid_cells = (uint32_t*)dalloc(c, ncells * 4, &rc); if (rc) return rc;
+    c->mid_cap = (uint32_t)ncells;  // two lists: small mid cells, then wave-sized ones
+    c->mid_cells = (uint32_t*)dalloc(c, 2 * ncells * 4, &rc); if (rc) return rc;
     c->big_cells = (uint4*)dalloc(c, ncells * 16, &rc); if (rc) return rc;
     c->big_chunks = (uint2*)dalloc(c, nchunks * 8, &rc); if (rc) return rc;
     c->chunk_top = (unsigned long long*)dalloc(c, nchunks * SLOTS * 8, &rc); if (rc) return rc;
@@ -1095,18 +1116,22 @@ int mb_run(Ctx* c, float2* vout) {
     prof_begin(c, CG_STAGE_MB_CELLS);
     hipLaunchKernelGGL(k_cells, dim3(F / CELL_BLOCK), dim3(CELL_BLOCK), 0, c->stream, c->cell_end,
                        c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums,
-                       c->slot_idx, c->slot_pos, c->mid_cells, c->big_cells, c->big_chunks,
+                       c->slot_idx, c->slot_pos, c->mid_cells, c->mid_cap, c->big_cells, c->big_chunks,
                        c->big_count, c->blk_graph, c->desc);
-    // the wave-per-cell tier and the crowded-cell chunks touch disjoint cells:
-    // the first runs on the side stream beside the other two
+    // the mid tiers and the crowded-cell chunks touch disjoint cells: the
+    // mid tiers run on the side stream beside the other two
     const bool fork = c->side != nullptr;
     if (fork) {
         CG_HIP(c, hipEventRecord(c->ev_fork, c->stream));
         CG_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
     }
-    hipLaunchKernelGGL(k_cells_mid, dim3(MID_BLOCKS), dim3(256), 0, fork ? c->side : c->stream, c->cell_end,
-                       c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->slot_idx,
-                       c->slot_pos, c->mid_cells, c->big_count, c->blk_graph, c->desc);
+    hipLaunchKernelGGL(k_cells_mid<64>, dim3(MID_BLOCKS), dim3(256), 0, fork ? c->side : c->stream,
+                       c->cell_end, c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums,
+                       c->slot_idx, c->slot_pos, c->mid_cells + c->mid_cap, c->big_count + 3, c->blk_graph,
+                       c->desc);
+    hipLaunchKernelGGL(k_cells_mid<MID_GROUP>, dim3(MID_BLOCKS), dim3(256), 0, fork ? c->side : c->stream,
+                       c->cell_end, c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums,
+                       c->slot_idx, c->slot_pos, c->mid_cells, c->big_count + 2, c->blk_graph, c->desc);
     if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
     hipLaunchKernelGGL(k_cells_chunk, dim3(1024), dim3(256), 0, c->stream, c->cell_end, c->cell_begin,
                        c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->big_chunks,
