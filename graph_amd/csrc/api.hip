@@ -239,8 +239,10 @@ void free_packed(Ctx* c) {
     }
     for (int q = 0; q < 2; ++q) { release(c, c->rk[q]); release(c, c->rv[q]); }
     c->sorted = nullptr;
-    release(c, c->mb_rs.hist); release(c, c->mb_rs.tot); c->mb_rs = RadixSort();
-    release(c, c->col_rs.hist); release(c, c->col_rs.tot); c->col_rs = RadixSort();
+    release(c, c->mb_rs.hist); release(c, c->mb_rs.tot); release(c, c->mb_rs.blk); release(c, c->mb_rs.key0);
+    c->mb_rs = RadixSort();
+    release(c, c->col_rs.hist); release(c, c->col_rs.tot); release(c, c->col_rs.blk); release(c, c->col_rs.key0);
+    c->col_rs = RadixSort();
     release(c, c->cell_begin); release(c, c->cell_end);
     release(c, c->sums); release(c, c->cent); release(c, c->rp[0]); release(c, c->rp[1]); c->sorted_pos = nullptr;
     release(c, c->slot_idx); release(c, c->slot_pos); release(c, c->mid_cells); release(c, c->big_cells); release(c, c->big_chunks); release(c, c->chunk_top);

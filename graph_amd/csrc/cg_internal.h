@@ -157,6 +157,13 @@ struct RadixSort {
     uint32_t n_hist = 0;            // (1 << width) * nb
     uint32_t* hist = nullptr;       // [n_hist] per-pass [digit][block] counts
     uint32_t* tot = nullptr;        // [1 << width] digit totals of the current pass
+    // Graph segments (the many-body sort of a packed set, whose input is
+    // graph-major): block b sorts positions [blk[b].x, blk[b].y) of its graph,
+    // whose blocks are [blk[b].z, blk[b].w), on digits of key - key0[b], so a
+    // pass covers the largest graph's key range rather than the whole set's.
+    // Null: one segment, blocks of RB_TILE pairs from position 0, key0 = 0.
+    uint4* blk = nullptr;
+    uint32_t* key0 = nullptr;
 };
 
 struct Ctx {
@@ -297,7 +304,9 @@ int center_run(Ctx* c);
 int cluster_run(Ctx* c, float2* vout);
 void center_chunks_host(const std::vector<uint32_t>& k0, const std::vector<uint32_t>& np,
                         std::vector<uint4>& chunks, std::vector<uint32_t>& first);
-int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range);  // k_manybody.hip
+int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range,
+                const std::vector<uint4>* seg_blk = nullptr,
+                const std::vector<uint32_t>* seg_key0 = nullptr);  // k_manybody.hip
 void radix_run(Ctx* c, const RadixSort& rs, uint32_t* const (&k)[2], uint32_t* const (&v)[2],
                bool hist0_ready, float2* const* pay = nullptr);
 int mb_alloc(Ctx* c);

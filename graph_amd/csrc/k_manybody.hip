@@ -57,6 +57,21 @@ constexpr int RB_TILE = 256 * RB_ITEMS;
 constexpr int RB_MAXW = 9;               // digits of up to 9 bits: 18-bit cell keys in two passes
 constexpr int RB_RADIX = 1 << RB_MAXW;
 
+// Block b's positions [*bb, *be) and key offset (see RadixSort::blk).
+__device__ inline void radix_block(const uint4* __restrict__ blk, const uint32_t* __restrict__ key0,
+                                   uint32_t np, uint32_t& bb, uint32_t& be, uint32_t& k0) {
+    if (blk) {
+        const uint4 q = blk[blockIdx.x];
+        bb = q.x;
+        be = q.y;
+        k0 = key0[blockIdx.x];
+    } else {
+        bb = blockIdx.x * RB_TILE;
+        be = min(np, bb + RB_TILE);
+        k0 = 0u;
+    }
+}
+
 // Keys of the points in the previous tick's cell order, and the first digit's
 // block histogram.  `visit` may alias `vout` (each lane rewrites what it read).
 // Also clears cell_end (n_cells words) and the four crowded-cell counters for
@@ -69,10 +84,14 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
                                                   int width, uint32_t nb, uint32_t* __restrict__ hist,
                                                   uint32_t* __restrict__ cell_end, uint32_t n_cells,
                                                   uint32_t* __restrict__ big_count,
-                                                  float2* __restrict__ pout) {
+                                                  float2* __restrict__ pout,
+                                                  const uint4* __restrict__ blk,
+                                                  const uint32_t* __restrict__ key0) {
     __shared__ uint32_t h[RB_RADIX];
     const int t = threadIdx.x;
     const uint32_t radix = 1u << width, mask = radix - 1;
+    uint32_t bb, be, k0;
+    radix_block(blk, key0, np, bb, be, k0);
     for (uint32_t f = blockIdx.x * 256 + t; f < n_cells; f += nb * 256) cell_end[f] = 0u;
     if (blockIdx.x == 0 && t < 4) big_count[t] = 0u;
     for (uint32_t q = t; q < radix; q += 256) h[q] = 0;
@@ -83,7 +102,7 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
     float2 qq[RB_ITEMS];
 #pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
-        const uint32_t k = min(blockIdx.x * RB_TILE + j * 256 + t, np - 1);
+        const uint32_t k = min(bb + j * 256 + t, np - 1);
         ii[j] = visit[k];
     }
 #pragma unroll
@@ -95,13 +114,13 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
     }
 #pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
-        const uint32_t k = blockIdx.x * RB_TILE + j * 256 + t;
-        if (k < np) {
+        const uint32_t k = bb + j * 256 + t;
+        if (k < be) {
             const uint32_t key = finest_key(desc[gg[j]], qq[j]);
             kout[k] = key;
             vout[k] = ii[j];
             pout[k] = qq[j];  // the position rides the sort: cell order without a second gather
-            atomicAdd(&h[key & mask], 1u);
+            atomicAdd(&h[(key - k0) & mask], 1u);
         }
     }
     __syncthreads();
@@ -109,19 +128,23 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
 }
 
 __global__ void __launch_bounds__(256) k_radix_up(const uint32_t* __restrict__ kin, uint32_t np, int shift,
-                                                  int width, uint32_t nb, uint32_t* __restrict__ hist) {
+                                                  int width, uint32_t nb, uint32_t* __restrict__ hist,
+                                                  const uint4* __restrict__ blk,
+                                                  const uint32_t* __restrict__ key0) {
     __shared__ uint32_t h[RB_RADIX];
     const int t = threadIdx.x;
     const uint32_t radix = 1u << width, mask = radix - 1;
+    uint32_t bb, be, k0;
+    radix_block(blk, key0, np, bb, be, k0);
     for (uint32_t q = t; q < radix; q += 256) h[q] = 0;
     __syncthreads();
     uint32_t keys[RB_ITEMS];
 #pragma unroll
-    for (int j = 0; j < RB_ITEMS; ++j) keys[j] = kin[min(blockIdx.x * RB_TILE + j * 256 + t, np - 1)];
+    for (int j = 0; j < RB_ITEMS; ++j) keys[j] = kin[min(bb + j * 256 + t, np - 1)];
 #pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
-        const uint32_t k = blockIdx.x * RB_TILE + j * 256 + t;
-        if (k < np) atomicAdd(&h[(keys[j] >> shift) & mask], 1u);
+        const uint32_t k = bb + j * 256 + t;
+        if (k < be) atomicAdd(&h[((keys[j] - k0) >> shift) & mask], 1u);
     }
     __syncthreads();
     for (uint32_t q = t; q < radix; q += 256) hist[q * nb + blockIdx.x] = h[q];
@@ -135,18 +158,37 @@ __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ kout,
                                                     uint32_t* __restrict__ vout,
                                                     const float2* __restrict__ pin,
-                                                    float2* __restrict__ pout) {
+                                                    float2* __restrict__ pout,
+                                                    const uint4* __restrict__ blk,
+                                                    const uint32_t* __restrict__ key0) {
     __shared__ uint32_t base[RB_RADIX];
     __shared__ uint32_t wc[4][RB_RADIX];
     __shared__ uint32_t sw[4];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t radix = 1u << width, mask = radix - 1;
     const unsigned long long lt = (1ull << lane) - 1ull;
+    uint32_t bb, be, k0;
+    radix_block(blk, key0, np, bb, be, k0);
+    // the block's segment: blocks [fb, lb) from output position `start`
+    const uint4 seg = blk ? blk[blockIdx.x] : make_uint4(0u, 0u, 0u, nb);
+    const uint32_t fb = seg.z, lb = seg.w, start = blk ? blk[fb].x : 0u;
     {
-        // the digits' offsets: exclusive scan of the totals (2 digits per lane),
-        // plus this block's offset within each digit's row
+        // the digits' offsets: exclusive scan of the segment's digit totals
+        // (2 digits per lane; the rows hold exclusive prefixes over all blocks,
+        // so a segment's count is a difference), plus this block's offset
+        // within its segment's row
         const uint32_t d0 = 2 * t, d1 = 2 * t + 1;
-        const uint32_t v0 = d0 < radix ? totals[d0] : 0u, v1 = d1 < radix ? totals[d1] : 0u;
+        uint32_t v0 = 0u, v1 = 0u, o0 = 0u, o1 = 0u;
+        if (d0 < radix) {
+            const uint32_t f = blk ? hist[d0 * nb + fb] : 0u;
+            v0 = (lb < nb ? hist[d0 * nb + lb] : totals[d0]) - f;
+            o0 = hist[d0 * nb + blockIdx.x] - f;
+        }
+        if (d1 < radix) {
+            const uint32_t f = blk ? hist[d1 * nb + fb] : 0u;
+            v1 = (lb < nb ? hist[d1 * nb + lb] : totals[d1]) - f;
+            o1 = hist[d1 * nb + blockIdx.x] - f;
+        }
         uint32_t inc = v0 + v1;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -156,28 +198,28 @@ __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__
         if (lane == 63) sw[wave] = inc;
         __syncthreads();
         const uint32_t before = (wave > 0 ? sw[0] : 0u) + (wave > 1 ? sw[1] : 0u) + (wave > 2 ? sw[2] : 0u);
-        const uint32_t ex = before + inc - v0 - v1;
-        if (d0 < radix) base[d0] = ex + hist[d0 * nb + blockIdx.x];
-        if (d1 < radix) base[d1] = ex + v0 + hist[d1 * nb + blockIdx.x];
+        const uint32_t ex = start + before + inc - v0 - v1;
+        if (d0 < radix) base[d0] = ex + o0;
+        if (d1 < radix) base[d1] = ex + v0 + o1;
     }
     // all of the lane's items in flight before the ranking rounds
     uint32_t keys[RB_ITEMS], vals[RB_ITEMS];
     float2 pays[RB_ITEMS];
 #pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
-        const uint32_t k = min(blockIdx.x * RB_TILE + j * 256 + t, np - 1);
+        const uint32_t k = min(bb + j * 256 + t, np - 1);
         keys[j] = kin[k];
         vals[j] = vin[k];
         pays[j] = pin ? pin[k] : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
-        const uint32_t k = blockIdx.x * RB_TILE + j * 256 + t;
-        const bool valid = k < np;
+        const uint32_t k = bb + j * 256 + t;
+        const bool valid = k < be;
         const uint32_t key = valid ? keys[j] : 0u;
         const uint32_t val = vals[j];
         const float2 pay = pays[j];
-        const uint32_t dg = (key >> shift) & mask;
+        const uint32_t dg = ((key - k0) >> shift) & mask;
         for (uint32_t q = t; q < radix; q += 256) {
 #pragma unroll
             for (int w = 0; w < 4; ++w) wc[w][q] = 0u;
@@ -1033,14 +1075,27 @@ __global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ s
 }  // namespace
 
 // Digits of at most RB_MAXW bits, as few passes as cover key_range, for n pairs.
-int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range) {
+int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range, const std::vector<uint4>* seg_blk,
+                const std::vector<uint32_t>* seg_key0) {
     int rc = CG_OK;
     int bits = 1;
     while (bits < 32 && (1ull << bits) < key_range) ++bits;
     rs.passes = (bits + RB_MAXW - 1) / RB_MAXW;
     rs.width = (bits + rs.passes - 1) / rs.passes;
     rs.n = n;
-    rs.nb = (uint32_t)(((size_t)(n ? n : 1) + RB_TILE - 1) / RB_TILE);
+    if (seg_blk && !seg_blk->empty()) {
+        rs.nb = (uint32_t)seg_blk->size();
+        rs.blk = (uint4*)dalloc(c, seg_blk->size() * 16, &rc);
+        rs.key0 = (uint32_t*)dalloc(c, seg_key0->size() * 4, &rc);
+        if (rc) return rc;
+        CG_HIP(c, hipMemcpyAsync(rs.blk, seg_blk->data(), seg_blk->size() * 16, hipMemcpyHostToDevice,
+                                 c->stream));
+        CG_HIP(c, hipMemcpyAsync(rs.key0, seg_key0->data(), seg_key0->size() * 4, hipMemcpyHostToDevice,
+                                 c->stream));
+        CG_HIP(c, hipStreamSynchronize(c->stream));  // the host tables are the caller's temporaries
+    } else {
+        rs.nb = (uint32_t)(((size_t)(n ? n : 1) + RB_TILE - 1) / RB_TILE);
+    }
     rs.n_hist = (1u << rs.width) * rs.nb;
     rs.hist = (uint32_t*)dalloc(c, (size_t)rs.n_hist * 4, &rc);
     if (rc) return rc;
@@ -1056,8 +1111,29 @@ int mb_alloc(Ctx* c) {
         c->rk[q] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
         c->rv[q] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
     }
-    rc = radix_alloc(c, c->mb_rs, c->NP, F);
-    if (rc) return rc;
+    {
+        // one segment per graph with present points, in slot order (the
+        // present list and every tick's cell order are graph-major)
+        std::vector<uint4> blk;
+        std::vector<uint32_t> key0;
+        uint64_t range = 1;
+        for (uint32_t s = 0; s < c->G; ++s) {
+            const GDesc& d = c->h_desc[s];
+            if (d.np == 0) continue;
+            range = std::max<uint64_t>(range, (uint64_t)d.gf * d.gf);
+            const uint32_t fb = (uint32_t)blk.size(), nbk = (d.np + RB_TILE - 1) / RB_TILE;
+            for (uint32_t q = 0; q < nbk; ++q) {
+                const uint32_t b0 = d.k0 + q * RB_TILE;
+                blk.push_back(make_uint4(b0, std::min(d.k0 + d.np, b0 + RB_TILE), fb, fb + nbk));
+                key0.push_back(d.f0);
+            }
+        }
+        // a lone graph at offset 0 is the plain sort
+        const bool one = !blk.empty() && blk.front().z == 0 && blk.front().w == blk.size() &&
+                         blk.front().x == 0 && key0.front() == 0;
+        rc = one ? radix_alloc(c, c->mb_rs, c->NP, range) : radix_alloc(c, c->mb_rs, c->NP, range, &blk, &key0);
+        if (rc) return rc;
+    }
     c->sorted = c->rv[c->mb_rs.passes & 1];
     for (int q = 0; q < 2; ++q) {
         c->rp[q] = (float2*)dalloc(c, np * 8, &rc);
@@ -1093,11 +1169,11 @@ void radix_run(Ctx* c, const RadixSort& rs, uint32_t* const (&k)[2], uint32_t* c
     for (int p = 0; p < rs.passes; ++p) {
         if (p || !hist0_ready)
             hipLaunchKernelGGL(k_radix_up, dim3(rs.nb), dim3(256), 0, c->stream, k[p & 1], rs.n, p * W, W,
-                               rs.nb, rs.hist);
+                               rs.nb, rs.hist, rs.blk, rs.key0);
         hipLaunchKernelGGL(k_scan_rows, dim3(1u << W), dim3(256), 0, c->stream, rs.hist, rs.nb, rs.tot);
         hipLaunchKernelGGL(k_radix_down, dim3(rs.nb), dim3(256), 0, c->stream, k[p & 1], v[p & 1], rs.n,
                            p * W, W, rs.nb, rs.hist, rs.tot, k[(p + 1) & 1], v[(p + 1) & 1],
-                           pay ? pay[p & 1] : nullptr, pay ? pay[(p + 1) & 1] : nullptr);
+                           pay ? pay[p & 1] : nullptr, pay ? pay[(p + 1) & 1] : nullptr, rs.blk, rs.key0);
     }
 }
 
@@ -1108,7 +1184,7 @@ int mb_run(Ctx* c, float2* vout) {
     const RadixSort& rs = c->mb_rs;
     hipLaunchKernelGGL(k_bin_keys, dim3(rs.nb), dim3(256), 0, c->stream, c->sorted, c->pos[0], c->pos[1],
                        c->gid, c->desc, np, c->rk[0], c->rv[0], rs.width, rs.nb, rs.hist,
-                       c->cell_end, F, c->big_count, c->rp[0]);
+                       c->cell_end, F, c->big_count, c->rp[0], rs.blk, rs.key0);
     radix_run(c, rs, c->rk, c->rv, true, c->rp);
     hipLaunchKernelGGL(k_bin_bounds, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream,
                        c->rk[rs.passes & 1], np, c->cell_begin, c->cell_end);
