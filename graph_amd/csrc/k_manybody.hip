@@ -659,7 +659,8 @@ __device__ inline float4 centroid(long long sx, long long sy, long long n, float
 // One block per tile of one graph's level `lin` (work item {graph slot, lin,
 // T | nout << 16, tile}); produces the `nout` coarser levels of that tile down
 // to one cell per tile.  At the finest level (write_in) an empty cell (end 0)
-// has zero sums.
+// has zero sums.  Only the finest stage's last level keeps its sums (the
+// top-level stage starts from them); every level keeps its centroids.
 __global__ void __launch_bounds__(256) k_pyramid(const uint4* __restrict__ work,
                                                  const GDesc* __restrict__ desc,
                                                  const uint32_t* __restrict__ cell_end,
@@ -710,9 +711,11 @@ __global__ void __launch_bounds__(256) k_pyramid(const uint4* __restrict__ work,
             nxt[3 * c + 1] = v[1];
             nxt[3 * c + 2] = v[2];
             size_t rm = out0 + (size_t)(ty * half + y) * gout + (tx * half + x);
-            sums[3 * rm] = v[0];
-            sums[3 * rm + 1] = v[1];
-            sums[3 * rm + 2] = v[2];
+            if (write_in && lev == nout) {  // the top-level stage reads this level's sums
+                sums[3 * rm] = v[0];
+                sums[3 * rm + 1] = v[1];
+                sums[3 * rm + 2] = v[2];
+            }
             cent[rm] = centroid(v[0], v[1], v[2], d.fx_inv);
         }
         __syncthreads();
