@@ -250,8 +250,8 @@ def main() -> int:
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
-              "(each graph or request would be assigned to one GPU; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only; multi-GPU runs are not part of it "
+              "(each graph or request would be assigned to one GPU)", file=sys.stderr)
         return 2
 
     import torch

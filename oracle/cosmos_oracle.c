@@ -1,8 +1,8 @@
 /*
  * cosmos_oracle.c — CPU restatement of cosmos.gl's force-simulation tick.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the smoke
- * check and bench.py's cpu_baseline leg may load it.  The product path
+ * TEST INFRASTRUCTURE ONLY.  The GPU results are compared with this file's:
+ * only tests/, the smoke check and bench.py's cpu_baseline leg may load it.  The product path
  * (graph_amd/) never calls it and has no CPU fallback.
  *
  * Two kinds of functions live here:

@@ -1,7 +1,8 @@
 """ctypes binding of the oracle (``oracle/liboracle.so``).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and bench.py's
-cpu_baseline leg, as the checker.  Nothing in graph_amd/ imports it.
+cpu_baseline leg, as the answer the GPU results are compared with.  Nothing in
+graph_amd/ imports it.
 
 Functions are thin wrappers over ``cosmos_oracle.c``; see that file for the
 reference file:line each one restates.

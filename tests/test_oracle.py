@@ -1,4 +1,4 @@
-"""Pins the oracle (oracle/cosmos_oracle.c) before it is trusted as the checker.
+"""Pins the oracle (oracle/cosmos_oracle.c) before the GPU results are compared with it.
 
 The reference ships no tests, fixtures or golden vectors for this path, so
 every check here comes from mathematics the reference's formulas fix: closed
