@@ -28,13 +28,16 @@ constexpr uint32_t BIG_CELL = 16; // finest cells above this many points use a w
 constexpr uint32_t CELL_BLOCK = 256;  // each graph's finest cells start on a multiple of this
 
 // Link degree tiers: points with at most LANE_MAX entries run one lane each;
-// up to T64_MAX entries one 64-lane workgroup each; above that 256-lane
-// workgroups, at most 4 entries per lane: a point above HUB_SEG entries is
-// cut into segments of at most HUB_SEG.
+// up to T64_MAX entries one 64-lane workgroup each (at most 4 entries per
+// lane); above that 256-lane workgroups, at most HUB_E entries per lane: a
+// point above HUB_SEG entries is cut into segments of at most HUB_SEG.  (Six
+// per lane measured best: fewer, longer segments need fewer workgroups and
+// fix-up terms; longer chunks cost more per sweep.)
 constexpr uint32_t LANE_MAX = 32;
 constexpr uint32_t LANE_SPLIT = 8;  // lane-tier schedule classes: <= 2, <= LANE_SPLIT entries, then the rest
 constexpr uint32_t T64_MAX = 256;
-constexpr uint32_t HUB_SEG = 1024;   // entries per segment of a split 256-lane point
+constexpr int HUB_E = 6;
+constexpr uint32_t HUB_SEG = 256 * HUB_E;  // entries per segment of a split 256-lane point
 constexpr int HUB_ROUNDS = 5;        // segment/fix-up rounds per pass for split points
 
 // Passes of one tick, in order (src/index.ts:2012-2078).  Bit p of GDesc::passes

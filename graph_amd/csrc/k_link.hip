@@ -10,8 +10,9 @@
 // velocity accumulated so far.  Points are scheduled by live degree:
 //   - up to LANE_MAX entries: one lane per point, the serial loop as written;
 //   - up to T64_MAX entries: one 64-lane workgroup per point; above: 256
-//     lanes.  The chain is cut into one chunk per lane, at most 4 entries,
-//     whose terms the lane gathers once into registers; each
+//     lanes.  The chain is cut into one chunk per lane, at most 4 entries
+//     (64 lanes) or HUB_E (256 lanes), whose terms the lane gathers once into
+//     registers; each
 //     sweep runs every chunk from a guessed start velocity, carrying the
 //     chunk's end velocity and 2x2 Jacobian, and an affine prefix scan over
 //     the chunks yields the next guesses (multiple shooting with
@@ -487,7 +488,7 @@ __global__ void __launch_bounds__(64) k_hub_final(int pass, const uint2* __restr
     hub_finish(sched[unit_b[hb.x].x], f.end, d, pass, pos0, pos1, flags, vout, stats, HUB_ROUNDS, ok);
 }
 
-// Chunked tiers: one workgroup of B lanes per work unit of at most 4 B
+// Chunked tiers: one workgroup of B lanes per work unit of at most E B
 // entries.  unit_a = {first entry, len, -, graph}; unit_b = {slot, warm offset,
 // split point or NO_HUB, segment}.  Each lane gathers its chunk's terms
 // (q.x, q.y, D, K) once into registers (D the rest length, K = linkSpring *
@@ -501,7 +502,7 @@ __global__ void __launch_bounds__(64) k_hub_final(int pass, const uint2* __restr
 // {end, start used} and its end's Jacobian for the fix-up.  Round r > 0
 // reruns only the segments after the first of split points whose fix-up has
 // not converged.
-template <int B>
+template <int B, int E>
 __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restrict__ unit_a,
                                                   const uint4* __restrict__ unit_b, uint32_t u_first,
                                                   int round, const uint32_t* __restrict__ sched,
@@ -563,19 +564,19 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
     const float2* __restrict__ pin = bsel ? pos1 : pos0;
     float2* __restrict__ wslot = warm + V.y + t;
     const uint32_t len = U.y;
-    const uint32_t c = (len + B - 1) / B;  // <= 4 by construction
+    const uint32_t c = (len + B - 1) / B;  // <= E by construction
     const uint32_t lo = min(len, (uint32_t)t * c), hi = min(len, lo + c);
     const uint32_t steps = hi - lo;
     const int last = (int)((len - 1) / c);
     const float2 pp = pin[p];
     // the lane's entries [lo, hi) of the unit, gathered once
-    float4 reg[4];
+    float4 reg[E];
     {
-        uint4 en[4];
+        uint4 en[E];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) en[s] = ent[U.x + min(lo + s, len - 1)];
+        for (int s = 0; s < E; ++s) en[s] = ent[U.x + min(lo + s, len - 1)];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
+        for (int s = 0; s < E; ++s) {
             const float2 q = pin[en[s].x];
             const float r = __uint_as_float(en[s].y), sv = __uint_as_float(en[s].z),
                         bv = __uint_as_float(en[s].w);
@@ -603,7 +604,7 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
         v = g;
         J[0] = 1.f; J[1] = 0.f; J[2] = 0.f; J[3] = 1.f;
 #pragma unroll
-        for (int s = 0; s < 4; ++s) spring_step(reg[s], pp, v, J, (uint32_t)s < steps);
+        for (int s = 0; s < E; ++s) spring_step(reg[s], pp, v, J, (uint32_t)s < steps);
         // chunk map  s -> J (s - g) + v;  exclusive prefix applied to the unit's start
         const Aff m{J[0], J[1], J[2], J[3], v.x - (J[0] * g.x + J[2] * g.y), v.y - (J[1] * g.x + J[3] * g.y)};
         const Aff ex = block_scan<B>(m, s_wave, s_wave_inc, tot);
@@ -901,7 +902,7 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         CG_HIP(c, hipStreamSynchronize(c->stream));
         // A 64-lane point is one unit; a 256-lane point of d entries is
         // ceil(d / HUB_SEG) near-equal segments, one unit each, in chain order.
-        // Every unit holds at most 4 entries per lane.
+        // Every unit holds at most 4 (64 lanes) or HUB_E (256 lanes) entries per lane.
         std::vector<uint4> ua, ub;
         std::vector<uint32_t> slot_u0(nch + 1, 0), slot_graph(nch ? nch : 1, 0), slot_lanes(nch ? nch : 1, 0);
         std::vector<uint2> hh;
@@ -924,9 +925,9 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
             uint32_t eb = hs[k];
             for (uint32_t q = 0; q < S; ++q) {
                 const uint32_t len = dd / S + (q < dd % S ? 1u : 0u);
-                if (len == 0 || len > 4 * B) {
+                if (len == 0 || len > (B == 256u ? (uint32_t)HUB_E : 4u) * B) {
                     cleanup();
-                    return fail(c, CG_ERR_INVALID, "link schedule: a work unit outside 1..4 entries per lane");
+                    return fail(c, CG_ERR_INVALID, "link schedule: a work unit outside its tier's entries per lane");
                 }
                 ua.push_back(make_uint4(eb, len, 0u, gs));
                 ub.push_back(make_uint4(L.n_lane + k, 0u, hub_id, q));
@@ -1012,7 +1013,7 @@ int link_run(Ctx* c, int dir, float2* vout) {
     // sweep counters only while requested (same-address atomics per block)
     uint32_t* stats = (c->prof_mask & CG_PROFILE_LINK_STATS) ? c->link_stats : nullptr;
     if (L.n_units > us)
-        hipLaunchKernelGGL(k_link_chunk<256>, dim3(L.n_units - us), dim3(256), 0, c->stream, pass, L.unit_a,
+        hipLaunchKernelGGL((k_link_chunk<256, HUB_E>), dim3(L.n_units - us), dim3(256), 0, c->stream, pass, L.unit_a,
                            L.unit_b, us, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm,
                            L.wjac, L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
     const bool fork = L.n_hubs && c->side;
@@ -1024,7 +1025,7 @@ int link_run(Ctx* c, int dir, float2* vout) {
             s = c->side;
         }
         for (int r = 1; r < HUB_ROUNDS; ++r)
-            hipLaunchKernelGGL(k_link_chunk<256>, dim3(L.n_units - us), dim3(256), 0, s, pass, L.unit_a, L.unit_b,
+            hipLaunchKernelGGL((k_link_chunk<256, HUB_E>), dim3(L.n_units - us), dim3(256), 0, s, pass, L.unit_a, L.unit_b,
                                us, r, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm, L.wjac,
                                L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
         hipLaunchKernelGGL(k_hub_final, dim3(L.n_hubs), dim3(64), 0, s, pass, L.hubs, L.n_hubs, L.unit_a,
@@ -1034,11 +1035,11 @@ int link_run(Ctx* c, int dir, float2* vout) {
         if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
     if (us > u256)
-        hipLaunchKernelGGL(k_link_chunk<256>, dim3(us - u256), dim3(256), 0, c->stream, pass, L.unit_a,
+        hipLaunchKernelGGL((k_link_chunk<256, HUB_E>), dim3(us - u256), dim3(256), 0, c->stream, pass, L.unit_a,
                            L.unit_b, u256, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm,
                            L.wjac, L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
     if (L.n_u64)
-        hipLaunchKernelGGL(k_link_chunk<64>, dim3(L.n_u64), dim3(64), 0, c->stream, pass, L.unit_a, L.unit_b,
+        hipLaunchKernelGGL((k_link_chunk<64, 4>), dim3(L.n_u64), dim3(64), 0, c->stream, pass, L.unit_a, L.unit_b,
                            0u, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm, L.wjac,
                            L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
     if (L.n_lane)

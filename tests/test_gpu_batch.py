@@ -26,7 +26,7 @@ def spec(kind: str, seed: int) -> dict:
         links = datagen.barabasi_albert(n, 2, seed)
         p = workloads.sim_params(S)
         sizes = None
-    elif kind == "hubs":  # chung-lu: lane, 64-lane and 1024-lane link tiers
+    elif kind == "hubs":  # chung-lu: lane, 64-lane and 256-lane link tiers
         n, S = 20000, 4096.0
         links = datagen.chung_lu(n, 2.1, 1.0, 5 * n, seed)
         p = workloads.sim_params(S, center=0.3)

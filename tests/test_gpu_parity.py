@@ -180,7 +180,7 @@ def test_link_forces_vs_serial_oracle(case):
 
 def test_link_forces_after_ticks_vs_oracle():
     """Link forces from a live state, 40 ticks into a layout (warm starts in
-    play; the points above 1024 entries per direction, up to 41,950, are split
+    play; the points above 1536 entries per direction, up to 41,950, are split
     into segments and fixed up), against the serial oracle for every point."""
     S, n = 8192.0, 200_000
     links = datagen.chung_lu(n, 2.1, 1.0, 1_000_000, 7)
