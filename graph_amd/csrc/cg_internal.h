@@ -33,7 +33,7 @@ constexpr uint32_t CELL_BLOCK = 256;  // each graph's finest cells start on a mu
 // point above HUB_SEG entries is cut into segments of at most HUB_SEG.  (Six
 // per lane measured best: fewer, longer segments need fewer workgroups and
 // fix-up terms; longer chunks cost more per sweep.)
-constexpr uint32_t LANE_MAX = 32;
+constexpr uint32_t LANE_MAX = 64;  // 24, 32, 48, 96, 128 measured slower at cfg1
 constexpr uint32_t LANE_SPLIT = 8;  // lane-tier schedule classes: <= 2, <= LANE_SPLIT entries, then the rest
 constexpr uint32_t T64_MAX = 256;
 constexpr int HUB_E = 6;
