@@ -1004,18 +1004,14 @@ int link_run(Ctx* c, int dir, float2* vout) {
     if (!c->any_links) return CG_OK;
     const LinkDir& L = c->dir[dir];
     const int pass = P_LINK_IN + dir;
-    // The split points' first round runs alone on the main stream; their later
-    // rounds (each opening with the fix-up of the one before) and the final
-    // fix-up run on the side stream while the whole 256-lane points, the
-    // 64-lane and the lane tiers fill the GPU on the main stream, which joins
-    // the side stream at the end.  Rounds after a point holds return at once.
+    // The split points' rounds (each after the first opening with the fix-up
+    // of the one before) and the final fix-up run on the side stream while the
+    // whole 256-lane points, the 64-lane and the lane tiers fill the GPU on
+    // the main stream, which joins the side stream at the end.  Rounds after a
+    // point holds return at once.
     const uint32_t u256 = L.n_u64, us = L.first_split_unit;
     // sweep counters only while requested (same-address atomics per block)
     uint32_t* stats = (c->prof_mask & CG_PROFILE_LINK_STATS) ? c->link_stats : nullptr;
-    if (L.n_units > us)
-        hipLaunchKernelGGL((k_link_chunk<256, HUB_E>), dim3(L.n_units - us), dim3(256), 0, c->stream, pass, L.unit_a,
-                           L.unit_b, us, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm,
-                           L.wjac, L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
     const bool fork = L.n_hubs && c->side;
     if (L.n_hubs) {
         hipStream_t s = c->stream;
@@ -1024,6 +1020,9 @@ int link_run(Ctx* c, int dir, float2* vout) {
             CG_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
             s = c->side;
         }
+        hipLaunchKernelGGL((k_link_chunk<256, HUB_E>), dim3(L.n_units - us), dim3(256), 0, s, pass, L.unit_a,
+                           L.unit_b, us, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm,
+                           L.wjac, L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
         for (int r = 1; r < HUB_ROUNDS; ++r)
             hipLaunchKernelGGL((k_link_chunk<256, HUB_E>), dim3(L.n_units - us), dim3(256), 0, s, pass, L.unit_a, L.unit_b,
                                us, r, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm, L.wjac,
