@@ -272,7 +272,8 @@ __global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __r
     const float2* __restrict__ pin = bsel ? pos1 : pos0;
     const float2 pp = pin[p];
     const float2 v = mx <= 2 ? lane_chain<2>(d, ent, b, e, pin, pp)
-                             : (mx <= 4 ? lane_chain<4>(d, ent, b, e, pin, pp) : lane_chain<8>(d, ent, b, e, pin, pp));
+                             : (mx <= 4 ? lane_chain<4>(d, ent, b, e, pin, pp)
+                                        : (mx <= 16 ? lane_chain<8>(d, ent, b, e, pin, pp) : lane_chain<16>(d, ent, b, e, pin, pp)));
     if (vout) {
         vout[p] = v;
         return;
