@@ -403,12 +403,14 @@ __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ blk_graph,
                                                       const GDesc* __restrict__ desc) {
     const uint32_t f = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    // most cells are empty: their lanes leave after this one load (alignment
+    // padding cells are never filled, so they read 0 as well)
+    const uint32_t e = end_[f];
+    if (e == 0) return;
     const GDesc& d = desc[blk_graph[blockIdx.x]];
     if (!(d.passes & (1u << P_MANYBODY))) return;
     const uint32_t mc = f - d.f0;
     if (mc >= d.gf * d.gf) return;  // alignment padding
-    const uint32_t e = end_[f];
-    if (e == 0) return;
     const uint32_t b = begin[f], n = e - b;
     const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
     if (n > BIG_CELL && n <= MID_SMALL) {
