@@ -948,7 +948,9 @@ constexpr int MB_TILE = 1024;  // staged cells per level (16 KB)
 // small and every lane's reads hit it); other blocks, and windows above
 // MB_TILE cells, read the level from global memory.  Either way the same
 // values are summed in the same order.
-__global__ void __launch_bounds__(256) k_mb_force(const uint32_t* __restrict__ sv,
+// Eight waves per SIMD: the compiler fits the kernel in 64 VGPRs without spills
+// (88 unbounded, five waves).
+__global__ void __launch_bounds__(256, 8) k_mb_force(const uint32_t* __restrict__ sv,
                                                   const float2* __restrict__ sorted_pos,
                                                   const float4* __restrict__ cent,
                                                   const int32_t* __restrict__ slot_idx,
