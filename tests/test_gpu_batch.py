@@ -115,6 +115,24 @@ def test_alone_equals_continuous_mix():
         assert np.isnan(ref.reshape(-1, 2)[~present]).all()
 
 
+def test_graph_without_present_points_in_a_mix():
+    """A graph whose points are all absent (no sort segment, no link units)
+    between two others: they still match their runs alone bit for bit, and it
+    stays all NaN."""
+    a, b = spec("ba", 11), spec("hubs", 12)
+    empty = spec("ba", 13)
+    empty["xy"] = np.full_like(empty["xy"], np.nan)
+    with _lib.Context() as ctx:
+        ga, ge, gb = load(ctx, a), load(ctx, empty), load(ctx, b)
+        ctx.step_graphs([ga, ge, gb], 20)
+        xa, xe, xb = (ctx.graph_get_positions(g) for g in (ga, ge, gb))
+    assert np.isnan(xe).all()
+    for s, x in ((a, xa), (b, xb)):
+        same = np.array_equal(bits(alone(s, 20)), bits(x))
+        say(f"{s['kind']}: bit-identical={same}")
+        assert same
+
+
 def test_unstepped_graphs_keep_state_and_alpha():
     a, b = spec("ba", 7), spec("nolinks", 8)
     with _lib.Context() as ctx:
