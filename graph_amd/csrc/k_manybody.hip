@@ -907,7 +907,8 @@ __device__ __forceinline__ float2 near_field(const float4* fin, int x0, int y0, 
                 }
             }
             if (sampled > 0.f) {
-                float hw = others / sampled;
+                // sampled is 1..8: a reciprocal (1 ulp) instead of an IEEE divide
+                float hw = others * __builtin_amdgcn_rcpf(sampled);
                 wx += hw * sxp;
                 wy += hw * syp;
             }
