@@ -12,14 +12,14 @@
 //   - up to T64_MAX entries: one 64-lane workgroup per point; above: 256
 //     lanes.  The chain is cut into one chunk per lane, at most 4 entries
 //     (64 lanes) or HUB_E (256 lanes), whose terms the lane gathers once into
-//     registers; each
-//     sweep runs every chunk from a guessed start velocity, carrying the
-//     chunk's end velocity and 2x2 Jacobian, and an affine prefix scan over
-//     the chunks yields the next guesses (multiple shooting with
-//     Newton-accurate chunk maps).  Sweeps repeat until no start moves more
-//     than the tolerance; the last chunk's end, moved to its corrected start,
-//     is the point's velocity.  Converged starts are kept per point as the
-//     next call's first guesses.
+//     registers; each sweep runs every chunk from a guessed start velocity,
+//     carrying the chunk's end velocity and 2x2 Jacobian, and an affine
+//     prefix scan over the chunks yields the next guesses (multiple shooting
+//     with Newton-accurate chunk maps).  Sweeps repeat until no start moves
+//     more than FIX_ACCEPT tolerances (or the correction stalls near the
+//     tolerance, or MAX_SWEEPS); the last chunk's end, moved to its corrected
+//     start, is the point's velocity.  Converged starts are kept per point as
+//     the next call's first guesses.
 //   - a point above HUB_SEG entries is cut into segments, one 256-lane
 //     workgroup each (one CU no longer carries a whole hub): each segment is
 //     solved as above from a guessed start and reports its end and end
