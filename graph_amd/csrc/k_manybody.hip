@@ -1,12 +1,14 @@
 // k_manybody.hip — grid-pyramid many-body repulsion with the Monte-Carlo
 // near field, as one deterministic pipeline per tick:
 //
-//   1. LSD radix sort of the present points by finest cell, visiting them in
-//      the previous tick's cell order; cell bounds from the sorted keys
+//   1. LSD radix sort of the present points by finest cell (segmented per
+//      graph in a packed set), visiting them in the previous tick's cell
+//      order, positions riding along; cell bounds from the sorted keys
 //   2. per finest cell: fixed-point sums of (x, y), count, and the top-8
-//      near-field slots by (ticket, index) — one lane per cell, one wave per
-//      cell above BIG_CELL points (beside the crowded tier, on the side
-//      stream), chunked blocks plus a merge above BIG_CHUNK points
+//      near-field slots by (ticket, index) — one lane per cell, 16 lanes per
+//      cell above BIG_CELL points and a wave above MID_SMALL (both beside the
+//      crowded tier, on the side stream), chunked blocks plus a merge above
+//      BIG_CHUNK points
 //   3. 2x2 merges up the pyramid in the same fixed point, f32 centroids
 //      (cx, cy, N) per level
 //   4. per point (in cell order): far field over every level + near field,
