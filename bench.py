@@ -284,19 +284,11 @@ def main() -> int:
     p = wl["params"]
     seeds = datagen.uniform(args.warmup + args.steps + 1, 0.0, 1.0, 99)
 
-    # warmup; its second half with every stage bracketed (the first ticks start
-    # from index order and cold link guesses): per-stage breakdown, picks the
-    # dominant kernel
-    w0 = args.warmup // 2
-    ctx.step(p, w0, seeds=seeds[:w0])
-    ctx.set_profiling(_lib.STAGES)
-    ctx.step(p, args.warmup - w0, seeds=seeds[w0: args.warmup])
-    breakdown = ctx.get_profile()
-    single = {s: v for s, v in breakdown.items() if s in SINGLE_KERNEL}
-    dominant = max(single, key=lambda s: single[s][0]) if single else "mb_force"
+    # warmup: the production path, so the tick graphs are captured here (8 ticks
+    # per graph from 8 warm-up ticks on), not in the timed region
+    ctx.step(p, args.warmup, seeds=seeds[: args.warmup])
 
     # timed region: the production path (tick sequences replayed from captured graphs)
-    ctx.set_profiling([])
     ctx.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -306,6 +298,14 @@ def main() -> int:
     t1 = time.perf_counter()
     el = t1 - t0
     ticks_per_s = args.steps / el
+    # after the timed region: 10 ticks with every stage bracketed (per-stage
+    # breakdown; picks the dominant kernel)
+    ctx.set_profiling(_lib.STAGES)
+    ctx.step(p, 10, seeds=datagen.uniform(10, 0.0, 1.0, 97))
+    breakdown = ctx.get_profile()
+    ctx.set_profiling([])
+    single = {s: v for s, v in breakdown.items() if s in SINGLE_KERNEL}
+    dominant = max(single, key=lambda s: single[s][0]) if single else "mb_force"
     # the dominant kernel's duration: HIP events on the library stream around each of
     # its launches over an equal run of ticks right after (events force direct launches)
     prof_seeds = datagen.uniform(args.steps, 0.0, 1.0, 98)
