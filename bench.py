@@ -284,8 +284,9 @@ def main() -> int:
     p = wl["params"]
     seeds = datagen.uniform(args.warmup + args.steps + 1, 0.0, 1.0, 99)
 
-    # warmup: the production path, so the tick graphs are captured here (8 ticks
-    # per graph from 8 warm-up ticks on), not in the timed region
+    # capture and instantiate the tick graphs the timed region replays (runs no
+    # tick), then the warm-up ticks on the production path
+    ctx.prepare(p)
     ctx.step(p, args.warmup, seeds=seeds[: args.warmup])
 
     # timed region: the production path (tick sequences replayed from captured graphs)

@@ -96,6 +96,8 @@ _SIGNATURES = {
     "cg_eval_force": (C.c_int, [_CTX, C.POINTER(CgSimParams), C.c_int32, C.c_float, C.c_float, _F]),
     "cg_debug_slots": (C.c_int, [_CTX, _I, C.c_uint32, _U, _U]),
     "cg_debug_link_csr": (C.c_int, [_CTX, C.c_int32, _U, _U, _F, _F, _U, C.c_uint32, _U]),
+    "cg_debug_cells": (C.c_int, [_CTX, C.c_uint32, _U, _F, _U, C.c_uint32]),
+    "cg_debug_collision_keys": (C.c_int, [_CTX, _U, _U, C.c_uint32, _U, _U]),
     "cg_device_bytes": (C.c_uint64, [_CTX]),
     "cg_set_profiling": (C.c_int, [_CTX, C.c_uint32]),
     "cg_debug_link_stats": (C.c_int, [_CTX, _U]),
@@ -114,6 +116,7 @@ _SIGNATURES = {
     "cg_graph_get_positions": (C.c_int, [_CTX, C.c_uint32, _F, C.c_uint32]),
     "cg_step_graphs": (C.c_int, [_CTX, _U, C.c_uint32, C.c_uint32]),
     "cg_sync": (C.c_int, [_CTX]),
+    "cg_prepare": (C.c_int, [_CTX, C.POINTER(CgSimParams)]),
     "cg_set_capture": (C.c_int, [_CTX, C.c_int32]),
     "cg_drag_points": (C.c_int, [_CTX, _U, _F, C.c_uint32]),
     "cg_get_positions_at": (C.c_int, [_CTX, _U, C.c_uint32, _F]),
@@ -288,6 +291,10 @@ class Context:
         self._check(self._l.cg_step(self._ctx, C.byref(params), nticks, fptr(s), dptr(tr)))
         return tr
 
+    def prepare(self, params: CgSimParams) -> None:
+        """Packs graph 0 and captures the tick graphs cg_step replays, running no tick."""
+        self._check(self._l.cg_prepare(self._ctx, C.byref(params)))
+
     def get_positions(self) -> np.ndarray:
         out = np.empty(2 * self.n, dtype=np.float32)
         self._check(self._l.cg_get_positions(self._ctx, fptr(out), self.n))
@@ -323,6 +330,31 @@ class Context:
         out = np.empty(gf.value * gf.value * 8, dtype=np.int32)
         self._check(self._l.cg_debug_slots(self._ctx, iptr(out), out.size, C.byref(gf), C.byref(L)))
         return out, gf.value, L.value
+
+    def debug_cells(self, level: int, members: bool = False):
+        """(counts[g*g], centroids[g*g, 2], members or None) of pyramid level
+        `level` from the last many-body pass; members (finest level only) are
+        the present points grouped by row-major cell."""
+        g = 4 << level
+        counts = np.empty(g * g, dtype=np.uint32)
+        cent = np.empty(2 * g * g, dtype=np.float32)
+        mem = np.empty(max(self.n, 1), dtype=np.uint32) if members else None
+        self._check(self._l.cg_debug_cells(self._ctx, level, uptr(counts), fptr(cent), uptr(mem),
+                                           0 if mem is None else mem.size))
+        if mem is not None:
+            mem = mem[: int(counts.sum(dtype=np.int64))]
+        return counts, cent.reshape(-1, 2), mem
+
+    def debug_collision_keys(self):
+        """(keys, points, G): the collision pass's sorted (grid * G^2 + cell,
+        point) pairs of graph 0."""
+        cnt, G = C.c_uint32(), C.c_uint32()
+        self._check(self._l.cg_debug_collision_keys(self._ctx, None, None, 0, C.byref(cnt), C.byref(G)))
+        k = np.empty(max(cnt.value, 1), dtype=np.uint32)
+        p = np.empty(max(cnt.value, 1), dtype=np.uint32)
+        self._check(self._l.cg_debug_collision_keys(self._ctx, uptr(k), uptr(p), k.size, C.byref(cnt),
+                                                    C.byref(G)))
+        return k[: cnt.value], p[: cnt.value], G.value
 
     def debug_link_csr(self, direction: int):
         E = C.c_uint32()

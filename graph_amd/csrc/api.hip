@@ -106,13 +106,13 @@ __global__ void __launch_bounds__(1024) k_desc_advance(const uint32_t* __restric
 
 __global__ void k_set_word(uint32_t* __restrict__ a, uint32_t v) { *a = v; }
 
-// drag-point.frag:28-31: listed points (graph-local indices below n) take new positions.
-__global__ void k_drag(float2* __restrict__ pos, uint32_t p0, uint32_t n, const uint8_t* __restrict__ flags,
-                       const uint32_t* __restrict__ idx, const float2* __restrict__ xy, uint32_t count) {
+// drag-point.frag:28-31: listed points (graph-local indices of present points,
+// filtered on the host) take new positions.
+__global__ void k_drag(float2* __restrict__ pos, uint32_t p0, const uint32_t* __restrict__ idx,
+                       const float2* __restrict__ xy, uint32_t count) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count || idx[k] >= n) return;
-    const uint32_t i = p0 + idx[k];
-    if (!(flags[i] & F_ABSENT)) pos[i] = xy[k];
+    if (k >= count) return;
+    pos[p0 + idx[k]] = xy[k];
 }
 
 // Tracked positions: NaN past the point count (absent points hold NaN already).
@@ -265,9 +265,75 @@ int upload(Ctx* c, T* dst, const std::vector<T>& src) {
     return CG_OK;
 }
 
+// What a repack replaces: the previous packing's device-resident state (the
+// position buffers and the link schedules with their warm starts), the
+// placement of every graph in it and the set's sizes.  A repack that fails
+// restores it, so positions stay readable and a later repack starts from them.
+struct PackSnap {
+    float2* pos[2] = {nullptr, nullptr};
+    LinkDir dir[2];
+    std::vector<uint32_t> order;
+    std::vector<GDesc> h_desc;
+    uint32_t G = 0, N = 0, NP = 0, F = 0, C = 0, CG = 0, NPC = 0;
+    struct Place {
+        uint32_t slot, p0, k0, f0, c0, col_c0, cur, w0[2], wn[2];
+    };
+    std::vector<Place> place;  // by graph id
+};
+
+PackSnap pack_snapshot(const Ctx* c) {
+    PackSnap s;
+    s.pos[0] = c->pos[0];
+    s.pos[1] = c->pos[1];
+    s.dir[0] = c->dir[0];
+    s.dir[1] = c->dir[1];
+    s.order = c->order;
+    s.h_desc = c->h_desc;
+    s.G = c->G; s.N = c->N; s.NP = c->NP; s.F = c->F; s.C = c->C; s.CG = c->CG; s.NPC = c->NPC;
+    for (const Graph& g : c->graphs)
+        s.place.push_back({g.slot, g.p0, g.k0, g.f0, g.c0, g.col_c0, g.cur, {g.w0[0], g.w0[1]},
+                           {g.wn[0], g.wn[1]}});
+    return s;
+}
+
+// Undoes a failed repack: frees everything it allocated and puts the previous
+// position buffers, link schedules and placement back.  The other packed
+// arrays (scratch and host-derived tables) stay freed; `dirty` stays set, so
+// the next step repacks from the restored state.
+void pack_rollback(Ctx* c, const PackSnap& s) {
+    (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    free_packed(c);
+    for (int b = 0; b < 2; ++b)
+        if (c->pos[b] != s.pos[b]) dfree(c, c->pos[b]);
+    for (int d = 0; d < 2; ++d)
+        if (c->dir[d].ent != s.dir[d].ent || c->dir[d].warm != s.dir[d].warm) link_free_dir(c, c->dir[d]);
+    c->pos[0] = s.pos[0];
+    c->pos[1] = s.pos[1];
+    c->dir[0] = s.dir[0];
+    c->dir[1] = s.dir[1];
+    c->order = s.order;
+    c->h_desc = s.h_desc;
+    c->G = s.G; c->N = s.N; c->NP = s.NP; c->F = s.F; c->C = s.C; c->CG = s.CG; c->NPC = s.NPC;
+    for (size_t id = 0; id < c->graphs.size() && id < s.place.size(); ++id) {
+        Graph& g = c->graphs[id];
+        const PackSnap::Place& q = s.place[id];
+        g.slot = q.slot; g.p0 = q.p0; g.k0 = q.k0; g.f0 = q.f0; g.c0 = q.c0; g.col_c0 = q.col_c0;
+        g.cur = q.cur;
+        for (int d = 0; d < 2; ++d) {
+            g.w0[d] = q.w0[d];
+            g.wn[d] = q.wn[d];
+        }
+    }
+    c->desc = nullptr;
+    c->desc_pending = false;
+    c->dirty = true;
+}
+
 // Lays every live graph out in fresh packed arrays (id order), carrying the
-// positions of graphs that were already packed over on the device.
-int pack(Ctx* c) {
+// positions of graphs that were already packed over on the device.  `old`
+// holds the previous packing; the caller frees or restores it.
+int pack_body(Ctx* c, const PackSnap& old) {
     CG_HIP(c, hipStreamSynchronize(c->stream));
     if (c->side) CG_HIP(c, hipStreamSynchronize(c->side));
     c->desc_pending = false;
@@ -317,10 +383,10 @@ int pack(Ctx* c) {
             st[s] = d;
         }
     }
-    float2* old_pos[2] = {c->pos[0], c->pos[1]};
+    float2* const* old_pos = old.pos;
     c->pos[0] = c->pos[1] = nullptr;
     free_packed(c);
-    LinkDir old_dirs[2] = {c->dir[0], c->dir[1]};
+    c->dir[0] = c->dir[1] = LinkDir();
     c->G = G;
     c->N = (uint32_t)N;
     c->NP = (uint32_t)NP;
@@ -491,10 +557,30 @@ int pack(Ctx* c) {
 
     rc = mb_alloc(c); if (rc) return rc;
     rc = col_alloc(c); if (rc) return rc;
-    rc = link_pack(c, old_dirs); if (rc) return rc;
+    rc = link_pack(c, old.dir); if (rc) return rc;
     CG_HIP(c, hipStreamSynchronize(c->stream));
-    dfree(c, old_pos[0]);
-    dfree(c, old_pos[1]);
+    return CG_OK;
+}
+
+// Repacks the graph set.  On success the previous packing is freed and the
+// host copies of newly uploaded positions are released; on failure (budget,
+// allocation, device error) the previous packing is restored and the error
+// returned.
+int pack(Ctx* c) {
+    PackSnap old = pack_snapshot(c);
+    const int rc = pack_body(c, old);
+    if (rc) {
+        const std::string msg = c->err;
+        pack_rollback(c, old);
+        c->err = msg;
+        return rc;
+    }
+    dfree(c, old.pos[0]);
+    dfree(c, old.pos[1]);
+    for (int d = 0; d < 2; ++d) {
+        LinkDir L = old.dir[d];
+        link_free_dir(c, L);
+    }
     for (uint32_t id : c->order) {
         Graph& g = c->graphs[id];
         if (g.xy_pending) {
@@ -507,7 +593,9 @@ int pack(Ctx* c) {
 }
 
 // The parameter-dependent part of a descriptor (alpha, seed and the force constants).
-void fill_dynamic(GDesc& d, const cg_sim_params& p, float a, float seed) {
+// `extent`: the largest |coordinate| the graph's present points may hold (the
+// integrate pass clamps to [0, S], but positions as set or dragged are not).
+void fill_dynamic(GDesc& d, const cg_sim_params& p, float extent, float a, float seed) {
     const float S = p.space_size;
     d.S = S;
     for (uint32_t l = 0; l < (uint32_t)MAX_LEVELS; ++l)
@@ -530,11 +618,13 @@ void fill_dynamic(GDesc& d, const cg_sim_params& p, float a, float seed) {
     d.col_radius = p.collision_radius;
     d.col_padding = p.collision_padding;
     d.cluster_k = p.cluster;
-    // fixed point of the cell sums: np * 16 S * 2^k < 2^63
+    // fixed point of the cell sums: np * 16 X * 2^k < 2^63 with X = max(S, extent),
+    // so no position a point can hold overflows a sum (k < 0 for huge extents)
+    const double X = std::max<double>(std::fabs((double)S), (double)extent);
     int bits_s = 0, bits_n = 0;
-    while (bits_s < 128 && std::ldexp(1.0, bits_s) < (double)S) ++bits_s;
+    while (bits_s < 160 && std::ldexp(1.0, bits_s) < X) ++bits_s;
     while (bits_n < 32 && (1ull << bits_n) < (unsigned long long)d.np + 1) ++bits_n;
-    const int k = std::max(0, std::min(40, 59 - bits_s - bits_n));
+    const int k = std::max(-100, std::min(40, 59 - bits_s - bits_n));
     d.fx_scale = std::ldexp(1.f, k);
     d.fx_inv = std::ldexp(1.f, -k);
 }
@@ -687,7 +777,7 @@ int run_ticks(Ctx* c, const std::vector<uint32_t>& ids, uint32_t nticks, int64_t
                 }
                 float seed = 0.f;
                 if (live) seed = ((int64_t)s == sel_slot && seeds) ? seeds[t0 + tt] : rng_unit(g);
-                fill_dynamic(d, p, (float)g.alpha, seed);
+                fill_dynamic(d, p, g.extent, (float)g.alpha, seed);
                 d.passes = passes;
                 d.inbuf = inbuf;
                 D[s] = d;
@@ -762,14 +852,19 @@ int graph_set_positions(Ctx* c, Graph& g, const float* xy, uint32_t n) {
     g.xy.assign(2 * (size_t)n, 0.f);
     g.flags.assign(n, 0);
     uint32_t np = 0;
+    float ext = 0.f;
     for (uint32_t i = 0; i < n; ++i) {
         float px = xy[2 * i], py = xy[2 * i + 1];
         bool absent = std::isnan(px) || std::isnan(py);
         g.xy[2 * i] = absent ? NAN : px;
         g.xy[2 * i + 1] = absent ? NAN : py;
         if (absent) g.flags[i] = F_ABSENT;
-        else ++np;
+        else {
+            ++np;
+            ext = std::max(ext, std::max(std::fabs(px), std::fabs(py)));
+        }
     }
+    g.extent = ext;
     for (uint32_t k : g.pinned)
         if (k < n) g.flags[k] |= F_PINNED;
     g.np = np;
@@ -887,21 +982,31 @@ int io_reserve(Ctx* c, uint32_t count) {
 
 int graph_drag_points(Ctx* c, Graph& g, const uint32_t* idx, const float* xy, uint32_t count) {
     if (count && (!idx || !xy)) return fail(c, CG_ERR_INVALID, "drag_points: null input");
-    if (count == 0) return CG_OK;
+    // only present points below the point count move (absent points stay absent)
+    std::vector<uint32_t> hi;
+    std::vector<float> hxy;
+    for (uint32_t k = 0; k < count; ++k) {
+        if (idx[k] >= g.n || (g.flags[idx[k]] & F_ABSENT)) continue;
+        hi.push_back(idx[k]);
+        hxy.push_back(xy[2 * k]);
+        hxy.push_back(xy[2 * k + 1]);
+        g.extent = std::max(g.extent, std::max(std::fabs(xy[2 * k]), std::fabs(xy[2 * k + 1])));
+    }
+    if (hi.empty()) return CG_OK;
     if (g.xy_pending) {  // not on the device yet: edit the pending copy
-        for (uint32_t k = 0; k < count; ++k)
-            if (idx[k] < g.n && !(g.flags[idx[k]] & F_ABSENT)) {
-                g.xy[2 * (size_t)idx[k]] = xy[2 * k];
-                g.xy[2 * (size_t)idx[k] + 1] = xy[2 * k + 1];
-            }
+        for (size_t k = 0; k < hi.size(); ++k) {
+            g.xy[2 * (size_t)hi[k]] = hxy[2 * k];
+            g.xy[2 * (size_t)hi[k] + 1] = hxy[2 * k + 1];
+        }
         return CG_OK;
     }
-    int rc = io_reserve(c, count);
+    const uint32_t cnt = (uint32_t)hi.size();
+    int rc = io_reserve(c, cnt);
     if (rc) return rc;
-    CG_HIP(c, hipMemcpyAsync(c->io_idx, idx, (size_t)count * 4, hipMemcpyHostToDevice, c->stream));
-    CG_HIP(c, hipMemcpyAsync(c->io_xy, xy, (size_t)count * 8, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_drag, dim3(blocks_for(count, 256)), dim3(256), 0, c->stream, c->pos[g.cur], g.p0, g.n,
-                       c->flags, c->io_idx, c->io_xy, count);
+    CG_HIP(c, hipMemcpyAsync(c->io_idx, hi.data(), (size_t)cnt * 4, hipMemcpyHostToDevice, c->stream));
+    CG_HIP(c, hipMemcpyAsync(c->io_xy, hxy.data(), (size_t)cnt * 8, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_drag, dim3(blocks_for(cnt, 256)), dim3(256), 0, c->stream, c->pos[g.cur], g.p0,
+                       c->io_idx, c->io_xy, cnt);
     CG_HIP(c, hipGetLastError());
     CG_HIP(c, hipStreamSynchronize(c->stream));
     return CG_OK;
@@ -1092,6 +1197,28 @@ int cg_step(cg_ctx* x, const cg_sim_params* p, uint32_t nticks, const float* see
     Graph& g = default_graph(c);
     set_params(c, g, *p);
     return run_ticks(c, {0u}, nticks, 0, seeds, alpha_trace);
+}
+
+int cg_prepare(cg_ctx* x, const cg_sim_params* p) {
+    if (!x || !p) return CG_ERR_INVALID;
+    Ctx* c = &x->c;
+    CG_HIP(c, hipSetDevice(c->device));
+    Graph& g = default_graph(c);
+    set_params(c, g, *p);
+    if (c->dirty) {
+        int rc = pack(c);
+        if (rc) return rc;
+    }
+    if (!c->capture || c->prof_mask || !p->enable_simulation) return CG_OK;
+    // the plan cg_step runs for graph 0 alone (the other graphs are not stepped)
+    const TickPlan tp{graph_passes(c->graphs[0])};
+    if (tp.any == 0) return CG_OK;
+    c->desc = c->desc_cur;
+    hipGraphExec_t e = nullptr;
+    int rc = tick_graph(c, tp, GRAPH_TICKS, &e);
+    if (!rc) rc = tick_graph(c, tp, 1, &e);
+    c->desc = c->desc_base;
+    return rc;
 }
 
 int cg_get_positions(cg_ctx* x, float* xy, uint32_t n) {
@@ -1287,7 +1414,7 @@ int cg_eval_force(cg_ctx* x, const cg_sim_params* p, int32_t force, float alpha,
     for (uint32_t s = 0; s < c->G; ++s) {
         const Graph& h = c->graphs[c->order[s]];
         GDesc d = c->h_desc[s];
-        fill_dynamic(d, h.params, alpha, seed);
+        fill_dynamic(d, h.params, h.extent, alpha, seed);
         d.passes = (s == g.slot) ? (1u << pass) : 0u;
         d.inbuf = h.cur ? ((1u << P_COUNT) - 1) : 0u;
         table[s] = d;
@@ -1340,6 +1467,87 @@ int cg_debug_slots(cg_ctx* x, int32_t* slots, uint32_t capacity, uint32_t* fines
             v = (empty || v < 0) ? -1 : v - (int32_t)g.p0;
         }
     }
+    return CG_OK;
+}
+
+int cg_debug_cells(cg_ctx* x, uint32_t level, uint32_t* counts, float* centroid, uint32_t* members,
+                   uint32_t capacity) {
+    CG_CTX(x);
+    if (c->graphs.empty() || !c->graphs[0].live) return fail(c, CG_ERR_STATE, "cg_debug_cells: no graph");
+    const Graph& g = c->graphs[0];
+    if (c->dirty || !c->cent || g.np == 0) return fail(c, CG_ERR_STATE, "cg_debug_cells: no many-body pass yet");
+    if (level >= g.L) return fail(c, CG_ERR_INVALID, "cg_debug_cells: level out of range");
+    const uint32_t gl = 4u << level;
+    const size_t cells = (size_t)gl * gl;
+    std::vector<float4> cent(cells);
+    CG_HIP(c, hipMemcpyAsync(cent.data(), c->cent + g.c0 + level_off(level), cells * sizeof(float4),
+                             hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> b, e, sv;
+    const bool finest = level == g.L - 1;
+    if (members) {
+        if (!finest) return fail(c, CG_ERR_INVALID, "cg_debug_cells: members exist for the finest level only");
+        if (capacity < g.np) return fail(c, CG_ERR_INVALID, "cg_debug_cells: capacity");
+        const size_t F = (size_t)g.gf * g.gf;
+        b.resize(F);
+        e.resize(F);
+        sv.resize(c->NP);
+        CG_HIP(c, hipMemcpyAsync(b.data(), c->cell_begin + g.f0, F * 4, hipMemcpyDeviceToHost, c->stream));
+        CG_HIP(c, hipMemcpyAsync(e.data(), c->cell_end + g.f0, F * 4, hipMemcpyDeviceToHost, c->stream));
+        CG_HIP(c, hipMemcpyAsync(sv.data(), c->sorted, (size_t)c->NP * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t rm = 0; rm < cells; ++rm) {
+        const float4 q = cent[rm];
+        if (counts) counts[rm] = (uint32_t)q.z;
+        if (centroid) {
+            centroid[2 * rm] = q.z > 0.f ? q.x : 0.f;
+            centroid[2 * rm + 1] = q.z > 0.f ? q.y : 0.f;
+        }
+    }
+    if (members) {
+        // row-major cell order; the sorted array holds each finest cell's
+        // points contiguously (Morton cell order, [begin, end) per cell)
+        size_t w = 0;
+        for (size_t rm = 0; rm < cells; ++rm) {
+            const uint32_t m = morton2((uint32_t)(rm % gl), (uint32_t)(rm / gl));
+            if (e[m] == 0) continue;
+            for (uint32_t k = b[m]; k < e[m]; ++k) {
+                if (w >= g.np || sv[k] < g.p0 || sv[k] >= g.p0 + g.n)
+                    return fail(c, CG_ERR_STATE, "cg_debug_cells: a cell holds a point of another graph");
+                members[w++] = sv[k] - g.p0;
+            }
+        }
+        if (w != g.np) return fail(c, CG_ERR_STATE, "cg_debug_cells: cells do not cover every present point");
+    }
+    return CG_OK;
+}
+
+int cg_debug_collision_keys(cg_ctx* x, uint32_t* keys, uint32_t* points, uint32_t capacity, uint32_t* count,
+                            uint32_t* grid) {
+    CG_CTX(x);
+    if (c->graphs.empty() || !c->graphs[0].live) return fail(c, CG_ERR_STATE, "cg_debug_collision_keys: no graph");
+    const Graph& g = c->graphs[0];
+    if (c->dirty || !collides(g) || !c->col_keys[0])
+        return fail(c, CG_ERR_STATE, "cg_debug_collision_keys: no collision pass yet");
+    if (grid) *grid = g.col_G;
+    const uint32_t nk = 4 * c->NPC;
+    const int fin = c->col_rs.passes & 1;
+    std::vector<uint32_t> hk(nk), hv(nk);
+    CG_HIP(c, hipMemcpyAsync(hk.data(), c->col_keys[fin], (size_t)nk * 4, hipMemcpyDeviceToHost, c->stream));
+    CG_HIP(c, hipMemcpyAsync(hv.data(), c->col_vals[fin], (size_t)nk * 4, hipMemcpyDeviceToHost, c->stream));
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    const uint32_t lo = g.col_c0, hi = g.col_c0 + 4 * g.col_G * g.col_G;
+    uint32_t w = 0;
+    for (uint32_t k = 0; k < nk; ++k) {
+        if (hk[k] < lo || hk[k] >= hi) continue;
+        if (w < capacity) {
+            if (keys) keys[w] = hk[k] - lo;
+            if (points) points[w] = hv[k] - g.p0;
+        }
+        ++w;
+    }
+    if (count) *count = w;
+    if ((keys || points) && w > capacity) return fail(c, CG_ERR_INVALID, "cg_debug_collision_keys: capacity");
     return CG_OK;
 }
 

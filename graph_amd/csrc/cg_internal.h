@@ -105,6 +105,7 @@ struct Graph {
     // host copies
     std::vector<float> xy;                   // positions as set (absent = NaN), until packed
     bool xy_pending = false;
+    float extent = 0.f;                      // largest |coordinate| ever set or dragged (present points)
     std::vector<uint8_t> flags;
     std::vector<uint32_t> pinned;
     std::vector<float> jitter;
@@ -318,7 +319,8 @@ void pyramid_work_host(uint32_t slot, uint32_t gf, uint32_t L, std::vector<uint4
                        std::vector<uint4>& w1);
 int link_build_graph(Ctx* c, Graph& g);                           // k_link.hip
 void link_free_graph(Ctx* c, Graph& g);
-int link_pack(Ctx* c, LinkDir (&old)[2]);
+int link_pack(Ctx* c, const LinkDir (&old)[2]);
+void link_free_dir(Ctx* c, LinkDir& L);
 int link_run(Ctx* c, int dir, float2* vout);
 int col_prepare_graph(Ctx* c, Graph& g);                          // k_collision.hip
 int col_run(Ctx* c, float2* vout);

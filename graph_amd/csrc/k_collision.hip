@@ -9,7 +9,7 @@
 // aggregates of all four grids per point and integrates.
 //
 // Reference: ForceCollision/index.ts:61-164 (grid geometry, sizes),
-// :264-345 (run); build-grid.vert:28-62; force-collision-spatial.frag:103-230.
+// :264-345 (run); build-grid.vert:28-62; force-collision-spatial.frag:41-168.
 #include <algorithm>
 #include <cmath>
 

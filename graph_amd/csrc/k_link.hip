@@ -802,12 +802,16 @@ int link_build_graph(Ctx* c, Graph& gr) {
 // regions are laid out graph by graph, each graph's chunked points in index
 // order, so their layout depends only on the graph; a graph whose links did
 // not change carries its region over from the previous packing (old).
-int link_pack(Ctx* c, LinkDir (&old)[2]) {
-    auto free_dir = [&](LinkDir& L) {
-        dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff); dfree(c, L.unit_a); dfree(c, L.unit_b);
-        dfree(c, L.hubs); dfree(c, L.seg_out); dfree(c, L.hub_done); dfree(c, L.warm); dfree(c, L.wjac);
-        L = LinkDir();
-    };
+void link_free_dir(Ctx* c, LinkDir& L) {
+    dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff); dfree(c, L.unit_a); dfree(c, L.unit_b);
+    dfree(c, L.hubs); dfree(c, L.seg_out); dfree(c, L.hub_done); dfree(c, L.warm); dfree(c, L.wjac);
+    L = LinkDir();
+}
+
+// Builds the packed link schedules into c->dir, reading the warm starts of
+// `old` (the previous packing's schedules, which the caller frees once the
+// whole repack has succeeded, or keeps if it fails).
+int link_pack(Ctx* c, const LinkDir (&old)[2]) {
     for (int d = 0; d < 2; ++d) c->dir[d] = LinkDir();
     c->any_links = false;
     const uint32_t G = c->G, NP = c->NP;
@@ -815,10 +819,7 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         const Graph& g = c->graphs[c->order[s]];
         if (g.links_ready && g.m > 0) c->any_links = true;
     }
-    if (!c->any_links || NP == 0) {
-        for (int d = 0; d < 2; ++d) free_dir(old[d]);
-        return CG_OK;
-    }
+    if (!c->any_links || NP == 0) return CG_OK;
     int rc = CG_OK;
     std::vector<void*> tmp;
     auto talloc = [&](size_t bytes) -> void* {
@@ -992,7 +993,6 @@ int link_pack(Ctx* c, LinkDir (&old)[2]) {
         // keep the staging vectors alive until the copies above have run
         CG_HIP(c, hipStreamSynchronize(c->stream));
     }
-    for (int d = 0; d < 2; ++d) free_dir(old[d]);
     if (!c->link_stats && !rc) {
         c->link_stats = (uint32_t*)dalloc(c, 64, &rc);
         if (!rc) CG_HIP(c, hipMemsetAsync(c->link_stats, 0, 64, c->stream));

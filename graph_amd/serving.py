@@ -83,6 +83,12 @@ def footprint_bytes(n: int, m: int, collision: bool) -> int:
     return int(1.3 * (per_point * n + per_cell * cells + per_link * m + col)) + (1 << 20)
 
 
+def collides(p: _lib.CgSimParams) -> bool:
+    """Whether the collision pass runs for these parameters: simulationCollision
+    is truthy (src/index.ts:2066), as the library decides it."""
+    return p.collision != 0 and not math.isnan(p.collision)
+
+
 def alpha_ticks(decay: float, alpha0: float = 1.0) -> int:
     """Ticks until alpha < ALPHA_MIN under Store.addAlpha (Store/index.ts:328-332)."""
     f = 1.0 - math.pow(ALPHA_MIN, 1.0 / decay)
@@ -126,7 +132,7 @@ class BatchServer:
         active: dict[int, tuple[Job, int, int, float, int]] = {}  # gid -> job, left, total, start, bytes
         held = 0
         results: list[Result] = []
-        stats = dict(segments=0, ticks=0, max_active=0, point_ticks=0, admitted_late=0)
+        stats = dict(segments=0, ticks=0, max_active=0, point_ticks=0, rejected=[])
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         while pending or active:
@@ -134,8 +140,15 @@ class BatchServer:
             # admit every arrived job that fits, in arrival order (no overtaking)
             while pending and pending[0].arrival <= t:
                 job = pending[0]
-                need = footprint_bytes(job.n, job.m, job.sizes is not None)
-                if active and held + need > self.budget:
+                need = footprint_bytes(job.n, job.m, collides(job.params))
+                if need > self.budget:
+                    # could never fit, even alone: refused, not run
+                    pending.pop(0)
+                    stats["rejected"].append(job.jid)
+                    if log:
+                        log(f"job {job.jid} n={job.n} refused: footprint {need} B exceeds the budget")
+                    continue
+                if held + need > self.budget:
                     break
                 pending.pop(0)
                 gid = ctx.graph_add()

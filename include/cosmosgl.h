@@ -137,6 +137,11 @@ int cg_get_alpha(cg_ctx* ctx, double* alpha);
 int cg_step(cg_ctx* ctx, const cg_sim_params* params, uint32_t nticks,
             const float* seeds, double* alpha_trace);
 
+/* Packs graph 0 with these parameters and captures the HIP graphs cg_step
+ * will replay for them, without running a tick (no reference counterpart:
+ * it moves capture and instantiation out of a caller's first timed steps). */
+int cg_prepare(cg_ctx* ctx, const cg_sim_params* params);
+
 /* Copies n positions to xy (2n floats); absent points read back NaN. */
 int cg_get_positions(cg_ctx* ctx, float* xy, uint32_t n);
 
@@ -177,6 +182,24 @@ int cg_eval_force(cg_ctx* ctx, const cg_sim_params* params, int32_t force, float
  * receive the pyramid geometry. */
 int cg_debug_slots(cg_ctx* ctx, int32_t* slots, uint32_t capacity, uint32_t* finest,
                    uint32_t* levels);
+
+/* Level aggregation of the last many-body pass (calculate-level.vert:28-53 per
+ * level, the level cells of force-level.frag:66).  level in [0, L): the grid
+ * has g = 4 << level cells per side.  counts: g*g present-point counts per cell
+ * (row-major cells); centroid: NULL or 2*g*g floats, each non-empty cell's mean
+ * position (0, 0 for empty cells).  members: NULL, or at the finest level
+ * (level L-1) np graph-local point indices grouped by finest cell in row-major
+ * cell order (within a cell the order is unspecified); capacity >= np. */
+int cg_debug_cells(cg_ctx* ctx, uint32_t level, uint32_t* counts, float* centroid, uint32_t* members,
+                   uint32_t capacity);
+
+/* Spatial-hash keys of the last collision pass (build-grid.vert:46-55), in the
+ * order the device sorted them: keys[k] = o * G*G + row-major cell of offset
+ * grid o (offsets (0,0), (0.5,0), (0,0.5), (0.5,0.5) of a cell) and points[k]
+ * the graph-local point; 4 * np pairs, ascending by (key, point).  grid
+ * receives G (cells per side).  count receives the number of pairs. */
+int cg_debug_collision_keys(cg_ctx* ctx, uint32_t* keys, uint32_t* points, uint32_t capacity,
+                            uint32_t* count, uint32_t* grid);
 
 /* Link CSR of one direction (0 = INCOMING instance, 1 = OUTGOING) in the
  * reference's entry order: offsets[n+1], nbr[E], bias[E], strength[E], and

@@ -77,6 +77,20 @@ int32_t or_cell(float p, float cell_size, int32_t g) {
     return (int32_t)f;
 }
 
+/* Both axes' cells of n points at one grid — calculate-level.vert:47-48 (and
+ * build-grid.vert:50-55 with an offset of ox, oy cells added to the position in
+ * f32 first): cells[2i] = cx, cells[2i+1] = cy; absent points get -1. */
+void or_cells(uint32_t n, const float* pos, const uint8_t* absent, float cell_size, int32_t g, float ox,
+              float oy, int32_t* cells) {
+    volatile float offx = ox * cell_size, offy = oy * cell_size;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (absent && absent[i]) { cells[2 * i] = -1; cells[2 * i + 1] = -1; continue; }
+        volatile float qx = pos[2 * i] + offx, qy = pos[2 * i + 1] + offy;
+        cells[2 * i] = or_cell(qx, cell_size, g);
+        cells[2 * i + 1] = or_cell(qy, cell_size, g);
+    }
+}
+
 /* f32 cell size the reference uploads: adjustedSpaceSize / gridSize computed in
  * double and rounded to the f32 uniform (ForceManyBody/index.ts:483, 531, 583). */
 float or_cell_size(float space_size, uint32_t g) {
@@ -435,7 +449,7 @@ void or_collision_grid(float space_size, double radius, double padding, double m
 }
 
 /* collision — ForceCollision/index.ts:264-345, build-grid.vert:28-62,
- * force-collision-spatial.frag:103-230.  sizes: resolved per-point sizes. */
+ * force-collision-spatial.frag:41-168.  sizes: resolved per-point sizes. */
 void or_collision(uint32_t n, const double* pos, const uint8_t* absent, const float* sizes,
                   float space_size, double alpha, double strength, double radius, double padding,
                   double max_size, double* v) {

@@ -44,6 +44,7 @@ def lib():
         l.or_geometry.argtypes = [C.c_uint32, U, U]
         l.or_cell.argtypes = [C.c_float, C.c_float, C.c_int32]
         l.or_cell.restype = C.c_int32
+        l.or_cells.argtypes = [C.c_uint32, F, B, C.c_float, C.c_int32, C.c_float, C.c_float, I]
         l.or_cell_size.argtypes = [C.c_float, C.c_uint32]
         l.or_cell_size.restype = C.c_float
         l.or_slots.argtypes = [C.c_uint32, F, B, C.c_float, C.c_float, I]
@@ -109,6 +110,15 @@ def cell(p: float, h: float, g: int) -> int:
 
 def cell_size(S: float, g: int) -> float:
     return lib().or_cell_size(S, g)
+
+
+def cells(xy, h: float, g: int, absent=None, offset=(0.0, 0.0)) -> np.ndarray:
+    """Per-point (cx, cy) at a grid of g cells of size h (or_cells); -1 for absent."""
+    xy = _f32(xy).reshape(-1)
+    n = xy.size // 2
+    out = np.empty(2 * max(n, 1), np.int32)
+    lib().or_cells(n, _p(xy, F), _p(_u8(absent), B), h, g, offset[0], offset[1], _p(out, I))
+    return out[: 2 * n].reshape(-1, 2)
 
 
 def slots(xy, S: float, seed: float, absent=None) -> np.ndarray:

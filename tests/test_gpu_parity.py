@@ -343,11 +343,13 @@ def test_ticks_vs_oracle_pass_order(center):
     """Whole ticks (gravity, centre, many-body, both link directions, each
     integrated) against the oracle's passes composed with positions stored in
     f32 between passes, as the reference's position textures hold them.
-    Repulsion is 0 so the many-body pass integrates a zero velocity: the check
-    then covers the pass order and buffer hand-offs (with centre off, gravity
-    runs inside the many-body binning) without the cell-boundary
-    discontinuities of the quadtree.  Per point |dp| <= 1e-5 * spaceSize per
-    tick."""
+    Repulsion is 0 so the many-body pass integrates a zero velocity: over
+    several ticks the check then covers the pass order and the buffer
+    hand-offs between passes without the cell-boundary discontinuities of the
+    pyramid (a point one rounding away from a cell edge may bin differently
+    in f32 and f64).  The many-body pass inside whole ticks is checked per
+    tick from identical states in test_gpu_cells.py.  Per point
+    |dp| <= 1e-5 * spaceSize per tick."""
     S, n, ticks = 4096.0, 10_000, 3
     xy = datagen.uniform_square(n, S, 5)
     xy[2 * 7: 2 * 9] = np.nan

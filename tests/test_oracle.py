@@ -443,3 +443,158 @@ def test_cluster_unclustered_and_empty_clusters_get_no_pull(oracle):
     # a point whose cluster has no present member and no custom position: 0/0, no pull
     v3 = oracle.cluster(xy, np.array([0, 0, 0], np.int32), 1.0, 1.0, absent=np.array([1, 1, 1], np.uint8))
     assert np.array_equal(v3, np.zeros(6))
+
+
+@pytest.mark.parametrize("k", [1, 5, 37, 1000])
+@pytest.mark.parametrize("where", ["level0", "level2"])
+def test_far_field_scales_with_cell_count(oracle, k, where):
+    """force-level.frag:54 weights a far cell by its count N: k points in one
+    far cell pull like one point of mass k at their centroid.  The query point
+    is alone in its neighbourhood, so its whole velocity is that one term,
+    alpha*K*N*d/|d|^2 (|d| >= 1).  Catches a dropped N factor, a centroid that
+    is the sum rather than the mean, or a cell charged at two levels."""
+    rng = np.random.default_rng(k)
+    S, alpha, K = 4096.0, 0.8, 0.5
+    if where == "level0":
+        q = (100.0, 120.0)
+        box = (3400.0, 3500.0)   # level-0 cell (3, 3): charged at level 0
+    else:
+        q = (600.0, 600.0)       # level-0 (0,0), level-1 (1,1), level-2 (2,2) at 16x16 finest
+        box = (1300.0, 300.0)    # level-1 (2,0) is next to q's: charged at level 2, cell (5, 1)
+    n = k + 1
+    for coincident in (True, False):
+        xy = np.empty(2 * n, np.float32)
+        xy[0], xy[1] = q
+        if coincident:
+            pts = np.tile(np.array(box, np.float32), (k, 1))
+        else:
+            pts = (np.array(box) + rng.uniform(0.0, 200.0, (k, 2))).astype(np.float32)
+        xy[2:] = pts.reshape(-1)
+        if where == "level2" and n < 17:
+            # pad with absent points so the finest grid is 16x16 (L = 3) whatever k is
+            xy = np.concatenate([xy, np.full(2 * (17 - n), np.nan, np.float32)])
+        absent = oracle.absent_mask(xy)
+        gf, L = oracle.geometry(xy.size // 2)
+        if where == "level2":
+            assert gf >= 16 and L >= 3
+        got = oracle.manybody(xy, S, alpha, K, 0.3, absent=absent, eval_idx=np.array([0], np.uint32))
+        c = pts.astype(np.float64).mean(axis=0)
+        d = np.array(q, np.float64) - c
+        want = alpha * K * k * d / float(d @ d)
+        np.testing.assert_allclose(got[:2], want, rtol=1e-12)
+
+
+def _collision_by_hand(xy, sizes, S, alpha, strength):
+    """force-collision-spatial.frag:41-168 and build-grid.vert:46-55 written
+    out per offset grid with Python dicts (cell -> member list)."""
+    P = np.asarray(xy, np.float64).reshape(-1, 2)
+    sizes = np.asarray(sizes, np.float64)
+    n = P.shape[0]
+    maxs = max(4.0, float(np.max(sizes)))
+    cs_eff = max(2.0 * (maxs / 2.0), 8.0)
+    G = int(min(512, max(1, math.floor(S / cs_eff))))
+    h = float(np.float32(S / G))
+    out = np.zeros((n, 2))
+    for ox, oy in ((0.0, 0.0), (0.5, 0.0), (0.0, 0.5), (0.5, 0.5)):
+        def cell(p):
+            qx = float(np.float32(np.float32(p[0]) + np.float32(ox * h)))
+            qy = float(np.float32(np.float32(p[1]) + np.float32(oy * h)))
+            return (min(max(int(math.floor(float(np.float32(qx / h)))), 0), G - 1),
+                    min(max(int(math.floor(float(np.float32(qy / h)))), 0), G - 1))
+        members = {}
+        for i in range(n):
+            members.setdefault(cell(P[i]), []).append(i)
+        for i in range(n):
+            ri = sizes[i] / 2.0
+            mx, my = cell(P[i])
+            u = np.zeros(2)
+            total = 0.0
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    others = [j for j in members.get((mx + dx, my + dy), []) if j != i]
+                    if not others:
+                        continue
+                    total += len(others)
+                    avg = P[others].mean(axis=0)
+                    rho = ri + float(np.mean([sizes[j] for j in others])) / 2.0
+                    dv = P[i] - avg
+                    d = math.hypot(*dv)
+                    if 0.001 < d < rho:
+                        f = min(alpha * strength * math.sqrt((rho - d) / rho) * rho * 0.5 * 0.25 * len(others),
+                                rho * 0.5)
+                        u += f * dv / d
+            if total > 2.0:
+                u *= 2.0 / total
+            nu = math.hypot(*u)
+            if nu > 0.1 * ri:
+                u *= 0.1 * ri / nu
+            out[i] += u
+    return out
+
+
+def test_collision_three_others_in_one_cell_by_hand(oracle):
+    """Four points inside [96, 100)^2 share one cell of every offset grid
+    (cell size 8): each grid averages the three others, damps by 2/3 and,
+    with a small alpha*strength, stays under the per-pass clamp, so the
+    velocity is 4 * (2/3) * f * (p - avg)/|p - avg| in closed form."""
+    S = 4096.0
+    xy = np.array([96.5, 96.5, 97.5, 96.5, 96.5, 98.0, 99.0, 99.0], np.float64)
+    sizes = np.full(4, 4.0, np.float32)
+    alpha, strength = 0.1, 0.5
+    v = oracle.collision(xy, sizes, S, alpha, strength).reshape(-1, 2)
+    P = xy.reshape(-1, 2)
+    for i in range(4):
+        others = [j for j in range(4) if j != i]
+        avg = P[others].mean(axis=0)
+        d = np.linalg.norm(P[i] - avg)
+        rho = 4.0
+        assert 0.001 < d < rho
+        f = alpha * strength * math.sqrt((rho - d) / rho) * rho * 0.5 * 0.25 * 3
+        u = (2.0 / 3.0) * f * (P[i] - avg) / d
+        assert np.linalg.norm(u) < 0.2          # the clamp does not bind
+        np.testing.assert_allclose(v[i], 4 * u, rtol=1e-12)
+    # with alpha * strength = 1 the per-pass clamp binds: |v| = 4 * 0.1 * r
+    v = oracle.collision(xy, sizes, S, 1.0, 1.0).reshape(-1, 2)
+    np.testing.assert_allclose(np.hypot(v[:, 0], v[:, 1]), 4 * 0.2, rtol=1e-12)
+
+
+def test_collision_neighbour_cells_match_written_out_rule(oracle):
+    """Random clusters straddling cell edges (own cell, 3x3 neighbours, cells
+    at the grid edge, mixed sizes): the oracle equals the rule written out
+    with Python member lists.  Catches a self contribution not removed, a
+    neighbour cell skipped, a wrong damping total or a clamp on the sum of
+    the four grids instead of each."""
+    rng = np.random.default_rng(8)
+    S = 1024.0
+    for trial in range(6):
+        n = 40
+        centre = rng.uniform(20, S - 20, 2) if trial < 4 else np.array([2.0, S - 3.0])
+        xy = (centre + rng.normal(0, 6.0, (n, 2))).clip(0, S).astype(np.float32).astype(np.float64)
+        sizes = rng.uniform(2.0, 9.0, n).astype(np.float32)
+        got = oracle.collision(xy.reshape(-1), sizes, S, 0.7, 0.9).reshape(-1, 2)
+        want = _collision_by_hand(xy, sizes, S, 0.7, 0.9)
+        np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12)
+
+
+def test_cells_vector_form_equals_scalar_cell(oracle):
+    """or_cells (used to check the GPU's binning) agrees with the pinned scalar
+    or_cell point by point, at each offset of the collision grids, and maps
+    absent points to -1."""
+    rng = np.random.default_rng(3)
+    for S, g in ((4096.0, 256), (5000.0, 64), (1024.0, 7)):
+        h = oracle.cell_size(S, g)
+        xy = rng.uniform(-0.1 * S, 1.1 * S, 400).astype(np.float32)
+        xy[:20] = (np.arange(20) * h).astype(np.float32)   # exact cell boundaries
+        xy[20] = S
+        xy[40:42] = np.nan
+        absent = oracle.absent_mask(xy)
+        for off in ((0.0, 0.0), (0.5, 0.0), (0.0, 0.5), (0.5, 0.5)):
+            got = oracle.cells(xy, h, g, absent, off)
+            P = xy.reshape(-1, 2)
+            for i in range(P.shape[0]):
+                if absent[i]:
+                    assert tuple(got[i]) == (-1, -1)
+                    continue
+                qx = np.float32(P[i, 0]) + np.float32(np.float32(off[0]) * np.float32(h))
+                qy = np.float32(P[i, 1]) + np.float32(np.float32(off[1]) * np.float32(h))
+                assert tuple(got[i]) == (oracle.cell(float(qx), h, g), oracle.cell(float(qy), h, g))
