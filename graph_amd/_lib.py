@@ -117,6 +117,8 @@ _SIGNATURES = {
     "cg_step_graphs": (C.c_int, [_CTX, _U, C.c_uint32, C.c_uint32]),
     "cg_sync": (C.c_int, [_CTX]),
     "cg_prepare": (C.c_int, [_CTX, C.POINTER(CgSimParams)]),
+    "cg_eval_manybody_noalloc": (C.c_int, [_CTX, C.POINTER(CgSimParams), C.c_float, C.c_float, _F, _F,
+                                           C.c_uint32, _F]),
     "cg_set_capture": (C.c_int, [_CTX, C.c_int32]),
     "cg_drag_points": (C.c_int, [_CTX, _U, _F, C.c_uint32]),
     "cg_get_positions_at": (C.c_int, [_CTX, _U, C.c_uint32, _F]),
@@ -322,6 +324,17 @@ class Context:
     def eval_force(self, params: CgSimParams, force: int, alpha: float, seed: float = 0.0) -> np.ndarray:
         out = np.zeros(2 * self.n, dtype=np.float32)
         self._check(self._l.cg_eval_force(self._ctx, C.byref(params), force, alpha, seed, fptr(out)))
+        return out
+
+    def eval_manybody_noalloc(self, params: CgSimParams, alpha: float, seed: float, xy, jitter=None):
+        """Many-body velocity of the given positions through the allocation-free
+        entry point (the static device scratch; no allocation)."""
+        xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
+        jit = None if jitter is None else np.ascontiguousarray(jitter, dtype=np.float32).reshape(-1)
+        n = xy.size // 2
+        out = np.zeros(2 * n, dtype=np.float32)
+        self._check(self._l.cg_eval_manybody_noalloc(self._ctx, C.byref(params), alpha, seed, fptr(xy),
+                                                     fptr(jit), n, fptr(out)))
         return out
 
     def debug_slots(self):

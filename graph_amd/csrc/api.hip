@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <new>
 
 #include "cg_internal.h"
@@ -21,6 +22,32 @@ struct cg_ctx {
 };
 
 namespace cg {
+
+// Static scratch of the allocation-free entry point (cg_eval_manybody_noalloc):
+// every device the library runs on holds one copy until the process ends.  In
+// host code `scratch` is only a handle for hipGetSymbolAddress; the device
+// address of each device's copy is looked up once per device, with that device
+// current, when a context is created on it, and used from then on.
+constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The calling thread's current device must be `device`.
+int scratch_setup(Ctx* c, int device) {
+    if (device < 0 || device >= MAX_DEVICES) return fail(c, CG_ERR_INVALID, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(c, e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[device] = (uint8_t*)ptr;
+    }
+    c->scratch = g_scratch_ptr[device];
+    return CG_OK;
+}
 
 int fail(Ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -34,6 +61,16 @@ int hip_fail(Ctx* c, hipError_t e, const char* what) {
 // Device allocation charged against the context's HBM budget.  Sets *rc only
 // on failure, so a chain of calls can share one status.
 void* dalloc(Ctx* c, size_t bytes, int* rc) {
+    if (c->arena) {  // carve from the static scratch: 256-byte aligned, never freed
+        const size_t off = (c->arena_used + 255) & ~(size_t)255;
+        if (off + (bytes ? bytes : 16) > c->arena_cap) {
+            *rc = fail(c, CG_ERR_INVALID, "the input needs more than the " + std::to_string(c->arena_cap) +
+                                              "-byte static scratch");
+            return nullptr;
+        }
+        c->arena_used = off + (bytes ? bytes : 16);
+        return c->arena + off;
+    }
     if (c->budget && c->held + bytes > c->budget) {
         *rc = fail(c, CG_ERR_BUDGET, "device allocation of " + std::to_string(bytes) +
                                          " bytes exceeds the context budget");
@@ -51,7 +88,7 @@ void* dalloc(Ctx* c, size_t bytes, int* rc) {
 }
 
 void dfree(Ctx* c, void* p) {
-    if (!p) return;
+    if (!p || c->arena) return;
     for (size_t k = 0; k < c->allocs.size(); ++k) {
         if (c->allocs[k].first == p) {
             c->held -= c->allocs[k].second;
@@ -121,6 +158,40 @@ __global__ void k_gather(const float2* __restrict__ pos, uint32_t p0, uint32_t n
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     out[k] = idx[k] < n ? pos[p0 + idx[k]] : make_float2(__int_as_float(0x7fc00000), __int_as_float(0x7fc00000));
+}
+
+// Present list, flags and graph slots of one graph at offset 0 (the
+// allocation-free entry point): one block; thread t scans a contiguous chunk,
+// a block scan of the chunk counts places each present point in index order.
+// A point with a NaN coordinate is absent (position-utils.ts:17-24).
+__global__ void __launch_bounds__(1024) k_present_list(const float2* __restrict__ pos, uint32_t n,
+                                                       uint32_t* __restrict__ present,
+                                                       uint8_t* __restrict__ flags,
+                                                       uint32_t* __restrict__ gid) {
+    __shared__ uint32_t s[1024];
+    const uint32_t t = threadIdx.x, chunk = (n + 1023) / 1024;
+    const uint32_t b = min(n, t * chunk), e = min(n, b + chunk);
+    uint32_t cnt = 0;
+    for (uint32_t i = b; i < e; ++i) {
+        const float2 p = pos[i];
+        const bool absent = isnan(p.x) || isnan(p.y);
+        flags[i] = absent ? F_ABSENT : 0;
+        gid[i] = 0;
+        cnt += absent ? 0u : 1u;
+    }
+    s[t] = cnt;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    uint32_t w = s[t] - cnt;
+    for (uint32_t i = b; i < e; ++i) {
+        const float2 p = pos[i];
+        if (!(isnan(p.x) || isnan(p.y))) present[w++] = i;
+    }
 }
 
 constexpr uint64_t GRAPH_SEED = 0x5eed5eedULL;   // every graph's generator starts here
@@ -1105,6 +1176,10 @@ int cg_create(cg_ctx** out, const cg_create_params* params) {
         cg_destroy(x);
         return CG_ERR_DEVICE;
     }
+    // the static scratch's address on this device (used only by the
+    // allocation-free entry point, which reports its absence)
+    if (scratch_setup(c, c->device) != CG_OK) c->scratch = nullptr;
+    c->err.clear();
     *out = x;
     return CG_OK;
 }
@@ -1440,6 +1515,95 @@ int cg_eval_force(cg_ctx* x, const cg_sim_params* p, int32_t force, float alpha,
     (void)hipStreamSynchronize(c->stream);
     dfree(c, dv);
     return rc;
+}
+
+int cg_eval_manybody_noalloc(cg_ctx* x, const cg_sim_params* p, float alpha, float seed, const float* xy,
+                             const float* jitter, uint32_t n, float* v) {
+    CG_CTX(x);
+    if (!p) return fail(c, CG_ERR_INVALID, "cg_eval_manybody_noalloc: null params");
+    if (n == 0) return CG_OK;
+    if (!xy || !v) return fail(c, CG_ERR_INVALID, "cg_eval_manybody_noalloc: null positions or output");
+    if (n > CG_NOALLOC_MAX_POINTS)
+        return fail(c, CG_ERR_INVALID, "cg_eval_manybody_noalloc: " + std::to_string(n) +
+                                           " points exceed the static scratch's maximum of " +
+                                           std::to_string(CG_NOALLOC_MAX_POINTS));
+    if (!c->scratch)
+        return fail(c, CG_ERR_STATE, "cg_eval_manybody_noalloc: no static scratch address on this device");
+    // present count and extent on the host (the graph's descriptor needs both)
+    uint32_t np = 0;
+    float ext = 0.f;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float px = xy[2 * (size_t)i], py = xy[2 * (size_t)i + 1];
+        if (std::isnan(px) || std::isnan(py)) continue;
+        ++np;
+        ext = std::max(ext, std::max(std::fabs(px), std::fabs(py)));
+    }
+    std::memset(v, 0, (size_t)n * 8);
+    if (np == 0) return CG_OK;
+    uint32_t gf = 0, L = 0;
+    geometry(n, &gf, &L);
+    // A context of one graph at offset 0 whose buffers are carved from the
+    // scratch: the same launchers and kernels as cg_eval_force, no allocation.
+    Ctx s;
+    s.device = c->device;
+    s.stream = c->stream;
+    s.arena = c->scratch;
+    s.arena_cap = SCRATCH_BYTES;
+    s.G = 1;
+    s.N = n;
+    s.NP = np;
+    s.F = (uint32_t)pad_cells(gf);
+    s.C = level_off(L);
+    int rc = CG_OK;
+    s.pos[0] = (float2*)dalloc(&s, (size_t)n * 8, &rc);
+    s.pos[1] = s.pos[0];  // evaluation mode writes no positions
+    s.jitter = (float2*)dalloc(&s, (size_t)n * 8, &rc);
+    s.flags = (uint8_t*)dalloc(&s, n, &rc);
+    s.gid = (uint32_t*)dalloc(&s, (size_t)n * 4, &rc);
+    s.present = (uint32_t*)dalloc(&s, (size_t)np * 4, &rc);
+    s.desc_cur = (GDesc*)dalloc(&s, sizeof(GDesc), &rc);
+    s.desc = s.desc_cur;
+    s.f0s = (uint32_t*)dalloc(&s, 4, &rc);
+    s.blk_graph = (uint32_t*)dalloc(&s, (size_t)(s.F / CELL_BLOCK) * 4, &rc);
+    uint4 w0[PYR_W0_MAX], w1[PYR_W1_MAX];
+    pyramid_work(0, gf, L, w0, &s.n_pyr[0], w1, &s.n_pyr[1]);
+    s.pyr[0] = (uint4*)dalloc(&s, (size_t)(s.n_pyr[0] + 1) * 16, &rc);
+    s.pyr[1] = (uint4*)dalloc(&s, (size_t)(s.n_pyr[1] + 1) * 16, &rc);
+    float2* dv = (float2*)dalloc(&s, (size_t)n * 8, &rc);
+    if (rc) return fail(c, rc, s.err);
+    GDesc d{};
+    d.n = n;
+    d.np = np;
+    d.gf = gf;
+    d.L = L;
+    d.col_h = 1.f;
+    fill_dynamic(d, *p, ext, alpha, seed);
+    d.passes = 1u << P_MANYBODY;
+    d.inbuf = 0;
+    CG_HIP(c, hipMemcpyAsync(s.pos[0], xy, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    if (jitter)
+        CG_HIP(c, hipMemcpyAsync(s.jitter, jitter, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    else
+        CG_HIP(c, hipMemsetAsync(s.jitter, 0, (size_t)n * 8, c->stream));
+    CG_HIP(c, hipMemsetAsync(s.f0s, 0, 4, c->stream));
+    CG_HIP(c, hipMemsetAsync(s.blk_graph, 0, (size_t)(s.F / CELL_BLOCK) * 4, c->stream));
+    CG_HIP(c, hipMemsetAsync(dv, 0, (size_t)n * 8, c->stream));
+    CG_HIP(c, hipMemcpyAsync(s.desc_cur, &d, sizeof(GDesc), hipMemcpyHostToDevice, c->stream));
+    CG_HIP(c, hipMemcpyAsync(s.pyr[0], w0, (size_t)s.n_pyr[0] * 16, hipMemcpyHostToDevice, c->stream));
+    if (s.n_pyr[1])
+        CG_HIP(c, hipMemcpyAsync(s.pyr[1], w1, (size_t)s.n_pyr[1] * 16, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_present_list, dim3(1), dim3(1024), 0, c->stream, s.pos[0], n, s.present, s.flags,
+                       s.gid);
+    CG_HIP(c, hipGetLastError());
+    rc = mb_alloc(&s, gf);  // carves the binning and pyramid buffers; first visit order = present list
+    if (!rc) rc = mb_run(&s, dv);
+    if (rc) {
+        (void)hipStreamSynchronize(c->stream);
+        return fail(c, rc, s.err);
+    }
+    CG_HIP(c, hipMemcpyAsync(v, dv, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    return CG_OK;
 }
 
 int cg_debug_slots(cg_ctx* x, int32_t* slots, uint32_t capacity, uint32_t* finest,

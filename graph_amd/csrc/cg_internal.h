@@ -172,6 +172,11 @@ struct RadixSort {
 
 struct Ctx {
     int device = 0;
+    uint8_t* scratch = nullptr;     // this device's static scratch array (api.hip), set up by cg_create
+    // arena mode: dalloc carves from [arena, arena + arena_cap) and dfree is a
+    // no-op (the allocation-free entry point's buffers)
+    uint8_t* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;     // second stream (default priority) for the hub link tier
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -313,10 +318,12 @@ int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range,
                 const std::vector<uint32_t>* seg_key0 = nullptr);  // k_manybody.hip
 void radix_run(Ctx* c, const RadixSort& rs, uint32_t* const (&k)[2], uint32_t* const (&v)[2],
                bool hist0_ready, float2* const* pay = nullptr);
-int mb_alloc(Ctx* c);
+int mb_alloc(Ctx* c, uint32_t single_gf = 0);
 int mb_run(Ctx* c, float2* vout);
 void pyramid_work_host(uint32_t slot, uint32_t gf, uint32_t L, std::vector<uint4>& w0,
                        std::vector<uint4>& w1);
+constexpr uint32_t PYR_W0_MAX = 256, PYR_W1_MAX = 16;  // work items per graph of each stage
+void pyramid_work(uint32_t slot, uint32_t gf, uint32_t L, uint4* w0, uint32_t* n0, uint4* w1, uint32_t* n1);
 int link_build_graph(Ctx* c, Graph& g);                           // k_link.hip
 void link_free_graph(Ctx* c, Graph& g);
 int link_pack(Ctx* c, const LinkDir (&old)[2]);

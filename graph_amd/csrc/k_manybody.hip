@@ -1113,7 +1113,9 @@ int radix_alloc(Ctx* c, RadixSort& rs, uint32_t n, uint64_t key_range, const std
     return rc;
 }
 
-int mb_alloc(Ctx* c) {
+// single_gf != 0: the set is one graph at offset 0 with that finest grid (the
+// plain sort; no host tables are built).
+int mb_alloc(Ctx* c, uint32_t single_gf) {
     int rc = CG_OK;
     const size_t np = c->NP ? c->NP : 1;
     const size_t F = c->F ? c->F : CELL_BLOCK, C = c->C ? c->C : 1;
@@ -1121,7 +1123,10 @@ int mb_alloc(Ctx* c) {
         c->rk[q] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
         c->rv[q] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
     }
-    {
+    if (single_gf) {
+        rc = radix_alloc(c, c->mb_rs, c->NP, (uint64_t)single_gf * single_gf);
+        if (rc) return rc;
+    } else {
         // one segment per graph with present points, in slot order (the
         // present list and every tick's cell order are graph-major)
         std::vector<uint4> blk;
@@ -1247,21 +1252,32 @@ int mb_run(Ctx* c, float2* vout) {
 
 // Pyramid work lists for one graph slot (stage 0: finest level in tiles of up
 // to 32x32; stage 1: the remaining top levels of graphs finer than 128^2).
-void pyramid_work_host(uint32_t slot, uint32_t gf, uint32_t L, std::vector<uint4>& w0,
-                       std::vector<uint4>& w1) {
+void pyramid_work(uint32_t slot, uint32_t gf, uint32_t L, uint4* w0, uint32_t* n0, uint4* w1, uint32_t* n1) {
     const uint32_t lf = L - 1;
     const uint32_t T1 = std::min<uint32_t>(32u, gf / 4);
     uint32_t nout1 = 0;
     while ((1u << nout1) < T1) ++nout1;
-    const uint32_t tiles = (gf / T1) * (gf / T1);
-    for (uint32_t t = 0; t < tiles; ++t) w0.push_back(make_uint4(slot, lf, T1 | (nout1 << 16), t));
+    const uint32_t tiles = (gf / T1) * (gf / T1);  // at most (512 / 32)^2 = PYR_W0_MAX
+    for (uint32_t t = 0; t < tiles; ++t) w0[t] = make_uint4(slot, lf, T1 | (nout1 << 16), t);
+    *n0 = tiles;
+    *n1 = 0;
     const uint32_t g2 = gf / T1;
     if (g2 > 4) {
         const uint32_t lin2 = lf - nout1, T2 = g2 / 4;
         uint32_t nout2 = 0;
         while ((1u << nout2) < T2) ++nout2;
-        for (uint32_t t = 0; t < 16; ++t) w1.push_back(make_uint4(slot, lin2, T2 | (nout2 << 16), t));
+        for (uint32_t t = 0; t < 16; ++t) w1[t] = make_uint4(slot, lin2, T2 | (nout2 << 16), t);
+        *n1 = 16;
     }
+}
+
+void pyramid_work_host(uint32_t slot, uint32_t gf, uint32_t L, std::vector<uint4>& w0,
+                       std::vector<uint4>& w1) {
+    uint4 a[PYR_W0_MAX], b[PYR_W1_MAX];
+    uint32_t na = 0, nb = 0;
+    pyramid_work(slot, gf, L, a, &na, b, &nb);
+    w0.insert(w0.end(), a, a + na);
+    w1.insert(w1.end(), b, b + nb);
 }
 
 }  // namespace cg

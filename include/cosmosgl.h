@@ -177,6 +177,24 @@ const char* cg_last_error(const cg_ctx* ctx);
 int cg_eval_force(cg_ctx* ctx, const cg_sim_params* params, int32_t force, float alpha,
                   float seed, float* v);
 
+/* Allocation-free variant of cg_eval_force(CG_FORCE_MANYBODY).  It has no
+ * counterpart in the reference; it is for callers that cannot allocate device
+ * memory (code that must not call hipMalloc: capture-adjacent setup, tight
+ * serving loops, harnesses that forbid allocation).  The many-body velocity of
+ * one graph of n points given here, not through cg_set_positions: xy (2n
+ * floats, NaN = absent), jitter (2n floats or NULL = none), result in v (2n
+ * floats, 0 for absent points) — the same bits cg_eval_force returns for a
+ * context holding these positions and jitter.  Every intermediate lives in one
+ * static device array of this library (64 MiB per device, held until the
+ * process ends); the call allocates nothing and leaves the context's own
+ * memory untouched (ctx gives the device, the stream and cg_last_error).
+ * n above CG_NOALLOC_MAX_POINTS is refused with CG_ERR_INVALID.  Calls that
+ * overlap on one device (two contexts, two threads) race on the shared array;
+ * the call is synchronous. */
+#define CG_NOALLOC_MAX_POINTS 262144u
+int cg_eval_manybody_noalloc(cg_ctx* ctx, const cg_sim_params* params, float alpha, float seed,
+                             const float* xy, const float* jitter, uint32_t n, float* v);
+
 /* Near-field slots of the last many-body evaluation: finest^2 * 8 point
  * indices (row-major cells, ascending ticket, -1 = empty).  finest/levels
  * receive the pyramid geometry. */
