@@ -26,33 +26,81 @@ METRIC = "ticks/s at 1M pts/5M links; batched layouts/s + p50/p99 latency, 1 GPU
 HBM_PEAK_GBS = 8000.0
 VALU_FP32_PEAK_TFLOPS = 157.3
 
-# single-kernel stages whose events bracket exactly one kernel launch
-SINGLE_KERNEL = ("gravity", "mb_force", "mouse")
+# The kernels each stage's HIP events bracket (one bracketed launch = one pass;
+# the link passes fork a helper stream and join it before their closing event).
+STAGE_KERNELS = {
+    "gravity": "k_pointwise<GRAVITY>",
+    "mb_sort": "k_bin_keys + k_radix_up/k_scan_rows/k_radix_down per digit + k_bin_bounds",
+    "mb_cells": "k_cells + k_cells_mid<64>/<16> (helper stream) + k_cells_chunk + k_cells_merge",
+    "mb_pyramid": "k_pyramid x2",
+    "mb_force": "k_mb_force",
+    "link_in": "link pass IN: k_link_lane + k_link_chunk<64,4> + k_link_chunk<256,6> + k_hub_final (two streams)",
+    "link_out": "link pass OUT: k_link_lane + k_link_chunk<64,4> + k_link_chunk<256,6> + k_hub_final (two streams)",
+    "collision": "k_col_keys + radix passes + k_fill32 + k_col_bounds + k_col_cells + k_col_force",
+}
 
 
-def stage_work(stage: str, wl: dict, ctx_info: dict) -> tuple[str, float]:
-    """Algorithmic bytes (or flops) of one launch of a stage."""
-    n, np_, L, gf = ctx_info["n"], ctx_info["np"], ctx_info["L"], ctx_info["gf"]
-    if stage in ("gravity", "center", "mouse"):
-        # present index 4, position in 8, flags 1, position out 8, key 4, value 4
-        return "hbm", 29.0 * np_
+def stage_work(stage: str, info: dict) -> tuple[str, float]:
+    """Algorithmic work of one pass of a stage: the compulsory HBM traffic of the
+    reference's dataflow in SoA f32/int32 (each array counted once per pass that
+    consumes or produces it; cache-served re-reads not counted), or the FLOPs of
+    the far/near-field pass.  np present points, E directed entries of the pass's
+    direction, C_f finest cells, C_tot level cells."""
+    np_, L, Cf, Ct = info["np"], info["L"], info["gf"] ** 2, info["C_tot"]
+    if stage in ("gravity", "center", "mouse", "cluster"):
+        return "hbm", 16.0 * np_                      # position in 8, position out 8
+    if stage == "mb_sort":
+        return "hbm", 16.0 * np_                      # position in 8; cell key 4 + point index 4 out
+    if stage == "mb_cells":
+        return "hbm", 12.0 * np_ + 80.0 * Cf          # index 4 + position 8 in; finest sums 16 + 8 slots x 8 out
+    if stage == "mb_pyramid":
+        return "hbm", 16.0 * Cf + 16.0 * Ct           # finest sums in; every level's (x, y, N) out
     if stage == "mb_force":
         # flops per far-field cell: d (2), |d|^2 (3), 1/l (1), N/l (1), accumulate (4) = 11,
         # over 9.75 level-0 cells (mean over query cells of the 4x4 grid) and 27 per finer
         # level; per near-field pair 13 (the same plus min/sqrt of the clamp), at most 72 pairs
         far = 9.75 + 27 * (L - 1)
-        return "valu", (11.0 * far + 13.0 * ctx_info.get("near_pairs", 72)) * np_
+        return "valu", (11.0 * far + 13.0 * info.get("near_pairs", 72)) * np_
     if stage in ("link_in", "link_out"):
-        E = ctx_info["E_live"][0 if stage == "link_in" else 1]
-        # entry 16 + gathered neighbour position 8 per entry; per point 25
-        return "hbm", 24.0 * E + 25.0 * np_
+        E = info["E_live"][0 if stage == "link_in" else 1]
+        return "hbm", 24.0 * E + 16.0 * np_          # per entry: neighbour, draw, bias, strength 16 + gathered position 8
+    if stage == "collision":
+        return "hbm", 92.0 * np_                      # four offset grids: keys, cell sums, sizes, positions
     return "hbm", 0.0
 
 
+def level_cells(L: int) -> int:
+    """Cells of a pyramid of L levels (4^2, 8^2, ..., (4 << (L-1))^2)."""
+    return sum((4 << lv) ** 2 for lv in range(L))
+
+
+def stage_rooflines(breakdown: dict, info: dict) -> dict:
+    """Per stage: ms per pass, algorithmic bytes (or flops) per pass, achieved rate
+    and its fraction of the MI355X peak (HBM 8 TB/s; FP32 VALU 157.3 TFLOP/s)."""
+    out = {}
+    for st, (ms, cnt) in breakdown.items():
+        if cnt == 0:
+            continue
+        kind, work = stage_work(st, info)
+        sec = ms / cnt / 1e3
+        if work <= 0 or sec <= 0:
+            continue
+        if kind == "hbm":
+            rate = work / sec / 1e9
+            out[st] = {"ms_per_pass": sec * 1e3, "bound": "hbm", "bytes_per_pass": work,
+                       "achieved": rate, "unit": "GB/s", "frac": rate / HBM_PEAK_GBS}
+        else:
+            rate = work / sec / 1e12
+            out[st] = {"ms_per_pass": sec * 1e3, "bound": "valu", "flops_per_pass": work,
+                       "achieved": rate, "unit": "TFLOP/s", "frac": rate / VALU_FP32_PEAK_TFLOPS}
+    return out
+
+
 def profiled_traffic(kernel: str) -> tuple[float | None, str | None]:
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE runs of `bench.py --traffic-probe`, scaled by the
-    calibration kernels there), or None when it holds no figure for the kernel."""
+    """HBM bytes per pass of a stage (or launch of a kernel) from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs of
+    `bench.py --traffic-probe`, scaled by the calibration kernels there), or
+    None when it holds no figure for it."""
     path = ROOT / "profiles" / "traffic_cfg1.md"
     if not path.exists():
         return None, None
@@ -133,6 +181,120 @@ def force_error_log(wl: dict, ticks: int, every: int, sample: int, log) -> dict:
     return {"ticks": ticks, "every": every, "sample_points": int(idx.size),
             "sample": f"{sample} random points and the 64 longest link chains per direction",
             "tolerance": 1e-5, "max_error_over_S": worst, "within_tolerance": worst <= 1e-5, "log": rows}
+
+
+def densest_cells_sample(xy: np.ndarray, S: float, g: int, cells: int, cap: int) -> np.ndarray:
+    """Points of the `cells` most populated cells of a g x g grid over [0, S)^2
+    (at most `cap` points), by the oracle's cell rule."""
+    from oracle import oracle as O
+
+    c = O.cells(xy, O.cell_size(S, g), g, O.absent_mask(xy))
+    ok = c[:, 0] >= 0
+    key = c[:, 1].astype(np.int64) * g + c[:, 0]
+    key[~ok] = -1
+    cnt = np.bincount(key[ok], minlength=g * g)
+    top = np.argsort(cnt)[-cells:]
+    return np.flatnonzero(np.isin(key, top))[:cap]
+
+
+def bench_cfg2(ticks: int, every: int, sample: int, log) -> dict:
+    """The 4M-point SBM graph with collision: `ticks` ticks from alpha 1, timed in
+    chunks of `every` ticks (each bracketed by synchronisations); between chunks,
+    outside the timing, every pass (gravity, many-body, link IN, link OUT,
+    collision) is evaluated on the GPU from the live state and by the FP64 oracle
+    from the same state on a sample: random points, the points of the most
+    crowded finest and collision cells and the longest link chains."""
+    import torch
+
+    from graph_amd import _lib, datagen, workloads
+    from oracle import oracle as O
+
+    t_gen = time.perf_counter()
+    wl = workloads.cfg2()
+    n, S, p = wl["n"], wl["S"], wl["params"]
+    m = int(wl["links"].size // 2)
+    log(f"cfg2: {n} points, {m} links generated in {time.perf_counter() - t_gen:.1f}s")
+    ctx = _lib.Context(device=0)
+    ctx.set_positions(wl["xy"])
+    ctx.set_jitter(wl["jitter"])
+    ctx.set_links(wl["links"], None, wl["r_in"], wl["r_out"])
+    ctx.set_point_sizes(wl["sizes"])
+    ctx.set_alpha(1.0)
+    ctx.prepare(p)
+    sizes = wl["sizes"]
+    max_size = max(float(p.point_default_size), float(sizes.max()))
+    G, h = O.collision_grid(S, float(p.collision_radius), float(p.collision_padding), max_size)
+    Lk = O.Links(n, wl["links"], wl["r_in"], wl["r_out"])
+    lk = wl["links"].reshape(-1, 2).astype(np.int64)
+    hubs = np.concatenate([np.argsort(np.bincount(lk[:, c], minlength=n))[-64:] for c in (0, 1)])
+    rnd = np.random.default_rng(6).choice(n, sample, replace=False)
+    lo, hi = p.link_dist_random_variation_range[0], p.link_dist_random_variation_range[1]
+    seeds = datagen.uniform(ticks, 0.0, 1.0, 111)
+    rows, worst, chunk_s = [], {}, []
+    for t in range(0, ticks + 1, every):
+        if t:
+            ctx.sync()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.step(p, every, seeds=seeds[t - every: t])
+            ctx.sync()
+            torch.cuda.synchronize()
+            chunk_s.append(time.perf_counter() - t0)
+        xy = ctx.get_positions()
+        a = float(np.float32(ctx.get_alpha()))
+        seed = 0.25 + (t % 97) / 200.0
+        idx = np.unique(np.concatenate([rnd, hubs, densest_cells_sample(xy, S, 512, 8, 512),
+                                        densest_cells_sample(xy, S, G, 8, 512)])).astype(np.uint32)
+
+        def err(got, want):
+            d = got.reshape(-1, 2)[idx].astype(np.float64) - want.reshape(-1, 2)[idx]
+            return float(np.hypot(d[:, 0], d[:, 1]).max() / S)
+
+        e = {"gravity": err(ctx.eval_force(p, _lib.FORCE_GRAVITY, a), O.gravity(xy, S, a, p.gravity)),
+             "manybody": err(ctx.eval_force(p, _lib.FORCE_MANYBODY, a, seed),
+                             O.manybody(xy, S, a, p.repulsion, seed, jitter=wl["jitter"], eval_idx=idx))}
+        for d, name in ((0, "link_in"), (1, "link_out")):
+            e[name] = err(ctx.eval_force(p, _lib.FORCE_LINK_IN + d, a),
+                          Lk.force(xy, d, a, p.link_spring, p.link_distance, lo, hi))
+        e["collision"] = err(ctx.eval_force(p, _lib.FORCE_COLLISION, a),
+                             O.collision(xy, sizes, S, a, p.collision, p.collision_radius, p.collision_padding,
+                                         max_size=max_size))
+        for k, v in e.items():
+            worst[k] = max(worst.get(k, 0.0), v)
+        rows.append({"tick": t, "alpha": a, "sample_points": int(idx.size),
+                     **{k: float(f"{v:.3e}") for k, v in e.items()}})
+        if t % 100 == 0:
+            log(f"cfg2 force error at tick {t}: " + ", ".join(f"{k} {v:.2e}" for k, v in e.items()))
+    # stage breakdown over 20 direct-launch ticks after the run
+    ctx.set_profiling(_lib.STAGES)
+    ctx.step(p, 20, seeds=datagen.uniform(20, 0.0, 1.0, 112))
+    breakdown = ctx.get_profile()
+    ctx.set_profiling([])
+    dev_gib = ctx.device_bytes() / 2**30
+    ctx.close()
+    info = dict(n=n, np=n, L=8, gf=512, C_tot=level_cells(8), E_live=[m, m])
+    timed = chunk_s[1:]  # the first chunk also runs the first ticks' one-off work
+    tps = len(timed) * every / sum(timed)
+    return {
+        "workload": wl["name"] + f" ({m} links, log-normal radii max size {max_size:.1f}, "
+                    f"collision strength {p.collision:g})",
+        "points": n, "links": m, "space_size": S,
+        "space_size_note": "the config's 16384; Store.adjustSpaceSize (Store/index.ts:209-233) would halve "
+                           "it to 8192 on a WebGL device whose maxTextureDimension2D is 16384 (none here)",
+        "collision_grid": G, "collision_cell": h, "finest_grid": 512, "levels": 8,
+        "ticks": ticks, "ticks_per_s": tps, "ms_per_tick": 1e3 / tps, "point_ticks_per_s": tps * n,
+        "timed_window": f"ticks {2 * every}-{ticks}: {len(timed)} chunks of {every} ticks, each between "
+                        f"synchronisations (the error evaluations between chunks are outside the timing)",
+        "stage_ms_per_tick": {st: v[0] / 20.0 for st, v in breakdown.items()},
+        "stages": stage_rooflines(breakdown, info),
+        "device_gib": dev_gib,
+        "force_error": {"every": every, "tolerance": 1e-5, "max_error_over_S": worst,
+                        "within_tolerance": max(worst.values()) <= 1e-5,
+                        "sample": f"{sample} random points, the 64 longest link chains per direction, the "
+                                  f"points of the 8 most crowded finest cells and of the 8 most crowded "
+                                  f"collision cells (up to 512 each)",
+                        "log": rows},
+    }
 
 
 def bench_batched(count: int, ticks: int, sample: int, log) -> dict:
@@ -243,10 +405,12 @@ def main() -> int:
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
     ap.add_argument("--serve-segment", type=int, default=100)
     ap.add_argument("--error-ticks", type=int, default=1000)
+    ap.add_argument("--cfg2-ticks", type=int, default=500)
+    ap.add_argument("--cfg2-every", type=int, default=50, help="ticks between force-error checkpoints")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc: calibration kernels of known traffic (256 MiB), then 5 "
                          "direct-launch ticks of the 1M/5M workload; prints nothing else")
-    ap.add_argument("--skip", default="", help="comma list of legs to skip: errors,batched,serving")
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: errors,cfg2,batched,serving")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -299,16 +463,20 @@ def main() -> int:
     t1 = time.perf_counter()
     el = t1 - t0
     ticks_per_s = args.steps / el
-    # after the timed region: 10 ticks with every stage bracketed (per-stage
-    # breakdown; picks the dominant kernel)
+    # after the timed region: 50 ticks with every stage bracketed (the per-stage
+    # breakdown and rooflines; picks the dominant stage by time per tick)
     ctx.set_profiling(_lib.STAGES)
-    ctx.step(p, 10, seeds=datagen.uniform(10, 0.0, 1.0, 97))
+    ctx.step(p, 50, seeds=datagen.uniform(50, 0.0, 1.0, 97))
     breakdown = ctx.get_profile()
     ctx.set_profiling([])
-    single = {s: v for s, v in breakdown.items() if s in SINGLE_KERNEL}
-    dominant = max(single, key=lambda s: single[s][0]) if single else "mb_force"
-    # the dominant kernel's duration: HIP events on the library stream around each of
-    # its launches over an equal run of ticks right after (events force direct launches)
+    gf, L = 512, 8
+    m = int(wl["links"].size // 2)
+    info = dict(n=n, np=n, L=L, gf=gf, C_tot=level_cells(L), E_live=[m, m])
+    modelled = {st: v for st, v in breakdown.items() if stage_work(st, info)[1] > 0}
+    dominant = max(modelled, key=lambda st: modelled[st][0])
+    # the dominant stage's duration: HIP events on the library stream around each
+    # of its passes (a link pass joins its helper stream before the closing event)
+    # over an equal run of ticks right after (events force direct launches)
     prof_seeds = datagen.uniform(args.steps, 0.0, 1.0, 98)
     ctx.set_profiling([dominant])
     t2 = time.perf_counter()
@@ -317,27 +485,27 @@ def main() -> int:
     direct_ms = (time.perf_counter() - t2) / args.steps * 1e3
     ctx.set_profiling([])
 
-    gf, L = 512, 8
-    info = dict(n=n, np=n, L=L, gf=gf, E_live=[int(wl["links"].size // 2)] * 2)
-    kind, work = stage_work(dominant, wl, info)
+    kind, work = stage_work(dominant, info)
     k_ms, k_cnt = timed.get(dominant, (0.0, 0))
     k_avg_s = (k_ms / max(k_cnt, 1)) / 1000.0
     if kind == "hbm":
         achieved = work / k_avg_s / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": work}
     else:
         achieved = work / k_avg_s / 1e12
         roof = {"bound": "valu", "achieved": achieved, "peak": VALU_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / VALU_FP32_PEAK_TFLOPS, "traffic": None}
-    roof["kernel"] = {"gravity": "k_pointwise<GRAVITY>", "mb_force": "k_mb_force"}.get(dominant, dominant)
-    roof["traffic"], src = profiled_traffic(roof["kernel"])
+                "frac": achieved / VALU_FP32_PEAK_TFLOPS, "traffic": None, "algorithmic_flops": work}
+    roof["kernel"] = STAGE_KERNELS.get(dominant, dominant)
+    roof["stage"] = dominant
+    roof["traffic"], src = profiled_traffic(dominant)
     if src:
-        roof["traffic_unit"] = "bytes per launch"
+        roof["traffic_unit"] = "bytes per pass"
         roof["traffic_source"] = src
     roof["avg_ms"] = k_avg_s * 1e3
-    roof["timing"] = (f"HIP events around each of {k_cnt} launches over {args.steps} direct-launch ticks "
-                      f"run after the timed region ({direct_ms:.3f} ms/tick there)")
+    roof["timing"] = (f"HIP events around each of {k_cnt} passes over {args.steps} direct-launch ticks "
+                      f"run after the timed region ({direct_ms:.3f} ms/tick there); dominant = the stage "
+                      f"with the most time per tick over 50 stage-timed ticks")
 
     out = {
         "metric": METRIC,
@@ -355,7 +523,8 @@ def main() -> int:
         "config": {"workload": wl["name"], "points": n, "links": int(wl["links"].size // 2),
                    "space_size": wl["S"], "finest_grid": gf, "levels": L, "ticks_per_step": 1},
         "point_ticks_per_s": ticks_per_s * n,
-        "stage_ms_per_tick": {s: v[0] / max(v[1], 1) for s, v in breakdown.items()},
+        "stage_ms_per_tick": {st: v[0] / 50.0 for st, v in breakdown.items()},
+        "stages": stage_rooflines(breakdown, info),
         "roofline": roof,
     }
     ctx.close()
@@ -363,6 +532,8 @@ def main() -> int:
     skip = set(filter(None, args.skip.split(",")))
     if "errors" not in skip:
         out["force_error"] = force_error_log(wl, args.error_ticks, 50, 4096, log)
+    if "cfg2" not in skip:
+        out["cfg2"] = bench_cfg2(args.cfg2_ticks, args.cfg2_every, 4096, log)
     if "batched" not in skip:
         out["batched"] = bench_batched(args.batch_graphs, args.batch_ticks, args.batch_sample, log)
     if "serving" not in skip:

@@ -117,6 +117,7 @@ _SIGNATURES = {
     "cg_step_graphs": (C.c_int, [_CTX, _U, C.c_uint32, C.c_uint32]),
     "cg_sync": (C.c_int, [_CTX]),
     "cg_prepare": (C.c_int, [_CTX, C.POINTER(CgSimParams)]),
+    "cg_graph_eval_force": (C.c_int, [_CTX, C.c_uint32, C.c_int32, C.c_float, C.c_float, _F]),
     "cg_eval_manybody_noalloc": (C.c_int, [_CTX, C.POINTER(CgSimParams), C.c_float, C.c_float, _F, _F,
                                            C.c_uint32, _F]),
     "cg_set_capture": (C.c_int, [_CTX, C.c_int32]),
@@ -324,6 +325,12 @@ class Context:
     def eval_force(self, params: CgSimParams, force: int, alpha: float, seed: float = 0.0) -> np.ndarray:
         out = np.zeros(2 * self.n, dtype=np.float32)
         self._check(self._l.cg_eval_force(self._ctx, C.byref(params), force, alpha, seed, fptr(out)))
+        return out
+
+    def graph_eval_force(self, gid: int, force: int, alpha: float, seed: float = 0.0) -> np.ndarray:
+        """One force pass of graph `gid` inside the packed set, with its stored parameters."""
+        out = np.zeros(2 * self._sizes.get(gid, 0), dtype=np.float32)
+        self._check(self._l.cg_graph_eval_force(self._ctx, gid, force, alpha, seed, fptr(out)))
         return out
 
     def eval_manybody_noalloc(self, params: CgSimParams, alpha: float, seed: float, xy, jitter=None):

@@ -1121,6 +1121,84 @@ int pass_of_force(int32_t force) {
 }  // namespace
 }  // namespace cg
 
+namespace cg {
+namespace {
+
+// One force pass of graph `id` from its current state (cg_eval_force and
+// cg_graph_eval_force); the other graphs of the set run nothing.
+int eval_force(Ctx* c, uint32_t id, const cg_sim_params* p, int32_t force, float alpha, float seed, float* v) {
+    Graph* gp = lookup(c, id);
+    if (!gp) return fail(c, CG_ERR_INVALID, "eval_force: unknown graph id " + std::to_string(id));
+    if (gp->n && !v) return fail(c, CG_ERR_INVALID, "eval_force: null output");
+    const int pass = pass_of_force(force);
+    if (pass < 0) return fail(c, CG_ERR_INVALID, "eval_force: unknown force");
+    if (p) set_params(c, *gp, *p);
+    int rc = CG_OK;
+    if (c->dirty) {
+        rc = pack(c);
+        if (rc) return rc;
+    }
+    Graph& g = c->graphs[id];
+    if (g.n == 0) return CG_OK;
+    std::memset(v, 0, (size_t)g.n * 8);
+    const bool runs = g.np && ((pass < P_LINK_IN) || (pass == P_COLLISION && collides(g)) ||
+                               (pass == P_CLUSTER && clustered(g)) ||
+                               ((pass == P_LINK_IN || pass == P_LINK_OUT) && g.links_ready && g.m > 0));
+    if (!runs) return CG_OK;
+    std::vector<GDesc> table(c->G);
+    for (uint32_t s = 0; s < c->G; ++s) {
+        const Graph& h = c->graphs[c->order[s]];
+        GDesc d = c->h_desc[s];
+        fill_dynamic(d, h.params, h.extent, alpha, seed);
+        d.passes = (s == g.slot) ? (1u << pass) : 0u;
+        d.inbuf = h.cur ? ((1u << P_COUNT) - 1) : 0u;
+        table[s] = d;
+    }
+    rc = write_table(c, table);
+    if (rc) return rc;
+    float2* dv = (float2*)dalloc(c, (size_t)c->N * 8, &rc);
+    if (rc) return rc;
+    // A link pass leaves its converged chunk starts as the next tick's warm
+    // starts; an evaluation must leave the simulation's state as it found it,
+    // so they are saved and restored around it.
+    LinkDir* L = (pass == P_LINK_IN || pass == P_LINK_OUT) ? &c->dir[pass - P_LINK_IN] : nullptr;
+    const size_t wl = (L && L->warm) ? L->warm_len : 0;
+    uint8_t* keep = wl ? (uint8_t*)dalloc(c, wl * (sizeof(float2) + sizeof(float4)), &rc) : nullptr;
+    if (rc) {
+        dfree(c, dv);
+        return rc;
+    }
+    hipError_t e = hipMemsetAsync(dv, 0, (size_t)c->N * 8, c->stream);
+    if (e == hipSuccess && keep) e = hipMemcpyAsync(keep, L->warm, wl * 8, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess && keep)
+        e = hipMemcpyAsync(keep + wl * 8, L->wjac, wl * 16, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) rc = hip_fail(c, e, "eval_force clear");
+    if (!rc) {
+        if (pass <= P_CENTER) rc = pw_run(c, pass, dv);
+        else if (pass == P_MANYBODY) rc = mb_run(c, dv);
+        else if (pass == P_COLLISION) rc = col_run(c, dv);
+        else if (pass == P_CLUSTER) rc = cluster_run(c, dv);
+        else rc = link_run(c, pass - P_LINK_IN, dv);
+    }
+    if (keep) {
+        e = hipMemcpyAsync(L->warm, keep, wl * 8, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(L->wjac, keep + wl * 8, wl * 16, hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess && rc == CG_OK) rc = hip_fail(c, e, "eval_force warm-start restore");
+    }
+    if (rc == CG_OK) {
+        e = hipMemcpyAsync(v, dv + g.p0, (size_t)g.n * 8, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, e, "eval_force readback");
+    }
+    (void)hipStreamSynchronize(c->stream);
+    dfree(c, keep);
+    dfree(c, dv);
+    return rc;
+}
+
+}  // namespace
+}  // namespace cg
+
 using namespace cg;
 
 #define CG_CTX(x)                       \
@@ -1463,58 +1541,18 @@ int cg_sync(cg_ctx* x) {
 
 // ---- inspection (graph 0) --------------------------------------------------
 
-int cg_eval_force(cg_ctx* x, const cg_sim_params* p, int32_t force, float alpha, float seed,
-                  float* v) {
+
+int cg_eval_force(cg_ctx* x, const cg_sim_params* p, int32_t force, float alpha, float seed, float* v) {
     if (!x || !p) return CG_ERR_INVALID;
     Ctx* c = &x->c;
     CG_HIP(c, hipSetDevice(c->device));
-    Graph& g0 = default_graph(c);
-    if (g0.n && !v) return fail(c, CG_ERR_INVALID, "cg_eval_force: null output");
-    const int pass = pass_of_force(force);
-    if (pass < 0) return fail(c, CG_ERR_INVALID, "cg_eval_force: unknown force");
-    set_params(c, g0, *p);
-    int rc = CG_OK;
-    if (c->dirty) {
-        rc = pack(c);
-        if (rc) return rc;
-    }
-    Graph& g = c->graphs[0];
-    if (g.n == 0) return CG_OK;
-    std::memset(v, 0, (size_t)g.n * 8);
-    const bool runs = g.np && ((pass < P_LINK_IN) || (pass == P_COLLISION && collides(g)) ||
-                               (pass == P_CLUSTER && clustered(g)) ||
-                               ((pass == P_LINK_IN || pass == P_LINK_OUT) && g.links_ready && g.m > 0));
-    if (!runs) return CG_OK;
-    std::vector<GDesc> table(c->G);
-    for (uint32_t s = 0; s < c->G; ++s) {
-        const Graph& h = c->graphs[c->order[s]];
-        GDesc d = c->h_desc[s];
-        fill_dynamic(d, h.params, h.extent, alpha, seed);
-        d.passes = (s == g.slot) ? (1u << pass) : 0u;
-        d.inbuf = h.cur ? ((1u << P_COUNT) - 1) : 0u;
-        table[s] = d;
-    }
-    rc = write_table(c, table);
-    if (rc) return rc;
-    float2* dv = (float2*)dalloc(c, (size_t)c->N * 8, &rc);
-    if (rc) return rc;
-    hipError_t e = hipMemsetAsync(dv, 0, (size_t)c->N * 8, c->stream);
-    if (e != hipSuccess) rc = hip_fail(c, e, "cg_eval_force clear");
-    if (!rc) {
-        if (pass <= P_CENTER) rc = pw_run(c, pass, dv);
-        else if (pass == P_MANYBODY) rc = mb_run(c, dv);
-        else if (pass == P_COLLISION) rc = col_run(c, dv);
-        else if (pass == P_CLUSTER) rc = cluster_run(c, dv);
-        else rc = link_run(c, pass - P_LINK_IN, dv);
-    }
-    if (rc == CG_OK) {
-        e = hipMemcpyAsync(v, dv + g.p0, (size_t)g.n * 8, hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) rc = hip_fail(c, e, "cg_eval_force readback");
-    }
-    (void)hipStreamSynchronize(c->stream);
-    dfree(c, dv);
-    return rc;
+    default_graph(c);
+    return eval_force(c, 0, p, force, alpha, seed, v);
+}
+
+int cg_graph_eval_force(cg_ctx* x, uint32_t id, int32_t force, float alpha, float seed, float* v) {
+    CG_CTX(x);
+    return eval_force(c, id, nullptr, force, alpha, seed, v);
 }
 
 int cg_eval_manybody_noalloc(cg_ctx* x, const cg_sim_params* p, float alpha, float seed, const float* xy,

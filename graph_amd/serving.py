@@ -126,7 +126,9 @@ class BatchServer:
     def close(self) -> None:
         self.ctx.close()
 
-    def run(self, jobs: list[Job], log=None) -> tuple[list[Result], dict]:
+    def run(self, jobs: list[Job], log=None, on_segment=None) -> tuple[list[Result], dict]:
+        """Serves `jobs` (arrival times relative to the call).  on_segment(ctx,
+        {graph id: job}) runs after every segment, before retirements."""
         ctx = self.ctx
         pending = sorted(jobs, key=lambda j: j.arrival)
         active: dict[int, tuple[Job, int, int, float, int]] = {}  # gid -> job, left, total, start, bytes
@@ -168,6 +170,8 @@ class BatchServer:
             stats["segments"] += 1
             stats["ticks"] += k
             stats["point_ticks"] += k * sum(v[0].n for v in active.values())
+            if on_segment:
+                on_segment(ctx, {gid: v[0] for gid, v in active.items()})
             for gid in ids:
                 job, left, total, start, need = active[gid]
                 left -= k

@@ -101,7 +101,9 @@ def cfg4_jobs(rate: float = 40.0, duration: float = 120.0, seed: int = 31, n_lo:
     spaceSize of next_pow2(4 sqrt(n) 8); parameters as in the 10k workload."""
     from .serving import Job
 
-    k = int(rate * duration * 3) + 16
+    # draws sized for the full 120 s window whatever `duration` is, so a shorter
+    # window is the first part of the same request stream
+    k = int(rate * max(duration, 120.0) * 3) + 16
     u = datagen.uniform(6 * k, 0.0, 1.0, seed).astype(np.float64).reshape(6, k)
     gaps = -np.log(1.0 - u[0]) / rate
     arrivals = np.cumsum(gaps)

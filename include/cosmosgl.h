@@ -176,6 +176,9 @@ const char* cg_last_error(const cg_ctx* ctx);
  * integrated).  v: 2n floats; absent points get 0. */
 int cg_eval_force(cg_ctx* ctx, const cg_sim_params* params, int32_t force, float alpha,
                   float seed, float* v);
+/* The same for graph `id` of the set, with its stored parameters, evaluated in
+ * place inside the packed set. */
+int cg_graph_eval_force(cg_ctx* ctx, uint32_t id, int32_t force, float alpha, float seed, float* v);
 
 /* Allocation-free variant of cg_eval_force(CG_FORCE_MANYBODY).  It has no
  * counterpart in the reference; it is for callers that cannot allocate device
