@@ -97,7 +97,7 @@ _SIGNATURES = {
     "cg_debug_slots": (C.c_int, [_CTX, _I, C.c_uint32, _U, _U]),
     "cg_debug_link_csr": (C.c_int, [_CTX, C.c_int32, _U, _U, _F, _F, _U, C.c_uint32, _U]),
     "cg_debug_cells": (C.c_int, [_CTX, C.c_uint32, _U, _F, _U, C.c_uint32]),
-    "cg_debug_collision_keys": (C.c_int, [_CTX, _U, _U, C.c_uint32, _U, _U]),
+    "cg_debug_collision_cells": (C.c_int, [_CTX, _U, _U, _U, _U, C.c_uint32, _U, _U]),
     "cg_device_bytes": (C.c_uint64, [_CTX]),
     "cg_set_profiling": (C.c_int, [_CTX, C.c_uint32]),
     "cg_debug_link_stats": (C.c_int, [_CTX, _U]),
@@ -365,16 +365,22 @@ class Context:
             mem = mem[: int(counts.sum(dtype=np.int64))]
         return counts, cent.reshape(-1, 2), mem
 
-    def debug_collision_keys(self):
-        """(keys, points, G): the collision pass's sorted (grid * G^2 + cell,
-        point) pairs of graph 0."""
+    def debug_collision_cells(self):
+        """(keys, points, cells, counts, G) of graph 0's last collision pass: the
+        sorted (base cell, point) pairs, every point's cell in each of the four
+        offset grids (cells[i, o]; UINT32_MAX for absent points) and the point
+        count of every cell aggregate (counts[o, cell])."""
         cnt, G = C.c_uint32(), C.c_uint32()
-        self._check(self._l.cg_debug_collision_keys(self._ctx, None, None, 0, C.byref(cnt), C.byref(G)))
+        self._check(self._l.cg_debug_collision_cells(self._ctx, None, None, None, None, 0, C.byref(cnt),
+                                                     C.byref(G)))
         k = np.empty(max(cnt.value, 1), dtype=np.uint32)
         p = np.empty(max(cnt.value, 1), dtype=np.uint32)
-        self._check(self._l.cg_debug_collision_keys(self._ctx, uptr(k), uptr(p), k.size, C.byref(cnt),
-                                                    C.byref(G)))
-        return k[: cnt.value], p[: cnt.value], G.value
+        cells = np.empty(4 * max(self.n, 1), dtype=np.uint32)
+        counts = np.empty(4 * G.value * G.value, dtype=np.uint32)
+        self._check(self._l.cg_debug_collision_cells(self._ctx, uptr(k), uptr(p), uptr(cells), uptr(counts),
+                                                     k.size, C.byref(cnt), C.byref(G)))
+        return (k[: cnt.value], p[: cnt.value], cells[: 4 * self.n].reshape(-1, 4),
+                counts.reshape(4, -1), G.value)
 
     def debug_link_csr(self, direction: int):
         E = C.c_uint32()

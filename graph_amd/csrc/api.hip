@@ -303,8 +303,9 @@ void free_packed(Ctx* c) {
     release(c, c->flags); release(c, c->present); release(c, c->gid); release(c, c->jitter);
     release(c, c->size); release(c, c->desc_base); c->desc = nullptr;
     release(c, c->f0s); release(c, c->colc0s); release(c, c->blk_graph);
+    release(c, c->col_ssize);
     for (int b = 0; b < 2; ++b) {
-        release(c, c->col_keys[b]); release(c, c->col_vals[b]);
+        release(c, c->col_keys[b]); release(c, c->col_vals[b]); release(c, c->col_pos[b]);
         release(c, c->pyr[b]);
         c->n_pyr[b] = 0;
     }
@@ -432,7 +433,7 @@ int pack_body(Ctx* c, const PackSnap& old) {
             NPC += g.np;
         }
     }
-    if (N >= (1ull << 31) || F * SLOTS >= (1ull << 32) || C >= (1ull << 31) || CG >= (1ull << 32) ||
+    if (N >= (1ull << 31) || F * SLOTS >= (1ull << 32) || C >= (1ull << 31) || CG >= (1ull << 28) ||
         4 * NPC >= (1ull << 31))
         return fail(c, CG_ERR_INVALID, "the graph set exceeds the 32-bit index space of one context");
     std::vector<uint32_t> old_p0(G);
@@ -1724,23 +1725,57 @@ int cg_debug_cells(cg_ctx* x, uint32_t level, uint32_t* counts, float* centroid,
     return CG_OK;
 }
 
-int cg_debug_collision_keys(cg_ctx* x, uint32_t* keys, uint32_t* points, uint32_t capacity, uint32_t* count,
-                            uint32_t* grid) {
+int cg_debug_collision_cells(cg_ctx* x, uint32_t* keys, uint32_t* points, uint32_t* cells, uint32_t* counts,
+                             uint32_t capacity, uint32_t* count, uint32_t* grid) {
     CG_CTX(x);
-    if (c->graphs.empty() || !c->graphs[0].live) return fail(c, CG_ERR_STATE, "cg_debug_collision_keys: no graph");
+    if (c->graphs.empty() || !c->graphs[0].live) return fail(c, CG_ERR_STATE, "cg_debug_collision_cells: no graph");
     const Graph& g = c->graphs[0];
     if (c->dirty || !collides(g) || !c->col_keys[0])
-        return fail(c, CG_ERR_STATE, "cg_debug_collision_keys: no collision pass yet");
+        return fail(c, CG_ERR_STATE, "cg_debug_collision_cells: no collision pass yet");
     if (grid) *grid = g.col_G;
-    const uint32_t nk = 4 * c->NPC;
+    if (count) *count = g.np;
+    if ((keys || points) && capacity < g.np) return fail(c, CG_ERR_INVALID, "cg_debug_collision_cells: capacity");
+    const uint32_t npc = c->NPC;
     const int fin = c->col_rs.passes & 1;
-    std::vector<uint32_t> hk(nk), hv(nk);
-    CG_HIP(c, hipMemcpyAsync(hk.data(), c->col_keys[fin], (size_t)nk * 4, hipMemcpyDeviceToHost, c->stream));
-    CG_HIP(c, hipMemcpyAsync(hv.data(), c->col_vals[fin], (size_t)nk * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> hk(npc), hv(npc);
+    CG_HIP(c, hipMemcpyAsync(hk.data(), c->col_keys[fin], (size_t)npc * 4, hipMemcpyDeviceToHost, c->stream));
+    CG_HIP(c, hipMemcpyAsync(hv.data(), c->col_vals[fin], (size_t)npc * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> pc;
+    if (cells) {
+        // each point's four cells from its current positions, by the kernels' cell function
+        std::vector<GDesc> table(c->G);
+        for (uint32_t s = 0; s < c->G; ++s) {
+            const Graph& h = c->graphs[c->order[s]];
+            GDesc d = c->h_desc[s];
+            fill_dynamic(d, h.params, h.extent, (float)h.alpha, 0.f);
+            d.passes = 0;
+            d.inbuf = h.cur ? ((1u << P_COUNT) - 1) : 0u;
+            table[s] = d;
+        }
+        int rc = write_table(c, table);
+        if (rc) return rc;
+        uint32_t* dc = (uint32_t*)dalloc(c, (size_t)npc * 16, &rc);
+        if (rc) return rc;
+        rc = col_point_cells(c, dc);
+        pc.resize(4 * (size_t)npc);
+        hipError_t e = rc ? hipSuccess : hipMemcpyAsync(pc.data(), dc, (size_t)npc * 16, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dfree(c, dc);
+        if (rc) return rc;
+        if (e != hipSuccess) return hip_fail(c, e, "cg_debug_collision_cells readback");
+    }
+    if (counts) {
+        // the count channel of the four grids' cell aggregates
+        const size_t nc = 4 * (size_t)g.col_G * g.col_G;
+        std::vector<float4> agg(nc);
+        CG_HIP(c, hipMemcpyAsync(agg.data(), c->col_cell + g.col_c0, nc * 16, hipMemcpyDeviceToHost, c->stream));
+        CG_HIP(c, hipStreamSynchronize(c->stream));
+        for (size_t q = 0; q < nc; ++q) counts[q] = (uint32_t)agg[q].w;
+    }
     CG_HIP(c, hipStreamSynchronize(c->stream));
-    const uint32_t lo = g.col_c0, hi = g.col_c0 + 4 * g.col_G * g.col_G;
+    const uint32_t lo = g.col_c0 / 4, hi = lo + g.col_G * g.col_G;
     uint32_t w = 0;
-    for (uint32_t k = 0; k < nk; ++k) {
+    for (uint32_t k = 0; k < npc; ++k) {
         if (hk[k] < lo || hk[k] >= hi) continue;
         if (w < capacity) {
             if (keys) keys[w] = hk[k] - lo;
@@ -1748,8 +1783,18 @@ int cg_debug_collision_keys(cg_ctx* x, uint32_t* keys, uint32_t* points, uint32_
         }
         ++w;
     }
-    if (count) *count = w;
-    if ((keys || points) && w > capacity) return fail(c, CG_ERR_INVALID, "cg_debug_collision_keys: capacity");
+    if (w != g.np) return fail(c, CG_ERR_STATE, "cg_debug_collision_cells: sorted pairs do not cover the graph");
+    if (cells) {
+        for (size_t q = 0; q < 4 * (size_t)g.n; ++q) cells[q] = 0xffffffffu;
+        // the collision present list is graph-major: graph 0's points come first
+        uint32_t k0 = 0;
+        for (uint32_t s = 0; s < g.slot; ++s)
+            if (c->h_desc[s].col_G) k0 += c->h_desc[s].np;
+        std::vector<uint32_t> cp(g.np);
+        CG_HIP(c, hipMemcpy(cp.data(), c->cpresent + k0, (size_t)g.np * 4, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < g.np; ++k)
+            for (int o = 0; o < 4; ++o) cells[4 * (size_t)(cp[k] - g.p0) + o] = pc[4 * (size_t)(k0 + k) + o];
+    }
     return CG_OK;
 }
 

@@ -238,12 +238,14 @@ struct Ctx {
     // collision (graphs with simulationCollision on)
     uint32_t NPC = 0;               // their present points
     uint32_t* cpresent = nullptr;   // [NPC]
-    uint32_t* col_keys[2] = {nullptr, nullptr};  // [4 NPC]
-    uint32_t* col_vals[2] = {nullptr, nullptr};
+    uint32_t* col_keys[2] = {nullptr, nullptr};  // [NPC] base-grid cell keys, ping-pong
+    uint32_t* col_vals[2] = {nullptr, nullptr};  // [NPC] point indices
+    float2* col_pos[2] = {nullptr, nullptr};     // [NPC] positions riding the sort
+    float* col_ssize = nullptr;     // [NPC] sizes in sorted order
     RadixSort col_rs;
-    uint32_t* col_begin = nullptr;  // [CG]
+    uint32_t* col_begin = nullptr;  // [CG / 4] per base cell
     uint32_t* col_end = nullptr;
-    float4* col_cell = nullptr;     // [CG]
+    float4* col_cell = nullptr;     // [CG] the four grids' cell aggregates
 
     // clusters: point -> packed cluster, per-cluster targets, present members by cluster
     uint32_t CC = 0;                // packed clusters
@@ -332,6 +334,8 @@ int link_run(Ctx* c, int dir, float2* vout);
 int col_prepare_graph(Ctx* c, Graph& g);                          // k_collision.hip
 int col_run(Ctx* c, float2* vout);
 int col_alloc(Ctx* c);
+int col_point_cells(Ctx* c, uint32_t* out);                       // [4 NPC] cells per point, per grid
+constexpr uint32_t RADIX_TILE = 2048;  // pairs per block of the radix sort (k_manybody.hip)
 
 // ---------------------------------------------------------------------------
 // device helpers

@@ -1,12 +1,18 @@
 // k_collision.hip — spatial-hash collision over four offset grids.
 //
-// Per offset o in {(0,0), (.5,0), (0,.5), (.5,.5)}: integer row-major cell key
-// per present point, one radix sort of (key, index) over all four grids and all
-// graphs, segment bounds, and a fixed-order per-cell reduction of
-// (x, y, size, count).  Sums are kept
-// relative to the cell's origin so the force pass can remove the point's own
-// contribution without f32 cancellation.  The force pass sweeps the 3x3 cell
-// aggregates of all four grids per point and integrates.
+// build-grid.vert bins every point into four grids of the same cell size h,
+// offset by o in {(0,0), (.5,0), (0,.5), (.5,.5)} cells: cell = clamp(floor((p +
+// o h) / h), 0, G-1) per axis.  Here each point is sorted once, by its cell of
+// the unshifted (base) grid, its position riding the sort; a point's cell in a
+// shifted grid is its base cell or the next one along each shifted axis, so a
+// cell of grid o gathers the points of at most 4 base cells and keeps those
+// whose own grid-o cell (the exact formula above) is that cell.  Per cell, in
+// a fixed order (base cells ascending, points by index within each), the sums
+// of (x - x0, y - y0, size) and the count; relative to the cell's origin so
+// the force pass can remove the point's own contribution without f32
+// cancellation.  The force pass visits the points in base-cell order (a wave
+// reads the same few cells), sweeps the 3x3 cell aggregates of all four grids
+// and integrates.
 //
 // Reference: ForceCollision/index.ts:61-164 (grid geometry, sizes),
 // :264-345 (run); build-grid.vert:28-62; force-collision-spatial.frag:41-168.
@@ -30,77 +36,130 @@ __device__ inline int col_cell(float p, float o, float h, int G) {
     return cell_of(__fadd_rn(p, __fmul_rn(o, h)), h, G);
 }
 
-// One key per (point, offset grid): the graph's collision block, then the grid,
-// then the row-major cell.  Work item t = o * npc + k.
+// One base-grid key per present collision point (the graph's block of base
+// cells, col_c0 / 4, then the row-major cell), its index and position.
 __global__ void k_col_keys(const float2* __restrict__ pos0, const float2* __restrict__ pos1,
                            const uint32_t* __restrict__ cpresent, uint32_t npc,
                            const uint32_t* __restrict__ gid, const GDesc* __restrict__ desc,
-                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 4 * npc) return;
-    const uint32_t o = t / npc, k = t - o * npc;
+                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                           float2* __restrict__ pays) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npc) return;
     const uint32_t i = cpresent[k];
     const GDesc& d = desc[gid[i]];
     const float2 p = ((d.inbuf >> P_COLLISION) & 1u) ? pos1[i] : pos0[i];
     const int G = (int)d.col_G;
-    int cx = col_cell(p.x, c_off[o][0], d.col_h, G), cy = col_cell(p.y, c_off[o][1], d.col_h, G);
-    keys[t] = d.col_c0 + o * (uint32_t)(G * G) + (uint32_t)(cy * G + cx);
-    vals[t] = i;
+    const int cx = col_cell(p.x, 0.f, d.col_h, G), cy = col_cell(p.y, 0.f, d.col_h, G);
+    keys[k] = d.col_c0 / 4u + (uint32_t)(cy * G + cx);
+    vals[k] = i;
+    pays[k] = p;
 }
 
-__global__ void k_col_bounds(const uint32_t* __restrict__ sk, uint32_t nk,
-                             uint32_t* __restrict__ begin, uint32_t* __restrict__ end) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+// Base-cell bounds from the sorted keys (end cleared first; end 0 = empty) and
+// the sizes in sorted order.
+__global__ void k_col_bounds(const uint32_t* __restrict__ sk, const uint32_t* __restrict__ sv, uint32_t nk,
+                             const float* __restrict__ size, uint32_t* __restrict__ begin,
+                             uint32_t* __restrict__ end, float* __restrict__ ssize) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nk) return;
-    uint32_t key = sk[k];
+    const uint32_t key = sk[k];
     if (k == 0 || sk[k - 1] != key) begin[key] = k;
     if (k == nk - 1 || sk[k + 1] != key) end[key] = k + 1;
+    ssize[k] = size[sv[k]];
 }
 
-// One lane per cell of every graph's four grids: (sum(x - x0), sum(y - y0),
-// sum(size), count) in f64, stored as f32, with (x0, y0) = cell index * h.
-__global__ void k_col_cells(const uint32_t* __restrict__ begin, const uint32_t* __restrict__ end,
-                            const uint32_t* __restrict__ sv, const float2* __restrict__ pos0,
-                            const float2* __restrict__ pos1, const float* __restrict__ size,
-                            const uint32_t* __restrict__ colc0s, const GDesc* __restrict__ desc,
-                            uint32_t G, uint32_t CG, float4* __restrict__ cells) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= CG) return;
-    const GDesc& d = desc[find_graph(colc0s, G, c)];
-    const float2* __restrict__ pin = ((d.inbuf >> P_COLLISION) & 1u) ? pos1 : pos0;
-    const uint32_t GG = d.col_G * d.col_G;
-    const uint32_t local = (c - d.col_c0) % GG;
-    const uint32_t e = end[c], b = e ? begin[c] : 0u;  // end 0: empty (begin not written)
-    float x0 = (float)(local % d.col_G) * d.col_h, y0 = (float)(local / d.col_G) * d.col_h;
-    double sx = 0.0, sy = 0.0, ss = 0.0;
-    for (uint32_t k = b; k < e; ++k) {
-        uint32_t i = sv[k];
-        float2 p = pin[i];
-        sx += (double)p.x - (double)x0;
-        sy += (double)p.y - (double)y0;
-        ss += (double)size[i];
+// CELL_LANES lanes per cell of every graph's four grids: (sum(x - x0),
+// sum(y - y0), sum(size), count) in f64, stored as f32, with (x0, y0) = cell
+// index * h.  The lanes stride over the points of the cell's base cells
+// (batches of COL_BATCH loads in flight), keep those whose own cell of this
+// grid is this cell, and reduce in a fixed tree, so the result depends only on
+// the points' order in the sort (by index within a base cell).
+constexpr int CELL_LANES = 16;
+constexpr int COL_BATCH = 4;
+__global__ void __launch_bounds__(256) k_col_cells(const uint32_t* __restrict__ begin,
+                                                   const uint32_t* __restrict__ end,
+                                                   const float2* __restrict__ spos,
+                                                   const float* __restrict__ ssize,
+                                                   const uint32_t* __restrict__ colc0s,
+                                                   const GDesc* __restrict__ desc, uint32_t G,
+                                                   uint32_t CG, float4* __restrict__ cells) {
+    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = gt / CELL_LANES, sub = gt % CELL_LANES;
+    const bool valid = c < CG;
+    double ax = 0.0, ay = 0.0, as = 0.0;
+    uint32_t n = 0;
+    if (valid) {
+        const GDesc& d = desc[find_graph(colc0s, G, c)];
+        const int Gc = (int)d.col_G;
+        const uint32_t GG = d.col_G * d.col_G;
+        const uint32_t local = c - d.col_c0;
+        const int o = (int)(local / GG);
+        const uint32_t cell = local - (uint32_t)o * GG;
+        const int cx = (int)(cell % d.col_G), cy = (int)(cell / d.col_G);
+        const float h = d.col_h;
+        const float x0 = (float)cx * h, y0 = (float)cy * h;
+        const float ox = c_off[o][0], oy = c_off[o][1];
+        const int sx = ox != 0.f ? 1 : 0, sy = oy != 0.f ? 1 : 0;  // base cells cx - sx .. cx
+        const uint32_t b0 = d.col_c0 / 4u;
+        for (int by = max(cy - sy, 0); by <= cy; ++by) {
+            for (int bx = max(cx - sx, 0); bx <= cx; ++bx) {
+                const uint32_t bc = b0 + (uint32_t)(by * Gc + bx);
+                const uint32_t e = end[bc];
+                if (e == 0) continue;
+                for (uint32_t k = begin[bc] + sub; k < e; k += COL_BATCH * CELL_LANES) {
+                    float2 pp[COL_BATCH];
+                    float sz[COL_BATCH];
+#pragma unroll
+                    for (int u = 0; u < COL_BATCH; ++u) {
+                        const uint32_t kk = min(k + u * CELL_LANES, e - 1);
+                        pp[u] = spos[kk];
+                        sz[u] = ssize[kk];
+                    }
+#pragma unroll
+                    for (int u = 0; u < COL_BATCH; ++u) {
+                        if (k + u * CELL_LANES >= e) break;
+                        const float2 p = pp[u];
+                        if ((sx && col_cell(p.x, ox, h, Gc) != cx) || (sy && col_cell(p.y, oy, h, Gc) != cy))
+                            continue;
+                        ax += (double)p.x - (double)x0;
+                        ay += (double)p.y - (double)y0;
+                        as += (double)sz[u];
+                        ++n;
+                    }
+                }
+            }
+        }
     }
-    cells[c] = make_float4((float)sx, (float)sy, (float)ss, (float)(e - b));
+    // fixed-tree reduction over the cell's lanes (aligned groups of CELL_LANES)
+#pragma unroll
+    for (int w = CELL_LANES / 2; w > 0; w >>= 1) {
+        ax += __shfl_down(ax, w, CELL_LANES);
+        ay += __shfl_down(ay, w, CELL_LANES);
+        as += __shfl_down(as, w, CELL_LANES);
+        n += __shfl_down(n, w, CELL_LANES);
+    }
+    if (valid && sub == 0) cells[c] = make_float4((float)ax, (float)ay, (float)as, (float)n);
 }
 
+// One lane per present collision point, in base-cell order.
 __global__ void __launch_bounds__(256) k_col_force(const float4* __restrict__ cells,
                                                    float2* __restrict__ pos0,
                                                    float2* __restrict__ pos1,
-                                                   const uint32_t* __restrict__ cpresent,
+                                                   const uint32_t* __restrict__ sv,
+                                                   const float2* __restrict__ spos,
+                                                   const float* __restrict__ ssize,
                                                    uint32_t npc, const uint32_t* __restrict__ gid,
                                                    const GDesc* __restrict__ desc,
-                                                   const float* __restrict__ size,
                                                    const uint8_t* __restrict__ flags,
                                                    float2* __restrict__ vout) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npc) return;
-    const uint32_t i = cpresent[k];
-    const uint32_t gslot = gid[i];
-    const GDesc& a = desc[gslot];
+    const uint32_t i = sv[k];
+    const GDesc& a = desc[gid[i]];
     if (!(a.passes & (1u << P_COLLISION))) return;
     const bool bsel = (a.inbuf >> P_COLLISION) & 1u;
-    const float2 p = (bsel ? pos1 : pos0)[i];
-    const float si = size[i];
+    const float2 p = spos[k];
+    const float si = ssize[k];
     const float ri = (a.col_radius > 0.f ? a.col_radius : si * 0.5f) + a.col_padding;
     const int G = (int)a.col_G;
     const size_t GG = (size_t)G * G;
@@ -110,13 +169,21 @@ __global__ void __launch_bounds__(256) k_col_force(const float4* __restrict__ ce
         const int mx = col_cell(p.x, c_off[o][0], a.col_h, G);
         const int my = col_cell(p.y, c_off[o][1], a.col_h, G);
         float ux = 0.f, uy = 0.f, total = 0.f;
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int cx = mx + dx;
-            if (cx < 0 || cx >= G) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int cy = my + dy;
-                if (cy < 0 || cy >= G) continue;
-                float4 cd = cg_[cy * G + cx];
+        // the 3x3 cells' aggregates, all loads in flight before any use (cells
+        // outside the grid read as empty)
+        float4 nb[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int cx = mx + q / 3 - 1, cy = my + q % 3 - 1;
+            const bool in = cx >= 0 && cx < G && cy >= 0 && cy < G;
+            nb[q] = in ? cg_[cy * G + cx] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int dx = q / 3 - 1, dy = q % 3 - 1;
+            const int cx = mx + dx, cy = my + dy;
+            {
+                float4 cd = nb[q];
                 float cnt = cd.w;
                 if (cnt < 0.5f) continue;
                 const float x0 = (float)cx * a.col_h, y0 = (float)cy * a.col_h;
@@ -129,20 +196,24 @@ __global__ void __launch_bounds__(256) k_col_force(const float4* __restrict__ ce
                 }
                 if (cnt < 0.5f) continue;
                 total += cnt;
-                // average of the other points in the cell, relative to p
-                float ddx = (p.x - x0) - rx / cnt;
-                float ddy = (p.y - y0) - ry / cnt;
-                float as = ss / cnt;
+                // average of the other points in the cell, relative to p (the
+                // hardware reciprocal and square root, ~1 ulp: the divisions
+                // and lengths of the shader, well inside the force tolerance)
+                const float ic = __builtin_amdgcn_rcpf(cnt);
+                float ddx = (p.x - x0) - rx * ic;
+                float ddy = (p.y - y0) - ry * ic;
+                float as = ss * ic;
                 float ro = (a.col_radius > 0.f ? a.col_radius : as * 0.5f) + a.col_padding;
                 float rho = ri + ro;
-                float dd = sqrtf(ddx * ddx + ddy * ddy);
+                float dd = __builtin_amdgcn_sqrtf(ddx * ddx + ddy * ddy);
                 if (dd < rho && dd > 0.001f) {
-                    float ov = (rho - dd) / rho;
-                    float soft = sqrtf(ov) * rho * 0.5f;
+                    const float id = __builtin_amdgcn_rcpf(dd);
+                    float ov = (rho - dd) * __builtin_amdgcn_rcpf(rho);
+                    float soft = __builtin_amdgcn_sqrtf(ov) * rho * 0.5f;
                     float f = a.alpha * a.col_strength * soft * 0.25f * cnt;
                     f = fminf(f, rho * 0.5f);
-                    ux += f * (ddx / dd);
-                    uy += f * (ddy / dd);
+                    ux += f * (ddx * id);
+                    uy += f * (ddy * id);
                 } else if (dd <= 0.001f && cnt > 0.f) {
                     // golden-angle kick by the point's index in its graph
                     float ang = (float)(i - a.p0) * 0.618033988749895f;
@@ -155,14 +226,14 @@ __global__ void __launch_bounds__(256) k_col_force(const float4* __restrict__ ce
             }
         }
         if (total > 2.f) {
-            float damp = 2.f / total;
+            float damp = 2.f * __builtin_amdgcn_rcpf(total);
             ux *= damp;
             uy *= damp;
         }
         float maxc = ri * 0.1f;
-        float corr = sqrtf(ux * ux + uy * uy);
+        float corr = __builtin_amdgcn_sqrtf(ux * ux + uy * uy);
         if (corr > maxc) {
-            float s = maxc / corr;
+            float s = maxc * __builtin_amdgcn_rcpf(corr);
             ux *= s;
             uy *= s;
         }
@@ -176,6 +247,23 @@ __global__ void __launch_bounds__(256) k_col_force(const float4* __restrict__ ce
     }
     float2* __restrict__ pout = bsel ? pos0 : pos1;
     pout[i] = (flags[i] & F_PINNED) ? p : integrate(p, v, a.friction, a.S);
+}
+
+// The four grids' cells of every present collision point (inspection), by
+// the same cell function the cell and force kernels use.
+__global__ void k_col_point_cells(const float2* __restrict__ pos0, const float2* __restrict__ pos1,
+                                  const uint32_t* __restrict__ cpresent, uint32_t npc,
+                                  const uint32_t* __restrict__ gid, const GDesc* __restrict__ desc,
+                                  uint32_t* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npc) return;
+    const uint32_t i = cpresent[k];
+    const GDesc& d = desc[gid[i]];
+    const float2 p = ((d.inbuf >> P_COLLISION) & 1u) ? pos1[i] : pos0[i];
+    const int G = (int)d.col_G;
+    for (int o = 0; o < 4; ++o)
+        out[4 * (size_t)k + o] = (uint32_t)(col_cell(p.y, c_off[o][1], d.col_h, G) * G +
+                                            col_cell(p.x, c_off[o][0], d.col_h, G));
 }
 
 }  // namespace
@@ -207,38 +295,66 @@ int col_prepare_graph(Ctx* c, Graph& g) {
 }
 
 int col_run(Ctx* c, float2* vout) {
-    const uint32_t npc = c->NPC, CG = c->CG;
+    const uint32_t npc = c->NPC, CG = c->CG, CB = c->CG / 4;
     if (npc == 0 || CG == 0) return CG_OK;
-    const uint32_t nk = 4 * npc;
-    hipLaunchKernelGGL(k_col_keys, dim3(blocks_for(nk, 256)), dim3(256), 0, c->stream, c->pos[0],
-                       c->pos[1], c->cpresent, npc, c->gid, c->desc, c->col_keys[0], c->col_vals[0]);
-    radix_run(c, c->col_rs, c->col_keys, c->col_vals, false);
+    hipLaunchKernelGGL(k_col_keys, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->pos[0], c->pos[1],
+                       c->cpresent, npc, c->gid, c->desc, c->col_keys[0], c->col_vals[0], c->col_pos[0]);
+    radix_run(c, c->col_rs, c->col_keys, c->col_vals, false, c->col_pos);
     const int fin = c->col_rs.passes & 1;
-    hipLaunchKernelGGL(k_fill32, dim3(blocks_for(CG, 256)), dim3(256), 0, c->stream, c->col_end, CG, 0u);
-    hipLaunchKernelGGL(k_col_bounds, dim3(blocks_for(nk, 256)), dim3(256), 0, c->stream, c->col_keys[fin],
-                       nk, c->col_begin, c->col_end);
-    hipLaunchKernelGGL(k_col_cells, dim3(blocks_for(CG, 256)), dim3(256), 0, c->stream, c->col_begin,
-                       c->col_end, c->col_vals[fin], c->pos[0], c->pos[1], c->size, c->colc0s, c->desc,
-                       c->G, CG, c->col_cell);
+    hipLaunchKernelGGL(k_fill32, dim3(blocks_for(CB, 256)), dim3(256), 0, c->stream, c->col_end, CB, 0u);
+    hipLaunchKernelGGL(k_col_bounds, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->col_keys[fin],
+                       c->col_vals[fin], npc, c->size, c->col_begin, c->col_end, c->col_ssize);
+    hipLaunchKernelGGL(k_col_cells, dim3(blocks_for((uint64_t)CG * CELL_LANES, 256)), dim3(256), 0, c->stream, c->col_begin,
+                       c->col_end, c->col_pos[fin], c->col_ssize, c->colc0s, c->desc, c->G, CG, c->col_cell);
     hipLaunchKernelGGL(k_col_force, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->col_cell,
-                       c->pos[0], c->pos[1], c->cpresent, npc, c->gid, c->desc, c->size, c->flags, vout);
+                       c->pos[0], c->pos[1], c->col_vals[fin], c->col_pos[fin], c->col_ssize, npc, c->gid, c->desc,
+                       c->flags, vout);
     CG_HIP(c, hipGetLastError());
     return CG_OK;
 }
 
-// Packed collision arrays for the graphs with collision on (called by the packer).
+int col_point_cells(Ctx* c, uint32_t* out) {
+    if (c->NPC == 0) return CG_OK;
+    hipLaunchKernelGGL(k_col_point_cells, dim3(blocks_for(c->NPC, 256)), dim3(256), 0, c->stream, c->pos[0],
+                       c->pos[1], c->cpresent, c->NPC, c->gid, c->desc, out);
+    CG_HIP(c, hipGetLastError());
+    return CG_OK;
+}
+
+// Packed collision arrays for the graphs with collision on (called by the
+// packer): the base-grid sort (segmented per graph: the present list of the
+// collision graphs is graph-major), cell bounds and the four grids' cells.
 int col_alloc(Ctx* c) {
     int rc = CG_OK;
-    const uint32_t nk = 4 * c->NPC;
-    if (nk == 0 || c->CG == 0) return CG_OK;
+    const uint32_t npc = c->NPC;
+    if (npc == 0 || c->CG == 0) return CG_OK;
     for (int b = 0; b < 2; ++b) {
-        c->col_keys[b] = (uint32_t*)dalloc(c, (size_t)nk * 4, &rc); if (rc) return rc;
-        c->col_vals[b] = (uint32_t*)dalloc(c, (size_t)nk * 4, &rc); if (rc) return rc;
+        c->col_keys[b] = (uint32_t*)dalloc(c, (size_t)npc * 4, &rc); if (rc) return rc;
+        c->col_vals[b] = (uint32_t*)dalloc(c, (size_t)npc * 4, &rc); if (rc) return rc;
+        c->col_pos[b] = (float2*)dalloc(c, (size_t)npc * 8, &rc); if (rc) return rc;
     }
-    c->col_begin = (uint32_t*)dalloc(c, (size_t)c->CG * 4, &rc); if (rc) return rc;
-    c->col_end = (uint32_t*)dalloc(c, (size_t)c->CG * 4, &rc); if (rc) return rc;
+    c->col_ssize = (float*)dalloc(c, (size_t)npc * 4, &rc); if (rc) return rc;
+    c->col_begin = (uint32_t*)dalloc(c, (size_t)(c->CG / 4) * 4, &rc); if (rc) return rc;
+    c->col_end = (uint32_t*)dalloc(c, (size_t)(c->CG / 4) * 4, &rc); if (rc) return rc;
     c->col_cell = (float4*)dalloc(c, (size_t)c->CG * 16, &rc); if (rc) return rc;
-    return radix_alloc(c, c->col_rs, nk, c->CG);
+    std::vector<uint4> blk;
+    std::vector<uint32_t> key0;
+    uint64_t range = 1;
+    uint32_t k0 = 0;
+    for (uint32_t s = 0; s < c->G; ++s) {
+        const GDesc& d = c->h_desc[s];
+        if (d.col_G == 0) continue;  // collision off for this graph
+        range = std::max<uint64_t>(range, (uint64_t)d.col_G * d.col_G);
+        const uint32_t fb = (uint32_t)blk.size(), nbk = (d.np + RADIX_TILE - 1) / RADIX_TILE;
+        for (uint32_t q = 0; q < nbk; ++q) {
+            const uint32_t b0 = k0 + q * RADIX_TILE;
+            blk.push_back(make_uint4(b0, std::min(k0 + d.np, b0 + RADIX_TILE), fb, fb + nbk));
+            key0.push_back(d.col_c0 / 4u);
+        }
+        k0 += d.np;
+    }
+    const bool one = blk.size() && blk.front().z == 0 && blk.front().w == blk.size() && key0.front() == 0;
+    return one ? radix_alloc(c, c->col_rs, npc, range) : radix_alloc(c, c->col_rs, npc, range, &blk, &key0);
 }
 
 }  // namespace cg

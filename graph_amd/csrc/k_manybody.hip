@@ -56,6 +56,7 @@ __device__ inline uint32_t finest_key(const GDesc& d, float2 q) {
 // atomics: on this part they serialise at a few ns each.
 constexpr int RB_ITEMS = 8;
 constexpr int RB_TILE = 256 * RB_ITEMS;
+static_assert(RB_TILE == (int)RADIX_TILE, "segment tables are built with RADIX_TILE");
 constexpr int RB_MAXW = 9;               // digits of up to 9 bits: 18-bit cell keys in two passes
 constexpr int RB_RADIX = 1 << RB_MAXW;
 

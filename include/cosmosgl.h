@@ -214,13 +214,17 @@ int cg_debug_slots(cg_ctx* ctx, int32_t* slots, uint32_t capacity, uint32_t* fin
 int cg_debug_cells(cg_ctx* ctx, uint32_t level, uint32_t* counts, float* centroid, uint32_t* members,
                    uint32_t capacity);
 
-/* Spatial-hash keys of the last collision pass (build-grid.vert:46-55), in the
- * order the device sorted them: keys[k] = o * G*G + row-major cell of offset
- * grid o (offsets (0,0), (0.5,0), (0,0.5), (0.5,0.5) of a cell) and points[k]
- * the graph-local point; 4 * np pairs, ascending by (key, point).  grid
- * receives G (cells per side).  count receives the number of pairs. */
-int cg_debug_collision_keys(cg_ctx* ctx, uint32_t* keys, uint32_t* points, uint32_t capacity,
-                            uint32_t* count, uint32_t* grid);
+/* Spatial hash of the last collision pass (build-grid.vert:46-55).  keys and
+ * points: the np (cell, point) pairs of the unshifted grid (row-major cells)
+ * in the order the device sorted them, ascending by (cell, point).  cells:
+ * NULL or 4*n entries, at [4i + o] the row-major cell of point i in offset
+ * grid o (offsets (0,0), (0.5,0), (0,0.5), (0.5,0.5) of a cell) as the cell
+ * and force kernels compute it from the current positions, UINT32_MAX for
+ * absent points.  counts: NULL or 4*G*G point counts of the four grids' cell
+ * aggregates the last pass built (grid o's row-major cells at o*G*G).  grid
+ * receives G (cells per side); count receives np. */
+int cg_debug_collision_cells(cg_ctx* ctx, uint32_t* keys, uint32_t* points, uint32_t* cells,
+                             uint32_t* counts, uint32_t capacity, uint32_t* count, uint32_t* grid);
 
 /* Link CSR of one direction (0 = INCOMING instance, 1 = OUTGOING) in the
  * reference's entry order: offsets[n+1], nbr[E], bias[E], strength[E], and

@@ -4,9 +4,9 @@
   per-cell membership as sets (the order inside a cell follows the previous
   tick's order by design), per-level cell counts of the pyramid, and the
   level centroids within 1e-5 * spaceSize of the f64 means;
-- spatial-hash collision keys (build-grid.vert:46-55) and their sort order:
-  the device's sorted (key, point) sequence equals the lexicographic sort of
-  the oracle's keys;
+- spatial-hash collision cells (build-grid.vert:46-55): every point's cell in
+  each of the four offset grids, and the device's sorted (base cell, point)
+  sequence equals the lexicographic sort of the oracle's keys;
 
 plus the context's behaviour around them: a repack refused by the HBM budget
 leaves the previous packing intact, positions far outside [0, spaceSize]
@@ -153,18 +153,25 @@ def test_collision_keys_and_sort_order_bit_exact(case):
     c.set_positions(xy)
     c.set_point_sizes(sizes)
     c.eval_force(params(S, collision=0.7), _lib.FORCE_COLLISION, 0.8)
-    keys, pts, Gd = c.debug_collision_keys()
+    keys, pts, cells, counts, Gd = c.debug_collision_cells()
     assert Gd == G
-    wk, wp = [], []
+    # the sorted (base cell, point) sequence
+    base = O.cells(xy, h, G, absent)[pres]
+    wk = base[:, 1].astype(np.int64) * G + base[:, 0]
+    order = np.lexsort((pres, wk))
+    assert keys.size == pres.size
+    assert np.array_equal(keys.astype(np.int64), wk[order]), "base-grid keys differ"
+    assert np.array_equal(pts.astype(np.int64), pres[order]), "collision sort order differs"
+    # every point's cell in each offset grid
     for o, off in enumerate(OFFSETS):
         cell = O.cells(xy, h, G, absent, off)[pres]
-        wk.append(o * G * G + cell[:, 1].astype(np.int64) * G + cell[:, 0])
-        wp.append(pres)
-    wk, wp = np.concatenate(wk), np.concatenate(wp)
-    order = np.lexsort((wp, wk))
-    assert keys.size == wk.size
-    assert np.array_equal(keys.astype(np.int64), wk[order]), "collision keys differ"
-    assert np.array_equal(pts.astype(np.int64), wp[order]), "collision sort order differs"
+        want = cell[:, 1].astype(np.int64) * G + cell[:, 0]
+        got = cells[pres, o].astype(np.int64)
+        assert np.array_equal(got, want), f"grid {o}: {int(np.sum(got != want))} points in other cells"
+        # the cell aggregates hold exactly those points
+        assert np.array_equal(counts[o].astype(np.int64), np.bincount(want, minlength=G * G)), \
+            f"grid {o}: aggregate counts differ"
+    assert (cells[absent == 1] == 0xFFFFFFFF).all()
     c.close()
 
 
