@@ -122,9 +122,16 @@ class BatchServer:
         self.budget = int(budget)
         self.segment = int(segment)
         self.ctx = _lib.Context(device=device, hbm_budget=self.budget)
+        self._cancel: set[int] = set()
 
     def close(self) -> None:
         self.ctx.close()
+
+    def cancel(self, jid: int) -> None:
+        """Withdraws job `jid`: a waiting job is never admitted, a resident one
+        leaves the device at the next segment boundary.  Either way it has no
+        result and is listed in the run's stats["cancelled"]."""
+        self._cancel.add(int(jid))
 
     def run(self, jobs: list[Job], log=None, on_segment=None) -> tuple[list[Result], dict]:
         """Serves `jobs` (arrival times relative to the call).  on_segment(ctx,
@@ -134,11 +141,19 @@ class BatchServer:
         active: dict[int, tuple[Job, int, int, float, int]] = {}  # gid -> job, left, total, start, bytes
         held = 0
         results: list[Result] = []
-        stats = dict(segments=0, ticks=0, max_active=0, point_ticks=0, rejected=[])
+        stats = dict(segments=0, ticks=0, max_active=0, point_ticks=0, rejected=[], cancelled=[])
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         while pending or active:
             t = now()
+            if self._cancel:
+                for gid in [g for g, v in active.items() if v[0].jid in self._cancel]:
+                    ctx.graph_remove(gid)
+                    held -= active[gid][4]
+                    stats["cancelled"].append(active.pop(gid)[0].jid)
+                for j in [j for j in pending if j.jid in self._cancel]:
+                    pending.remove(j)
+                    stats["cancelled"].append(j.jid)
             # admit every arrived job that fits, in arrival order (no overtaking)
             while pending and pending[0].arrival <= t:
                 job = pending[0]
@@ -159,6 +174,8 @@ class BatchServer:
                 active[gid] = (job, total, total, t, need)
                 held += need
             if not active:
+                if not pending:
+                    break
                 # idle until the next arrival
                 time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
                 continue

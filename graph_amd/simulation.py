@@ -250,29 +250,44 @@ class Graph:
         self._ctx.set_links(links, strength, r_in, r_out)
 
     def start(self, alpha: float = 1.0) -> None:
-        """src/index.ts:1457-1476."""
-        if not self.points_number:
+        """src/index.ts:1457-1476: (re)starts at `alpha`; onSimulationStart only
+        when the simulation was not already running."""
+        if not self.config["enableSimulation"] or not self.points_number:
             return
+        was_running = self.is_simulation_running
+        self.is_simulation_running = True
+        self.simulation_progress = 0.0
         self.alpha = float(alpha)
         self._ctx.set_alpha(self.alpha)
-        self.is_simulation_running = True
         cb = self.config.get("onSimulationStart")
-        if cb:
+        if cb and not was_running:
             cb()
 
     def stop(self) -> None:
+        """src/index.ts:1482-1490: stops and resets alpha and progress;
+        onSimulationEnd only if the simulation was active."""
+        was_active = self.is_simulation_running or self.alpha > 0 or self.simulation_progress > 0
         self.is_simulation_running = False
+        self.simulation_progress = 0.0
+        self.alpha = 0.0
+        self._ctx.set_alpha(0.0)
         cb = self.config.get("onSimulationEnd")
-        if cb:
+        if cb and was_active:
             cb()
 
     def pause(self) -> None:
+        """src/index.ts:1496-1501: keeps alpha and progress."""
+        if not self.is_simulation_running:
+            return
         self.is_simulation_running = False
         cb = self.config.get("onSimulationPause")
         if cb:
             cb()
 
     def unpause(self) -> None:
+        """src/index.ts:1510-1521."""
+        if not self.config["enableSimulation"] or self.is_simulation_running:
+            return
         self.is_simulation_running = True
         cb = self.config.get("onSimulationUnpause")
         if cb:
@@ -280,16 +295,22 @@ class Graph:
 
     def step(self) -> None:
         """Graph.step (src/index.ts:1528-1540): one tick even when paused."""
+        if not self.config["enableSimulation"] or not self.points_number:
+            return  # no points: pointsTextureSize is 0
         self._run_simulation_step(True)
 
     def frame(self) -> None:
-        """One rAF frame of the simulation (src/index.ts:2152-2177, 2237-2322)."""
+        """One rAF frame of the simulation (src/index.ts:2152-2177): the run ends
+        (end(), src/index.ts:2335-2341) before the frame's step once alpha has
+        fallen below ALPHA_MIN; the step then runs no forces."""
+        if not self.points_number:
+            return
         if self.is_simulation_running and self.alpha < ALPHA_MIN:
             self.is_simulation_running = False
+            self.simulation_progress = 1.0
             cb = self.config.get("onSimulationEnd")
             if cb:
                 cb()
-            return
         self._run_simulation_step(False)
 
     def run_ticks(self, nticks: int) -> np.ndarray:
