@@ -390,6 +390,43 @@ static napi_value js_step(napi_env env, napi_callback_info info) {
     return out;
 }
 
+/* evalForce(h, params, force, alpha, seed, out: Float32Array(2n)) — one force
+ * pass from the current positions, nothing integrated (cg_eval_force). */
+static napi_value js_eval_force(napi_env env, napi_callback_info info) {
+    size_t argc = 6;
+    napi_value argv[6];
+    CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    cg_ctx* ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    if (argc < 6) {
+        napi_throw_type_error(env, NULL, "cosmosgl.evalForce: (h, params, force, alpha, seed, out) expected");
+        return NULL;
+    }
+    cg_sim_params p;
+    cg_default_params(&p);
+    napi_valuetype vt;
+    napi_typeof(env, argv[1], &vt);
+    if (vt == napi_object) read_params(env, argv[1], &p);
+    int32_t force = 0;
+    double alpha = 1.0, seed = 0.0;
+    CHECK(env, napi_get_value_int32(env, argv[2], &force));
+    CHECK(env, napi_get_value_double(env, argv[3], &alpha));
+    CHECK(env, napi_get_value_double(env, argv[4], &seed));
+    void* out;
+    size_t len;
+    if (!get_typed(env, argv[5], napi_float32_array, &out, &len, 0)) {
+        napi_throw_type_error(env, NULL, "cosmosgl.evalForce: out must be a Float32Array");
+        return NULL;
+    }
+    /* cg_eval_force writes one pair per point: the length is checked first by
+     * cg_get_positions, which refuses a mismatched count before writing */
+    int rc = cg_get_positions(ctx, (float*)out, (uint32_t)(len / 2));
+    if (rc == CG_OK && len % 2) rc = CG_ERR_INVALID;
+    if (rc == CG_OK) rc = cg_eval_force(ctx, &p, force, (float)alpha, (float)seed, (float*)out);
+    if (rc != CG_OK) return throw_rc(env, ctx, rc);
+    return argv[5];
+}
+
 static napi_value js_get_positions(napi_env env, napi_callback_info info) {
     size_t argc = 2;
     napi_value argv[2];
@@ -449,6 +486,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"getAlpha", 0, js_get_alpha, 0, 0, 0, napi_enumerable, 0},
         {"step", 0, js_step, 0, 0, 0, napi_enumerable, 0},
         {"getPositions", 0, js_get_positions, 0, 0, 0, napi_enumerable, 0},
+        {"evalForce", 0, js_eval_force, 0, 0, 0, napi_enumerable, 0},
         {"defaultParams", 0, js_default_params, 0, 0, 0, napi_enumerable, 0},
     };
     if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) return NULL;

@@ -315,6 +315,45 @@ def test_napi_addon_matches_ctypes(tmp_path):
     assert np.array_equal(np.fromfile(tmp_path / "alpha.f64", np.float64), tr)
 
 
+def test_napi_addon_forces_vs_oracle(tmp_path):
+    """Node drives the addon (graph_amd/napi/run_force.js): every force pass it
+    evaluates through evalForce matches the oracle from the same inputs."""
+    import json
+    import shutil
+    import subprocess
+
+    node = shutil.which("node")
+    addon = _lib.PKG / "napi" / "cosmosgl.node"
+    if node is None or not addon.exists():
+        pytest.skip("node or the addon is not available")
+    S, n = 4096.0, 10_000
+    xy = datagen.uniform_square(n, S, 42)
+    links = datagen.barabasi_albert(n, 2, 42)
+    m = links.size // 2
+    r_in, r_out = datagen.uniform(m, 0, 1, 1), datagen.uniform(m, 0, 1, 2)
+    jit = datagen.jitter(n, 3)
+    sizes = (2 * datagen.lognormal(n, np.log(2), 0.5, 4)).astype(np.float32)
+    for name, arr in (("xy", xy), ("links", links), ("rin", r_in), ("rout", r_out), ("jitter", jit),
+                      ("sizes", sizes)):
+        arr.astype(np.float32).tofile(tmp_path / f"{name}.f32")
+    forces = [_lib.FORCE_GRAVITY, _lib.FORCE_MANYBODY, _lib.FORCE_LINK_IN, _lib.FORCE_LINK_OUT,
+              _lib.FORCE_COLLISION]
+    alpha, seed = 0.75, 0.40625
+    (tmp_path / "params.json").write_text(json.dumps({
+        "params": {"spaceSize": S, "simulationRepulsion": 0.5, "simulationGravity": 0.25,
+                   "simulationCollision": 0.6},
+        "alpha": alpha, "seed": seed, "forces": forces}))
+    subprocess.run([node, str(_lib.PKG / "napi" / "run_force.js"), str(tmp_path)], check=True, timeout=120)
+    got = {f: np.fromfile(tmp_path / f"v{f}.f32", np.float32) for f in forces}
+    Lk = O.Links(n, links, r_in, r_out)
+    check(got[_lib.FORCE_GRAVITY], O.gravity(xy, S, alpha, 0.25), S, "addon gravity")
+    check(got[_lib.FORCE_MANYBODY], O.manybody(xy, S, alpha, 0.5, seed, jitter=jit), S, "addon many-body")
+    for d in (0, 1):
+        check(got[_lib.FORCE_LINK_IN + d], Lk.force(xy, d, alpha, 1.0, 10.0, 1.0, 1.2), S, f"addon link {d}")
+    check(got[_lib.FORCE_COLLISION],
+          O.collision(xy, sizes, S, alpha, 0.6, max_size=max(4.0, float(sizes.max()))), S, "addon collision")
+
+
 def test_cluster_force_vs_oracle():
     # Clusters/force-cluster.frag: custom targets, centroid targets (unset or a negative
     # coordinate), unclustered points, absent points out of the centroids, per-point strength

@@ -117,9 +117,27 @@ def test_napi_addon_exports():
     assert names.split(",") == sorted(["create", "destroy", "setPositions", "setLinks", "setJitter",
                                         "setPointSizes", "setPinned", "setPointClusters",
                                         "setClusterPositions", "setPointClusterStrength", "setAlpha",
-                                        "getAlpha", "step", "getPositions", "defaultParams"])
+                                        "getAlpha", "step", "getPositions", "evalForce", "defaultParams"])
     import json
 
     d = json.loads(params)
     assert d["spaceSize"] == 4096 and d["simulationDecay"] == 5000
     assert abs(d["simulationCluster"] - 0.1) < 1e-7
+
+
+def test_typescript_dropin_matches_addon_exports():
+    """graph_amd/napi/ForceSimulationHip.ts declares exactly the addon's exports
+    (the property table of cosmosgl_napi.c) and calls nothing else."""
+    import re
+
+    napi = _lib.PKG / "napi"
+    exports = set(re.findall(r'\{"(\w+)", 0, js_\w+', (napi / "cosmosgl_napi.c").read_text()))
+    ts = (napi / "ForceSimulationHip.ts").read_text()
+    iface = ts[ts.index("interface CosmosGLAddon"):ts.index("}\n", ts.index("interface CosmosGLAddon"))]
+    declared = set(re.findall(r"^\s+(\w+) \(", iface, re.M))
+    called = set(re.findall(r"\baddon\.(\w+)\(", ts))
+    assert declared == exports - {"defaultParams"}, (declared ^ exports)
+    assert called <= exports, called - exports
+    # every key the drop-in passes to step() is one the addon reads
+    read = set(re.findall(r'read_\w+\(env, obj, "(\w+)"', (napi / "cosmosgl_napi.c").read_text()))
+    assert {"spaceSize", "runForces", "mouseRepulsion", "mousePosition"} <= read
