@@ -111,9 +111,13 @@ def profiled_traffic(kernel: str) -> tuple[float | None, str | None]:
     return None, None
 
 
-def cpu_baseline(wl: dict, seconds: float) -> dict:
+def oracle_ticks_per_s(wl: dict, seconds: float, threads: int) -> tuple[float, int, float, int]:
+    """Full FP64 oracle ticks of a workload for about `seconds` on `threads`
+    threads (0 = the OpenMP default): (ticks/s, ticks, seconds, threads)."""
     from oracle import oracle as O
 
+    O.set_threads(threads)
+    used = O.get_threads()
     p = wl["params"]
     pd = dict(space_size=wl["S"], gravity=p.gravity, center=0.0, repulsion=p.repulsion,
               link_spring=p.link_spring, link_distance=p.link_distance, lo=1.0, hi=1.2,
@@ -128,9 +132,57 @@ def cpu_baseline(wl: dict, seconds: float) -> dict:
         el = time.perf_counter() - t0
         if el >= seconds:
             break
-    return {"value": ticks / el, "unit": "ticks/s", "cores": 1, "kind": "port",
-            "sample": f"{ticks} full ticks of the same 1M-point/5M-link workload through the FP64 "
-                      f"oracle (oracle/cosmos_oracle.c, 1 thread) in {el:.1f} s"}
+    O.set_threads(0)
+    return ticks / el, ticks, el, used
+
+
+def cpu_baseline(wl: dict, wl0: dict, seconds: float) -> dict:
+    """The oracle's FP64 tick on the host: the 1M/5M workload on every host
+    thread OpenMP is given (the reported baseline), and beside it the same on
+    one thread and the 10k workload on every thread."""
+    v, t, el, used = oracle_ticks_per_s(wl, seconds, 0)
+    v1, t1, el1, _ = oracle_ticks_per_s(wl, seconds * 0.5, 1)
+    v0, t0, el0, used0 = oracle_ticks_per_s(wl0, seconds * 0.5, 0)
+    return {"value": v, "unit": "ticks/s", "cores": used, "kind": "port",
+            "sample": f"{t} full ticks of the same 1M-point/5M-link workload through the FP64 oracle "
+                      f"(oracle/cosmos_oracle.c, OpenMP over {used} threads) in {el:.1f} s",
+            "one_thread": {"value": v1, "unit": "ticks/s", "cores": 1,
+                           "sample": f"{t1} ticks of the 1M/5M workload on 1 thread in {el1:.1f} s"},
+            "cfg0": {"value": v0, "unit": "ticks/s", "cores": used0,
+                     "sample": f"{t0} ticks of the 10k-point BA workload on {used0} threads in {el0:.1f} s"},
+            "reference_cpu_path": "unavailable: the reference's shaders need Node >= 22, its npm packages "
+                                  "(luma.gl 9.3.6) and a headless WebGL2 (llvmpipe), none of which is "
+                                  "present here or on the GPU host"}
+
+
+def bench_cfg0(ticks: int, warmup: int) -> dict:
+    """The 10k-point BA workload on the GPU: `ticks` ticks (the run's length)
+    after `warmup`, timed between synchronisations."""
+    import torch
+
+    from graph_amd import _lib, datagen, workloads
+
+    wl = workloads.cfg0()
+    ctx = _lib.Context(device=0)
+    ctx.set_positions(wl["xy"])
+    ctx.set_jitter(wl["jitter"])
+    ctx.set_links(wl["links"], None, wl["r_in"], wl["r_out"])
+    ctx.set_alpha(1.0)
+    p = wl["params"]
+    seeds = datagen.uniform(warmup + ticks, 0.0, 1.0, 95)
+    ctx.prepare(p)
+    ctx.step(p, warmup, seeds=seeds[:warmup])
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(p, ticks, seeds=seeds[warmup:])
+    ctx.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ctx.close()
+    return {"workload": wl["name"], "points": wl["n"], "links": int(wl["links"].size // 2), "ticks": ticks,
+            "warmup": warmup, "ticks_per_s": ticks / el, "ms_per_tick": el / ticks * 1e3,
+            "point_ticks_per_s": ticks * wl["n"] / el}
 
 
 def force_error_log(wl: dict, ticks: int, every: int, sample: int, log) -> dict:
@@ -538,8 +590,9 @@ def main() -> int:
         out["batched"] = bench_batched(args.batch_graphs, args.batch_ticks, args.batch_sample, log)
     if "serving" not in skip:
         out["serving"] = bench_serving(args.serve_seconds, args.serve_budget_gib, args.serve_segment, log)
+    out["cfg0"] = bench_cfg0(300, 20)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(wl, workloads.cfg0(), args.cpu_seconds)
     print(json.dumps(out), flush=True)
     return 0
 

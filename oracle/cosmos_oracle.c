@@ -18,9 +18,20 @@
  * shader's IEEE single-precision steps.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Threads of the loops below.  Every parallel loop writes each output element
+ * from exactly one iteration and keeps each sum inside one iteration, so the
+ * results do not depend on the thread count. */
+static int g_default_threads = 0;  /* the OpenMP default (OMP_NUM_THREADS), kept for n = 0 */
+void or_set_threads(int n) {
+    if (!g_default_threads) g_default_threads = omp_get_max_threads();
+    omp_set_num_threads(n > 0 ? n : g_default_threads);
+}
+int or_get_threads(void) { return omp_get_max_threads(); }
 
 /* ------------------------------------------------------------------------ */
 /* integer references                                                        */
@@ -134,6 +145,7 @@ void or_slots(uint32_t n, const float* pos, const uint8_t* absent, float space_s
         tk_t t; t.tb = f32_bits(or_ticket(i, seed)); t.idx = i;
         buf[fill[cell_of[i]]++] = t;
     }
+#pragma omp parallel for schedule(dynamic, 256)
     for (uint32_t c = 0; c < cells; ++c) {
         uint32_t b = cnt[c], e = cnt[c + 1];
         qsort(buf + b, e - b, sizeof(tk_t), tk_cmp);
@@ -331,6 +343,7 @@ void or_manybody(uint32_t n, const double* pos, const uint8_t* absent, const flo
     float* hs = (float*)malloc(L * sizeof(float));
     float* pf = (float*)malloc((size_t)(n ? n : 1) * 2 * sizeof(float));
     for (uint32_t i = 0; i < 2 * n; ++i) pf[i] = (float)pos[i];
+#pragma omp parallel for schedule(dynamic, 1)
     for (uint32_t lv = 0; lv < L; ++lv) {
         uint32_t g = 4u << lv;
         hs[lv] = or_cell_size(space_size, g);
@@ -347,6 +360,7 @@ void or_manybody(uint32_t n, const double* pos, const uint8_t* absent, const flo
     or_slots(n, pf, absent, space_size, seed, slots);
     double ak = alpha * k;
     uint32_t cnt = eval ? n_eval : n;
+#pragma omp parallel for schedule(dynamic, 256)
     for (uint32_t t = 0; t < cnt; ++t) {
         uint32_t i = eval ? eval[t] : t;
         if (absent && absent[i]) { v[2 * i] = 0; v[2 * i + 1] = 0; continue; }
@@ -415,6 +429,7 @@ void or_link(uint32_t n, const double* pos, const uint8_t* absent,
              const float* bias, const float* strength,
              double spring, double distance, double lo, double hi, double alpha,
              double* v) {
+#pragma omp parallel for schedule(dynamic, 512)
     for (uint32_t p = 0; p < n; ++p) {
         double vx = 0, vy = 0;
         double px = pos[2 * p], py = pos[2 * p + 1];
@@ -469,6 +484,7 @@ void or_collision(uint32_t n, const double* pos, const uint8_t* absent, const fl
             double* c = grid + ((size_t)cy * G + cx) * 4;
             c[0] += pos[2 * i]; c[1] += pos[2 * i + 1]; c[2] += sizes[i]; c[3] += 1.0;
         }
+#pragma omp parallel for schedule(static, 1024)
         for (uint32_t i = 0; i < n; ++i) {
             if (absent && absent[i]) continue;
             double px = pos[2 * i], py = pos[2 * i + 1];

@@ -65,6 +65,8 @@ def lib():
         l.or_cluster.argtypes = [C.c_uint32, D, B, I, C.c_uint32, D, F, C.c_double, C.c_double, D]
         l.or_alpha_step.argtypes = [C.c_double, C.c_double]
         l.or_alpha_step.restype = C.c_double
+        l.or_set_threads.argtypes = [C.c_int]
+        l.or_get_threads.restype = C.c_int
         _lib = l
     return _lib
 
@@ -302,3 +304,13 @@ def tick(pos, alpha, seed, p, links=None, jitter=None, absent=None, pinned=None,
                       p.get("collision_padding", 0.0), absent=absent)
         pos = integrate(pos, v, fr, S, pinned, absent)
     return pos, alpha_step(alpha, p["decay"])
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's parallel loops (results do not depend on it);
+    0 restores the OpenMP default (OMP_NUM_THREADS)."""
+    lib().or_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().or_get_threads())

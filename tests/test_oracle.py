@@ -598,3 +598,22 @@ def test_cells_vector_form_equals_scalar_cell(oracle):
                 qx = np.float32(P[i, 0]) + np.float32(np.float32(off[0]) * np.float32(h))
                 qy = np.float32(P[i, 1]) + np.float32(np.float32(off[1]) * np.float32(h))
                 assert tuple(got[i]) == (oracle.cell(float(qx), h, g), oracle.cell(float(qy), h, g))
+
+
+def test_oracle_results_do_not_depend_on_threads(oracle):
+    """The parallel loops give the same bits on 1 thread and on several."""
+    from graph_amd import workloads
+
+    wl = workloads.cfg0()
+    S, n = wl["S"], wl["n"]
+    sizes = np.full(n, 6.0, np.float32)
+    pd = dict(space_size=S, gravity=0.25, center=0, repulsion=0.5, link_spring=1.0, link_distance=10.0,
+              lo=1.0, hi=1.2, friction=0.85, decay=5000.0, collision=0.5)
+    Lk = oracle.Links(n, wl["links"], wl["r_in"], wl["r_out"])
+    outs = []
+    for threads in (1, 4):
+        oracle.set_threads(threads)
+        pos, a = oracle.tick(wl["xy"], 1.0, 0.3, pd, links=Lk, jitter=wl["jitter"], sizes=sizes)
+        outs.append(pos)
+    oracle.set_threads(0)
+    assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))
