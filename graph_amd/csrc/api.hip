@@ -194,6 +194,41 @@ __global__ void __launch_bounds__(1024) k_present_list(const float2* __restrict_
     }
 }
 
+// A graph's staged per-point inputs and its place in the packed arrays.
+struct GStage {
+    const uint8_t* flags;
+    const float2* jit;
+    const float* size;        // null: no collision (size 0)
+    const uint32_t* pres;
+    uint32_t p0, k0, np, pad;
+};
+
+// The packed per-point arrays from every graph's staged inputs: point i
+// belongs to the graph whose range [p0, p0 + n) holds it (p0s ascending).
+__global__ void k_pack_points(const GStage* __restrict__ tab, const uint32_t* __restrict__ p0s, uint32_t G,
+                              uint32_t N, uint8_t* __restrict__ flags, uint32_t* __restrict__ gid,
+                              float2* __restrict__ jit, float* __restrict__ size) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const uint32_t s = find_graph(p0s, G, i);
+    const GStage& t = tab[s];
+    const uint32_t j = i - t.p0;
+    flags[i] = t.flags[j];
+    gid[i] = s;
+    jit[i] = t.jit[j];
+    size[i] = t.size ? t.size[j] : 0.f;
+}
+
+// A present list: entry k belongs to the table entry whose [k0, k0 + np)
+// holds it (k0s ascending); global index = p0 + the staged local index.
+__global__ void k_pack_present(const GStage* __restrict__ tab, const uint32_t* __restrict__ k0s, uint32_t G,
+                               uint32_t NP, uint32_t* __restrict__ present) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= NP) return;
+    const GStage& t = tab[find_graph(k0s, G, k)];
+    present[k] = t.p0 + t.pres[k - t.k0];
+}
+
 constexpr uint64_t GRAPH_SEED = 0x5eed5eedULL;   // every graph's generator starts here
 constexpr uint32_t DESC_ENTRIES = 1u << 16;      // descriptors per upload (ticks x graphs)
 constexpr uint32_t DESC_MAX_TICKS = 256;
@@ -337,6 +372,43 @@ int upload(Ctx* c, T* dst, const std::vector<T>& src) {
     return CG_OK;
 }
 
+// Uploads one graph's per-point inputs (flags, jitter, resolved collision
+// sizes, local present indices) to its own device arrays.  Runs when they
+// changed since the last packing, so a repack uploads nothing for graphs that
+// were already resident.
+int stage_graph(Ctx* c, Graph& g) {
+    dfree(c, g.d_flags); dfree(c, g.d_jit); dfree(c, g.d_size); dfree(c, g.d_pres);
+    g.d_flags = nullptr; g.d_jit = nullptr; g.d_size = nullptr; g.d_pres = nullptr;
+    int rc = CG_OK;
+    const size_t n = g.n ? g.n : 1;
+    g.d_flags = (uint8_t*)dalloc(c, n, &rc);
+    g.d_jit = (float2*)dalloc(c, n * 8, &rc);
+    g.d_pres = (uint32_t*)dalloc(c, (g.np ? g.np : 1) * 4, &rc);
+    if (collides(g)) g.d_size = (float*)dalloc(c, n * 4, &rc);
+    if (rc) return rc;
+    std::vector<uint32_t> pres;
+    pres.reserve(g.np);
+    for (uint32_t i = 0; i < g.n; ++i)
+        if (!(g.flags[i] & F_ABSENT)) pres.push_back(i);
+    if (g.n) {
+        CG_HIP(c, hipMemcpyAsync(g.d_flags, g.flags.data(), g.n, hipMemcpyHostToDevice, c->stream));
+        CG_HIP(c, hipMemcpyAsync(g.d_jit, g.jitter.data(), (size_t)g.n * 8, hipMemcpyHostToDevice, c->stream));
+        if (g.d_size)
+            CG_HIP(c, hipMemcpyAsync(g.d_size, g.col_size.data(), (size_t)g.n * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (!pres.empty())
+        CG_HIP(c, hipMemcpyAsync(g.d_pres, pres.data(), pres.size() * 4, hipMemcpyHostToDevice, c->stream));
+    CG_HIP(c, hipStreamSynchronize(c->stream));  // the host vectors are temporaries
+    g.staged = true;
+    return CG_OK;
+}
+
+void unstage_graph(Ctx* c, Graph& g) {
+    dfree(c, g.d_flags); dfree(c, g.d_jit); dfree(c, g.d_size); dfree(c, g.d_pres);
+    g.d_flags = nullptr; g.d_jit = nullptr; g.d_size = nullptr; g.d_pres = nullptr;
+    g.staged = false;
+}
+
 // What a repack replaces: the previous packing's device-resident state (the
 // position buffers and the link schedules with their warm starts), the
 // placement of every graph in it and the set's sizes.  A repack that fails
@@ -419,6 +491,12 @@ int pack_body(Ctx* c, const PackSnap& old) {
         if (collides(g) && (g.col_dirty || g.col_size.size() != g.n)) {
             rc = col_prepare_graph(c, g);
             if (rc) return rc;
+            g.staged = false;
+        }
+        if (g.staged && collides(g) != (g.d_size != nullptr)) g.staged = false;
+        if (!g.staged) {
+            rc = stage_graph(c, g);
+            if (rc) return rc;
         }
     }
     uint64_t N = 0, NP = 0, F = 0, C = 0, CG = 0, NPC = 0;
@@ -480,26 +558,21 @@ int pack_body(Ctx* c, const PackSnap& old) {
     c->blk_graph = (uint32_t*)dalloc(c, (F / CELL_BLOCK + 1) * 4, &rc);
     if (rc) return rc;
 
-    // host staging of the per-point arrays
-    std::vector<uint8_t> hflags(N);
-    std::vector<uint32_t> hgid(N), hpres, hcpres, hf0(G), hcol(G), k0s(G), nps(G), hblk(F / CELL_BLOCK);
-    std::vector<float> hjit(2 * N), hsize(N, 0.f);
+    // per-point arrays, assembled on the device from every graph's staged inputs
+    std::vector<uint32_t> hf0(G), hcol(G), k0s(G), nps(G), hblk(F / CELL_BLOCK);
+    std::vector<uint32_t> tp0(G), tk0(G), tck0;
+    std::vector<GStage> tab(G), ctab;
     std::vector<uint4> w0, w1;
-    hpres.reserve(NP);
-    hcpres.reserve(NPC);
+    uint32_t ck0 = 0;
     for (uint32_t s = 0; s < G; ++s) {
         const Graph& g = c->graphs[c->order[s]];
-        const bool col = collides(g);
-        if (g.n) {
-            std::memcpy(&hflags[g.p0], g.flags.data(), g.n);
-            std::fill(hgid.begin() + g.p0, hgid.begin() + g.p0 + g.n, s);
-            std::memcpy(&hjit[2 * (size_t)g.p0], g.jitter.data(), (size_t)g.n * 8);
-            if (col) std::memcpy(&hsize[g.p0], g.col_size.data(), (size_t)g.n * 4);
-        }
-        for (uint32_t i = 0; i < g.n; ++i) {
-            if (g.flags[i] & F_ABSENT) continue;
-            hpres.push_back(g.p0 + i);
-            if (col) hcpres.push_back(g.p0 + i);
+        tab[s] = GStage{g.d_flags, g.d_jit, g.d_size, g.d_pres, g.p0, g.k0, g.np, 0u};
+        tp0[s] = g.p0;
+        tk0[s] = g.k0;
+        if (collides(g)) {
+            ctab.push_back(GStage{g.d_flags, g.d_jit, g.d_size, g.d_pres, g.p0, ck0, g.np, 0u});
+            tck0.push_back(ck0);
+            ck0 += g.np;
         }
         hf0[s] = g.f0;
         std::fill(hblk.begin() + g.f0 / CELL_BLOCK, hblk.begin() + (g.f0 + pad_cells(g.gf)) / CELL_BLOCK, s);
@@ -508,12 +581,38 @@ int pack_body(Ctx* c, const PackSnap& old) {
         nps[s] = g.np;
         if (g.np) pyramid_work_host(s, g.gf, g.L, w0, w1);
     }
-    rc = upload(c, c->flags, hflags); if (rc) return rc;
-    rc = upload(c, c->gid, hgid); if (rc) return rc;
-    rc = upload(c, (float*)c->jitter, hjit); if (rc) return rc;
-    rc = upload(c, c->size, hsize); if (rc) return rc;
-    rc = upload(c, c->present, hpres); if (rc) return rc;
-    rc = upload(c, c->cpresent, hcpres); if (rc) return rc;
+    {
+        // the tables for the assembly kernels, freed once they have run
+        const size_t tb = (gg + ctab.size() + 1) * sizeof(GStage), wb = (2 * gg + tck0.size() + 1) * 4;
+        uint8_t* tmp = (uint8_t*)dalloc(c, tb + wb, &rc);
+        if (rc) return rc;
+        GStage* dtab = (GStage*)tmp;
+        GStage* dctab = dtab + gg;
+        uint32_t* dp0 = (uint32_t*)(tmp + tb);
+        uint32_t* dk0 = dp0 + gg;
+        uint32_t* dck0 = dk0 + gg;
+        hipError_t e = hipSuccess;
+        if (G) e = hipMemcpyAsync(dtab, tab.data(), G * sizeof(GStage), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess && !ctab.empty())
+            e = hipMemcpyAsync(dctab, ctab.data(), ctab.size() * sizeof(GStage), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess && G) e = hipMemcpyAsync(dp0, tp0.data(), G * 4, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess && G) e = hipMemcpyAsync(dk0, tk0.data(), G * 4, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess && !tck0.empty())
+            e = hipMemcpyAsync(dck0, tck0.data(), tck0.size() * 4, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess && N)
+            hipLaunchKernelGGL(k_pack_points, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, dtab, dp0, G,
+                               (uint32_t)N, c->flags, c->gid, c->jitter, c->size);
+        if (e == hipSuccess && NP)
+            hipLaunchKernelGGL(k_pack_present, dim3(blocks_for(NP, 256)), dim3(256), 0, c->stream, dtab, dk0, G,
+                               (uint32_t)NP, c->present);
+        if (e == hipSuccess && NPC)
+            hipLaunchKernelGGL(k_pack_present, dim3(blocks_for(NPC, 256)), dim3(256), 0, c->stream, dctab, dck0,
+                               (uint32_t)ctab.size(), (uint32_t)NPC, c->cpresent);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // the host tables are temporaries
+        dfree(c, tmp);
+        if (e != hipSuccess) return hip_fail(c, e, "pack: assembling the per-point arrays");
+    }
     rc = upload(c, c->f0s, hf0); if (rc) return rc;
     rc = upload(c, c->colc0s, hcol); if (rc) return rc;
     rc = upload(c, c->blk_graph, hblk); if (rc) return rc;
@@ -559,10 +658,15 @@ int pack_body(Ctx* c, const PackSnap& old) {
         rc = upload(c, c->pyr[1], w1); if (rc) return rc;
     }
 
-    // clusters: packed indices, strengths, targets and member chunks
+    // clusters: packed indices, strengths, targets and member chunks (the
+    // per-point arrays only when some graph has clusters: the pass reads them
+    // for those graphs' points only)
+    bool any_clustered = false;
+    for (uint32_t s = 0; s < G; ++s) any_clustered |= clustered(c->graphs[c->order[s]]);
     {
-        std::vector<int32_t> hpcl(N, -1);
-        std::vector<float> hpcs(N, 1.f);
+        const size_t ncl_pts = any_clustered ? N : 0;
+        std::vector<int32_t> hpcl(ncl_pts, -1);
+        std::vector<float> hpcs(ncl_pts, 1.f);
         std::vector<float2> hcpos;
         std::vector<uint32_t> members, first;
         std::vector<uint4> chunks;
@@ -596,8 +700,8 @@ int pack_body(Ctx* c, const PackSnap& old) {
         first.push_back((uint32_t)chunks.size());
         c->CC = (uint32_t)hcpos.size();
         c->n_cl_chunks = (uint32_t)chunks.size();
-        c->pcl = (int32_t*)dalloc(c, nn * 4, &rc);
-        c->pcs = (float*)dalloc(c, nn * 4, &rc);
+        c->pcl = (int32_t*)dalloc(c, (ncl_pts ? ncl_pts : 1) * 4, &rc);
+        c->pcs = (float*)dalloc(c, (ncl_pts ? ncl_pts : 1) * 4, &rc);
         c->cpos = (float2*)dalloc(c, (hcpos.size() + 1) * 8, &rc);
         c->cl_members = (uint32_t*)dalloc(c, (members.size() + 1) * 4, &rc);
         c->cl_chunks = (uint4*)dalloc(c, (chunks.size() + 1) * 16, &rc);
@@ -947,6 +1051,7 @@ int graph_set_positions(Ctx* c, Graph& g, const float* xy, uint32_t n) {
     if (!same_n) g.sizes.clear();
     g.xy_pending = true;
     g.col_dirty = true;
+    g.staged = false;
     c->dirty = true;
     if (g.links_set) return link_build_graph(c, g);
     return CG_OK;
@@ -971,6 +1076,7 @@ int graph_set_links(Ctx* c, Graph& g, const float* links, uint32_t m, const floa
 int graph_set_jitter(Ctx* c, Graph& g, const float* xy, uint32_t n) {
     if (n != g.n || (n && !xy)) return fail(c, CG_ERR_INVALID, "set_jitter: expected one pair per point");
     g.jitter.assign(xy, xy + 2 * (size_t)n);
+    g.staged = false;
     if (!c->dirty && n) {
         CG_HIP(c, hipMemcpyAsync(c->jitter + g.p0, g.jitter.data(), (size_t)n * 8, hipMemcpyHostToDevice,
                                  c->stream));
@@ -994,6 +1100,7 @@ int graph_set_pinned(Ctx* c, Graph& g, const uint32_t* idx, uint32_t count) {
     for (auto& f : g.flags) f &= (uint8_t)~F_PINNED;
     for (uint32_t k = 0; k < count; ++k)
         if (idx[k] < g.n) g.flags[idx[k]] |= F_PINNED;
+    g.staged = false;
     if (!c->dirty && g.n) {
         CG_HIP(c, hipMemcpyAsync(c->flags + g.p0, g.flags.data(), g.n, hipMemcpyHostToDevice, c->stream));
         CG_HIP(c, hipStreamSynchronize(c->stream));
@@ -1418,6 +1525,7 @@ int cg_graph_remove(cg_ctx* x, uint32_t id) {
     CG_GRAPH(g, id);
     CG_HIP(c, hipStreamSynchronize(c->stream));
     link_free_graph(c, *g);
+    unstage_graph(c, *g);
     *g = Graph();
     g->id = id;
     g->live = false;

@@ -124,6 +124,13 @@ struct Graph {
     float col_h = 0.f;
     bool col_dirty = true;
     std::vector<float> col_size;             // resolved sizes
+    // device copies of the graph's per-point inputs, staged when they change;
+    // the packer assembles the packed arrays from them on the device
+    bool staged = false;
+    uint8_t* d_flags = nullptr;              // [n]
+    float2* d_jit = nullptr;                 // [n]
+    float* d_size = nullptr;                 // [n] resolved collision sizes (collision graphs)
+    uint32_t* d_pres = nullptr;              // [np] local indices of the present points
     // placement in the packed arrays (valid after pack)
     uint32_t slot = 0;                       // index in the descriptor table
     uint32_t p0 = 0, k0 = 0, f0 = 0, c0 = 0, col_c0 = 0;

@@ -52,7 +52,7 @@ class Job:
 class Result:
     jid: int
     arrival: float
-    start: float
+    start: float     # admitted (batched) or started (sequential)
     finish: float
     ticks: int
     xy: np.ndarray = field(repr=False)
@@ -60,6 +60,14 @@ class Result:
     @property
     def latency(self) -> float:
         return self.finish - self.arrival
+
+    @property
+    def queue_wait(self) -> float:
+        return self.start - self.arrival
+
+    @property
+    def service(self) -> float:
+        return self.finish - self.start
 
 
 def finest_grid(n: int) -> int:
@@ -75,11 +83,11 @@ def footprint_bytes(n: int, m: int, collision: bool) -> int:
     repack makes (positions and link schedules exist twice while it runs)."""
     gf = finest_grid(n)
     cells = gf * gf
-    per_point = 16 + 1 + 4 + 8 + 4 + 4 + 16 + 8 + 32   # positions x2, flags, gid, jitter, size,
-    #                                                     present, keys/values x2, sorted, link schedule
+    per_point = 16 + 1 + 4 + 8 + 4 + 4 + 16 + 8 + 32 + 13  # positions x2, flags, gid, jitter, size,
+    #        present, keys/values x2, sorted, link schedule, staged flags/jitter/present
     per_cell = 8 + 96 + 4 + 54                          # bounds, 8 slots, crowded list, levels
     per_link = 2 * (16 + 16 + 16) + 24                  # CSR + schedule + unit/warm state, build
-    col = (4 * n * 16 + 4 * 512 * 512 * 24) if collision else 0
+    col = (n * 40 + 4 * 512 * 512 * 24) if collision else 0   # sort keys/values/positions x2, sizes x2, cells
     return int(1.3 * (per_point * n + per_cell * cells + per_link * m + col)) + (1 << 20)
 
 
@@ -141,9 +149,11 @@ class BatchServer:
         active: dict[int, tuple[Job, int, int, float, int]] = {}  # gid -> job, left, total, start, bytes
         held = 0
         results: list[Result] = []
-        stats = dict(segments=0, ticks=0, max_active=0, point_ticks=0, rejected=[], cancelled=[])
+        stats = dict(segments=0, ticks=0, max_active=0, point_ticks=0, rejected=[], cancelled=[],
+                     ingest_s=0.0, pack_s=0.0, tick_s=0.0, retire_s=0.0, idle_s=0.0, repacks=0)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
+        changed = True
         while pending or active:
             t = now()
             if self._cancel:
@@ -151,6 +161,7 @@ class BatchServer:
                     ctx.graph_remove(gid)
                     held -= active[gid][4]
                     stats["cancelled"].append(active.pop(gid)[0].jid)
+                    changed = True
                 for j in [j for j in pending if j.jid in self._cancel]:
                     pending.remove(j)
                     stats["cancelled"].append(j.jid)
@@ -168,27 +179,43 @@ class BatchServer:
                 if held + need > self.budget:
                     break
                 pending.pop(0)
+                ti = time.perf_counter()
                 gid = ctx.graph_add()
                 _load(ctx, gid, job)
+                stats["ingest_s"] += time.perf_counter() - ti
                 total = alpha_ticks(job.params.decay)
                 active[gid] = (job, total, total, t, need)
                 held += need
+                changed = True
             if not active:
                 if not pending:
                     break
                 # idle until the next arrival
+                ti = time.perf_counter()
                 time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
+                stats["idle_s"] += time.perf_counter() - ti
                 continue
             stats["max_active"] = max(stats["max_active"], len(active))
             k = min(self.segment, min(v[1] for v in active.values()))
             ids = list(active)
+            if changed:
+                # the set changed: repack now (no tick), so the time is its own
+                ti = time.perf_counter()
+                ctx.step_graphs(ids, 0)
+                ctx.sync()
+                stats["pack_s"] += time.perf_counter() - ti
+                stats["repacks"] += 1
+                changed = False
+            ti = time.perf_counter()
             ctx.step_graphs(ids, k)
             ctx.sync()
+            stats["tick_s"] += time.perf_counter() - ti
             stats["segments"] += 1
             stats["ticks"] += k
             stats["point_ticks"] += k * sum(v[0].n for v in active.values())
             if on_segment:
                 on_segment(ctx, {gid: v[0] for gid, v in active.items()})
+            ti = time.perf_counter()
             for gid in ids:
                 job, left, total, start, need = active[gid]
                 left -= k
@@ -200,9 +227,11 @@ class BatchServer:
                 ctx.graph_remove(gid)
                 del active[gid]
                 held -= need
+                changed = True
                 results.append(Result(job.jid, job.arrival, start, fin, total, xy))
                 if log:
                     log(f"job {job.jid} n={job.n} done at {fin:.2f}s latency {fin - job.arrival:.2f}s")
+            stats["retire_s"] += time.perf_counter() - ti
         stats["wall_s"] = now()
         return sorted(results, key=lambda r: r.jid), stats
 
@@ -241,7 +270,14 @@ def serve_sequential(jobs: list[Job], device: int = 0, log=None) -> tuple[list[R
 
 
 def latency_summary(results: list[Result]) -> dict:
+    """Completion latency (arrival to result), split into queue wait (arrival to
+    admission) and service (admission to result); layouts/s over the makespan."""
     lat = np.array([r.latency for r in results], dtype=np.float64)
+    wait = np.array([r.queue_wait for r in results], dtype=np.float64)
+    serv = np.array([r.service for r in results], dtype=np.float64)
     fin = max(r.finish for r in results) - min(r.arrival for r in results)
-    return dict(jobs=len(results), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
-                mean_s=float(lat.mean()), makespan_s=float(fin), layouts_per_s=len(results) / fin)
+    pct = lambda a, q: float(np.percentile(a, q))
+    return dict(jobs=len(results), p50_s=pct(lat, 50), p99_s=pct(lat, 99), mean_s=float(lat.mean()),
+                queue_wait_p50_s=pct(wait, 50), queue_wait_p99_s=pct(wait, 99),
+                service_p50_s=pct(serv, 50), service_p99_s=pct(serv, 99),
+                makespan_s=float(fin), layouts_per_s=len(results) / fin)
