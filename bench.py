@@ -160,10 +160,11 @@ def bench_cfg0(ticks: int, warmup: int) -> dict:
     after `warmup`, timed between synchronisations."""
     import torch
 
-    from graph_amd import _lib, datagen, workloads
+    from graph_amd import _lib, autotune, datagen, workloads
 
     wl = workloads.cfg0()
     ctx = _lib.Context(device=0)
+    tuning = autotune.apply(ctx, wl["n"], wl["links"].size / wl["n"])
     ctx.set_positions(wl["xy"])
     ctx.set_jitter(wl["jitter"])
     ctx.set_links(wl["links"], None, wl["r_in"], wl["r_out"])
@@ -181,7 +182,7 @@ def bench_cfg0(ticks: int, warmup: int) -> dict:
     el = time.perf_counter() - t0
     ctx.close()
     return {"workload": wl["name"], "points": wl["n"], "links": int(wl["links"].size // 2), "ticks": ticks,
-            "warmup": warmup, "ticks_per_s": ticks / el, "ms_per_tick": el / ticks * 1e3,
+            "warmup": warmup, "ticks_per_s": ticks / el, "ms_per_tick": el / ticks * 1e3, "tuning": tuning,
             "point_ticks_per_s": ticks * wl["n"] / el}
 
 
@@ -258,7 +259,7 @@ def bench_cfg2(ticks: int, every: int, sample: int, log) -> dict:
     crowded finest and collision cells and the longest link chains."""
     import torch
 
-    from graph_amd import _lib, datagen, workloads
+    from graph_amd import _lib, autotune, datagen, workloads
     from oracle import oracle as O
 
     t_gen = time.perf_counter()
@@ -267,6 +268,7 @@ def bench_cfg2(ticks: int, every: int, sample: int, log) -> dict:
     m = int(wl["links"].size // 2)
     log(f"cfg2: {n} points, {m} links generated in {time.perf_counter() - t_gen:.1f}s")
     ctx = _lib.Context(device=0)
+    tuning = autotune.apply(ctx, n, 2.0 * m / n)
     ctx.set_positions(wl["xy"])
     ctx.set_jitter(wl["jitter"])
     ctx.set_links(wl["links"], None, wl["r_in"], wl["r_out"])
@@ -333,7 +335,7 @@ def bench_cfg2(ticks: int, every: int, sample: int, log) -> dict:
         "points": n, "links": m, "space_size": S,
         "space_size_note": "the config's 16384; Store.adjustSpaceSize (Store/index.ts:209-233) would halve "
                            "it to 8192 on a WebGL device whose maxTextureDimension2D is 16384 (none here)",
-        "collision_grid": G, "collision_cell": h, "finest_grid": 512, "levels": 8,
+        "collision_grid": G, "collision_cell": h, "finest_grid": 512, "levels": 8, "tuning": tuning,
         "ticks": ticks, "ticks_per_s": tps, "ms_per_tick": 1e3 / tps, "point_ticks_per_s": tps * n,
         "timed_window": f"ticks {ticks - len(timed) * every}-{ticks}: {len(timed)} chunks of {every} ticks, each between "
                         f"synchronisations (the error evaluations between chunks are outside the timing)",
@@ -354,11 +356,13 @@ def bench_batched(count: int, ticks: int, sample: int, log) -> dict:
     `ticks` ticks each, against the plain one-graph-at-a-time loop (the
     single-graph entry points, one context per graph) timed on an evenly
     strided sample of the same graphs and scaled to the whole batch."""
-    from graph_amd import _lib, workloads
+    from graph_amd import _lib, autotune, serving, workloads
 
     ns, ms = workloads.cfg3_sizes(count)
     graphs = [workloads.cfg3_graph(i, int(ns[i]), int(ms[i])) for i in range(count)]
+    links_total = sum(g["links"].size // 2 for g in graphs)
     ctx = _lib.Context(device=0)
+    tuning = autotune.apply(ctx, int(ns.sum()), 2.0 * links_total / float(ns.sum()))
     ids = []
     for g in graphs:
         gid = ctx.graph_add()
@@ -385,6 +389,7 @@ def bench_batched(count: int, ticks: int, sample: int, log) -> dict:
     for i in picks:
         g = graphs[i]
         with _lib.Context(device=0) as c:
+            autotune.apply(c, g["n"], g["links"].size / g["n"])
             c.set_positions(g["xy"])
             c.set_jitter(g["jitter"])
             c.set_links(g["links"], None, g["r_in"], g["r_out"])
@@ -397,13 +402,29 @@ def bench_batched(count: int, ticks: int, sample: int, log) -> dict:
         same &= bool(np.array_equal(x.view(np.uint32), xb[i].view(np.uint32)))
     plain_total = float(np.mean(plain_s)) * count
     log(f"batched: plain loop {np.mean(plain_s) * 1e3:.1f} ms per graph on {len(picks)} graphs")
+    # completion latencies: the same graphs submitted together at t = 0 to the
+    # batch server with a resident cap of a quarter of the points, so graphs
+    # queue and are admitted in order as others finish
+    jobs = [serving.Job(jid=i, arrival=0.0, xy=g["xy"], links=g["links"], params=g["params"], r_in=g["r_in"],
+                        r_out=g["r_out"], jitter=g["jitter"], ticks=ticks) for i, g in enumerate(graphs)]
+    cap = int(ns.sum()) // 4
+    srv = serving.BatchServer(budget=200 * 2**30, segment=100, tuning=tuning, max_points=cap)
+    res, st = srv.run(jobs)
+    srv.close()
+    stream = serving.latency_summary(res)
+    stream.update(resident_point_cap=cap, repacks=st["repacks"], pack_s=st["pack_s"], tick_s=st["tick_s"])
+    log(f"batched: streamed with a {cap}-point cap: p50 {stream['p50_s']:.2f}s p99 {stream['p99_s']:.2f}s, "
+        f"{stream['layouts_per_s']:.1f} layouts/s")
+    same_stream = all(np.array_equal(r.xy.view(np.uint32), xb[r.jid].view(np.uint32)) for r in res if r.jid in xb)
     return {
         "workload": f"{count} Barabasi-Albert graphs, n log-uniform in [1000, 200000] (seed 23), "
                     f"m uniform in 1..4, {ticks} ticks each",
         "graphs": count, "points": int(ns.sum()), "ticks": ticks,
         "layouts_per_s": count / batch_s, "ms_per_tick": batch_s / ticks * 1e3,
         "point_ticks_per_s": float(ns.sum()) * ticks / batch_s,
-        "p50_latency_s": batch_s, "p99_latency_s": batch_s,
+        "latency_all_at_once_s": batch_s,
+        "p50_latency_s": stream["p50_s"], "p99_latency_s": stream["p99_s"],
+        "streamed": stream, "streamed_equals_batched": same_stream, "tuning": tuning,
         "pack_s": pack_s, "device_gib": dev_gib,
         "plain_layouts_per_s": count / plain_total,
         "plain_sample": f"{len(picks)} evenly strided graphs of the batch run one at a time through "
@@ -414,32 +435,46 @@ def bench_batched(count: int, ticks: int, sample: int, log) -> dict:
     }
 
 
-def bench_serving(window_s: float, budget_gib: float, segment: int, log) -> dict:
+def bench_serving(window_s: float, budget_gib: float, segment: int, stride: int, log) -> dict:
     """Continuous batching: Poisson arrivals at 40/s over `window_s` seconds,
-    each job laid out until alpha < 0.001, against the plain first-come
-    first-served path over the same jobs."""
-    from graph_amd import serving, workloads
+    each job laid out until alpha < 0.001, all of them through the batch
+    server; the plain first-come first-served path runs every `stride`-th job
+    (an evenly strided sample of i.i.d. requests), whose mean service time
+    gives its layouts/s (the sequential path is saturated: requests arrive
+    far faster than it serves them)."""
+    from graph_amd import autotune, serving, workloads
 
     jobs = workloads.cfg4_jobs(duration=window_s)
-    log(f"serving: {len(jobs)} jobs, {sum(j.n for j in jobs)} points, {sum(j.m for j in jobs)} links")
-    srv = serving.BatchServer(budget=int(budget_gib * 2**30), segment=segment)
+    pts, lnk = sum(j.n for j in jobs), sum(j.m for j in jobs)
+    log(f"serving: {len(jobs)} jobs, {pts} points, {lnk} links")
+    tuning = autotune.lookup(pts, 2.0 * lnk / pts)
+    srv = serving.BatchServer(budget=int(budget_gib * 2**30), segment=segment, tuning=tuning)
     res, st = srv.run(jobs)
     srv.close()
     log(f"serving: batched done in {st['wall_s']:.1f}s")
-    seq, seq_st = serving.serve_sequential(jobs)
-    log(f"serving: sequential done, busy {seq_st['busy_s']:.1f}s")
-    same = all(np.array_equal(a.xy.view(np.uint32), b.xy.view(np.uint32)) for a, b in zip(res, seq))
-    b, p = serving.latency_summary(res), serving.latency_summary(seq)
+    sample = jobs[::stride]
+    seq, seq_st = serving.serve_sequential(sample, tuned=True)
+    service = float(np.mean([r.service for r in seq]))
+    log(f"serving: sequential sample of {len(sample)} jobs, mean service {service:.3f}s")
+    by_id = {r.jid: r for r in res}
+    same = all(np.array_equal(a.xy.view(np.uint32), by_id[a.jid].xy.view(np.uint32)) for a in seq)
+    b = serving.latency_summary(res)
+    seq_rate = 1.0 / service
     return {
         "workload": f"Poisson arrivals 40/s over {window_s:g}s (seed 31), n log-uniform in [500, 500000], "
                     f"Chung-Lu links 2-8x points, 10% with collision, until alpha < 0.001 "
                     f"({serving.alpha_ticks(5000.0)} ticks)",
-        "jobs": len(jobs), "hbm_budget_gib": budget_gib, "segment_ticks": segment,
-        "batched": b, "sequential": p,
-        "speedup_layouts_per_s": b["layouts_per_s"] / p["layouts_per_s"],
-        "max_active": st["max_active"], "segments": st["segments"],
+        "jobs": len(jobs), "points": pts, "links": lnk, "hbm_budget_gib": budget_gib, "segment_ticks": segment,
+        "batched": b,
+        "batched_breakdown_s": {k: st[k] for k in ("ingest_s", "pack_s", "tick_s", "retire_s", "idle_s", "wall_s")},
+        "repacks": st["repacks"], "max_active": st["max_active"], "segments": st["segments"],
         "point_ticks_per_s": st["point_ticks"] / st["wall_s"],
-        "sequential_equals_batched": bool(same),
+        "sequential": {"sample_jobs": len(sample), "stride": stride, "mean_service_s": service,
+                       "layouts_per_s": seq_rate,
+                       "note": "plain path, one context per job, context creation to readback, on every "
+                               f"{stride}-th request; layouts/s = 1 / mean service time"},
+        "speedup_layouts_per_s": b["layouts_per_s"] / seq_rate,
+        "sequential_equals_batched": bool(same), "tuning": tuning,
     }
 
 
@@ -453,7 +488,8 @@ def main() -> int:
     ap.add_argument("--batch-graphs", type=int, default=2048)
     ap.add_argument("--batch-ticks", type=int, default=300)
     ap.add_argument("--batch-sample", type=int, default=32)
-    ap.add_argument("--serve-seconds", type=float, default=0.5)
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-stride", type=int, default=5, help="sequential path on every k-th request")
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
     ap.add_argument("--serve-segment", type=int, default=100)
     ap.add_argument("--error-ticks", type=int, default=1000)
@@ -472,7 +508,7 @@ def main() -> int:
 
     import torch
 
-    from graph_amd import _lib, datagen, workloads
+    from graph_amd import _lib, autotune, datagen, workloads
 
     wl = workloads.cfg1()
     n = wl["n"]
@@ -493,6 +529,7 @@ def main() -> int:
         ctx.close()
         return 0
     ctx = _lib.Context(device=0)
+    tuning = autotune.apply(ctx, n, wl["links"].size / n)  # the table's bit-neutral parameters
     ctx.set_positions(wl["xy"])
     ctx.set_jitter(wl["jitter"])
     ctx.set_links(wl["links"], None, wl["r_in"], wl["r_out"])
@@ -573,7 +610,8 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl["name"], "points": n, "links": int(wl["links"].size // 2),
-                   "space_size": wl["S"], "finest_grid": gf, "levels": L, "ticks_per_step": 1},
+                   "space_size": wl["S"], "finest_grid": gf, "levels": L, "ticks_per_step": 1,
+                   "tuning": tuning},
         "point_ticks_per_s": ticks_per_s * n,
         "stage_ms_per_tick": {st: v[0] / 50.0 for st, v in breakdown.items()},
         "stages": stage_rooflines(breakdown, info),
@@ -589,7 +627,8 @@ def main() -> int:
     if "batched" not in skip:
         out["batched"] = bench_batched(args.batch_graphs, args.batch_ticks, args.batch_sample, log)
     if "serving" not in skip:
-        out["serving"] = bench_serving(args.serve_seconds, args.serve_budget_gib, args.serve_segment, log)
+        out["serving"] = bench_serving(args.serve_seconds, args.serve_budget_gib, args.serve_segment,
+                                       args.serve_stride, log)
     out["cfg0"] = bench_cfg0(300, 20)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, workloads.cfg0(), args.cpu_seconds)

@@ -59,6 +59,21 @@ class CgSimParams(C.Structure):
     ]
 
 
+class CgTuning(C.Structure):
+    _fields_ = [
+        ("graph_ticks", C.c_uint32),
+        ("cell_lane_max", C.c_uint32),
+        ("cell_group_max", C.c_uint32),
+        ("force_tile", C.c_uint32),
+        ("link_lane_max", C.c_uint32),
+        ("link_group_max", C.c_uint32),
+        ("link_hub_entries", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class CosmosGLError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"cosmosgl error {code}: {msg}")
@@ -75,6 +90,9 @@ _CTX = C.c_void_p
 
 _SIGNATURES = {
     "cg_default_params": (None, [C.POINTER(CgSimParams)]),
+    "cg_default_tuning": (None, [C.POINTER(CgTuning)]),
+    "cg_set_tuning": (C.c_int, [_CTX, C.POINTER(CgTuning)]),
+    "cg_get_tuning": (C.c_int, [_CTX, C.POINTER(CgTuning)]),
     "cg_create": (C.c_int, [C.POINTER(_CTX), C.POINTER(CgCreateParams)]),
     "cg_destroy": (C.c_int, [_CTX]),
     "cg_set_positions": (C.c_int, [_CTX, _F, C.c_uint32]),
@@ -181,6 +199,12 @@ def dptr(a: np.ndarray | None):
         return None
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(_D)
+
+
+def default_tuning() -> dict:
+    t = CgTuning()
+    lib().cg_default_tuning(C.byref(t))
+    return t.as_dict()
 
 
 def default_params() -> CgSimParams:
@@ -293,6 +317,19 @@ class Context:
         tr = np.zeros(nticks, dtype=np.float64) if trace else None
         self._check(self._l.cg_step(self._ctx, C.byref(params), nticks, fptr(s), dptr(tr)))
         return tr
+
+    def set_tuning(self, tuning: dict) -> None:
+        """Free implementation parameters (cg_tuning); keys missing from `tuning` keep their defaults."""
+        t = CgTuning()
+        self._l.cg_default_tuning(C.byref(t))
+        for k, v in tuning.items():
+            setattr(t, k, int(v))
+        self._check(self._l.cg_set_tuning(self._ctx, C.byref(t)))
+
+    def get_tuning(self) -> dict:
+        t = CgTuning()
+        self._check(self._l.cg_get_tuning(self._ctx, C.byref(t)))
+        return t.as_dict()
 
     def prepare(self, params: CgSimParams) -> None:
         """Packs graph 0 and captures the tick graphs cg_step replays, running no tick."""

@@ -913,7 +913,6 @@ int tick_graph(Ctx* c, const TickPlan& tp, uint32_t n, hipGraphExec_t* out) {
     return CG_OK;
 }
 
-constexpr uint32_t GRAPH_TICKS = 8;  // ticks per captured graph (plus a one-tick graph for the rest)
 
 // Advances the graphs in `ids` by nticks.  The graph `sel` (or none) takes its
 // near-field seeds from `seeds` and reports alpha after each tick in `trace`.
@@ -982,7 +981,9 @@ int run_ticks(Ctx* c, const std::vector<uint32_t>& ids, uint32_t nticks, int64_t
             uint32_t r = 1;
             while (tt + r < T && plan[tt + r].any == plan[tt].any) ++r;
             for (uint32_t done = 0; done < r;) {
-                const uint32_t n = (cap && r - done >= GRAPH_TICKS) ? GRAPH_TICKS : 1u;
+                // graph_ticks ticks per captured graph, one-tick graphs for the rest
+                const uint32_t gt = c->tune.graph_ticks;
+                const uint32_t n = (cap && r - done >= gt) ? gt : 1u;
                 if (cap) {
                     hipGraphExec_t x = nullptr;
                     rc = tick_graph(c, plan[tt], n, &x);
@@ -1320,6 +1321,62 @@ using namespace cg;
 
 extern "C" {
 
+void cg_default_tuning(cg_tuning* t) {
+    if (!t) return;
+    const Tuning d;
+    t->graph_ticks = d.graph_ticks;
+    t->cell_lane_max = d.cell_lane_max;
+    t->cell_group_max = d.cell_group_max;
+    t->force_tile = d.force_tile;
+    t->link_lane_max = d.lane_max;
+    t->link_group_max = d.group_max;
+    t->link_hub_entries = d.hub_entries;
+}
+
+int cg_set_tuning(cg_ctx* x, const cg_tuning* t) {
+    CG_CTX(x);
+    if (!t) return fail(c, CG_ERR_INVALID, "cg_set_tuning: null tuning");
+    if (t->graph_ticks < 1 || t->graph_ticks > 256)
+        return fail(c, CG_ERR_INVALID, "cg_set_tuning: graph_ticks must be in [1, 256]");
+    if (t->cell_lane_max < 4 || t->cell_lane_max > t->cell_group_max || t->cell_group_max > 2048)
+        return fail(c, CG_ERR_INVALID, "cg_set_tuning: need 4 <= cell_lane_max <= cell_group_max <= 2048");
+    if (t->force_tile != 512 && t->force_tile != 1024)
+        return fail(c, CG_ERR_INVALID, "cg_set_tuning: force_tile must be 512 or 1024");
+    if (t->link_lane_max < 2 || t->link_lane_max >= t->link_group_max || t->link_group_max > T64_LIMIT)
+        return fail(c, CG_ERR_INVALID, "cg_set_tuning: need 2 <= link_lane_max < link_group_max <= 256");
+    if (t->link_hub_entries != 4 && t->link_hub_entries != 6 && t->link_hub_entries != 8)
+        return fail(c, CG_ERR_INVALID, "cg_set_tuning: link_hub_entries must be 4, 6 or 8");
+    Tuning n;
+    n.graph_ticks = t->graph_ticks;
+    n.cell_lane_max = t->cell_lane_max;
+    n.cell_group_max = t->cell_group_max;
+    n.force_tile = t->force_tile;
+    n.lane_max = t->link_lane_max;
+    n.group_max = t->link_group_max;
+    n.hub_entries = t->link_hub_entries;
+    const Tuning& o = c->tune;
+    const bool relayout = n.cell_lane_max != o.cell_lane_max || n.lane_max != o.lane_max ||
+                          n.group_max != o.group_max || n.hub_entries != o.hub_entries;
+    c->tune = n;
+    CG_HIP(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);  // captured graphs hold the old launch shapes
+    if (relayout) c->dirty = true;  // buffer sizes and link schedules follow the tuning
+    return CG_OK;
+}
+
+int cg_get_tuning(cg_ctx* x, cg_tuning* t) {
+    if (!x || !t) return CG_ERR_INVALID;
+    const Tuning& d = x->c.tune;
+    t->graph_ticks = d.graph_ticks;
+    t->cell_lane_max = d.cell_lane_max;
+    t->cell_group_max = d.cell_group_max;
+    t->force_tile = d.force_tile;
+    t->link_lane_max = d.lane_max;
+    t->link_group_max = d.group_max;
+    t->link_hub_entries = d.hub_entries;
+    return CG_OK;
+}
+
 void cg_default_params(cg_sim_params* p) {
     if (!p) return;
     std::memset(p, 0, sizeof(*p));
@@ -1476,7 +1533,7 @@ int cg_prepare(cg_ctx* x, const cg_sim_params* p) {
     if (tp.any == 0) return CG_OK;
     c->desc = c->desc_cur;
     hipGraphExec_t e = nullptr;
-    int rc = tick_graph(c, tp, GRAPH_TICKS, &e);
+    int rc = tick_graph(c, tp, c->tune.graph_ticks, &e);
     if (!rc) rc = tick_graph(c, tp, 1, &e);
     c->desc = c->desc_base;
     return rc;

@@ -24,21 +24,30 @@ constexpr uint8_t F_PINNED = 2;   // setPinnedPoints (Points/index.ts:1264-1304)
 
 constexpr int MAX_LEVELS = 8;     // 4^2 .. 512^2 (ForceManyBody/index.ts:19, 646-656)
 constexpr int SLOTS = 8;          // NEAR_FIELD_SLOTS (ForceManyBody/index.ts:35)
-constexpr uint32_t BIG_CELL = 16; // finest cells above this many points use a wave each
 constexpr uint32_t CELL_BLOCK = 256;  // each graph's finest cells start on a multiple of this
 
-// Link degree tiers: points with at most LANE_MAX entries run one lane each;
-// up to T64_MAX entries one 64-lane workgroup each (at most 4 entries per
-// lane); above that 256-lane workgroups, at most HUB_E entries per lane: a
-// point above HUB_SEG entries is cut into segments of at most HUB_SEG.  (Six
-// per lane measured best: fewer, longer segments need fewer workgroups and
-// fix-up terms; longer chunks cost more per sweep.)
-constexpr uint32_t LANE_MAX = 64;  // 24, 32, 48, 96, 128 measured slower at cfg1
+// Link degree tiers: points with at most Tuning::lane_max entries run one
+// lane each; up to Tuning::group_max entries one 64-lane workgroup each (at
+// most 4 entries per lane); above that 256-lane workgroups of at most
+// Tuning::hub_entries entries per lane: a point above 256 * hub_entries
+// entries is cut into segments of at most that many.
 constexpr uint32_t LANE_SPLIT = 8;  // lane-tier schedule classes: <= 2, <= LANE_SPLIT entries, then the rest
-constexpr uint32_t T64_MAX = 256;
-constexpr int HUB_E = 6;
-constexpr uint32_t HUB_SEG = 256 * HUB_E;  // entries per segment of a split 256-lane point
+constexpr uint32_t T64_LIMIT = 256; // 64 lanes x 4 entries
 constexpr int HUB_ROUNDS = 5;        // segment/fix-up rounds per pass for split points
+
+// Free implementation parameters (cg_tuning in the header; chosen by the
+// autotuner).  None changes the discretisation; the first four do not change
+// any result bit, the link tiers move chains between the serial and the
+// multiple-shooting solvers (results within the force tolerance).
+struct Tuning {
+    uint32_t graph_ticks = 8;      // ticks per captured HIP graph
+    uint32_t cell_lane_max = 16;   // finest cells up to this many points: one lane each
+    uint32_t cell_group_max = 64;  // up to this: 16 lanes each; above (to 2048): a wave each
+    uint32_t force_tile = 1024;    // staged cells per level in the force kernel (512 or 1024)
+    uint32_t lane_max = 64;        // link entries per direction up to which a point runs one lane
+    uint32_t group_max = 256;      // up to which it runs one 64-lane unit (<= T64_LIMIT)
+    uint32_t hub_entries = 6;      // entries per lane of 256-lane units (4, 6 or 8)
+};
 
 // Passes of one tick, in order (src/index.ts:2012-2078).  Bit p of GDesc::passes
 // says the graph runs pass p this tick; bit p of GDesc::inbuf names the position
@@ -146,7 +155,7 @@ struct LinkDir {
     uint32_t n_sched = 0;
     uint32_t n_lane = 0;         // lane-tier points (first in the schedule)
     // chunked-tier work units, in schedule order: one per 64-lane point, then
-    // one per segment of each 256-lane point (one segment up to HUB_SEG entries)
+    // one per segment of each 256-lane point (one segment up to 256 x hub_entries)
     uint32_t n_units = 0, n_u64 = 0;
     uint32_t first_split_unit = 0;  // split points come last (degree order)
     uint4* unit_a = nullptr;     // {first entry, entries, -, graph}
@@ -179,6 +188,7 @@ struct RadixSort {
 
 struct Ctx {
     int device = 0;
+    Tuning tune;
     uint8_t* scratch = nullptr;     // this device's static scratch array (api.hip), set up by cg_create
     // arena mode: dalloc carves from [arena, arena + arena_cap) and dfree is a
     // no-op (the allocation-free entry point's buffers)
@@ -228,7 +238,7 @@ struct Ctx {
     float2* sorted_pos = nullptr;   // [NP] positions in cell order (one of rp)
     int32_t* slot_idx = nullptr;    // [8 F] global point index or -1
     float2* slot_pos = nullptr;
-    uint32_t* mid_cells = nullptr;  // cells of BIG_CELL+1 .. BIG_CHUNK points: two lists of mid_cap
+    uint32_t* mid_cells = nullptr;  // cells of cell_lane_max+1 .. BIG_CHUNK points: two lists of mid_cap
     uint32_t mid_cap = 0;
     uint4* big_cells = nullptr;     // crowded cells {cell, first chunk, chunks, row-major cell}
     uint2* big_chunks = nullptr;    // their chunks {cell, chunk}

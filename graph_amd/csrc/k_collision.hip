@@ -74,8 +74,8 @@ __global__ void k_col_bounds(const uint32_t* __restrict__ sk, const uint32_t* __
 // (batches of COL_BATCH loads in flight), keep those whose own cell of this
 // grid is this cell, and reduce in a fixed tree, so the result depends only on
 // the points' order in the sort (by index within a base cell).
-constexpr int CELL_LANES = 16;
 constexpr int COL_BATCH = 4;
+template <int CELL_LANES>
 __global__ void __launch_bounds__(256) k_col_cells(const uint32_t* __restrict__ begin,
                                                    const uint32_t* __restrict__ end,
                                                    const float2* __restrict__ spos,
@@ -304,8 +304,16 @@ int col_run(Ctx* c, float2* vout) {
     hipLaunchKernelGGL(k_fill32, dim3(blocks_for(CB, 256)), dim3(256), 0, c->stream, c->col_end, CB, 0u);
     hipLaunchKernelGGL(k_col_bounds, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->col_keys[fin],
                        c->col_vals[fin], npc, c->size, c->col_begin, c->col_end, c->col_ssize);
-    hipLaunchKernelGGL(k_col_cells, dim3(blocks_for((uint64_t)CG * CELL_LANES, 256)), dim3(256), 0, c->stream, c->col_begin,
-                       c->col_end, c->col_pos[fin], c->col_ssize, c->colc0s, c->desc, c->G, CG, c->col_cell);
+    // lanes per cell from the points per base cell: 16 where cells are crowded
+    // (each lane then strides over a few points), 4 where most are sparse
+    if (npc >= 8u * CB)
+        hipLaunchKernelGGL(k_col_cells<16>, dim3(blocks_for((uint64_t)CG * 16, 256)), dim3(256), 0, c->stream,
+                           c->col_begin, c->col_end, c->col_pos[fin], c->col_ssize, c->colc0s, c->desc, c->G, CG,
+                           c->col_cell);
+    else
+        hipLaunchKernelGGL(k_col_cells<4>, dim3(blocks_for((uint64_t)CG * 4, 256)), dim3(256), 0, c->stream,
+                           c->col_begin, c->col_end, c->col_pos[fin], c->col_ssize, c->colc0s, c->desc, c->G, CG,
+                           c->col_cell);
     hipLaunchKernelGGL(k_col_force, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->col_cell,
                        c->pos[0], c->pos[1], c->col_vals[fin], c->col_pos[fin], c->col_ssize, npc, c->gid, c->desc,
                        c->flags, vout);

@@ -8,10 +8,11 @@
 //
 // The reference evaluates each point's links serially, every term seeing the
 // velocity accumulated so far.  Points are scheduled by live degree:
-//   - up to LANE_MAX entries: one lane per point, the serial loop as written;
-//   - up to T64_MAX entries: one 64-lane workgroup per point; above: 256
+//   - up to lane_max entries (Tuning): one lane per point, the serial loop
+//     as written;
+//   - up to group_max entries: one 64-lane workgroup per point; above: 256
 //     lanes.  The chain is cut into one chunk per lane, at most 4 entries
-//     (64 lanes) or HUB_E (256 lanes), whose terms the lane gathers once into
+//     (64 lanes) or hub_entries (256 lanes), whose terms the lane gathers once into
 //     registers; each sweep runs every chunk from a guessed start velocity,
 //     carrying the chunk's end velocity and 2x2 Jacobian, and an affine
 //     prefix scan over the chunks yields the next guesses (multiple shooting
@@ -20,7 +21,7 @@
 //     tolerance, or MAX_SWEEPS); the last chunk's end, moved to its corrected
 //     start, is the point's velocity.  Converged starts are kept per point as
 //     the next call's first guesses.
-//   - a point above HUB_SEG entries is cut into segments, one 256-lane
+//   - a point above 256 x hub_entries entries is cut into segments, one 256-lane
 //     workgroup each (one CU no longer carries a whole hub): each segment is
 //     solved as above from a guessed start and reports its end and end
 //     Jacobian; a fix-up composes the segments from the exact start 0 and
@@ -141,13 +142,13 @@ __device__ inline uint32_t live_degree(uint32_t p, const uint32_t* __restrict__ 
 // addresses, while a wave's chains stay of similar length.
 __global__ void k_sched_keys(const uint32_t* __restrict__ present, uint32_t np,
                              const uint32_t* __restrict__ gid, const GDesc* __restrict__ desc,
-                             const GLinkPtr* __restrict__ tab, uint32_t* __restrict__ keys,
+                             const GLinkPtr* __restrict__ tab, uint32_t lane_max, uint32_t* __restrict__ keys,
                              uint32_t* __restrict__ vals) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np) return;
     const uint32_t p = present[k];
     const uint32_t deg = live_degree(p, gid, desc, tab);
-    keys[k] = deg > LANE_MAX ? deg : (deg > LANE_SPLIT ? 2u : (deg > 2 ? 1u : 0u));
+    keys[k] = deg > lane_max ? deg : (deg > LANE_SPLIT ? 2u : (deg > 2 ? 1u : 0u));
     vals[k] = p;
 }
 
@@ -160,15 +161,15 @@ __global__ void k_sched_degree(const uint32_t* __restrict__ sched, uint32_t ns,
 }
 
 // Tier boundaries over the schedule's degrees (lane-tier degrees first, then
-// the chunked tiers ascending): first position above LANE_MAX and above
-// T64_MAX (counts pre-filled with ns).  One writer each.
-__global__ void k_tier_bounds(const uint32_t* __restrict__ skeys, uint32_t ns,
-                              uint32_t* __restrict__ counts) {
+// the chunked tiers ascending): first position above lane_max and above
+// group_max (counts pre-filled with ns).  One writer each.
+__global__ void k_tier_bounds(const uint32_t* __restrict__ skeys, uint32_t ns, uint32_t lane_max,
+                              uint32_t group_max, uint32_t* __restrict__ counts) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
     uint32_t d = skeys[k], prev = k ? skeys[k - 1] : 0u;
-    if (d > LANE_MAX && (k == 0 || prev <= LANE_MAX)) counts[0] = k;
-    if (d > T64_MAX && (k == 0 || prev <= T64_MAX)) counts[1] = k;
+    if (d > lane_max && (k == 0 || prev <= lane_max)) counts[0] = k;
+    if (d > group_max && (k == 0 || prev <= group_max)) counts[1] = k;
 }
 
 __global__ void k_slot_of(const uint32_t* __restrict__ sched, uint32_t ns, uint32_t* __restrict__ slot_of) {
@@ -867,7 +868,7 @@ int link_pack(Ctx* c, const LinkDir (&old)[2]) {
         L.soff = (uint32_t*)dalloc(c, ((size_t)ns + 1) * 4, &rc);
         if (rc) { cleanup(); return rc; }
         hipLaunchKernelGGL(k_sched_keys, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, c->present,
-                           ns, c->gid, c->desc, tab, k0, v0);
+                           ns, c->gid, c->desc, tab, c->tune.lane_max, k0, v0);
         size_t sb = tbytes;
         CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(d_tmp, sb, k0, k1, v0, L.sched, (int)ns, 0, 32,
                                                      c->stream));
@@ -878,7 +879,7 @@ int link_pack(Ctx* c, const LinkDir (&old)[2]) {
         CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, k1, L.soff, (int)ns + 1, c->stream));
         hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, c->stream, counts, 2u, ns);
         hipLaunchKernelGGL(k_tier_bounds, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, k1, ns,
-                           counts);
+                           c->tune.lane_max, c->tune.group_max, counts);
         uint32_t h[2] = {0, 0};
         CG_HIP(c, hipMemcpyAsync(h, counts, 8, hipMemcpyDeviceToHost, c->stream));
         CG_HIP(c, hipMemcpyAsync(&L.E_live, L.soff + ns, 4, hipMemcpyDeviceToHost, c->stream));
@@ -903,8 +904,10 @@ int link_pack(Ctx* c, const LinkDir (&old)[2]) {
                                      hipMemcpyDeviceToHost, c->stream));
         CG_HIP(c, hipStreamSynchronize(c->stream));
         // A 64-lane point is one unit; a 256-lane point of d entries is
-        // ceil(d / HUB_SEG) near-equal segments, one unit each, in chain order.
-        // Every unit holds at most 4 (64 lanes) or HUB_E (256 lanes) entries per lane.
+        // ceil(d / seg) near-equal segments of at most seg = 256 x hub_entries,
+        // one unit each, in chain order.  Every unit holds at most 4 (64 lanes)
+        // or hub_entries (256 lanes) entries per lane.
+        const uint32_t hub_e = c->tune.hub_entries, hub_seg = 256u * hub_e;
         std::vector<uint4> ua, ub;
         std::vector<uint32_t> slot_u0(nch + 1, 0), slot_graph(nch ? nch : 1, 0), slot_lanes(nch ? nch : 1, 0);
         std::vector<uint2> hh;
@@ -912,7 +915,7 @@ int link_pack(Ctx* c, const LinkDir (&old)[2]) {
         for (uint32_t k = 0; k < nch; ++k) {
             const uint32_t dd = hs[k + 1] - hs[k];
             const uint32_t B = k < n64 ? 64u : 256u;
-            const uint32_t S = B == 256u ? (dd + HUB_SEG - 1) / HUB_SEG : 1u;
+            const uint32_t S = B == 256u ? (dd + hub_seg - 1) / hub_seg : 1u;
             const uint32_t p = hsched[k];
             const uint32_t gs = (uint32_t)(std::upper_bound(h_p0.begin(), h_p0.end(), p) - h_p0.begin()) - 1;
             slot_graph[k] = gs;
@@ -927,7 +930,7 @@ int link_pack(Ctx* c, const LinkDir (&old)[2]) {
             uint32_t eb = hs[k];
             for (uint32_t q = 0; q < S; ++q) {
                 const uint32_t len = dd / S + (q < dd % S ? 1u : 0u);
-                if (len == 0 || len > (B == 256u ? (uint32_t)HUB_E : 4u) * B) {
+                if (len == 0 || len > (B == 256u ? hub_e : 4u) * B) {
                     cleanup();
                     return fail(c, CG_ERR_INVALID, "link schedule: a work unit outside its tier's entries per lane");
                 }
@@ -1001,7 +1004,8 @@ int link_pack(Ctx* c, const LinkDir (&old)[2]) {
     return rc;
 }
 
-int link_run(Ctx* c, int dir, float2* vout) {
+template <int HE>
+int link_run_e(Ctx* c, int dir, float2* vout) {
     if (!c->any_links) return CG_OK;
     const LinkDir& L = c->dir[dir];
     const int pass = P_LINK_IN + dir;
@@ -1021,11 +1025,11 @@ int link_run(Ctx* c, int dir, float2* vout) {
             CG_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
             s = c->side;
         }
-        hipLaunchKernelGGL((k_link_chunk<256, HUB_E>), dim3(L.n_units - us), dim3(256), 0, s, pass, L.unit_a,
+        hipLaunchKernelGGL((k_link_chunk<256, HE>), dim3(L.n_units - us), dim3(256), 0, s, pass, L.unit_a,
                            L.unit_b, us, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm,
                            L.wjac, L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
         for (int r = 1; r < HUB_ROUNDS; ++r)
-            hipLaunchKernelGGL((k_link_chunk<256, HUB_E>), dim3(L.n_units - us), dim3(256), 0, s, pass, L.unit_a, L.unit_b,
+            hipLaunchKernelGGL((k_link_chunk<256, HE>), dim3(L.n_units - us), dim3(256), 0, s, pass, L.unit_a, L.unit_b,
                                us, r, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm, L.wjac,
                                L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
         hipLaunchKernelGGL(k_hub_final, dim3(L.n_hubs), dim3(64), 0, s, pass, L.hubs, L.n_hubs, L.unit_a,
@@ -1035,7 +1039,7 @@ int link_run(Ctx* c, int dir, float2* vout) {
         if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
     if (us > u256)
-        hipLaunchKernelGGL((k_link_chunk<256, HUB_E>), dim3(us - u256), dim3(256), 0, c->stream, pass, L.unit_a,
+        hipLaunchKernelGGL((k_link_chunk<256, HE>), dim3(us - u256), dim3(256), 0, c->stream, pass, L.unit_a,
                            L.unit_b, u256, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm,
                            L.wjac, L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
     if (L.n_u64)
@@ -1049,6 +1053,14 @@ int link_run(Ctx* c, int dir, float2* vout) {
     if (fork) CG_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     CG_HIP(c, hipGetLastError());
     return CG_OK;
+}
+
+int link_run(Ctx* c, int dir, float2* vout) {
+    switch (c->tune.hub_entries) {
+        case 4: return link_run_e<4>(c, dir, vout);
+        case 8: return link_run_e<8>(c, dir, vout);
+        default: return link_run_e<6>(c, dir, vout);
+    }
 }
 
 }  // namespace cg

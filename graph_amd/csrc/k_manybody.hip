@@ -6,7 +6,7 @@
 //      order, positions riding along; cell bounds from the sorted keys
 //   2. per finest cell: fixed-point sums of (x, y), count, and the top-8
 //      near-field slots by (ticket, index) — one lane per cell, 16 lanes per
-//      cell above BIG_CELL points and a wave above MID_SMALL (both beside the
+//      cell above cell_lane_max points and a wave above cell_group_max (both beside the
 //      crowded tier, on the side stream), chunked blocks plus a merge above
 //      BIG_CHUNK points
 //   3. 2x2 merges up the pyramid in the same fixed point, f32 centroids
@@ -35,8 +35,7 @@ namespace {
 constexpr uint32_t TMAX = 0xffffffffu;
 constexpr uint32_t BIG_CHUNK = 2048;  // points per block in a crowded cell
 constexpr uint32_t MID_BLOCKS = 2048; // grid of each mid-cell pass (8192 waves)
-constexpr uint32_t MID_SMALL = 64;    // mid cells up to this many points take MID_GROUP lanes
-constexpr int MID_GROUP = 16;
+constexpr int MID_GROUP = 16;         // lanes per mid cell of at most Tuning::cell_group_max points
 
 // ---- 1. counting sort by finest cell ---------------------------------------------
 
@@ -386,7 +385,7 @@ __device__ inline void cell_points(uint32_t k0, uint32_t e, const uint32_t* __re
 
 // One lane per finest cell, one block per CELL_BLOCK cells of one graph.  Empty
 // cells write nothing: the pyramid reads their count, and their slots are
-// never read.  Cells above BIG_CELL points go to k_cells_mid: up to MID_SMALL
+// never read.  Cells above lane_max points go to k_cells_mid: up to group_max
 // points MID_GROUP lanes each (listed from mid_cells[0]), above it one wave
 // each (listed from mid_cells[mid_cap]); cells above BIG_CHUNK points are
 // queued in chunks of BIG_CHUNK points for k_cells_chunk and k_cells_merge.
@@ -404,7 +403,8 @@ __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict
                                                       uint2* __restrict__ big_chunks,
                                                       uint32_t* __restrict__ big_count,
                                                       const uint32_t* __restrict__ blk_graph,
-                                                      const GDesc* __restrict__ desc) {
+                                                      const GDesc* __restrict__ desc, uint32_t lane_max,
+                                                      uint32_t group_max) {
     const uint32_t f = blockIdx.x * CELL_BLOCK + threadIdx.x;
     // most cells are empty: their lanes leave after this one load (alignment
     // padding cells are never filled, so they read 0 as well)
@@ -416,11 +416,11 @@ __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict
     if (mc >= d.gf * d.gf) return;  // alignment padding
     const uint32_t b = begin[f], n = e - b;
     const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
-    if (n > BIG_CELL && n <= MID_SMALL) {
+    if (n > lane_max && n <= group_max) {
         mid_cells[atomicAdd(&big_count[2], 1u)] = f;  // MID_GROUP lanes each (k_cells_mid)
         return;
     }
-    if (n > MID_SMALL && n <= BIG_CHUNK) {
+    if (n > group_max && n <= BIG_CHUNK) {
         mid_cells[mid_cap + atomicAdd(&big_count[3], 1u)] = f;  // one wave each (k_cells_mid)
         return;
     }
@@ -470,8 +470,8 @@ __device__ inline void wave_top8(uint32_t (&tb)[SLOTS], uint32_t (&ti)[SLOTS],
     }
 }
 
-// G lanes per listed cell (MID_GROUP for cells of BIG_CELL+1 .. MID_SMALL
-// points, a wave for MID_SMALL+1 .. BIG_CHUNK): the group's lanes stride the
+// G lanes per listed cell (MID_GROUP for cells of cell_lane_max+1 ..
+// cell_group_max points, a wave above, to BIG_CHUNK): the group's lanes stride the
 // cell's segment with private top-8 lists and integer sums, then merge inside
 // the group.  Every lane of a group runs the same iterations, so the group
 // shuffles never read a lane that has left the loop.
@@ -944,7 +944,7 @@ __device__ inline void mb_finish(const GDesc& d, uint32_t i, float2 p, float2 r,
     pout[i] = (flags[i] & F_PINNED) ? p : integrate(p, v, d.friction, d.S);
 }
 
-constexpr int MB_TILE = 1024;  // staged cells per level (16 KB)
+// TILE: staged cells per level (16 KB at 1024; Tuning::force_tile)
 
 // One lane per point in cell order.  A block whose points all belong to one
 // running graph stages, level by level, the union of its points' 6x6 windows
@@ -954,6 +954,7 @@ constexpr int MB_TILE = 1024;  // staged cells per level (16 KB)
 // values are summed in the same order.
 // Eight waves per SIMD: the compiler fits the kernel in 64 VGPRs without spills
 // (88 unbounded, five waves).
+template <int MB_TILE>
 __global__ void __launch_bounds__(256, 8) k_mb_force(const uint32_t* __restrict__ sv,
                                                   const float2* __restrict__ sorted_pos,
                                                   const float4* __restrict__ cent,
@@ -1160,8 +1161,8 @@ int mb_alloc(Ctx* c, uint32_t single_gf) {
     c->cell_end = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
     c->slot_idx = (int32_t*)dalloc(c, F * SLOTS * 4, &rc); if (rc) return rc;
     c->slot_pos = (float2*)dalloc(c, F * SLOTS * 8, &rc); if (rc) return rc;
-    // crowded cells: at most NP / (BIG_CELL + 1) of them, and their chunks
-    const size_t ncells = np / (BIG_CELL + 1) + 1, nchunks = np / BIG_CHUNK + ncells + 1;
+    // crowded cells: at most NP / (cell_lane_max + 1) of them, and their chunks
+    const size_t ncells = np / (c->tune.cell_lane_max + 1) + 1, nchunks = np / BIG_CHUNK + ncells + 1;
     c->mid_cap = (uint32_t)ncells;  // two lists: small mid cells, then wave-sized ones
     c->mid_cells = (uint32_t*)dalloc(c, 2 * ncells * 4, &rc); if (rc) return rc;
     c->big_cells = (uint4*)dalloc(c, ncells * 16, &rc); if (rc) return rc;
@@ -1209,7 +1210,7 @@ int mb_run(Ctx* c, float2* vout) {
     hipLaunchKernelGGL(k_cells, dim3(F / CELL_BLOCK), dim3(CELL_BLOCK), 0, c->stream, c->cell_end,
                        c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums,
                        c->slot_idx, c->slot_pos, c->mid_cells, c->mid_cap, c->big_cells, c->big_chunks,
-                       c->big_count, c->blk_graph, c->desc);
+                       c->big_count, c->blk_graph, c->desc, c->tune.cell_lane_max, c->tune.cell_group_max);
     // the mid tiers and the crowded-cell chunks touch disjoint cells: the
     // mid tiers run on the side stream beside the other two
     const bool fork = c->side != nullptr;
@@ -1243,9 +1244,14 @@ int mb_run(Ctx* c, float2* vout) {
                            c->cell_end, c->sums, c->cent, 0);
     prof_end(c, CG_STAGE_MB_PYRAMID);
     prof_begin(c, CG_STAGE_MB_FORCE);
-    hipLaunchKernelGGL(k_mb_force, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->sorted,
-                       c->sorted_pos, c->cent, c->slot_idx, c->slot_pos, c->jitter, c->flags, c->gid,
-                       c->desc, np, c->pos[0], c->pos[1], vout);
+    if (c->tune.force_tile == 512)
+        hipLaunchKernelGGL(k_mb_force<512>, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->sorted,
+                           c->sorted_pos, c->cent, c->slot_idx, c->slot_pos, c->jitter, c->flags, c->gid,
+                           c->desc, np, c->pos[0], c->pos[1], vout);
+    else
+        hipLaunchKernelGGL(k_mb_force<1024>, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->sorted,
+                           c->sorted_pos, c->cent, c->slot_idx, c->slot_pos, c->jitter, c->flags, c->gid,
+                           c->desc, np, c->pos[0], c->pos[1], vout);
     prof_end(c, CG_STAGE_MB_FORCE);
     CG_HIP(c, hipGetLastError());
     return CG_OK;

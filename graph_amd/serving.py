@@ -38,6 +38,7 @@ class Job:
     r_out: np.ndarray | None = None
     jitter: np.ndarray | None = None
     sizes: np.ndarray | None = None
+    ticks: int | None = None   # ticks to run; None = until alpha < ALPHA_MIN
 
     @property
     def n(self) -> int:
@@ -126,10 +127,14 @@ class BatchServer:
     ends on the tick the first active job reaches its cutoff, so no job runs
     past it.  ``budget`` is the HBM budget admission works against."""
 
-    def __init__(self, budget: int, segment: int = 100, device: int = 0):
+    def __init__(self, budget: int, segment: int = 100, device: int = 0, tuning: dict | None = None,
+                 max_points: int | None = None):
         self.budget = int(budget)
         self.segment = int(segment)
+        self.max_points = max_points  # resident points cap (None: the HBM budget alone)
         self.ctx = _lib.Context(device=device, hbm_budget=self.budget)
+        if tuning:
+            self.ctx.set_tuning(tuning)
         self._cancel: set[int] = set()
 
     def close(self) -> None:
@@ -178,12 +183,14 @@ class BatchServer:
                     continue
                 if held + need > self.budget:
                     break
+                if self.max_points and active and sum(v[0].n for v in active.values()) + job.n > self.max_points:
+                    break
                 pending.pop(0)
                 ti = time.perf_counter()
                 gid = ctx.graph_add()
                 _load(ctx, gid, job)
                 stats["ingest_s"] += time.perf_counter() - ti
-                total = alpha_ticks(job.params.decay)
+                total = job.ticks if job.ticks is not None else alpha_ticks(job.params.decay)
                 active[gid] = (job, total, total, t, need)
                 held += need
                 changed = True
@@ -236,9 +243,10 @@ class BatchServer:
         return sorted(results, key=lambda r: r.jid), stats
 
 
-def serve_sequential(jobs: list[Job], device: int = 0, log=None) -> tuple[list[Result], dict]:
+def serve_sequential(jobs: list[Job], device: int = 0, log=None, tuned: bool = False) -> tuple[list[Result], dict]:
     """The plain path: each job alone through the single-graph entry points,
-    first come first served.  A job's service time is measured from context
+    first come first served (tuned: each job's context takes the bit-neutral
+    tuning of its own size from the autotuner's table).  A job's service time is measured from context
     creation (ingest and link build included, as the batched server's loop
     includes them) to its readback; a job starts when it has arrived and the
     previous one has finished."""
@@ -248,6 +256,10 @@ def serve_sequential(jobs: list[Job], device: int = 0, log=None) -> tuple[list[R
     for job in sorted(jobs, key=lambda j: j.arrival):
         s0 = time.perf_counter()
         with _lib.Context(device=device) as ctx:
+            if tuned:
+                from . import autotune
+
+                autotune.apply(ctx, job.n, 2.0 * job.m / max(job.n, 1))
             ctx.set_positions(job.xy)
             if job.jitter is not None:
                 ctx.set_jitter(job.jitter)
@@ -256,7 +268,7 @@ def serve_sequential(jobs: list[Job], device: int = 0, log=None) -> tuple[list[R
             if job.sizes is not None:
                 ctx.set_point_sizes(job.sizes)
             ctx.set_alpha(1.0)
-            total = alpha_ticks(job.params.decay)
+            total = job.ticks if job.ticks is not None else alpha_ticks(job.params.decay)
             ctx.step(job.params, total)
             xy = ctx.get_positions()
         service = time.perf_counter() - s0

@@ -240,7 +240,26 @@ int cg_debug_link_csr(cg_ctx* ctx, int32_t dir, uint32_t* offsets, uint32_t* nbr
  * points, max rounds, points stopped at the round cap}; out[12..15] zero. */
 int cg_debug_link_stats(cg_ctx* ctx, uint32_t* out16);
 
-/* Bytes of device memory the context holds. */
+/* Free implementation parameters of the tick (no reference counterpart):
+ * none changes the discretisation.  graph_ticks, the cell tiers and
+ * force_tile change no result bit; the link tiers move chains between the
+ * serial loop and the multiple-shooting solver (results within the force
+ * tolerance).  Chosen per workload by the autotuner (graph_amd/autotune.py);
+ * a change takes effect on the next step (the link and cell tiers repack). */
+typedef struct cg_tuning {
+    uint32_t graph_ticks;      /* ticks per captured HIP graph, 1..256 (default 8) */
+    uint32_t cell_lane_max;    /* finest cells of up to this many points: one lane each (16) */
+    uint32_t cell_group_max;   /* up to this: 16 lanes each; above, to 2048: a wave each (64) */
+    uint32_t force_tile;       /* level cells staged in LDS by the force kernel: 512 or 1024 (1024) */
+    uint32_t link_lane_max;    /* link entries per direction up to which a point runs one lane (64) */
+    uint32_t link_group_max;   /* up to which it runs one 64-lane workgroup, <= 256 (256) */
+    uint32_t link_hub_entries; /* entries per lane of 256-lane workgroups: 4, 6 or 8 (6) */
+} cg_tuning;
+void cg_default_tuning(cg_tuning* t);
+int cg_set_tuning(cg_ctx* ctx, const cg_tuning* t);
+int cg_get_tuning(cg_ctx* ctx, cg_tuning* t);
+
+/* Bytes of device memory the context holds. *//* Bytes of device memory the context holds. */
 uint64_t cg_device_bytes(const cg_ctx* ctx);
 
 /* Stage timing on the context's stream (no reference counterpart; used by the

@@ -141,3 +141,23 @@ def test_typescript_dropin_matches_addon_exports():
     # every key the drop-in passes to step() is one the addon reads
     read = set(re.findall(r'read_\w+\(env, obj, "(\w+)"', (napi / "cosmosgl_napi.c").read_text()))
     assert {"spaceSize", "runForces", "mouseRepulsion", "mousePosition"} <= read
+
+
+def test_tuning_struct_and_table_lookup(tmp_path):
+    from graph_amd import autotune
+
+    d = _lib.default_tuning()
+    assert d == {"graph_ticks": 8, "cell_lane_max": 16, "cell_group_max": 64, "force_tile": 1024,
+                 "link_lane_max": 64, "link_group_max": 256, "link_hub_entries": 6}
+    assert ctypes.sizeof(_lib.CgTuning) == 7 * 4
+    table = {"entries": [
+        {"points": 10_000, "mean_degree": 4.0, "tuning": dict(d, graph_ticks=32, link_lane_max=32)},
+        {"points": 1_000_000, "mean_degree": 10.0, "tuning": dict(d, force_tile=512, link_hub_entries=8)}]}
+    # nearest in (log2 points, log2 degree); bit-exact lookups keep the link tiers at their defaults
+    t = autotune.lookup(20_000, 4.0, table=table)
+    assert t["graph_ticks"] == 32 and t["link_lane_max"] == 64
+    t = autotune.lookup(2_000_000, 9.0, bit_exact=False, table=table)
+    assert t["force_tile"] == 512 and t["link_hub_entries"] == 8
+    assert autotune.lookup(5, 1.0, table={"entries": []}) == d
+    assert all(autotune.valid(dict(d, **{k: v})) for k, vs in autotune.BIT_NEUTRAL.items() for v in vs
+               if not (k == "cell_lane_max" and v > d["cell_group_max"]))
