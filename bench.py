@@ -497,8 +497,12 @@ def main() -> int:
     ap.add_argument("--cfg2-every", type=int, default=50, help="ticks between force-error checkpoints")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc: calibration kernels of known traffic (256 MiB), then 5 "
-                         "direct-launch ticks of the 1M/5M workload; prints nothing else")
-    ap.add_argument("--skip", default="", help="comma list of legs to skip: errors,cfg2,batched,serving")
+                         "direct-launch ticks of the 1M/5M workload; prints nothing else (= --probe cfg1)")
+    ap.add_argument("--probe", choices=["cfg1", "cfg2"], default=None,
+                    help="for rocprofv3: the calibration kernels, then --probe-ticks direct-launch ticks of "
+                         "one workload (cfg1: 1M/5M; cfg2: 4M SBM with collision); prints nothing else")
+    ap.add_argument("--probe-ticks", type=int, default=5)
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: errors,cfg2,batched,serving,cfg0")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -510,24 +514,29 @@ def main() -> int:
 
     from graph_amd import _lib, autotune, datagen, workloads
 
-    wl = workloads.cfg1()
-    n = wl["n"]
-    if args.traffic_probe:
+    if args.traffic_probe or args.probe:
         # calibration kernels of known traffic: a 256 MiB fill (writes 256 MiB), then an
         # elementwise scale of it into a second buffer (reads 256 MiB, writes 256 MiB)
         src = torch.ones(64 * 2**20, dtype=torch.float32, device="cuda")
         dst = src * 2.0
         torch.cuda.synchronize()
-        del dst
+        del dst, src
+        pw = workloads.cfg2() if args.probe == "cfg2" else workloads.cfg1()
         ctx = _lib.Context(device=0)
-        ctx.set_positions(wl["xy"])
-        ctx.set_jitter(wl["jitter"])
-        ctx.set_links(wl["links"], None, wl["r_in"], wl["r_out"])
+        autotune.apply(ctx, pw["n"], pw["links"].size / pw["n"])
+        ctx.set_positions(pw["xy"])
+        ctx.set_jitter(pw["jitter"])
+        ctx.set_links(pw["links"], None, pw["r_in"], pw["r_out"])
+        if pw.get("sizes") is not None:
+            ctx.set_point_sizes(pw["sizes"])
+        ctx.set_alpha(1.0)
         ctx.set_capture(False)
-        ctx.step(wl["params"], 5)
+        ctx.step(pw["params"], args.probe_ticks if args.probe else 5)
         ctx.sync()
         ctx.close()
         return 0
+    wl = workloads.cfg1()
+    n = wl["n"]
     ctx = _lib.Context(device=0)
     tuning = autotune.apply(ctx, n, wl["links"].size / n)  # the table's bit-neutral parameters
     ctx.set_positions(wl["xy"])
@@ -629,7 +638,8 @@ def main() -> int:
     if "serving" not in skip:
         out["serving"] = bench_serving(args.serve_seconds, args.serve_budget_gib, args.serve_segment,
                                        args.serve_stride, log)
-    out["cfg0"] = bench_cfg0(300, 20)
+    if "cfg0" not in skip:
+        out["cfg0"] = bench_cfg0(300, 20)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, workloads.cfg0(), args.cpu_seconds)
     print(json.dumps(out), flush=True)

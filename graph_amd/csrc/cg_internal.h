@@ -167,6 +167,8 @@ struct LinkDir {
     float2* warm = nullptr;      // last converged chunk starts, per graph regions
     float4* wjac = nullptr;      // split segments: each lane's prefix Jacobian at that start
                                  // (lane 0: the segment start it converged from)
+    float4* terms = nullptr;     // split segments: each lane's gathered terms (q.x, q.y, D, K),
+                                 // written by round 0, read by the later rounds of a pass
     size_t warm_len = 0;
 };
 

@@ -519,7 +519,8 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
                                                   const uint2* __restrict__ hubs, uint32_t n_hubs,
                                                   uint32_t* __restrict__ hub_done,
                                                   uint32_t* __restrict__ stats,
-                                                  float2* __restrict__ vout) {
+                                                  float2* __restrict__ vout,
+                                                  float4* __restrict__ terms) {
     __shared__ Aff s_wave[B / 64], s_wave_inc[B / 64];
     __shared__ float s_delta[B / 64];
     __shared__ float2 s_final;
@@ -571,9 +572,15 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
     const uint32_t steps = hi - lo;
     const int last = (int)((len - 1) / c);
     const float2 pp = pin[p];
-    // the lane's entries [lo, hi) of the unit, gathered once
+    // the lane's entries [lo, hi) of the unit, gathered once per pass: a split
+    // point's segments keep them in `terms` (slot-major, lanes contiguous) for
+    // the later rounds
     float4 reg[E];
-    {
+    float4* __restrict__ tslot = (split && terms) ? terms + (size_t)(u - u_first) * E * B + t : nullptr;
+    if (tslot && round > 0) {
+#pragma unroll
+        for (int s = 0; s < E; ++s) reg[s] = tslot[s * B];
+    } else {
         uint4 en[E];
 #pragma unroll
         for (int s = 0; s < E; ++s) en[s] = ent[U.x + min(lo + s, len - 1)];
@@ -583,6 +590,10 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
             const float r = __uint_as_float(en[s].y), sv = __uint_as_float(en[s].z),
                         bv = __uint_as_float(en[s].w);
             reg[s] = make_float4(q.x, q.y, (r * d.span + d.lo) * d.dist, d.spring_alpha * sv * bv);
+        }
+        if (tslot) {
+#pragma unroll
+            for (int s = 0; s < E; ++s) tslot[s * B] = reg[s];
         }
     }
     float2 g = *wslot;
@@ -806,6 +817,7 @@ int link_build_graph(Ctx* c, Graph& gr) {
 void link_free_dir(Ctx* c, LinkDir& L) {
     dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff); dfree(c, L.unit_a); dfree(c, L.unit_b);
     dfree(c, L.hubs); dfree(c, L.seg_out); dfree(c, L.hub_done); dfree(c, L.warm); dfree(c, L.wjac);
+    dfree(c, L.terms);
     L = LinkDir();
 }
 
@@ -970,6 +982,8 @@ int link_pack(Ctx* c, const LinkDir (&old)[2]) {
         L.hub_done = (uint32_t*)dalloc(c, (hh.size() ? hh.size() : 1) * 12, &rc);
         L.warm = (float2*)dalloc(c, (wpos ? wpos : 1) * 8, &rc);
         L.wjac = (float4*)dalloc(c, (wpos ? wpos : 1) * 16, &rc);
+        if (!hh.empty())
+            L.terms = (float4*)dalloc(c, (size_t)(L.n_units - L.first_split_unit) * hub_e * 256 * 16, &rc);
         if (rc) { cleanup(); return rc; }
         if (!ua.empty()) {
             CG_HIP(c, hipMemcpyAsync(L.unit_a, ua.data(), ua.size() * 16, hipMemcpyHostToDevice, c->stream));
@@ -1027,11 +1041,11 @@ int link_run_e(Ctx* c, int dir, float2* vout) {
         }
         hipLaunchKernelGGL((k_link_chunk<256, HE>), dim3(L.n_units - us), dim3(256), 0, s, pass, L.unit_a,
                            L.unit_b, us, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm,
-                           L.wjac, L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
+                           L.wjac, L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout, L.terms);
         for (int r = 1; r < HUB_ROUNDS; ++r)
             hipLaunchKernelGGL((k_link_chunk<256, HE>), dim3(L.n_units - us), dim3(256), 0, s, pass, L.unit_a, L.unit_b,
                                us, r, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm, L.wjac,
-                               L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
+                               L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout, L.terms);
         hipLaunchKernelGGL(k_hub_final, dim3(L.n_hubs), dim3(64), 0, s, pass, L.hubs, L.n_hubs, L.unit_a,
                            L.unit_b, L.sched, c->desc, c->pos[0], c->pos[1], c->flags,
                            L.seg_out + (size_t)((HUB_ROUNDS - 1) & 1) * 2 * L.n_units, L.warm, L.hub_done, stats,
@@ -1041,11 +1055,11 @@ int link_run_e(Ctx* c, int dir, float2* vout) {
     if (us > u256)
         hipLaunchKernelGGL((k_link_chunk<256, HE>), dim3(us - u256), dim3(256), 0, c->stream, pass, L.unit_a,
                            L.unit_b, u256, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm,
-                           L.wjac, L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
+                           L.wjac, L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout, nullptr);
     if (L.n_u64)
         hipLaunchKernelGGL((k_link_chunk<64, 4>), dim3(L.n_u64), dim3(64), 0, c->stream, pass, L.unit_a, L.unit_b,
                            0u, 0, L.sched, c->desc, c->pos[0], c->pos[1], c->flags, L.ent, L.warm, L.wjac,
-                           L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout);
+                           L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout, nullptr);
     if (L.n_lane)
         hipLaunchKernelGGL(k_link_lane, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, pass,
                            L.sched, L.soff, L.ent, L.n_lane, c->gid, c->desc, c->pos[0], c->pos[1],
