@@ -98,16 +98,17 @@ def stage_rooflines(breakdown: dict, info: dict) -> dict:
 
 def profiled_traffic(kernel: str) -> tuple[float | None, str | None]:
     """HBM bytes per pass of a stage (or launch of a kernel) from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs of
-    `bench.py --traffic-probe`, scaled by the calibration kernels there), or
-    None when it holds no figure for it."""
+    rocprofv3 --pmc summary (read requests by size and WRITE_SIZE over
+    `bench.py --probe cfg1`, checked against calibration kernels of known
+    traffic there), or None when it holds no figure for it."""
     path = ROOT / "profiles" / "traffic_cfg1.md"
     if not path.exists():
         return None, None
     for line in path.read_text().splitlines():
         parts = line.split()
         if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kernel:
-            return float(parts[2]), "profiles/traffic_cfg1.md (rocprofv3 --pmc FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"
+            return float(parts[2]), ("profiles/traffic_cfg1.md (rocprofv3 --pmc: 128/64/32-byte read requests "
+                                     "(= FETCH_SIZE x 2.0) + WRITE_SIZE x 1.0, per pass)")
     return None, None
 
 
@@ -478,6 +479,40 @@ def bench_serving(window_s: float, budget_gib: float, segment: int, stride: int,
     }
 
 
+def gather_probe() -> dict:
+    """Time per pass of 5M random gathers of 8-byte positions from the 1M-point
+    array (torch index_select with int32 indices, HIP events over 20
+    repetitions): indices uniform, then the 1M/5M graph's neighbour indices
+    in the CSR (owner) order the link passes read them.  Per gather: index 4 B
+    read, position 8 B gathered, 8 B written."""
+    import torch
+
+    from graph_amd import workloads
+
+    wl = workloads.cfg1()
+    n = wl["n"]
+    lk = wl["links"].reshape(-1, 2).astype(np.int64)
+    pos = torch.rand(n, 2, device="cuda", dtype=torch.float32)
+    cases = {"uniform": np.random.default_rng(7).integers(0, n, lk.shape[0])}
+    for d, name in ((0, "csr_in"), (1, "csr_out")):
+        cases[name] = lk[np.argsort(lk[:, d], kind="stable"), 1 - d]
+    out = {"points": n, "gathers": int(lk.shape[0]), "bytes_per_gather": 20}
+    for name, idx in cases.items():
+        t = torch.from_numpy(idx.astype(np.int32)).cuda()
+        for _ in range(3):
+            pos.index_select(0, t)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            pos.index_select(0, t)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / 20 * 1e3
+        out[name] = {"us": us, "gathers_per_s": idx.size / us * 1e6, "algorithmic_gb_s": idx.size * 20 / us / 1e3}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -502,6 +537,10 @@ def main() -> int:
                     help="for rocprofv3: the calibration kernels, then --probe-ticks direct-launch ticks of "
                          "one workload (cfg1: 1M/5M; cfg2: 4M SBM with collision); prints nothing else")
     ap.add_argument("--probe-ticks", type=int, default=5)
+    ap.add_argument("--gather-probe", action="store_true",
+                    help="the random-gather ceiling of the link passes: 5M gathers of 8-byte positions from "
+                         "the 1M-point array (torch index_select), uniform and in the 1M/5M graph's CSR "
+                         "order; prints one JSON line")
     ap.add_argument("--skip", default="", help="comma list of legs to skip: errors,cfg2,batched,serving,cfg0")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -514,6 +553,9 @@ def main() -> int:
 
     from graph_amd import _lib, autotune, datagen, workloads
 
+    if args.gather_probe:
+        print(json.dumps(gather_probe()), flush=True)
+        return 0
     if args.traffic_probe or args.probe:
         # calibration kernels of known traffic: a 256 MiB fill (writes 256 MiB), then an
         # elementwise scale of it into a second buffer (reads 256 MiB, writes 256 MiB)
