@@ -259,7 +259,7 @@ void cg_default_tuning(cg_tuning* t);
 int cg_set_tuning(cg_ctx* ctx, const cg_tuning* t);
 int cg_get_tuning(cg_ctx* ctx, cg_tuning* t);
 
-/* Bytes of device memory the context holds. *//* Bytes of device memory the context holds. */
+/* Bytes of device memory the context holds. */
 uint64_t cg_device_bytes(const cg_ctx* ctx);
 
 /* Stage timing on the context's stream (no reference counterpart; used by the
