@@ -26,7 +26,8 @@
 //     solved as above from a guessed start and reports its end and end
 //     Jacobian; a fix-up composes the segments from the exact start 0 and
 //     corrects the guesses (the same Newton step, one level up), for up to
-//     HUB_ROUNDS rounds.
+//     HUB_ROUNDS rounds, until the corrections hold within FIX_ACCEPT
+//     tolerances or the next one predicted from their rate is within one.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -286,6 +287,17 @@ __global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __r
 constexpr int MAX_SWEEPS = 8;
 constexpr float NOISE_FLOOR = 16.f;  // stalls below this many tolerances end the iteration
 constexpr float FIX_ACCEPT = 4.f;    // largest accepted start correction, in tolerances
+// A split point's fix-up correction d that follows a correction p is also
+// accepted when d * d / p (the next correction if the rounds kept only the
+// rate d / p they just showed; Newton's own rate, d ~ p^2, is faster) is
+// within one tolerance and d within RATE_ACCEPT tolerances: the output then
+// moves by d through the segment Jacobians and misses the solution by about
+// that next correction.  Split points only: their thousands of entries make
+// every spring step weak (bias = the neighbour's degree share), so their
+// rounds converge at Newton's rate; the chunk sweeps of strongly coupled
+// shorter chains keep the strict rule (measured: a rate rule there let an
+// all-multiple-shooting run of a 200k-point graph reach 2.7e-5 S).
+constexpr float RATE_ACCEPT = 64.f;
 // One serial spring step from an entry's terms (no change when !live),
 // also applying the step's Jacobian to J (column-major 2x2, J <- M J):
 //   u' = u + x t(u),  x = q - u,  t = K (1 - D / max(|x|, 0.99 D))
@@ -432,17 +444,19 @@ __device__ inline Composed hub_compose(uint2 hb, uint32_t lane, uint32_t want,
 
 // Whether split point h's segments hold after the round just evaluated: every
 // start correction within FIX_ACCEPT tolerances (the output applies the last
-// correction through the segment Jacobians, so its error is second order), or
-// a stall within NOISE_FLOOR tolerances (the f32 noise floor; a stall far
-// above is a slow Newton step, a spring crossing its clamp, and takes another
-// round).  hub_done holds [n_hubs] marks (the round that found the point
+// correction through the segment Jacobians, so its error is second order), a
+// correction whose rate-predicted successor is within one tolerance
+// (RATE_ACCEPT), or a stall within NOISE_FLOOR tolerances (the f32 noise floor;
+// a stall far above is a slow Newton step, a spring crossing its clamp, and
+// takes another round).  hub_done holds [n_hubs] marks (the round that found the point
 // holding, 0 while open), then the largest correction of the evaluation of
 // round e at [e & 1][h] ([2][n_hubs] f32).
 __device__ inline bool hub_holds(float dmax, int evaluated, uint32_t h, uint32_t n_hubs,
                                  const uint32_t* __restrict__ hub_done, float tol) {
     const float* __restrict__ dp = reinterpret_cast<const float*>(hub_done + n_hubs);
     const float prev = evaluated > 0 ? dp[((evaluated - 1) & 1) * n_hubs + h] : INFINITY;
-    return dmax <= FIX_ACCEPT * tol || (evaluated > 0 && dmax > 0.5f * prev && dmax <= NOISE_FLOOR * tol);
+    return dmax <= FIX_ACCEPT * tol || (evaluated > 0 && dmax > 0.5f * prev && dmax <= NOISE_FLOOR * tol) ||
+           (evaluated > 0 && dmax <= RATE_ACCEPT * tol && dmax * dmax <= tol * prev);
 }
 
 // The split point's velocity, integrated (or into vout), and the round counters.
