@@ -362,7 +362,8 @@ void free_packed(Ctx* c) {
     release(c, c->cl_chunks); release(c, c->cl_first); release(c, c->cl_part); release(c, c->cl_sum);
     c->CC = 0;
     c->n_cl_chunks = 0;
-    release(c, c->col_begin); release(c, c->col_end); release(c, c->col_cell);
+    release(c, c->col_begin); release(c, c->col_end); release(c, c->col_cell); release(c, c->col_crowd);
+    c->col_crowd_cap = 0;
 }
 
 template <typename T>

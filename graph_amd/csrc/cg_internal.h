@@ -265,6 +265,8 @@ struct Ctx {
     uint32_t* col_begin = nullptr;  // [CG / 4] per base cell
     uint32_t* col_end = nullptr;
     float4* col_cell = nullptr;     // [CG] the four grids' cell aggregates
+    uint32_t* col_crowd = nullptr;  // [1 + cap]: count, then the cells deferred to one block each
+    uint32_t col_crowd_cap = 0;
 
     // clusters: point -> packed cluster, per-cluster targets, present members by cluster
     uint32_t CC = 0;                // packed clusters

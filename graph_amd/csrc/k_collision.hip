@@ -8,9 +8,9 @@
 // cell of grid o gathers the points of at most 4 base cells and keeps those
 // whose own grid-o cell (the exact formula above) is that cell.  Per cell, in
 // a fixed order (base cells ascending, points by index within each), the sums
-// of (x - x0, y - y0, size) and the count; relative to the cell's origin so
-// the force pass can remove the point's own contribution without f32
-// cancellation.  The force pass visits the points in base-cell order (a wave
+// of (x - x0, y - y0, size) in f64 and the count; relative to the cell's
+// origin so the force pass can remove the point's own contribution without
+// f32 cancellation.  The force pass visits the points in base-cell order (a wave
 // reads the same few cells), sweeps the 3x3 cell aggregates of all four grids
 // and integrates.
 //
@@ -68,13 +68,44 @@ __global__ void k_col_bounds(const uint32_t* __restrict__ sk, const uint32_t* __
     ssize[k] = size[sv[k]];
 }
 
+// x / d for x < 2^24 and d >= 1: a float reciprocal estimate corrected to
+// the exact quotient.
+__device__ inline uint32_t udiv24(uint32_t x, uint32_t d) {
+    uint32_t q = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)d));
+    int32_t r = (int32_t)(x - q * d);
+    while (r < 0) {
+        --q;
+        r += (int32_t)d;
+    }
+    while (r >= (int32_t)d) {
+        ++q;
+        r -= (int32_t)d;
+    }
+    return q;
+}
+
 // CELL_LANES lanes per cell of every graph's four grids: (sum(x - x0),
 // sum(y - y0), sum(size), count) in f64, stored as f32, with (x0, y0) = cell
 // index * h.  The lanes stride over the points of the cell's base cells
 // (batches of COL_BATCH loads in flight), keep those whose own cell of this
 // grid is this cell, and reduce in a fixed tree, so the result depends only on
-// the points' order in the sort (by index within a base cell).
+// the points' order in the sort (by index within a base cell) and the lanes
+// per cell.  Every term is the exact f64 difference of two f32 values (or an
+// f32 size), so a cell's sums are exact, and independent of the lanes, while
+// n terms whose magnitudes span less than 2^30 / n (always away from the
+// grid's first row and column, where coordinates near 0 can be tiny).  A wave
+// finds its graph once when all its cells belong to one graph.
+// A cell whose base cells hold more than crowd_limit<CELL_LANES> points is
+// deferred to k_col_crowd (one wave each: a hub's neighbours can pile
+// thousands of points into one cell, which a few lanes would sum serially);
+// crowd[0] counts them.  Measured: at 4 lanes per cell, deferring above 128
+// points cut the serving mix's cell pass 590 -> ~100 us; at 16 lanes, cells of
+// 512-2048 points (the 4M SBM graph's contracting communities) cost less
+// in place than deferred.
 constexpr int COL_BATCH = 4;
+constexpr uint32_t COL_CROWD_MIN = 128;
+template <int CELL_LANES>
+constexpr uint32_t crowd_limit() { return CELL_LANES >= 16 ? 2048u : COL_CROWD_MIN; }
 template <int CELL_LANES>
 __global__ void __launch_bounds__(256) k_col_cells(const uint32_t* __restrict__ begin,
                                                    const uint32_t* __restrict__ end,
@@ -82,31 +113,56 @@ __global__ void __launch_bounds__(256) k_col_cells(const uint32_t* __restrict__ 
                                                    const float* __restrict__ ssize,
                                                    const uint32_t* __restrict__ colc0s,
                                                    const GDesc* __restrict__ desc, uint32_t G,
-                                                   uint32_t CG, float4* __restrict__ cells) {
+                                                   uint32_t CG, float4* __restrict__ cells,
+                                                   uint32_t* __restrict__ crowd, uint32_t crowd_cap) {
     const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t c = gt / CELL_LANES, sub = gt % CELL_LANES;
     const bool valid = c < CG;
+    // the graph of the wave's first and last cells; per lane only if they differ
+    const uint32_t cw0 = __builtin_amdgcn_readfirstlane(valid ? c : CG - 1);
+    const uint32_t cw1 = __builtin_amdgcn_readlane(valid ? c : CG - 1, 63);
+    const uint32_t gw0 = find_graph(colc0s, G, cw0);
+    const uint32_t gw1 = find_graph(colc0s, G, cw1);
+    uint32_t gs = gw0;
+    if (gw0 != gw1) gs = find_graph(colc0s, G, valid ? c : CG - 1);
     double ax = 0.0, ay = 0.0, as = 0.0;
     uint32_t n = 0;
+    bool crowded = false;
     if (valid) {
-        const GDesc& d = desc[find_graph(colc0s, G, c)];
+        const GDesc& d = desc[gs];
         const int Gc = (int)d.col_G;
         const uint32_t GG = d.col_G * d.col_G;
         const uint32_t local = c - d.col_c0;
-        const int o = (int)(local / GG);
+        const int o = (int)udiv24(local, GG);
         const uint32_t cell = local - (uint32_t)o * GG;
-        const int cx = (int)(cell % d.col_G), cy = (int)(cell / d.col_G);
+        const int cy = (int)udiv24(cell, d.col_G), cx = (int)cell - cy * Gc;
         const float h = d.col_h;
         const float x0 = (float)cx * h, y0 = (float)cy * h;
         const float ox = c_off[o][0], oy = c_off[o][1];
         const int sx = ox != 0.f ? 1 : 0, sy = oy != 0.f ? 1 : 0;  // base cells cx - sx .. cx
         const uint32_t b0 = d.col_c0 / 4u;
-        for (int by = max(cy - sy, 0); by <= cy; ++by) {
-            for (int bx = max(cx - sx, 0); bx <= cx; ++bx) {
-                const uint32_t bc = b0 + (uint32_t)(by * Gc + bx);
-                const uint32_t e = end[bc];
-                if (e == 0) continue;
-                for (uint32_t k = begin[bc] + sub; k < e; k += COL_BATCH * CELL_LANES) {
+        // the base cells' point ranges, in order (empty ranges for cells off the grid)
+        uint32_t rb[4], re[4], total = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int by = cy - sy + (q >> 1), bx = cx - sx + (q & 1);
+            const bool use = (q >> 1) <= sy && (q & 1) <= sx && by >= 0 && bx >= 0;
+            // q = 1 and 3 exist only with sx, q = 2 and 3 only with sy
+            const bool in = use && ((q & 1) == 0 || sx) && ((q >> 1) == 0 || sy);
+            const uint32_t bc = b0 + (uint32_t)(max(by, 0) * Gc + max(bx, 0));
+            const uint32_t e = in ? end[bc] : 0u;
+            const uint32_t b = e ? begin[bc] : 0u;
+            rb[q] = b;
+            re[q] = e;
+            total += e - b;
+        }
+        crowded = total > crowd_limit<CELL_LANES>();
+#pragma unroll 1
+        for (int q = 0; q < 4 && !crowded; ++q) {
+            const uint32_t e = re[q];
+            if (e == 0) continue;
+            {
+                for (uint32_t k = rb[q] + sub; k < e; k += COL_BATCH * CELL_LANES) {
                     float2 pp[COL_BATCH];
                     float sz[COL_BATCH];
 #pragma unroll
@@ -130,6 +186,20 @@ __global__ void __launch_bounds__(256) k_col_cells(const uint32_t* __restrict__ 
             }
         }
     }
+    // crowded cells to the list: one atomic per wave
+    const bool push = crowded && sub == 0;
+    const uint64_t pm = __ballot(push);
+    if (pm) {
+        const int first = __ffsll((long long)pm) - 1;
+        const int lane = (int)(threadIdx.x & 63u);
+        uint32_t base = 0;
+        if (lane == first) base = atomicAdd(crowd, (uint32_t)__popcll(pm));
+        base = __shfl(base, first, 64);
+        if (push) {
+            const uint32_t idx = base + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull));
+            if (idx < crowd_cap) crowd[1 + idx] = c;
+        }
+    }
     // fixed-tree reduction over the cell's lanes (aligned groups of CELL_LANES)
 #pragma unroll
     for (int w = CELL_LANES / 2; w > 0; w >>= 1) {
@@ -138,7 +208,77 @@ __global__ void __launch_bounds__(256) k_col_cells(const uint32_t* __restrict__ 
         as += __shfl_down(as, w, CELL_LANES);
         n += __shfl_down(n, w, CELL_LANES);
     }
-    if (valid && sub == 0) cells[c] = make_float4((float)ax, (float)ay, (float)as, (float)n);
+    if (valid && sub == 0 && !crowded) cells[c] = make_float4((float)ax, (float)ay, (float)as, (float)n);
+}
+
+// The deferred crowded cells, one wave each (grid-stride over the list): the
+// same sums as k_col_cells, each lane striding over the points of the cell's
+// base cells, then a fixed tree over the wave.
+__global__ void __launch_bounds__(256) k_col_crowd(const uint32_t* __restrict__ crowd, uint32_t crowd_cap,
+                                                   const uint32_t* __restrict__ begin,
+                                                   const uint32_t* __restrict__ end,
+                                                   const float2* __restrict__ spos,
+                                                   const float* __restrict__ ssize,
+                                                   const uint32_t* __restrict__ colc0s,
+                                                   const GDesc* __restrict__ desc, uint32_t G,
+                                                   float4* __restrict__ cells) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t count = min(crowd[0], crowd_cap);
+    for (uint32_t j = wave; j < count; j += waves) {
+        const uint32_t c = crowd[1 + j];
+        const GDesc& d = desc[find_graph(colc0s, G, c)];
+        const int Gc = (int)d.col_G;
+        const uint32_t GG = d.col_G * d.col_G;
+        const uint32_t local = c - d.col_c0;
+        const int o = (int)udiv24(local, GG);
+        const uint32_t cell = local - (uint32_t)o * GG;
+        const int cy = (int)udiv24(cell, d.col_G), cx = (int)cell - cy * Gc;
+        const float h = d.col_h;
+        const float x0 = (float)cx * h, y0 = (float)cy * h;
+        const float ox = c_off[o][0], oy = c_off[o][1];
+        const int sx = ox != 0.f ? 1 : 0, sy = oy != 0.f ? 1 : 0;
+        const uint32_t b0 = d.col_c0 / 4u;
+        double ax = 0.0, ay = 0.0, as = 0.0;
+        uint32_t n = 0;
+        for (int by = max(cy - sy, 0); by <= cy; ++by) {
+            for (int bx = max(cx - sx, 0); bx <= cx; ++bx) {
+                const uint32_t bc = b0 + (uint32_t)(by * Gc + bx);
+                const uint32_t e = end[bc];
+                if (e == 0) continue;
+                for (uint32_t k = begin[bc] + lane; k < e; k += COL_BATCH * 64) {
+                    float2 pp[COL_BATCH];
+                    float sz[COL_BATCH];
+#pragma unroll
+                    for (int u = 0; u < COL_BATCH; ++u) {
+                        const uint32_t kk = min(k + u * 64, e - 1);
+                        pp[u] = spos[kk];
+                        sz[u] = ssize[kk];
+                    }
+#pragma unroll
+                    for (int u = 0; u < COL_BATCH; ++u) {
+                        if (k + u * 64 >= e) break;
+                        const float2 p = pp[u];
+                        if ((sx && col_cell(p.x, ox, h, Gc) != cx) || (sy && col_cell(p.y, oy, h, Gc) != cy))
+                            continue;
+                        ax += (double)p.x - (double)x0;
+                        ay += (double)p.y - (double)y0;
+                        as += (double)sz[u];
+                        ++n;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int w = 32; w > 0; w >>= 1) {
+            ax += __shfl_down(ax, w, 64);
+            ay += __shfl_down(ay, w, 64);
+            as += __shfl_down(as, w, 64);
+            n += __shfl_down(n, w, 64);
+        }
+        if (lane == 0) cells[c] = make_float4((float)ax, (float)ay, (float)as, (float)n);
+    }
 }
 
 // One lane per present collision point, in base-cell order.
@@ -302,18 +442,23 @@ int col_run(Ctx* c, float2* vout) {
     radix_run(c, c->col_rs, c->col_keys, c->col_vals, false, c->col_pos);
     const int fin = c->col_rs.passes & 1;
     hipLaunchKernelGGL(k_fill32, dim3(blocks_for(CB, 256)), dim3(256), 0, c->stream, c->col_end, CB, 0u);
+    hipLaunchKernelGGL(k_fill32, dim3(1), dim3(64), 0, c->stream, c->col_crowd, 1u, 0u);
     hipLaunchKernelGGL(k_col_bounds, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->col_keys[fin],
                        c->col_vals[fin], npc, c->size, c->col_begin, c->col_end, c->col_ssize);
     // lanes per cell from the points per base cell: 16 where cells are crowded
-    // (each lane then strides over a few points), 4 where most are sparse
+    // (each lane then strides over a few points), 4 where most are sparse;
+    // crowded cells then take a wave each
     if (npc >= 8u * CB)
         hipLaunchKernelGGL(k_col_cells<16>, dim3(blocks_for((uint64_t)CG * 16, 256)), dim3(256), 0, c->stream,
                            c->col_begin, c->col_end, c->col_pos[fin], c->col_ssize, c->colc0s, c->desc, c->G, CG,
-                           c->col_cell);
+                           c->col_cell, c->col_crowd, c->col_crowd_cap);
     else
         hipLaunchKernelGGL(k_col_cells<4>, dim3(blocks_for((uint64_t)CG * 4, 256)), dim3(256), 0, c->stream,
                            c->col_begin, c->col_end, c->col_pos[fin], c->col_ssize, c->colc0s, c->desc, c->G, CG,
-                           c->col_cell);
+                           c->col_cell, c->col_crowd, c->col_crowd_cap);
+    hipLaunchKernelGGL(k_col_crowd, dim3(blocks_for(std::min<uint32_t>(c->col_crowd_cap, 16384u), 4)), dim3(256), 0, c->stream,
+                       c->col_crowd, c->col_crowd_cap, c->col_begin, c->col_end, c->col_pos[fin], c->col_ssize,
+                       c->colc0s, c->desc, c->G, c->col_cell);
     hipLaunchKernelGGL(k_col_force, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->col_cell,
                        c->pos[0], c->pos[1], c->col_vals[fin], c->col_pos[fin], c->col_ssize, npc, c->gid, c->desc,
                        c->flags, vout);
@@ -345,6 +490,11 @@ int col_alloc(Ctx* c) {
     c->col_begin = (uint32_t*)dalloc(c, (size_t)(c->CG / 4) * 4, &rc); if (rc) return rc;
     c->col_end = (uint32_t*)dalloc(c, (size_t)(c->CG / 4) * 4, &rc); if (rc) return rc;
     c->col_cell = (float4*)dalloc(c, (size_t)c->CG * 16, &rc); if (rc) return rc;
+    // each point's base cell is a base cell of at most 1 + 2 + 2 + 4 cells of
+    // the four grids, so at most 9 npc / (COL_CROWD_MIN + 1) cells can be
+    // deferred
+    c->col_crowd_cap = (uint32_t)(9ull * npc / (COL_CROWD_MIN + 1) + 1);
+    c->col_crowd = (uint32_t*)dalloc(c, ((size_t)c->col_crowd_cap + 1) * 4, &rc); if (rc) return rc;
     std::vector<uint4> blk;
     std::vector<uint32_t> key0;
     uint64_t range = 1;
