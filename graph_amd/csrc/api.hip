@@ -60,6 +60,33 @@ int hip_fail(Ctx* c, hipError_t e, const char* what) {
 
 // Device allocation charged against the context's HBM budget.  Sets *rc only
 // on failure, so a chain of calls can share one status.
+// Bytes a context keeps in freed-but-cached blocks beyond which frees go
+// straight to hipFree.
+constexpr uint64_t CACHE_MAX = 8ull << 30;
+
+namespace {
+// hipFree of one cached block (largest first), or false when none is left;
+// pending blocks only with `pending_too` (hipFree waits for the device).
+bool evict_one(Ctx* c, bool pending_too) {
+    if (!c->cache.empty()) {
+        auto it = std::prev(c->cache.end());
+        (void)hipFree(it->second);
+        c->held -= it->first;
+        c->cached -= it->first;
+        c->cache.erase(it);
+        return true;
+    }
+    if (pending_too && !c->pending.empty()) {
+        (void)hipFree(c->pending.back().first);
+        c->held -= c->pending.back().second;
+        c->cached -= c->pending.back().second;
+        c->pending.pop_back();
+        return true;
+    }
+    return false;
+}
+}  // namespace
+
 void* dalloc(Ctx* c, size_t bytes, int* rc) {
     if (c->arena) {  // carve from the static scratch: 256-byte aligned, never freed
         const size_t off = (c->arena_used + 255) & ~(size_t)255;
@@ -71,19 +98,38 @@ void* dalloc(Ctx* c, size_t bytes, int* rc) {
         c->arena_used = off + (bytes ? bytes : 16);
         return c->arena + off;
     }
-    if (c->budget && c->held + bytes > c->budget) {
+    const size_t want = ((bytes ? bytes : 16) + 255) & ~(size_t)255;
+    // a cached block of at least the size, at most twice it (+64 KiB for small ones)
+    auto it = c->cache.lower_bound(want);
+    if (it != c->cache.end() && it->first <= std::max<size_t>(2 * want, want + (64u << 10))) {
+        void* p = it->second;
+        const size_t sz = it->first;
+        c->cache.erase(it);
+        c->cached -= sz;
+        c->allocs.emplace_back(p, sz);
+        return p;
+    }
+    while (c->budget && c->held + want > c->budget && evict_one(c, true)) {
+    }
+    if (c->budget && c->held + want > c->budget) {
         *rc = fail(c, CG_ERR_BUDGET, "device allocation of " + std::to_string(bytes) +
                                          " bytes exceeds the context budget");
         return nullptr;
     }
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        while (evict_one(c, true)) {
+        }
+        e = hipMalloc(&p, want);
+    }
     if (e != hipSuccess) {
         *rc = hip_fail(c, e, "hipMalloc");
         return nullptr;
     }
-    c->held += bytes;
-    c->allocs.emplace_back(p, bytes);
+    c->held += want;
+    c->allocs.emplace_back(p, want);
     return p;
 }
 
@@ -91,13 +137,28 @@ void dfree(Ctx* c, void* p) {
     if (!p || c->arena) return;
     for (size_t k = 0; k < c->allocs.size(); ++k) {
         if (c->allocs[k].first == p) {
-            c->held -= c->allocs[k].second;
+            const size_t sz = c->allocs[k].second;
             c->allocs[k] = c->allocs.back();
             c->allocs.pop_back();
-            break;
+            if (c->cached + sz <= CACHE_MAX) {
+                c->pending.emplace_back(p, sz);
+                c->cached += sz;
+            } else {
+                c->held -= sz;
+                (void)hipFree(p);
+            }
+            return;
         }
     }
     (void)hipFree(p);
+}
+
+void cache_settle(Ctx* c) {
+    if (c->pending.empty()) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    for (auto& b : c->pending) c->cache.emplace(b.second, b.first);
+    c->pending.clear();
 }
 
 void prof_begin(Ctx* c, int stage) {
@@ -537,6 +598,7 @@ int pack_body(Ctx* c, const PackSnap& old) {
     float2* const* old_pos = old.pos;
     c->pos[0] = c->pos[1] = nullptr;
     free_packed(c);
+    cache_settle(c);  // nothing in flight reads the freed arrays: reusable below
     c->dir[0] = c->dir[1] = LinkDir();
     c->G = G;
     c->N = (uint32_t)N;
@@ -758,6 +820,7 @@ int pack(Ctx* c) {
         LinkDir L = old.dir[d];
         link_free_dir(c, L);
     }
+    cache_settle(c);  // the repack ended synchronised: the old packing is reusable
     for (uint32_t id : c->order) {
         Graph& g = c->graphs[id];
         if (g.xy_pending) {
@@ -1442,6 +1505,10 @@ int cg_destroy(cg_ctx* x) {
     if (c->side) (void)hipStreamDestroy(c->side);
     for (auto& a : c->allocs) (void)hipFree(a.first);
     c->allocs.clear();
+    for (auto& a : c->pending) (void)hipFree(a.first);
+    c->pending.clear();
+    for (auto& a : c->cache) (void)hipFree(a.second);
+    c->cache.clear();
     if (c->h_desc_pinned) (void)hipHostFree(c->h_desc_pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete x;

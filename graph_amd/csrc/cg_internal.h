@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/cosmosgl.h"
@@ -200,9 +201,17 @@ struct Ctx {
     hipStream_t side = nullptr;     // second stream (default priority) for the hub link tier
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint64_t budget = 0;
-    uint64_t held = 0;
+    uint64_t held = 0;              // live, pending and cached bytes (cg_device_bytes)
     std::string err;
     std::vector<std::pair<void*, size_t>> allocs;
+    // Freed blocks, kept for reuse instead of hipFree (which synchronises the
+    // device and costs ~0.15 ms): `pending` since the last settle (kernels
+    // enqueued before the free may still read them), `cache` once the
+    // context's streams have been seen idle (reusable).  Both count in held;
+    // a budget check or a failed hipMalloc evicts them with hipFree.
+    std::vector<std::pair<void*, size_t>> pending;
+    std::multimap<size_t, void*> cache;
+    uint64_t cached = 0;            // bytes in pending and cache
 
     std::vector<Graph> graphs;      // by id
     bool dirty = true;              // the graph set changed: repack before the next tick
@@ -322,6 +331,7 @@ int fail(Ctx* c, int code, const std::string& msg);
 int hip_fail(Ctx* c, hipError_t e, const char* what);
 void* dalloc(Ctx* c, size_t bytes, int* rc);
 void dfree(Ctx* c, void* p);
+void cache_settle(Ctx* c);     // synchronises the streams; freed blocks become reusable
 
 #define CG_HIP(c, expr)                                  \
     do {                                                 \

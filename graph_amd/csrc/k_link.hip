@@ -819,6 +819,7 @@ int link_build_graph(Ctx* c, Graph& gr) {
     }
     CG_HIP(c, hipStreamSynchronize(c->stream));
     cleanup();
+    cache_settle(c);  // the temporaries are reusable by the next build
     if (rc) return rc;
     gr.links_ready = true;
     return CG_OK;
