@@ -243,6 +243,12 @@ struct Ctx {
     RadixSort mb_rs;
     uint32_t* cell_begin = nullptr; // [F] first sorted position of each finest cell
     uint32_t* cell_end = nullptr;   // [F] one past its last (0 = empty)
+    // Per block of CELL_BLOCK finest cells (Morton-consecutive): a cell of it is
+    // occupied this pass (cell_occ), or it held points this pass or the last one,
+    // so its pyramid tile must be rewritten (cell_dirty).  A tile whose blocks
+    // are all clean keeps last pass's (empty) centroids.
+    uint8_t* cell_occ = nullptr;    // [F / CELL_BLOCK]
+    uint8_t* cell_dirty = nullptr;  // [F / CELL_BLOCK]
     long long* sums = nullptr;      // [3 C] fixed-point sum x, sum y, count per level cell
     float4* cent = nullptr;         // [C]
     float2* rp[2] = {nullptr, nullptr};  // [NP] position payload of the radix sort

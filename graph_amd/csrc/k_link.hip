@@ -256,17 +256,20 @@ __global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __r
                                                    float2* __restrict__ pos0,
                                                    float2* __restrict__ pos1,
                                                    const uint8_t* __restrict__ flags,
-                                                   float2* __restrict__ vout) {
+                                                   float2* __restrict__ vout, uint32_t one_graph) {
     // the longer-chain classes start first
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = r < count;
     const uint32_t j = in ? count - 1 - r : 0u;
     const uint32_t p = sched[j];
-    const GDesc& d = desc[gid[p]];
+    const uint32_t b = soff[j], e1 = soff[j + 1];
+    // a set of one graph reads its descriptor without the graph-slot load,
+    // one dependent memory latency less before the entry loads
+    const GDesc& d = desc[one_graph ? 0u : gid[p]];
     const bool active = in && (d.passes & (1u << pass));
-    const uint32_t b = soff[j], e = active ? soff[j + 1] : b;
+    const uint32_t e = active ? e1 : b;
     // batch depth from the wave's longest chain (every lane is still here)
-    uint32_t mx = e - b;
+    uint32_t mx = in ? e1 - b : 0u;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
     if (!active) return;
@@ -1078,7 +1081,7 @@ int link_run_e(Ctx* c, int dir, float2* vout) {
     if (L.n_lane)
         hipLaunchKernelGGL(k_link_lane, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, pass,
                            L.sched, L.soff, L.ent, L.n_lane, c->gid, c->desc, c->pos[0], c->pos[1],
-                           c->flags, vout);
+                           c->flags, vout, c->G == 1 ? 1u : 0u);
     if (fork) CG_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     CG_HIP(c, hipGetLastError());
     return CG_OK;

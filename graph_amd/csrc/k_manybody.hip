@@ -74,17 +74,32 @@ __device__ inline void radix_block(const uint4* __restrict__ blk, const uint32_t
     }
 }
 
+// The cells occupied at the previous pass (the run heads of its sorted keys,
+// still in `old`): each one's end is cleared, so a cell left empty reads 0,
+// and its block of CELL_BLOCK cells is marked dirty, so the pyramid rewrites
+// its tile even when the tile is empty now.  Only these cells can hold a stale
+// end: every other one has been 0 since the packing was built.
+__global__ void __launch_bounds__(256) k_mb_clear(const uint32_t* __restrict__ old, uint32_t np,
+                                                  uint32_t* __restrict__ cell_end,
+                                                  uint8_t* __restrict__ cell_dirty) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np) return;
+    const uint32_t key = old[k];
+    if (k == 0 || old[k - 1] != key) {
+        cell_end[key] = 0u;
+        cell_dirty[key / CELL_BLOCK] = 1u;
+    }
+}
+
 // Keys of the points in the previous tick's cell order, and the first digit's
 // block histogram.  `visit` may alias `vout` (each lane rewrites what it read).
-// Also clears cell_end (n_cells words) and the four crowded-cell counters for
-// this tick.
+// Also clears the four crowded-cell counters for this tick.
 __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const float2* __restrict__ pos0,
                                                   const float2* __restrict__ pos1,
                                                   const uint32_t* __restrict__ gid,
                                                   const GDesc* __restrict__ desc, uint32_t np,
                                                   uint32_t* __restrict__ kout, uint32_t* vout,
                                                   int width, uint32_t nb, uint32_t* __restrict__ hist,
-                                                  uint32_t* __restrict__ cell_end, uint32_t n_cells,
                                                   uint32_t* __restrict__ big_count,
                                                   float2* __restrict__ pout,
                                                   const uint4* __restrict__ blk,
@@ -94,7 +109,6 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
     const uint32_t radix = 1u << width, mask = radix - 1;
     uint32_t bb, be, k0;
     radix_block(blk, key0, np, bb, be, k0);
-    for (uint32_t f = blockIdx.x * 256 + t; f < n_cells; f += nb * 256) cell_end[f] = 0u;
     if (blockIdx.x == 0 && t < 4) big_count[t] = 0u;
     for (uint32_t q = t; q < radix; q += 256) h[q] = 0;
     __syncthreads();
@@ -251,14 +265,21 @@ __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__
 }
 
 // Cell bounds from the sorted keys: begin at each run's first position, end
-// one past its last (cell_end is cleared first; an empty cell has end 0).
+// one past its last (k_mb_clear has cleared the previous pass's cells; an empty
+// cell has end 0), and the occupied and dirty marks of each run's block.
 __global__ void __launch_bounds__(256) k_bin_bounds(const uint32_t* __restrict__ sk, uint32_t np,
                                                     uint32_t* __restrict__ begin,
-                                                    uint32_t* __restrict__ end) {
+                                                    uint32_t* __restrict__ end,
+                                                    uint8_t* __restrict__ cell_occ,
+                                                    uint8_t* __restrict__ cell_dirty) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np) return;
     const uint32_t key = sk[k];
-    if (k == 0 || sk[k - 1] != key) begin[key] = k;
+    if (k == 0 || sk[k - 1] != key) {
+        begin[key] = k;
+        cell_occ[key / CELL_BLOCK] = 1u;
+        cell_dirty[key / CELL_BLOCK] = 1u;
+    }
     if (k == np - 1 || sk[k + 1] != key) end[key] = k + 1;
 }
 
@@ -383,13 +404,12 @@ __device__ inline void cell_points(uint32_t k0, uint32_t e, const uint32_t* __re
     }
 }
 
-// One lane per finest cell, one block per CELL_BLOCK cells of one graph.  Empty
-// cells write nothing: the pyramid reads their count, and their slots are
-// never read.  Cells above lane_max points go to k_cells_mid: up to group_max
+// One lane for finest cell f.  Empty cells write nothing: the pyramid reads
+// their count, and their slots are never read.  Cells above lane_max points go to k_cells_mid: up to group_max
 // points MID_GROUP lanes each (listed from mid_cells[0]), above it one wave
 // each (listed from mid_cells[mid_cap]); cells above BIG_CHUNK points are
 // queued in chunks of BIG_CHUNK points for k_cells_chunk and k_cells_merge.
-__global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict__ end_,
+__device__ __forceinline__ void cell_lane(uint32_t f, const uint32_t* __restrict__ end_,
                                                       const uint32_t* __restrict__ begin,
                                                       const uint32_t* __restrict__ sv,
                                                       const float2* __restrict__ pos0,
@@ -405,12 +425,11 @@ __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ blk_graph,
                                                       const GDesc* __restrict__ desc, uint32_t lane_max,
                                                       uint32_t group_max) {
-    const uint32_t f = blockIdx.x * CELL_BLOCK + threadIdx.x;
     // most cells are empty: their lanes leave after this one load (alignment
     // padding cells are never filled, so they read 0 as well)
     const uint32_t e = end_[f];
     if (e == 0) return;
-    const GDesc& d = desc[blk_graph[blockIdx.x]];
+    const GDesc& d = desc[blk_graph[f / CELL_BLOCK]];
     if (!(d.passes & (1u << P_MANYBODY))) return;
     const uint32_t mc = f - d.f0;
     if (mc >= d.gf * d.gf) return;  // alignment padding
@@ -443,6 +462,84 @@ __global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint32_t* __restrict
     long long sx = 0, sy = 0;
     cell_points<1>(b, e, sv, sorted_pos, d, sx, sy, tb, ti);
     write_cell(d, rm, sx, sy, n, ti, pin, sums, slot_idx, slot_pos);
+}
+
+// One lane per finest cell, one workgroup per block of CELL_BLOCK cells (all
+// of one graph); a block with no occupied cell (cell_occ clear) leaves at once.
+__global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint8_t* __restrict__ cell_occ,
+                                                   const uint32_t* __restrict__ end_,
+                                                   const uint32_t* __restrict__ begin,
+                                                   const uint32_t* __restrict__ sv,
+                                                   const float2* __restrict__ pos0,
+                                                   const float2* __restrict__ pos1,
+                                                   const float2* __restrict__ sorted_pos,
+                                                   long long* __restrict__ sums,
+                                                   int32_t* __restrict__ slot_idx,
+                                                   float2* __restrict__ slot_pos,
+                                                   uint32_t* __restrict__ mid_cells, uint32_t mid_cap,
+                                                   uint4* __restrict__ big_cells,
+                                                   uint2* __restrict__ big_chunks,
+                                                   uint32_t* __restrict__ big_count,
+                                                   const uint32_t* __restrict__ blk_graph,
+                                                   const GDesc* __restrict__ desc, uint32_t lane_max,
+                                                   uint32_t group_max) {
+    if (!cell_occ[blockIdx.x]) return;  // uniform over the block
+    cell_lane(blockIdx.x * CELL_BLOCK + threadIdx.x, end_, begin, sv, pos0, pos1, sorted_pos, sums, slot_idx,
+              slot_pos, mid_cells, mid_cap, big_cells, big_chunks, big_count, blk_graph, desc, lane_max,
+              group_max);
+}
+
+// The same for sets whose finest cells far outnumber their points (many small
+// graphs: 4-16 cells per point, layouts that collapse into a few clumps): one
+// wave per block of CELL_BLOCK cells, a quarter of the waves, which leaves at
+// once when no cell of the block is occupied and otherwise queues the
+// occupied cells in LDS (a ballot per 64 cells) and runs them 64 at a time,
+// one lane per cell.  Which lane takes a cell changes no result.
+__global__ void __launch_bounds__(64) k_cells_q(const uint8_t* __restrict__ cell_occ,
+                                                const uint32_t* __restrict__ end_,
+                                                const uint32_t* __restrict__ begin,
+                                                const uint32_t* __restrict__ sv,
+                                                const float2* __restrict__ pos0,
+                                                const float2* __restrict__ pos1,
+                                                const float2* __restrict__ sorted_pos,
+                                                long long* __restrict__ sums,
+                                                int32_t* __restrict__ slot_idx,
+                                                float2* __restrict__ slot_pos,
+                                                uint32_t* __restrict__ mid_cells, uint32_t mid_cap,
+                                                uint4* __restrict__ big_cells,
+                                                uint2* __restrict__ big_chunks,
+                                                uint32_t* __restrict__ big_count,
+                                                const uint32_t* __restrict__ blk_graph,
+                                                const GDesc* __restrict__ desc, uint32_t lane_max,
+                                                uint32_t group_max) {
+    if (!cell_occ[blockIdx.x]) return;  // uniform over the wave
+    __shared__ uint32_t q[128];
+    const uint32_t lane = threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t qn = 0;  // queued cells (uniform over the wave)
+#pragma unroll 1
+    for (uint32_t j = 0; j < CELL_BLOCK; j += 64) {
+        const uint32_t f = blockIdx.x * CELL_BLOCK + j + lane;
+        const bool ne = end_[f] != 0u;
+        const unsigned long long m = __ballot(ne);
+        if (ne) q[qn + (uint32_t)__popcll(m & below)] = f;
+        qn += (uint32_t)__popcll(m);
+        if (qn >= 64) {
+            __syncthreads();
+            const uint32_t f0 = q[lane];
+            const uint32_t rest = qn - 64;
+            const uint32_t moved = lane < rest ? q[64 + lane] : 0u;
+            __syncthreads();
+            if (lane < rest) q[lane] = moved;
+            qn = rest;
+            cell_lane(f0, end_, begin, sv, pos0, pos1, sorted_pos, sums, slot_idx, slot_pos, mid_cells, mid_cap,
+                      big_cells, big_chunks, big_count, blk_graph, desc, lane_max, group_max);
+        }
+    }
+    __syncthreads();
+    if (lane < qn)
+        cell_lane(q[lane], end_, begin, sv, pos0, pos1, sorted_pos, sums, slot_idx, slot_pos, mid_cells, mid_cap,
+                  big_cells, big_chunks, big_count, blk_graph, desc, lane_max, group_max);
 }
 
 // The 8 smallest distinct tickets over the private lists of each aligned
@@ -666,16 +763,36 @@ __device__ inline float4 centroid(long long sx, long long sy, long long n, float
 // to one cell per tile.  At the finest level (write_in) an empty cell (end 0)
 // has zero sums.  Only the finest stage's last level keeps its sums (the
 // top-level stage starts from them); every level keeps its centroids.
-__global__ void __launch_bounds__(256) k_pyramid(const uint4* __restrict__ work,
-                                                 const GDesc* __restrict__ desc,
-                                                 const uint32_t* __restrict__ cell_end,
-                                                 long long* __restrict__ sums,
-                                                 float4* __restrict__ cent, int write_in) {
-    __shared__ long long s0[32 * 32 * 3];
-    __shared__ long long s1[16 * 16 * 3];
-    const uint4 wk = work[blockIdx.x];
-    const GDesc& d = desc[wk.x];
-    if (!(d.passes & (1u << P_MANYBODY))) return;
+// A finest-level tile of T >= 16 (whole blocks of CELL_BLOCK cells) whose
+// blocks are all clean is skipped: it held no point at this pass nor at the
+// last one, so every value it would write (empty centroids, zero sums) is
+// already in place, written when it was last processed or when the packing
+// marked every block dirty.  A processed tile clears its blocks' marks.
+// Whether work item wk's tile must be written this pass: its graph runs the
+// pass and, for a tracked finest-level tile, one of its blocks is dirty.
+// Sets the tile's first block and block count (0 when untracked).
+__device__ inline bool pyr_needed(uint4 wk, const GDesc& d, int write_in,
+                                  const uint8_t* __restrict__ cell_dirty, uint32_t& blk0, uint32_t& nblk) {
+    const int lin = (int)wk.y, T = (int)(wk.z & 0xffffu);
+    const int tiles_x = (4 << lin) / T;
+    const int tx = (int)wk.w % tiles_x, ty = (int)wk.w / tiles_x;
+    // the tile's blocks of finest cells: T x T Morton-consecutive cells from an
+    // aligned corner, so T * T / CELL_BLOCK whole blocks when T >= 16
+    const bool tracked = write_in && T * T >= (int)CELL_BLOCK;
+    blk0 = (d.f0 + morton2((uint32_t)(tx * T), (uint32_t)(ty * T))) / CELL_BLOCK;
+    nblk = tracked ? (uint32_t)(T * T) / CELL_BLOCK : 0u;
+    if (!(d.passes & (1u << P_MANYBODY))) return false;
+    if (!tracked) return true;
+    bool dirty = false;
+    for (uint32_t q = 0; q < nblk; ++q) dirty |= cell_dirty[blk0 + q] != 0;
+    return dirty;
+}
+
+// One tile of work item wk, by the whole workgroup (barriers inside).
+__device__ __forceinline__ void pyr_tile(uint4 wk, const GDesc& d, const uint32_t* __restrict__ cell_end,
+                                         long long* __restrict__ sums, float4* __restrict__ cent, int write_in,
+                                         uint8_t* __restrict__ cell_occ, uint8_t* __restrict__ cell_dirty,
+                                         uint32_t blk0, uint32_t nblk, long long* s0, long long* s1) {
     const int lin = (int)wk.y, T = (int)(wk.z & 0xffffu), nout = (int)(wk.z >> 16);
     const int gin = 4 << lin;
     const int tiles_x = gin / T;
@@ -728,6 +845,57 @@ __global__ void __launch_bounds__(256) k_pyramid(const uint4* __restrict__ work,
         cur = nxt;
         nxt = t;
         size = half;
+    }
+    // every lane read the marks before the barriers above
+    if (threadIdx.x < nblk) {
+        cell_occ[blk0 + threadIdx.x] = 0u;
+        cell_dirty[blk0 + threadIdx.x] = 0u;
+    }
+}
+
+// `span` work items per workgroup: with span 1 a workgroup takes its own tile
+// (or leaves at once); a larger span (sets of many mostly empty tiles) lists
+// the tiles of its range that need writing in LDS first and writes them one
+// after another, so the launch does not dispatch a workgroup per clean tile.
+__global__ void __launch_bounds__(256) k_pyramid(const uint4* __restrict__ work, uint32_t nwork,
+                                                 uint32_t span, const GDesc* __restrict__ desc,
+                                                 const uint32_t* __restrict__ cell_end,
+                                                 long long* __restrict__ sums,
+                                                 float4* __restrict__ cent, int write_in,
+                                                 uint8_t* __restrict__ cell_occ,
+                                                 uint8_t* __restrict__ cell_dirty) {
+    __shared__ long long s0[32 * 32 * 3];
+    __shared__ long long s1[16 * 16 * 3];
+    if (span == 1) {
+        const uint4 wk = work[blockIdx.x];
+        const GDesc& d = desc[wk.x];
+        uint32_t blk0, nblk;
+        if (!pyr_needed(wk, d, write_in, cell_dirty, blk0, nblk)) return;  // uniform over the block
+        pyr_tile(wk, d, cell_end, sums, cent, write_in, cell_occ, cell_dirty, blk0, nblk, s0, s1);
+        return;
+    }
+    __shared__ uint32_t s_list[256];
+    __shared__ uint32_t s_n;
+    const uint32_t w0 = blockIdx.x * span;
+    for (uint32_t base = 0; base < span; base += 256) {
+        if (threadIdx.x == 0) s_n = 0u;
+        __syncthreads();
+        const uint32_t w = w0 + base + threadIdx.x;
+        if (base + threadIdx.x < span && w < nwork) {
+            const uint4 wk = work[w];
+            uint32_t blk0, nblk;
+            if (pyr_needed(wk, desc[wk.x], write_in, cell_dirty, blk0, nblk)) s_list[atomicAdd(&s_n, 1u)] = w;
+        }
+        __syncthreads();
+        const uint32_t n = s_n;
+        for (uint32_t q = 0; q < n; ++q) {
+            const uint4 wk = work[s_list[q]];
+            const GDesc& d = desc[wk.x];
+            uint32_t blk0, nblk;
+            (void)pyr_needed(wk, d, write_in, cell_dirty, blk0, nblk);
+            pyr_tile(wk, d, cell_end, sums, cent, write_in, cell_occ, cell_dirty, blk0, nblk, s0, s1);
+        }
+        __syncthreads();  // the list is rewritten next round
     }
 }
 
@@ -1159,6 +1327,9 @@ int mb_alloc(Ctx* c, uint32_t single_gf) {
     c->sorted_pos = c->rp[c->mb_rs.passes & 1];  // the sort's position payload
     c->cell_begin = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
     c->cell_end = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
+    const size_t nblk = F / CELL_BLOCK + 1;
+    c->cell_occ = (uint8_t*)dalloc(c, nblk, &rc); if (rc) return rc;
+    c->cell_dirty = (uint8_t*)dalloc(c, nblk, &rc); if (rc) return rc;
     c->slot_idx = (int32_t*)dalloc(c, F * SLOTS * 4, &rc); if (rc) return rc;
     c->slot_pos = (float2*)dalloc(c, F * SLOTS * 8, &rc); if (rc) return rc;
     // crowded cells: at most NP / (cell_lane_max + 1) of them, and their chunks
@@ -1175,6 +1346,13 @@ int mb_alloc(Ctx* c, uint32_t single_gf) {
     if (c->NP)
         CG_HIP(c, hipMemcpyAsync(c->sorted, c->present, (size_t)c->NP * 4, hipMemcpyDeviceToDevice,
                                  c->stream));
+    // every cell starts empty and every block dirty, so the first pass writes
+    // every pyramid tile; the previous keys read by its clear all name cell 0
+    // (clearing a cell is always safe: the pass rewrites the occupied ones)
+    CG_HIP(c, hipMemsetAsync(c->cell_end, 0, F * 4, c->stream));
+    CG_HIP(c, hipMemsetAsync(c->cell_occ, 0, nblk, c->stream));
+    CG_HIP(c, hipMemsetAsync(c->cell_dirty, 1, nblk, c->stream));
+    CG_HIP(c, hipMemsetAsync(c->rk[c->mb_rs.passes & 1], 0, np * 4, c->stream));
     return rc;
 }
 
@@ -1199,18 +1377,37 @@ int mb_run(Ctx* c, float2* vout) {
     const uint32_t np = c->NP, F = c->F;
     prof_begin(c, CG_STAGE_MB_SORT);
     const RadixSort& rs = c->mb_rs;
+    // the previous pass's sorted keys (rk[passes & 1]) are read before the
+    // first digit's keys overwrite rk[0]
+    hipLaunchKernelGGL(k_mb_clear, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->rk[rs.passes & 1], np,
+                       c->cell_end, c->cell_dirty);
     hipLaunchKernelGGL(k_bin_keys, dim3(rs.nb), dim3(256), 0, c->stream, c->sorted, c->pos[0], c->pos[1],
                        c->gid, c->desc, np, c->rk[0], c->rv[0], rs.width, rs.nb, rs.hist,
-                       c->cell_end, F, c->big_count, c->rp[0], rs.blk, rs.key0);
+                       c->big_count, c->rp[0], rs.blk, rs.key0);
     radix_run(c, rs, c->rk, c->rv, true, c->rp);
     hipLaunchKernelGGL(k_bin_bounds, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream,
-                       c->rk[rs.passes & 1], np, c->cell_begin, c->cell_end);
+                       c->rk[rs.passes & 1], np, c->cell_begin, c->cell_end, c->cell_occ, c->cell_dirty);
     prof_end(c, CG_STAGE_MB_SORT);
     prof_begin(c, CG_STAGE_MB_CELLS);
-    hipLaunchKernelGGL(k_cells, dim3(F / CELL_BLOCK), dim3(CELL_BLOCK), 0, c->stream, c->cell_end,
-                       c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums,
-                       c->slot_idx, c->slot_pos, c->mid_cells, c->mid_cap, c->big_cells, c->big_chunks,
-                       c->big_count, c->blk_graph, c->desc, c->tune.cell_lane_max, c->tune.cell_group_max);
+    // pyramid workgroups: one per tile up to ~4096 of them, then `span` tiles each
+    auto span_for = [](uint32_t items) {
+        uint32_t sp = 1;
+        while (sp * 2 <= items / 4096u) sp *= 2;
+        return sp;
+    };
+    // a workgroup of CELL_BLOCK lanes per block of cells, or one wave per block
+    // when the set has more than 4M finest cells
+    const uint32_t nblocks = F / CELL_BLOCK;
+    if (nblocks > 16384u)
+        hipLaunchKernelGGL(k_cells_q, dim3(nblocks), dim3(64), 0, c->stream, c->cell_occ, c->cell_end,
+                           c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->slot_idx,
+                           c->slot_pos, c->mid_cells, c->mid_cap, c->big_cells, c->big_chunks, c->big_count,
+                           c->blk_graph, c->desc, c->tune.cell_lane_max, c->tune.cell_group_max);
+    else
+        hipLaunchKernelGGL(k_cells, dim3(nblocks), dim3(CELL_BLOCK), 0, c->stream, c->cell_occ, c->cell_end,
+                           c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->slot_idx,
+                           c->slot_pos, c->mid_cells, c->mid_cap, c->big_cells, c->big_chunks, c->big_count,
+                           c->blk_graph, c->desc, c->tune.cell_lane_max, c->tune.cell_group_max);
     // the mid tiers and the crowded-cell chunks touch disjoint cells: the
     // mid tiers run on the side stream beside the other two
     const bool fork = c->side != nullptr;
@@ -1236,12 +1433,15 @@ int mb_run(Ctx* c, float2* vout) {
     prof_end(c, CG_STAGE_MB_CELLS);
     prof_begin(c, CG_STAGE_MB_PYRAMID);
     // pyramid: finest -> tiles of up to 32x32 -> remaining top levels
-    if (c->n_pyr[0])
-        hipLaunchKernelGGL(k_pyramid, dim3(c->n_pyr[0]), dim3(256), 0, c->stream, c->pyr[0], c->desc,
-                           c->cell_end, c->sums, c->cent, 1);
+    if (c->n_pyr[0]) {
+        const uint32_t pspan = span_for(c->n_pyr[0]);
+        hipLaunchKernelGGL(k_pyramid, dim3((c->n_pyr[0] + pspan - 1) / pspan), dim3(256), 0, c->stream, c->pyr[0],
+                           c->n_pyr[0], pspan, c->desc, c->cell_end, c->sums, c->cent, 1, c->cell_occ,
+                           c->cell_dirty);
+    }
     if (c->n_pyr[1])
-        hipLaunchKernelGGL(k_pyramid, dim3(c->n_pyr[1]), dim3(256), 0, c->stream, c->pyr[1], c->desc,
-                           c->cell_end, c->sums, c->cent, 0);
+        hipLaunchKernelGGL(k_pyramid, dim3(c->n_pyr[1]), dim3(256), 0, c->stream, c->pyr[1], c->n_pyr[1], 1u,
+                           c->desc, c->cell_end, c->sums, c->cent, 0, c->cell_occ, c->cell_dirty);
     prof_end(c, CG_STAGE_MB_PYRAMID);
     prof_begin(c, CG_STAGE_MB_FORCE);
     if (c->tune.force_tile == 512)

@@ -82,18 +82,12 @@ def positions(case):
     raise ValueError(case)
 
 
-@pytest.mark.parametrize("case", ["cfg0", "cfg1", "clustered", "crowded", "boundary", "outside",
-                                  "odd_space"])
-def test_cell_assignment_and_levels_bit_exact(case):
-    S, xy = positions(case)
+def check_cells(c, xy, S, tag, gid=None):
+    """Per-level counts and centroids, and finest-cell membership as sets, of
+    the context's last many-body pass against the oracle's binning of xy."""
     n = xy.size // 2
-    say(f"cells {case}: n={n} S={S}")
     absent = O.absent_mask(xy)
     pres = np.flatnonzero(absent == 0).astype(np.int64)
-    c = _lib.Context()
-    c.set_positions(xy)
-    c.set_jitter(datagen.jitter(n, 6))
-    c.eval_force(params(S, repulsion=0.5), _lib.FORCE_MANYBODY, 0.9, 0.375)
     gf, L = O.geometry(n)
     P = xy.reshape(-1, 2)[pres].astype(np.float64)
     for lev in range(L):
@@ -104,20 +98,64 @@ def test_cell_assignment_and_levels_bit_exact(case):
         key = cell[:, 1].astype(np.int64) * g + cell[:, 0]
         want = np.bincount(key, minlength=g * g)
         assert np.array_equal(counts.astype(np.int64), want), \
-            f"level {lev}: counts differ in {int(np.sum(counts != want))} cells"
+            f"{tag} level {lev}: counts differ in {int(np.sum(counts != want))} cells"
         # centroids: the f64 mean of each cell's points
         occ = want > 0
         mean = np.zeros((g * g, 2))
         np.add.at(mean, key, P)
         mean[occ] /= want[occ, None]
         err = float(np.abs(cent[occ] - mean[occ]).max()) if occ.any() else 0.0
-        assert err <= TOL * max(S, float(np.abs(P).max())), f"level {lev}: centroid off by {err}"
+        assert err <= TOL * max(S, float(np.abs(P).max())), f"{tag} level {lev}: centroid off by {err}"
         if finest:
             dev_key = np.repeat(np.arange(g * g, dtype=np.int64), counts.astype(np.int64))
             dev = members.astype(np.int64)[np.lexsort((members, dev_key))]
             ora = pres[np.lexsort((pres, key))]
-            assert np.array_equal(dev, ora), f"finest-cell membership differs for {int(np.sum(dev != ora))} points"
-    say(f"cells {case}: {L} levels, {pres.size} present points bit-exact")
+            assert np.array_equal(dev, ora), f"{tag}: finest-cell membership differs for {int(np.sum(dev != ora))} points"
+    return L, pres.size
+
+
+@pytest.mark.parametrize("case", ["cfg0", "cfg1", "clustered", "crowded", "boundary", "outside",
+                                  "odd_space"])
+def test_cell_assignment_and_levels_bit_exact(case):
+    S, xy = positions(case)
+    n = xy.size // 2
+    say(f"cells {case}: n={n} S={S}")
+    c = _lib.Context()
+    c.set_positions(xy)
+    c.set_jitter(datagen.jitter(n, 6))
+    c.eval_force(params(S, repulsion=0.5), _lib.FORCE_MANYBODY, 0.9, 0.375)
+    L, npres = check_cells(c, xy, S, f"cells {case}")
+    say(f"cells {case}: {L} levels, {npres} present points bit-exact")
+    c.close()
+
+
+@pytest.mark.parametrize("n", [200, 600, 12_000])
+def test_cells_after_collapsing_ticks_bit_exact(n):
+    """A graph's layout collapses from a uniform square into a few clumps within
+    its first ticks, so most pyramid tiles empty out: every pass rewrites only
+    the tiles that held points at that pass or the one before, and the others
+    keep their empty centroids.  After the ticks, one more pass bins the final
+    positions; its cells, centroids and membership must equal the oracle's at
+    every level (finest grids of 32, 64 and 256: tiles below, at and above one
+    block of 256 finest cells)."""
+    S = float(1 << int(np.ceil(np.log2(32 * np.sqrt(n)))))
+    xy = datagen.uniform_square(n, S, 31)
+    links = datagen.barabasi_albert(n, 2, 31)
+    p = params(S, repulsion=0.5, gravity=0.25, link_spring=1.0, link_distance=10.0)
+    c = _lib.Context()
+    c.set_positions(xy)
+    c.set_jitter(datagen.jitter(n, 32))
+    c.set_links(links)
+    c.set_alpha(1.0)
+    say(f"collapse n={n} S={S}")
+    for t in (5, 15, 40):
+        c.step(p, t)
+        cur = c.get_positions()
+        c.eval_force(p, _lib.FORCE_MANYBODY, 0.5, 0.25)
+        L, npres = check_cells(c, cur, S, f"collapse n={n} after +{t} ticks")
+        gf, _ = O.geometry(n)
+        occ = np.unique(O.cells(cur, O.cell_size(S, gf), gf, O.absent_mask(cur)), axis=0).shape[0]
+        say(f"collapse n={n}: +{t} ticks, {occ} of {gf * gf} finest cells occupied, {L} levels bit-exact")
     c.close()
 
 
