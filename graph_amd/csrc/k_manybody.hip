@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
                                                   uint32_t* __restrict__ big_count,
                                                   float2* __restrict__ pout,
                                                   const uint4* __restrict__ blk,
-                                                  const uint32_t* __restrict__ key0) {
+                                                  const uint32_t* __restrict__ key0, uint32_t one_graph) {
     __shared__ uint32_t h[RB_RADIX];
     const int t = threadIdx.x;
     const uint32_t radix = 1u << width, mask = radix - 1;
@@ -113,7 +113,8 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
     for (uint32_t q = t; q < radix; q += 256) h[q] = 0;
     __syncthreads();
     // the lane's RB_ITEMS points go through each dependent load together
-    // (index, graph, position), so a wave waits three latencies, not 3 x RB_ITEMS
+    // (index, graph, position), so a wave waits three latencies, not 3 x RB_ITEMS;
+    // a set of one graph skips the graph loads
     uint32_t ii[RB_ITEMS], gg[RB_ITEMS];
     float2 qq[RB_ITEMS];
 #pragma unroll
@@ -122,7 +123,7 @@ __global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const f
         ii[j] = visit[k];
     }
 #pragma unroll
-    for (int j = 0; j < RB_ITEMS; ++j) gg[j] = gid[ii[j]];
+    for (int j = 0; j < RB_ITEMS; ++j) gg[j] = one_graph ? 0u : gid[ii[j]];
 #pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
         const GDesc& d = desc[gg[j]];
@@ -1134,7 +1135,7 @@ __global__ void __launch_bounds__(256, 8) k_mb_force(const uint32_t* __restrict_
                                                   const GDesc* __restrict__ desc, uint32_t np,
                                                   float2* __restrict__ pos0,
                                                   float2* __restrict__ pos1,
-                                                  float2* __restrict__ vout) {
+                                                  float2* __restrict__ vout, uint32_t one_graph) {
     __shared__ float4 s_tile[MB_TILE];
     __shared__ int s_box[4][4];
     __shared__ uint32_t s_g;
@@ -1143,7 +1144,7 @@ __global__ void __launch_bounds__(256, 8) k_mb_force(const uint32_t* __restrict_
     const bool valid = k < np;
     const uint32_t kk = valid ? k : np - 1;
     const uint32_t i = sv[kk];
-    const uint32_t g = gid[i];
+    const uint32_t g = one_graph ? 0u : gid[i];
     const bool run = (desc[g].passes >> P_MANYBODY) & 1u;
     if (t == 0) s_g = g;
     __syncthreads();
@@ -1383,7 +1384,7 @@ int mb_run(Ctx* c, float2* vout) {
                        c->cell_end, c->cell_dirty);
     hipLaunchKernelGGL(k_bin_keys, dim3(rs.nb), dim3(256), 0, c->stream, c->sorted, c->pos[0], c->pos[1],
                        c->gid, c->desc, np, c->rk[0], c->rv[0], rs.width, rs.nb, rs.hist,
-                       c->big_count, c->rp[0], rs.blk, rs.key0);
+                       c->big_count, c->rp[0], rs.blk, rs.key0, c->G == 1 ? 1u : 0u);
     radix_run(c, rs, c->rk, c->rv, true, c->rp);
     hipLaunchKernelGGL(k_bin_bounds, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream,
                        c->rk[rs.passes & 1], np, c->cell_begin, c->cell_end, c->cell_occ, c->cell_dirty);
@@ -1447,11 +1448,11 @@ int mb_run(Ctx* c, float2* vout) {
     if (c->tune.force_tile == 512)
         hipLaunchKernelGGL(k_mb_force<512>, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->sorted,
                            c->sorted_pos, c->cent, c->slot_idx, c->slot_pos, c->jitter, c->flags, c->gid,
-                           c->desc, np, c->pos[0], c->pos[1], vout);
+                           c->desc, np, c->pos[0], c->pos[1], vout, c->G == 1 ? 1u : 0u);
     else
         hipLaunchKernelGGL(k_mb_force<1024>, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->sorted,
                            c->sorted_pos, c->cent, c->slot_idx, c->slot_pos, c->jitter, c->flags, c->gid,
-                           c->desc, np, c->pos[0], c->pos[1], vout);
+                           c->desc, np, c->pos[0], c->pos[1], vout, c->G == 1 ? 1u : 0u);
     prof_end(c, CG_STAGE_MB_FORCE);
     CG_HIP(c, hipGetLastError());
     return CG_OK;
