@@ -42,11 +42,11 @@ __global__ void k_col_keys(const float2* __restrict__ pos0, const float2* __rest
                            const uint32_t* __restrict__ cpresent, uint32_t npc,
                            const uint32_t* __restrict__ gid, const GDesc* __restrict__ desc,
                            uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                           float2* __restrict__ pays) {
+                           float2* __restrict__ pays, uint32_t one_graph) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npc) return;
     const uint32_t i = cpresent[k];
-    const GDesc& d = desc[gid[i]];
+    const GDesc& d = desc[one_graph ? 0u : gid[i]];
     const float2 p = ((d.inbuf >> P_COLLISION) & 1u) ? pos1[i] : pos0[i];
     const int G = (int)d.col_G;
     const int cx = col_cell(p.x, 0.f, d.col_h, G), cy = col_cell(p.y, 0.f, d.col_h, G);
@@ -291,11 +291,11 @@ __global__ void __launch_bounds__(256) k_col_force(const float4* __restrict__ ce
                                                    uint32_t npc, const uint32_t* __restrict__ gid,
                                                    const GDesc* __restrict__ desc,
                                                    const uint8_t* __restrict__ flags,
-                                                   float2* __restrict__ vout) {
+                                                   float2* __restrict__ vout, uint32_t one_graph) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npc) return;
     const uint32_t i = sv[k];
-    const GDesc& a = desc[gid[i]];
+    const GDesc& a = desc[one_graph ? 0u : gid[i]];  // a set of one graph skips the slot load
     if (!(a.passes & (1u << P_COLLISION))) return;
     const bool bsel = (a.inbuf >> P_COLLISION) & 1u;
     const float2 p = spos[k];
@@ -438,7 +438,8 @@ int col_run(Ctx* c, float2* vout) {
     const uint32_t npc = c->NPC, CG = c->CG, CB = c->CG / 4;
     if (npc == 0 || CG == 0) return CG_OK;
     hipLaunchKernelGGL(k_col_keys, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->pos[0], c->pos[1],
-                       c->cpresent, npc, c->gid, c->desc, c->col_keys[0], c->col_vals[0], c->col_pos[0]);
+                       c->cpresent, npc, c->gid, c->desc, c->col_keys[0], c->col_vals[0], c->col_pos[0],
+                       c->G == 1 ? 1u : 0u);
     radix_run(c, c->col_rs, c->col_keys, c->col_vals, false, c->col_pos);
     const int fin = c->col_rs.passes & 1;
     hipLaunchKernelGGL(k_fill32, dim3(blocks_for(CB, 256)), dim3(256), 0, c->stream, c->col_end, CB, 0u);
@@ -461,7 +462,7 @@ int col_run(Ctx* c, float2* vout) {
                        c->colc0s, c->desc, c->G, c->col_cell);
     hipLaunchKernelGGL(k_col_force, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->col_cell,
                        c->pos[0], c->pos[1], c->col_vals[fin], c->col_pos[fin], c->col_ssize, npc, c->gid, c->desc,
-                       c->flags, vout);
+                       c->flags, vout, c->G == 1 ? 1u : 0u);
     CG_HIP(c, hipGetLastError());
     return CG_OK;
 }

@@ -57,11 +57,11 @@ __global__ void __launch_bounds__(256) k_pointwise(float2* __restrict__ pos0,
                                                    const uint32_t* __restrict__ gid,
                                                    const uint8_t* __restrict__ flags,
                                                    const GDesc* __restrict__ desc, uint32_t np,
-                                                   const double* __restrict__ csum) {
+                                                   const double* __restrict__ csum, uint32_t one_graph) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np) return;
     const uint32_t i = present[k];
-    const uint32_t g = gid[i];
+    const uint32_t g = one_graph ? 0u : gid[i];  // a set of one graph skips the slot load
     const GDesc& d = desc[g];
     if (!(d.passes & (1u << PASS))) return;
     const bool b = (d.inbuf >> PASS) & 1u;
@@ -258,7 +258,7 @@ int pw_run(Ctx* c, int pass, float2* vout) {
     }
 #define CG_PW(P)                                                                                   \
     hipLaunchKernelGGL(k_pointwise<P>, dim3(nb), dim3(256), 0, c->stream, c->pos[0], c->pos[1], vout, \
-                       c->present, c->gid, c->flags, c->desc, c->NP, c->center_sum)
+                       c->present, c->gid, c->flags, c->desc, c->NP, c->center_sum, c->G == 1 ? 1u : 0u)
     switch (pass) {
         case P_GRAVITY: CG_PW(P_GRAVITY); break;
         case P_CENTER: CG_PW(P_CENTER); break;
