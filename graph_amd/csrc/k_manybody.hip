@@ -81,67 +81,257 @@ __device__ inline void radix_block(const uint4* __restrict__ blk, const uint32_t
 // end: every other one has been 0 since the packing was built.
 __global__ void __launch_bounds__(256) k_mb_clear(const uint32_t* __restrict__ old, uint32_t np,
                                                   uint32_t* __restrict__ cell_end,
-                                                  uint8_t* __restrict__ cell_dirty) {
+                                                  uint8_t* __restrict__ cell_dirty,
+                                                  uint32_t* __restrict__ cell_count) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np) return;
     const uint32_t key = old[k];
     if (k == 0 || old[k - 1] != key) {
         cell_end[key] = 0u;
+        cell_count[key] = 0u;
         cell_dirty[key / CELL_BLOCK] = 1u;
     }
 }
 
-// Keys of the points in the previous tick's cell order, and the first digit's
-// block histogram.  `visit` may alias `vout` (each lane rewrites what it read).
-// Also clears the four crowded-cell counters for this tick.
-__global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* visit, const float2* __restrict__ pos0,
-                                                  const float2* __restrict__ pos1,
-                                                  const uint32_t* __restrict__ gid,
-                                                  const GDesc* __restrict__ desc, uint32_t np,
-                                                  uint32_t* __restrict__ kout, uint32_t* vout,
-                                                  int width, uint32_t nb, uint32_t* __restrict__ hist,
-                                                  uint32_t* __restrict__ big_count,
-                                                  float2* __restrict__ pout,
-                                                  const uint4* __restrict__ blk,
-                                                  const uint32_t* __restrict__ key0, uint32_t one_graph) {
-    __shared__ uint32_t h[RB_RADIX];
+// ---- counting sort by finest cell -----------------------------------------------
+//
+// The keys are finest cells (at most 2^18 per graph) and repeat heavily (a
+// layout collapses into clumps of up to tens of thousands of points per
+// cell), so the points are placed by a counting sort: (1) each point's key
+// and its rank among its cell's points; (2) an exclusive scan of the per-cell
+// counts gives every cell's first position (its begin; end = begin + count);
+// (3) each point goes to begin + rank.  A block of CB_TILE keys aggregates
+// them per cell in an LDS hash table (ranks inside the block from LDS
+// atomics) and adds each cell's block count to the cell's global count with
+// one atomic, whose old value is the block's base rank.  So the order of the
+// points inside a cell follows the order in which those atomics land and
+// varies from run to run; nothing computed from the cells depends on it
+// (integer sums, the set-valued slot rule, one lane per point in the force).
+constexpr int CB_ITEMS = 8;                  // keys per lane
+constexpr uint32_t CB_TILE = 256 * CB_ITEMS; // keys per block
+constexpr uint32_t CB_HASH = 4096;           // LDS hash slots (at most half of them used)
+constexpr uint32_t CB_EMPTY = 0xffffffffu;
+constexpr uint32_t SCAN_CELLS = 256 * 64;    // finest cells per block of the count scan
+
+// Keys of the points in the previous pass's cell order (positions as payload)
+// and their ranks in their cells.  Also clears the four crowded-cell counters.
+__global__ void __launch_bounds__(256) k_bin_count(const uint32_t* __restrict__ visit,
+                                                   const float2* __restrict__ pos0,
+                                                   const float2* __restrict__ pos1,
+                                                   const uint32_t* __restrict__ gid,
+                                                   const GDesc* __restrict__ desc, uint32_t np,
+                                                   uint32_t* __restrict__ cell_count,
+                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                   float2* __restrict__ pout, uint32_t* __restrict__ rank,
+                                                   uint32_t* __restrict__ big_count,
+                                                   uint8_t* __restrict__ cell_occ, uint8_t* __restrict__ cell_dirty,
+                                                   uint32_t one_graph) {
+    __shared__ uint32_t hk[CB_HASH], hc[CB_HASH];
     const int t = threadIdx.x;
-    const uint32_t radix = 1u << width, mask = radix - 1;
-    uint32_t bb, be, k0;
-    radix_block(blk, key0, np, bb, be, k0);
+    const uint32_t bb = blockIdx.x * CB_TILE;
     if (blockIdx.x == 0 && t < 4) big_count[t] = 0u;
-    for (uint32_t q = t; q < radix; q += 256) h[q] = 0;
-    __syncthreads();
-    // the lane's RB_ITEMS points go through each dependent load together
-    // (index, graph, position), so a wave waits three latencies, not 3 x RB_ITEMS;
-    // a set of one graph skips the graph loads
-    uint32_t ii[RB_ITEMS], gg[RB_ITEMS];
-    float2 qq[RB_ITEMS];
-#pragma unroll
-    for (int j = 0; j < RB_ITEMS; ++j) {
-        const uint32_t k = min(bb + j * 256 + t, np - 1);
-        ii[j] = visit[k];
+    for (uint32_t q = t; q < CB_HASH; q += 256) {
+        hk[q] = CB_EMPTY;
+        hc[q] = 0u;
     }
+    // the lane's items go through each dependent load together (index, graph,
+    // position); a set of one graph skips the graph loads
+    uint32_t ii[CB_ITEMS], gg[CB_ITEMS];
+    float2 qq[CB_ITEMS];
 #pragma unroll
-    for (int j = 0; j < RB_ITEMS; ++j) gg[j] = one_graph ? 0u : gid[ii[j]];
+    for (int j = 0; j < CB_ITEMS; ++j) ii[j] = visit[min(bb + j * 256 + t, np - 1)];
 #pragma unroll
-    for (int j = 0; j < RB_ITEMS; ++j) {
+    for (int j = 0; j < CB_ITEMS; ++j) gg[j] = one_graph ? 0u : gid[ii[j]];
+#pragma unroll
+    for (int j = 0; j < CB_ITEMS; ++j) {
         const GDesc& d = desc[gg[j]];
         qq[j] = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1[ii[j]] : pos0[ii[j]];
     }
+    __syncthreads();
+    const int lane = t & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t key[CB_ITEMS], slot[CB_ITEMS], r[CB_ITEMS];
 #pragma unroll
-    for (int j = 0; j < RB_ITEMS; ++j) {
-        const uint32_t k = bb + j * 256 + t;
-        if (k < be) {
-            const uint32_t key = finest_key(desc[gg[j]], qq[j]);
-            kout[k] = key;
-            vout[k] = ii[j];
-            pout[k] = qq[j];  // the position rides the sort: cell order without a second gather
-            atomicAdd(&h[(key - k0) & mask], 1u);
+    for (int j = 0; j < CB_ITEMS; ++j) {
+        key[j] = finest_key(desc[gg[j]], qq[j]);
+        slot[j] = 0u;
+        r[j] = 0u;
+        bool pending = bb + j * 256 + t < np;
+        // lanes holding the wave's first pending key go together: one of them
+        // finds the slot and adds their count (points visited in the previous
+        // cell order share few keys per wave); after a few rounds the rest go
+        // one by one
+        for (int round = 0; round < 4; ++round) {
+            const unsigned long long pm = __ballot(pending);
+            if (pm == 0ull) break;
+            const int first = (int)__ffsll((long long)pm) - 1;
+            const uint32_t lead = __shfl(key[j], first, 64);
+            const bool mine = pending && key[j] == lead;
+            const unsigned long long m = __ballot(mine);
+            uint32_t h = 0u, base = 0u;
+            if (lane == first) {
+                // open addressing: at most CB_TILE keys in CB_HASH slots, so a
+                // free slot or the key's own is always found
+                h = (lead * 2654435761u) >> 20;
+                for (;;) {
+                    const uint32_t o = atomicCAS(&hk[h], CB_EMPTY, lead);
+                    if (o == CB_EMPTY || o == lead) break;
+                    h = (h + 1u) & (CB_HASH - 1u);
+                }
+                base = atomicAdd(&hc[h], (uint32_t)__popcll(m));
+            }
+            h = __shfl(h, first, 64);
+            base = __shfl(base, first, 64);
+            if (mine) {
+                slot[j] = h;
+                r[j] = base + (uint32_t)__popcll(m & below);
+                pending = false;
+            }
+        }
+        if (pending) {
+            uint32_t h = (key[j] * 2654435761u) >> 20;
+            for (;;) {
+                const uint32_t o = atomicCAS(&hk[h], CB_EMPTY, key[j]);
+                if (o == CB_EMPTY || o == key[j]) break;
+                h = (h + 1u) & (CB_HASH - 1u);
+            }
+            slot[j] = h;
+            r[j] = atomicAdd(&hc[h], 1u);
         }
     }
     __syncthreads();
-    for (uint32_t q = t; q < radix; q += 256) hist[q * nb + blockIdx.x] = h[q];
+    // one global atomic per cell of the block: its old value is the block's
+    // base; the cell's block of CELL_BLOCK cells is marked occupied and dirty
+    for (uint32_t q = t; q < CB_HASH; q += 256) {
+        const uint32_t kk = hk[q];
+        if (kk != CB_EMPTY) {
+            hc[q] = atomicAdd(&cell_count[kk], hc[q]);
+            cell_occ[kk / CELL_BLOCK] = 1u;
+            cell_dirty[kk / CELL_BLOCK] = 1u;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CB_ITEMS; ++j) {
+        const uint32_t k = bb + j * 256 + t;
+        if (k < np) {
+            kout[k] = key[j];
+            vout[k] = ii[j];
+            pout[k] = qq[j];
+            rank[k] = hc[slot[j]] + r[j];
+        }
+    }
+}
+
+// Block-exclusive scan of one value per thread over 256 threads (4 waves).
+__device__ inline uint32_t block_exscan256(uint32_t v, uint32_t* s_w, uint32_t& total) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += u;
+    }
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    const uint32_t before = (w > 0 ? s_w[0] : 0u) + (w > 1 ? s_w[1] : 0u) + (w > 2 ? s_w[2] : 0u);
+    total = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    return before + inc - v;
+}
+
+// Count scan, step 1: the points of each range of SCAN_CELLS cells.
+__global__ void __launch_bounds__(256) k_count_sums(const uint32_t* __restrict__ cell_count, uint32_t F,
+                                                    const uint8_t* __restrict__ cell_occ,
+                                                    uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t c0 = blockIdx.x * SCAN_CELLS + threadIdx.x * 64u;
+    uint32_t v = 0u;
+    if (c0 < F && cell_occ[c0 / CELL_BLOCK]) {  // unoccupied blocks of cells count 0
+        const uint4* __restrict__ p = reinterpret_cast<const uint4*>(cell_count + c0);
+        const uint32_t nq = min(64u, F - c0) / 4u;  // F and c0 are multiples of CELL_BLOCK
+        for (uint32_t q = 0; q < nq; ++q) {
+            const uint4 x = p[q];
+            v += x.x + x.y + x.z + x.w;
+        }
+    }
+    uint32_t total;
+    (void)block_exscan256(v, s_w, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Count scan, step 2 (one block): exclusive prefixes of the range sums, in place.
+__global__ void __launch_bounds__(256) k_count_scan(uint32_t* __restrict__ sums, uint32_t nr) {
+    __shared__ uint32_t s_w[4];
+    uint32_t carry = 0u;
+    for (uint32_t base = 0; base < nr; base += 256 * 4) {
+        const uint32_t i0 = base + 4u * threadIdx.x;
+        uint32_t v[4], sum = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[q] = i0 + q < nr ? sums[i0 + q] : 0u;
+            sum += v[q];
+        }
+        uint32_t total;
+        uint32_t run = carry + block_exscan256(sum, s_w, total);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (i0 + q < nr) sums[i0 + q] = run;
+            run += v[q];
+        }
+        carry += total;
+        __syncthreads();  // s_w is rewritten next round
+    }
+}
+
+// Count scan, step 3: every occupied cell's bounds (begin = the points of the
+// cells before it, end = begin + count).  Empty cells keep end 0 (k_mb_clear
+// cleared the last pass's cells); unoccupied blocks of cells are not read.
+__global__ void __launch_bounds__(256) k_count_bounds(const uint32_t* __restrict__ cell_count, uint32_t F,
+                                                      const uint8_t* __restrict__ cell_occ,
+                                                      const uint32_t* __restrict__ sums,
+                                                      uint32_t* __restrict__ begin, uint32_t* __restrict__ end) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t c0 = blockIdx.x * SCAN_CELLS + threadIdx.x * 64u;
+    const uint32_t nq = (c0 < F && cell_occ[c0 / CELL_BLOCK]) ? min(64u, F - c0) / 4u : 0u;
+    const uint4* __restrict__ p = reinterpret_cast<const uint4*>(cell_count + c0);
+    uint32_t v = 0u;
+    for (uint32_t q = 0; q < nq; ++q) {
+        const uint4 x = p[q];
+        v += x.x + x.y + x.z + x.w;
+    }
+    uint32_t total;
+    uint32_t run = sums[blockIdx.x] + block_exscan256(v, s_w, total);
+    for (uint32_t q = 0; q < nq; ++q) {
+        const uint4 x = p[q];
+        const uint32_t cs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (cs[u]) {
+                const uint32_t f = c0 + 4u * q + u;
+                begin[f] = run;
+                end[f] = run + cs[u];
+            }
+            run += cs[u];
+        }
+    }
+}
+
+// Counting sort, step 4: each point to its cell's begin plus its rank, with
+// its key, index and position.
+__global__ void __launch_bounds__(256) k_bin_place(const uint32_t* __restrict__ kin,
+                                                   const uint32_t* __restrict__ vin,
+                                                   const float2* __restrict__ pin,
+                                                   const uint32_t* __restrict__ rank, uint32_t np,
+                                                   const uint32_t* __restrict__ begin,
+                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                   float2* __restrict__ pout) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np) return;
+    const uint32_t key = kin[k];
+    const uint32_t at = begin[key] + rank[k];
+    kout[at] = key;
+    vout[at] = vin[k];
+    pout[at] = pin[k];
 }
 
 __global__ void __launch_bounds__(256) k_radix_up(const uint32_t* __restrict__ kin, uint32_t np, int shift,
@@ -263,25 +453,6 @@ __global__ void __launch_bounds__(256) k_radix_down(const uint32_t* __restrict__
         for (uint32_t q = t; q < radix; q += 256) base[q] += wc[0][q] + wc[1][q] + wc[2][q] + wc[3][q];
         __syncthreads();
     }
-}
-
-// Cell bounds from the sorted keys: begin at each run's first position, end
-// one past its last (k_mb_clear has cleared the previous pass's cells; an empty
-// cell has end 0), and the occupied and dirty marks of each run's block.
-__global__ void __launch_bounds__(256) k_bin_bounds(const uint32_t* __restrict__ sk, uint32_t np,
-                                                    uint32_t* __restrict__ begin,
-                                                    uint32_t* __restrict__ end,
-                                                    uint8_t* __restrict__ cell_occ,
-                                                    uint8_t* __restrict__ cell_dirty) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= np) return;
-    const uint32_t key = sk[k];
-    if (k == 0 || sk[k - 1] != key) {
-        begin[key] = k;
-        cell_occ[key / CELL_BLOCK] = 1u;
-        cell_dirty[key / CELL_BLOCK] = 1u;
-    }
-    if (k == np - 1 || sk[k + 1] != key) end[key] = k + 1;
 }
 
 // Exclusive scan of each digit's row of the [digit][block] histogram table in
@@ -1294,32 +1465,15 @@ int mb_alloc(Ctx* c, uint32_t single_gf) {
         c->rk[q] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
         c->rv[q] = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
     }
-    if (single_gf) {
-        rc = radix_alloc(c, c->mb_rs, c->NP, (uint64_t)single_gf * single_gf);
-        if (rc) return rc;
-    } else {
-        // one segment per graph with present points, in slot order (the
-        // present list and every tick's cell order are graph-major)
-        std::vector<uint4> blk;
-        std::vector<uint32_t> key0;
-        uint64_t range = 1;
-        for (uint32_t s = 0; s < c->G; ++s) {
-            const GDesc& d = c->h_desc[s];
-            if (d.np == 0) continue;
-            range = std::max<uint64_t>(range, (uint64_t)d.gf * d.gf);
-            const uint32_t fb = (uint32_t)blk.size(), nbk = (d.np + RB_TILE - 1) / RB_TILE;
-            for (uint32_t q = 0; q < nbk; ++q) {
-                const uint32_t b0 = d.k0 + q * RB_TILE;
-                blk.push_back(make_uint4(b0, std::min(d.k0 + d.np, b0 + RB_TILE), fb, fb + nbk));
-                key0.push_back(d.f0);
-            }
-        }
-        // a lone graph at offset 0 is the plain sort
-        const bool one = !blk.empty() && blk.front().z == 0 && blk.front().w == blk.size() &&
-                         blk.front().x == 0 && key0.front() == 0;
-        rc = one ? radix_alloc(c, c->mb_rs, c->NP, range) : radix_alloc(c, c->mb_rs, c->NP, range, &blk, &key0);
-        if (rc) return rc;
-    }
+    (void)single_gf;
+    // the counting sort writes its keys, indices and positions to index 1 of
+    // each ping-pong pair (the visiting order and the payload it reads are
+    // index 1 of the previous pass, its first step writes index 0)
+    c->mb_rs = RadixSort();
+    c->mb_rs.passes = 1;
+    c->bin_rank = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
+    c->cell_count = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
+    c->scan_sums = (uint32_t*)dalloc(c, (F / SCAN_CELLS + 2) * 4, &rc); if (rc) return rc;
     c->sorted = c->rv[c->mb_rs.passes & 1];
     for (int q = 0; q < 2; ++q) {
         c->rp[q] = (float2*)dalloc(c, np * 8, &rc);
@@ -1351,6 +1505,7 @@ int mb_alloc(Ctx* c, uint32_t single_gf) {
     // every pyramid tile; the previous keys read by its clear all name cell 0
     // (clearing a cell is always safe: the pass rewrites the occupied ones)
     CG_HIP(c, hipMemsetAsync(c->cell_end, 0, F * 4, c->stream));
+    CG_HIP(c, hipMemsetAsync(c->cell_count, 0, F * 4, c->stream));
     CG_HIP(c, hipMemsetAsync(c->cell_occ, 0, nblk, c->stream));
     CG_HIP(c, hipMemsetAsync(c->cell_dirty, 1, nblk, c->stream));
     CG_HIP(c, hipMemsetAsync(c->rk[c->mb_rs.passes & 1], 0, np * 4, c->stream));
@@ -1377,17 +1532,21 @@ int mb_run(Ctx* c, float2* vout) {
     if (c->NP == 0) return CG_OK;
     const uint32_t np = c->NP, F = c->F;
     prof_begin(c, CG_STAGE_MB_SORT);
-    const RadixSort& rs = c->mb_rs;
-    // the previous pass's sorted keys (rk[passes & 1]) are read before the
-    // first digit's keys overwrite rk[0]
-    hipLaunchKernelGGL(k_mb_clear, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->rk[rs.passes & 1], np,
-                       c->cell_end, c->cell_dirty);
-    hipLaunchKernelGGL(k_bin_keys, dim3(rs.nb), dim3(256), 0, c->stream, c->sorted, c->pos[0], c->pos[1],
-                       c->gid, c->desc, np, c->rk[0], c->rv[0], rs.width, rs.nb, rs.hist,
-                       c->big_count, c->rp[0], rs.blk, rs.key0, c->G == 1 ? 1u : 0u);
-    radix_run(c, rs, c->rk, c->rv, true, c->rp);
-    hipLaunchKernelGGL(k_bin_bounds, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream,
-                       c->rk[rs.passes & 1], np, c->cell_begin, c->cell_end, c->cell_occ, c->cell_dirty);
+    // the previous pass's sorted keys (rk[1]) name the cells to clear before
+    // this pass counts into them
+    hipLaunchKernelGGL(k_mb_clear, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->rk[1], np, c->cell_end,
+                       c->cell_dirty, c->cell_count);
+    hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(np, CB_TILE)), dim3(256), 0, c->stream, c->sorted, c->pos[0],
+                       c->pos[1], c->gid, c->desc, np, c->cell_count, c->rk[0], c->rv[0], c->rp[0], c->bin_rank,
+                       c->big_count, c->cell_occ, c->cell_dirty, c->G == 1 ? 1u : 0u);
+    const uint32_t nr = (F + SCAN_CELLS - 1) / SCAN_CELLS;
+    hipLaunchKernelGGL(k_count_sums, dim3(nr), dim3(256), 0, c->stream, c->cell_count, F, c->cell_occ,
+                       c->scan_sums);
+    hipLaunchKernelGGL(k_count_scan, dim3(1), dim3(256), 0, c->stream, c->scan_sums, nr);
+    hipLaunchKernelGGL(k_count_bounds, dim3(nr), dim3(256), 0, c->stream, c->cell_count, F, c->cell_occ,
+                       c->scan_sums, c->cell_begin, c->cell_end);
+    hipLaunchKernelGGL(k_bin_place, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->rk[0], c->rv[0],
+                       c->rp[0], c->bin_rank, np, c->cell_begin, c->rk[1], c->rv[1], c->rp[1]);
     prof_end(c, CG_STAGE_MB_SORT);
     prof_begin(c, CG_STAGE_MB_CELLS);
     // pyramid workgroups: one per tile up to ~4096 of them, then `span` tiles each
