@@ -30,7 +30,7 @@ VALU_FP32_PEAK_TFLOPS = 157.3
 # the link passes fork a helper stream and join it before their closing event).
 STAGE_KERNELS = {
     "gravity": "k_pointwise<GRAVITY>",
-    "mb_sort": "k_bin_keys + k_radix_up/k_scan_rows/k_radix_down per digit + k_bin_bounds",
+    "mb_sort": "k_mb_clear + k_bin_count + k_count_sums/k_count_scan/k_count_bounds + k_bin_place",
     "mb_cells": "k_cells + k_cells_mid<64>/<16> (helper stream) + k_cells_chunk + k_cells_merge",
     "mb_pyramid": "k_pyramid x2",
     "mb_force": "k_mb_force",

@@ -251,7 +251,7 @@ struct Ctx {
     uint8_t* cell_dirty = nullptr;  // [F / CELL_BLOCK]
     uint32_t* cell_count = nullptr; // [F] points per finest cell this pass (0 outside it)
     uint32_t* bin_rank = nullptr;   // [NP] each point's rank in its cell, in visiting order
-    uint32_t* scan_sums = nullptr;  // [F / SCAN_CELLS + 1] per-range counts, then their prefixes
+    uint32_t* scan_sums = nullptr;  // [F / 1024 + 2] per-range counts of the count scan, then their prefixes
     long long* sums = nullptr;      // [3 C] fixed-point sum x, sum y, count per level cell
     float4* cent = nullptr;         // [C]
     float2* rp[2] = {nullptr, nullptr};  // [NP] position payload of the radix sort

@@ -111,7 +111,8 @@ constexpr int CB_ITEMS = 8;                  // keys per lane
 constexpr uint32_t CB_TILE = 256 * CB_ITEMS; // keys per block
 constexpr uint32_t CB_HASH = 4096;           // LDS hash slots (at most half of them used)
 constexpr uint32_t CB_EMPTY = 0xffffffffu;
-constexpr uint32_t SCAN_CELLS = 256 * 64;    // finest cells per block of the count scan
+constexpr uint32_t SCAN_CPT_MIN = 4;        // cells per thread of the count scan: 4 ..
+constexpr uint32_t SCAN_CPT_MAX = 256;      // .. 256, so that at most ~4096 blocks cover the cells
 
 // Keys of the points in the previous pass's cell order (positions as payload)
 // and their ranks in their cells.  Also clears the four crowded-cell counters.
@@ -239,16 +240,20 @@ __device__ inline uint32_t block_exscan256(uint32_t v, uint32_t* s_w, uint32_t& 
     return before + inc - v;
 }
 
-// Count scan, step 1: the points of each range of SCAN_CELLS cells.
+// Count scan, step 1: the points of each range of 256 cpt cells (cpt cells
+// per thread, cpt a power of two in [4, 256]; F is a multiple of CELL_BLOCK).
+// A thread's range lies inside one block of CELL_BLOCK cells, or covers whole
+// ones, and unoccupied blocks count 0 without being read.
 __global__ void __launch_bounds__(256) k_count_sums(const uint32_t* __restrict__ cell_count, uint32_t F,
-                                                    const uint8_t* __restrict__ cell_occ,
+                                                    uint32_t cpt, const uint8_t* __restrict__ cell_occ,
                                                     uint32_t* __restrict__ sums) {
     __shared__ uint32_t s_w[4];
-    const uint32_t c0 = blockIdx.x * SCAN_CELLS + threadIdx.x * 64u;
+    const uint32_t c0 = (blockIdx.x * 256u + threadIdx.x) * cpt;
     uint32_t v = 0u;
-    if (c0 < F && cell_occ[c0 / CELL_BLOCK]) {  // unoccupied blocks of cells count 0
-        const uint4* __restrict__ p = reinterpret_cast<const uint4*>(cell_count + c0);
-        const uint32_t nq = min(64u, F - c0) / 4u;  // F and c0 are multiples of CELL_BLOCK
+    for (uint32_t cb = c0; cb < min(F, c0 + cpt); cb += min(cpt, CELL_BLOCK)) {
+        if (!cell_occ[cb / CELL_BLOCK]) continue;
+        const uint4* __restrict__ p = reinterpret_cast<const uint4*>(cell_count + cb);
+        const uint32_t nq = min(cpt, CELL_BLOCK) / 4u;
         for (uint32_t q = 0; q < nq; ++q) {
             const uint4 x = p[q];
             v += x.x + x.y + x.z + x.w;
@@ -287,31 +292,38 @@ __global__ void __launch_bounds__(256) k_count_scan(uint32_t* __restrict__ sums,
 // cells before it, end = begin + count).  Empty cells keep end 0 (k_mb_clear
 // cleared the last pass's cells); unoccupied blocks of cells are not read.
 __global__ void __launch_bounds__(256) k_count_bounds(const uint32_t* __restrict__ cell_count, uint32_t F,
-                                                      const uint8_t* __restrict__ cell_occ,
+                                                      uint32_t cpt, const uint8_t* __restrict__ cell_occ,
                                                       const uint32_t* __restrict__ sums,
                                                       uint32_t* __restrict__ begin, uint32_t* __restrict__ end) {
     __shared__ uint32_t s_w[4];
-    const uint32_t c0 = blockIdx.x * SCAN_CELLS + threadIdx.x * 64u;
-    const uint32_t nq = (c0 < F && cell_occ[c0 / CELL_BLOCK]) ? min(64u, F - c0) / 4u : 0u;
-    const uint4* __restrict__ p = reinterpret_cast<const uint4*>(cell_count + c0);
+    const uint32_t c0 = (blockIdx.x * 256u + threadIdx.x) * cpt;
+    const uint32_t c1 = min(F, c0 + cpt), step = min(cpt, CELL_BLOCK);
     uint32_t v = 0u;
-    for (uint32_t q = 0; q < nq; ++q) {
-        const uint4 x = p[q];
-        v += x.x + x.y + x.z + x.w;
+    for (uint32_t cb = c0; cb < c1; cb += step) {
+        if (!cell_occ[cb / CELL_BLOCK]) continue;
+        const uint4* __restrict__ p = reinterpret_cast<const uint4*>(cell_count + cb);
+        for (uint32_t q = 0; q < step / 4u; ++q) {
+            const uint4 x = p[q];
+            v += x.x + x.y + x.z + x.w;
+        }
     }
     uint32_t total;
     uint32_t run = sums[blockIdx.x] + block_exscan256(v, s_w, total);
-    for (uint32_t q = 0; q < nq; ++q) {
-        const uint4 x = p[q];
-        const uint32_t cs[4] = {x.x, x.y, x.z, x.w};
+    for (uint32_t cb = c0; cb < c1; cb += step) {
+        if (!cell_occ[cb / CELL_BLOCK]) continue;
+        const uint4* __restrict__ p = reinterpret_cast<const uint4*>(cell_count + cb);
+        for (uint32_t q = 0; q < step / 4u; ++q) {
+            const uint4 x = p[q];
+            const uint32_t cs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (cs[u]) {
-                const uint32_t f = c0 + 4u * q + u;
-                begin[f] = run;
-                end[f] = run + cs[u];
+            for (int u = 0; u < 4; ++u) {
+                if (cs[u]) {
+                    const uint32_t f = cb + 4u * q + u;
+                    begin[f] = run;
+                    end[f] = run + cs[u];
+                }
+                run += cs[u];
             }
-            run += cs[u];
         }
     }
 }
@@ -1473,7 +1485,7 @@ int mb_alloc(Ctx* c, uint32_t single_gf) {
     c->mb_rs.passes = 1;
     c->bin_rank = (uint32_t*)dalloc(c, np * 4, &rc); if (rc) return rc;
     c->cell_count = (uint32_t*)dalloc(c, F * 4, &rc); if (rc) return rc;
-    c->scan_sums = (uint32_t*)dalloc(c, (F / SCAN_CELLS + 2) * 4, &rc); if (rc) return rc;
+    c->scan_sums = (uint32_t*)dalloc(c, (F / (256 * SCAN_CPT_MIN) + 2) * 4, &rc); if (rc) return rc;
     c->sorted = c->rv[c->mb_rs.passes & 1];
     for (int q = 0; q < 2; ++q) {
         c->rp[q] = (float2*)dalloc(c, np * 8, &rc);
@@ -1539,11 +1551,14 @@ int mb_run(Ctx* c, float2* vout) {
     hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(np, CB_TILE)), dim3(256), 0, c->stream, c->sorted, c->pos[0],
                        c->pos[1], c->gid, c->desc, np, c->cell_count, c->rk[0], c->rv[0], c->rp[0], c->bin_rank,
                        c->big_count, c->cell_occ, c->cell_dirty, c->G == 1 ? 1u : 0u);
-    const uint32_t nr = (F + SCAN_CELLS - 1) / SCAN_CELLS;
-    hipLaunchKernelGGL(k_count_sums, dim3(nr), dim3(256), 0, c->stream, c->cell_count, F, c->cell_occ,
+    // cells per scan thread: the fewest (4 .. 256) that keep the ranges to ~4096
+    uint32_t cpt = SCAN_CPT_MIN;
+    while (cpt < SCAN_CPT_MAX && (uint64_t)F > 4096ull * 256u * cpt) cpt *= 2;
+    const uint32_t nr = (uint32_t)(((uint64_t)F + 256u * cpt - 1) / (256u * cpt));
+    hipLaunchKernelGGL(k_count_sums, dim3(nr), dim3(256), 0, c->stream, c->cell_count, F, cpt, c->cell_occ,
                        c->scan_sums);
     hipLaunchKernelGGL(k_count_scan, dim3(1), dim3(256), 0, c->stream, c->scan_sums, nr);
-    hipLaunchKernelGGL(k_count_bounds, dim3(nr), dim3(256), 0, c->stream, c->cell_count, F, c->cell_occ,
+    hipLaunchKernelGGL(k_count_bounds, dim3(nr), dim3(256), 0, c->stream, c->cell_count, F, cpt, c->cell_occ,
                        c->scan_sums, c->cell_begin, c->cell_end);
     hipLaunchKernelGGL(k_bin_place, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->rk[0], c->rv[0],
                        c->rp[0], c->bin_rank, np, c->cell_begin, c->rk[1], c->rv[1], c->rp[1]);
