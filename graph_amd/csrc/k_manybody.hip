@@ -1,9 +1,9 @@
 // k_manybody.hip — grid-pyramid many-body repulsion with the Monte-Carlo
 // near field, as one deterministic pipeline per tick:
 //
-//   1. LSD radix sort of the present points by finest cell (segmented per
-//      graph in a packed set), visiting them in the previous tick's cell
-//      order, positions riding along; cell bounds from the sorted keys
+//   1. counting sort of the present points by finest cell, visiting them in
+//      the previous pass's cell order, positions riding along; cell bounds
+//      from the scan of the per-cell counts
 //   2. per finest cell: fixed-point sums of (x, y), count, and the top-8
 //      near-field slots by (ticket, index) — one lane per cell, 16 lanes per
 //      cell above cell_lane_max points and a wave above cell_group_max (both beside the
@@ -14,8 +14,8 @@
 //   4. per point (in cell order): far field over every level + near field,
 //      fused with the integrate pass
 //
-// Within a cell, points keep the previous tick's order; nothing depends on it
-// anyway.  Sums are integer (positions as multiples of 2^-k, k chosen per graph
+// Within a cell, the points' order follows the order in which the counting
+// sort's block atomics land; nothing depends on it.  Sums are integer (positions as multiples of 2^-k, k chosen per graph
 // so no sum overflows), so they are exact and order-free, and the slot choice
 // is a set function (the 8 smallest distinct tickets, the smallest index per
 // ticket).  No float atomics anywhere: a graph gives the same bits alone or
@@ -48,7 +48,8 @@ __device__ inline uint32_t finest_key(const GDesc& d, float2 q) {
     return d.f0 + morton2(cx, cy);
 }
 
-// LSD radix sort of (cell key, point index), RB_ITEMS keys per lane, digits of
+// LSD radix sort of (cell key, point index) — the collision pass's sort, whose
+// cell sums need points in index order inside a cell — RB_ITEMS keys per lane, digits of
 // `width` <= RB_MAXW bits.  Per pass: block digit histograms (LDS atomics only),
 // an exclusive scan of the [digit][block] table, then a stable scatter whose
 // in-block ranks come from ballot matching in position order.  No global
