@@ -412,6 +412,7 @@ void free_packed(Ctx* c) {
     release(c, c->col_rs.hist); release(c, c->col_rs.tot); release(c, c->col_rs.blk); release(c, c->col_rs.key0);
     c->col_rs = RadixSort();
     release(c, c->cell_begin); release(c, c->cell_end); release(c, c->cell_occ); release(c, c->cell_dirty);
+    release(c, c->cell_count); release(c, c->bin_rank); release(c, c->scan_sums);
     release(c, c->sums); release(c, c->cent); release(c, c->rp[0]); release(c, c->rp[1]); c->sorted_pos = nullptr;
     release(c, c->slot_idx); release(c, c->slot_pos); release(c, c->mid_cells); release(c, c->big_cells); release(c, c->big_chunks); release(c, c->chunk_top);
     release(c, c->big_count);
