@@ -31,7 +31,7 @@ VALU_FP32_PEAK_TFLOPS = 157.3
 STAGE_KERNELS = {
     "gravity": "k_pointwise<GRAVITY>",
     "mb_sort": "k_mb_clear + k_bin_count + k_count_sums/k_count_scan/k_count_bounds + k_bin_place",
-    "mb_cells": "k_cells + k_cells_mid<64>/<16> (helper stream) + k_cells_chunk + k_cells_merge",
+    "mb_cells": "k_cells_seg + k_cells_join",
     "mb_pyramid": "k_pyramid x2",
     "mb_force": "k_mb_force",
     "link_in": "link pass IN: k_link_lane + k_link_chunk<64,4> + k_link_chunk<256,6> + k_hub_final (two streams)",

@@ -62,8 +62,7 @@ class CgSimParams(C.Structure):
 class CgTuning(C.Structure):
     _fields_ = [
         ("graph_ticks", C.c_uint32),
-        ("cell_lane_max", C.c_uint32),
-        ("cell_group_max", C.c_uint32),
+        ("cell_items", C.c_uint32),
         ("force_tile", C.c_uint32),
         ("link_lane_max", C.c_uint32),
         ("link_group_max", C.c_uint32),

@@ -41,8 +41,7 @@ TABLE = Path(__file__).resolve().parent / "tuning.json"
 
 BIT_NEUTRAL = {
     "graph_ticks": [4, 8, 16, 32],
-    "cell_lane_max": [8, 16, 32],
-    "cell_group_max": [32, 64, 128],
+    "cell_items": [4, 8],
     "force_tile": [512, 1024],
 }
 LINK = {
@@ -54,7 +53,7 @@ LINK = {
 
 def valid(t: dict) -> bool:
     """The constraints cg_set_tuning enforces."""
-    return (4 <= t["cell_lane_max"] <= t["cell_group_max"] <= 2048
+    return (t["cell_items"] in (4, 8) and t["force_tile"] in (512, 1024)
             and 2 <= t["link_lane_max"] < t["link_group_max"] <= 256)
 
 

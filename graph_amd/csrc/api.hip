@@ -414,8 +414,7 @@ void free_packed(Ctx* c) {
     release(c, c->cell_begin); release(c, c->cell_end); release(c, c->cell_occ); release(c, c->cell_dirty);
     release(c, c->cell_count); release(c, c->bin_rank); release(c, c->scan_sums);
     release(c, c->sums); release(c, c->cent); release(c, c->rp[0]); release(c, c->rp[1]); c->sorted_pos = nullptr;
-    release(c, c->slot_idx); release(c, c->slot_pos); release(c, c->mid_cells); release(c, c->big_cells); release(c, c->big_chunks); release(c, c->chunk_top);
-    release(c, c->big_count);
+    release(c, c->slot_idx); release(c, c->slot_pos); release(c, c->cell_parts);
     release(c, c->center_part); release(c, c->center_sum); release(c, c->center_chunks);
     release(c, c->center_first);
     c->n_center_chunks = 0;
@@ -1390,8 +1389,7 @@ void cg_default_tuning(cg_tuning* t) {
     if (!t) return;
     const Tuning d;
     t->graph_ticks = d.graph_ticks;
-    t->cell_lane_max = d.cell_lane_max;
-    t->cell_group_max = d.cell_group_max;
+    t->cell_items = d.cell_items;
     t->force_tile = d.force_tile;
     t->link_lane_max = d.lane_max;
     t->link_group_max = d.group_max;
@@ -1403,8 +1401,8 @@ int cg_set_tuning(cg_ctx* x, const cg_tuning* t) {
     if (!t) return fail(c, CG_ERR_INVALID, "cg_set_tuning: null tuning");
     if (t->graph_ticks < 1 || t->graph_ticks > 256)
         return fail(c, CG_ERR_INVALID, "cg_set_tuning: graph_ticks must be in [1, 256]");
-    if (t->cell_lane_max < 4 || t->cell_lane_max > t->cell_group_max || t->cell_group_max > 2048)
-        return fail(c, CG_ERR_INVALID, "cg_set_tuning: need 4 <= cell_lane_max <= cell_group_max <= 2048");
+    if (t->cell_items != 4 && t->cell_items != 8)
+        return fail(c, CG_ERR_INVALID, "cg_set_tuning: cell_items must be 4 or 8");
     if (t->force_tile != 512 && t->force_tile != 1024)
         return fail(c, CG_ERR_INVALID, "cg_set_tuning: force_tile must be 512 or 1024");
     if (t->link_lane_max < 2 || t->link_lane_max >= t->link_group_max || t->link_group_max > T64_LIMIT)
@@ -1413,19 +1411,17 @@ int cg_set_tuning(cg_ctx* x, const cg_tuning* t) {
         return fail(c, CG_ERR_INVALID, "cg_set_tuning: link_hub_entries must be 4, 6 or 8");
     Tuning n;
     n.graph_ticks = t->graph_ticks;
-    n.cell_lane_max = t->cell_lane_max;
-    n.cell_group_max = t->cell_group_max;
+    n.cell_items = t->cell_items;
     n.force_tile = t->force_tile;
     n.lane_max = t->link_lane_max;
     n.group_max = t->link_group_max;
     n.hub_entries = t->link_hub_entries;
     const Tuning& o = c->tune;
-    const bool relayout = n.cell_lane_max != o.cell_lane_max || n.lane_max != o.lane_max ||
-                          n.group_max != o.group_max || n.hub_entries != o.hub_entries;
+    const bool relayout = n.lane_max != o.lane_max || n.group_max != o.group_max || n.hub_entries != o.hub_entries;
     c->tune = n;
     CG_HIP(c, hipStreamSynchronize(c->stream));
     drop_graphs(c);  // captured graphs hold the old launch shapes
-    if (relayout) c->dirty = true;  // buffer sizes and link schedules follow the tuning
+    if (relayout) c->dirty = true;  // the link schedules follow the tuning
     return CG_OK;
 }
 
@@ -1433,8 +1429,7 @@ int cg_get_tuning(cg_ctx* x, cg_tuning* t) {
     if (!x || !t) return CG_ERR_INVALID;
     const Tuning& d = x->c.tune;
     t->graph_ticks = d.graph_ticks;
-    t->cell_lane_max = d.cell_lane_max;
-    t->cell_group_max = d.cell_group_max;
+    t->cell_items = d.cell_items;
     t->force_tile = d.force_tile;
     t->link_lane_max = d.lane_max;
     t->link_group_max = d.group_max;

@@ -37,13 +37,12 @@ constexpr uint32_t T64_LIMIT = 256; // 64 lanes x 4 entries
 constexpr int HUB_ROUNDS = 5;        // segment/fix-up rounds per pass for split points
 
 // Free implementation parameters (cg_tuning in the header; chosen by the
-// autotuner).  None changes the discretisation; the first four do not change
+// autotuner).  None changes the discretisation; the first three do not change
 // any result bit, the link tiers move chains between the serial and the
 // multiple-shooting solvers (results within the force tolerance).
 struct Tuning {
     uint32_t graph_ticks = 8;      // ticks per captured HIP graph
-    uint32_t cell_lane_max = 16;   // finest cells up to this many points: one lane each
-    uint32_t cell_group_max = 64;  // up to this: 16 lanes each; above (to 2048): a wave each
+    uint32_t cell_items = 8;       // sorted points per lane of the cell pass (4 or 8)
     uint32_t force_tile = 1024;    // staged cells per level in the force kernel (512 or 1024)
     uint32_t lane_max = 64;        // link entries per direction up to which a point runs one lane
     uint32_t group_max = 256;      // up to which it runs one 64-lane unit (<= T64_LIMIT)
@@ -258,12 +257,7 @@ struct Ctx {
     float2* sorted_pos = nullptr;   // [NP] positions in cell order (one of rp)
     int32_t* slot_idx = nullptr;    // [8 F] global point index or -1
     float2* slot_pos = nullptr;
-    uint32_t* mid_cells = nullptr;  // cells of cell_lane_max+1 .. BIG_CHUNK points: two lists of mid_cap
-    uint32_t mid_cap = 0;
-    uint4* big_cells = nullptr;     // crowded cells {cell, first chunk, chunks, row-major cell}
-    uint2* big_chunks = nullptr;    // their chunks {cell, chunk}
-    unsigned long long* chunk_top = nullptr;  // [8 per chunk] top-8 (ticket << 32 | index)
-    uint32_t* big_count = nullptr;  // [0] crowded cells, [1] their chunks, [2] mid cells
+    uint4* cell_parts = nullptr;    // [2 per wave of the cell pass] partial cells at a wave's two ends
     uint4* pyr[2] = {nullptr, nullptr};  // pyramid work lists {graph slot, level, tile, levels}
     uint32_t n_pyr[2] = {0, 0};
     double* center_part = nullptr;  // [2 chunks] per-chunk partial sums

@@ -4,11 +4,11 @@
 //   1. counting sort of the present points by finest cell, visiting them in
 //      the previous pass's cell order, positions riding along; cell bounds
 //      from the scan of the per-cell counts
-//   2. per finest cell: fixed-point sums of (x, y), count, and the top-8
-//      near-field slots by (ticket, index) — one lane per cell, 16 lanes per
-//      cell above cell_lane_max points and a wave above cell_group_max (both beside the
-//      crowded tier, on the side stream), chunked blocks plus a merge above
-//      BIG_CHUNK points
+//   2. per finest cell (a run of equal keys in the sorted points), over the
+//      points rather than the cells: fixed-point sums of (x, y), count, and
+//      the top-8 near-field slots by (ticket, index) — one wave per 64 N
+//      sorted points, runs inside a lane finished by it, runs across lanes by
+//      segmented scans, runs across waves by a merge of the waves' partials
 //   3. 2x2 merges up the pyramid in the same fixed point, f32 centroids
 //      (cx, cy, N) per level
 //   4. per point (in cell order): far field over every level + near field,
@@ -31,11 +31,6 @@
 
 namespace cg {
 namespace {
-
-constexpr uint32_t TMAX = 0xffffffffu;
-constexpr uint32_t BIG_CHUNK = 2048;  // points per block in a crowded cell
-constexpr uint32_t MID_BLOCKS = 2048; // grid of each mid-cell pass (8192 waves)
-constexpr int MID_GROUP = 16;         // lanes per mid cell of at most Tuning::cell_group_max points
 
 // ---- 1. counting sort by finest cell ---------------------------------------------
 
@@ -116,7 +111,7 @@ constexpr uint32_t SCAN_CPT_MIN = 4;        // cells per thread of the count sca
 constexpr uint32_t SCAN_CPT_MAX = 256;      // .. 256, so that at most ~4096 blocks cover the cells
 
 // Keys of the points in the previous pass's cell order (positions as payload)
-// and their ranks in their cells.  Also clears the four crowded-cell counters.
+// and their ranks in their cells.
 __global__ void __launch_bounds__(256) k_bin_count(const uint32_t* __restrict__ visit,
                                                    const float2* __restrict__ pos0,
                                                    const float2* __restrict__ pos1,
@@ -125,13 +120,11 @@ __global__ void __launch_bounds__(256) k_bin_count(const uint32_t* __restrict__ 
                                                    uint32_t* __restrict__ cell_count,
                                                    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                    float2* __restrict__ pout, uint32_t* __restrict__ rank,
-                                                   uint32_t* __restrict__ big_count,
                                                    uint8_t* __restrict__ cell_occ, uint8_t* __restrict__ cell_dirty,
                                                    uint32_t one_graph) {
     __shared__ uint32_t hk[CB_HASH], hc[CB_HASH];
     const int t = threadIdx.x;
     const uint32_t bb = blockIdx.x * CB_TILE;
-    if (blockIdx.x == 0 && t < 4) big_count[t] = 0u;
     for (uint32_t q = t; q < CB_HASH; q += 256) {
         hk[q] = CB_EMPTY;
         hc[q] = 0u;
@@ -508,425 +501,490 @@ __global__ void __launch_bounds__(256) k_scan_rows(uint32_t* __restrict__ hist, 
     if (t == 0) totals[blockIdx.x] = carry;
 }
 
-// ---- 2. finest cells: sums + slots ----------------------------------------------
-
-// Insert (t, i) into a top-8 list of distinct tickets, ascending.  An equal
-// ticket keeps the smaller index, whatever the visiting order: the result is
-// that of drawing points in ascending index with the depth test's first-drawn
-// rule (build-nearfield-slots.vert:85-101).
-__device__ inline void top8_insert(uint32_t (&tb)[SLOTS], uint32_t (&ti)[SLOTS], uint32_t t,
-                                   uint32_t i) {
-    if (t > tb[SLOTS - 1]) return;
-    bool dup = false;
-#pragma unroll
-    for (int s = 0; s < SLOTS; ++s) {
-        const bool eq = tb[s] == t;
-        dup |= eq;
-        ti[s] = (eq && i < ti[s]) ? i : ti[s];
-    }
-    if (dup) return;
-#pragma unroll
-    for (int s = 0; s < SLOTS; ++s) {
-        bool sw = t < tb[s];
-        uint32_t a = tb[s], b = ti[s];
-        tb[s] = sw ? t : a;
-        ti[s] = sw ? i : b;
-        t = sw ? a : t;
-        i = sw ? b : i;
-    }
-}
-
+// ---- 2. finest cells over runs of the sorted points -------------------------------
+//
+// One wave per 64 N consecutive sorted points (N per lane, in order), so the
+// work follows the points, not the cells: a cell's points are one run of
+// equal keys.  A run inside one lane is finished by that lane: sums in
+// sequence, and each point's slot is its rank among the run's distinct
+// tickets.  A run across lanes is a segment of the wave: its sums come from a
+// segmented scan, its top-8 from 8 rounds of a segmented minimum over the
+// lanes' next candidates (each lane pops its candidate when the round's
+// minimum has its ticket).  A run across waves leaves a partial record at
+// each wave's end it crosses, and k_cells_join merges them.
 // Position as an integer multiple of 2^-k (exact for |p| >= 2^(23-k)).
 __device__ inline long long fixed(float v, float scale) {
     return __double2ll_rn((double)v * (double)scale);
 }
 
-__device__ inline void write_cell(const GDesc& d, uint32_t rm, long long sx, long long sy, uint32_t cnt,
-                                  const uint32_t (&ti)[SLOTS], const float2* __restrict__ pin,
-                                  long long* __restrict__ sums, int32_t* __restrict__ slot_idx,
-                                  float2* __restrict__ slot_pos) {
-    const size_t c = (size_t)d.c0 + level_off(d.L - 1) + rm;
-    sums[3 * c + 0] = sx;
-    sums[3 * c + 1] = sy;
-    sums[3 * c + 2] = (long long)cnt;
-    const size_t sb = ((size_t)d.f0 + rm) * SLOTS;
-#pragma unroll
-    for (int s = 0; s < SLOTS; ++s) {
-        bool ok = ti[s] != TMAX;
-        slot_idx[sb + s] = ok ? (int32_t)ti[s] : -1;
-        slot_pos[sb + s] = ok ? pin[ti[s]] : make_float2(0.f, 0.f);
-    }
+constexpr uint32_t KEY_NONE = 0xffffffffu;  // no cell: past the end, or a graph without the pass
+constexpr unsigned long long TOP_NONE = ~0ull;
+
+struct __align__(16) CellPart {
+    unsigned long long top[SLOTS];  // (ticket << 32 | index) ascending, TOP_NONE past the end
+    float2 pos[SLOTS];              // their positions
+    long long sx, sy;
+    uint32_t key, cnt, pad0, pad1;
+};
+static_assert(sizeof(CellPart) == 160, "ten uint4 per partial");
+
+// The fields of one graph the cell pass reads.
+struct SegGraph {
+    uint32_t passes, p0, f0, c0, gf, L;
+    float seed, fxs;
+};
+__device__ inline SegGraph seg_graph(const GDesc* __restrict__ desc, uint32_t g) {
+    const GDesc& d = desc[g];
+    SegGraph s;
+    s.passes = d.passes; s.p0 = d.p0; s.f0 = d.f0; s.c0 = d.c0; s.gf = d.gf; s.L = d.L;
+    s.seed = d.seed; s.fxs = d.fx_scale;
+    return s;
 }
 
-// Accumulates the points at sorted positions k0, k0 + STRIDE, ... below e into
-// one lane's integer sums and top-8 list (positions from the sort's payload,
-// in cell order).  Loads go out CELL_BATCH at a time so a lane waits on one
-// memory latency per batch rather than per point; the sums are exact and the
-// top-8 rule is order-free, so the batching changes no result.
-constexpr int CELL_BATCH = 8;
-template <uint32_t STRIDE>
-__device__ inline void cell_points(uint32_t k0, uint32_t e, const uint32_t* __restrict__ sv,
-                                   const float2* __restrict__ sorted_pos, const GDesc& d, long long& sx,
-                                   long long& sy, uint32_t (&tb)[SLOTS], uint32_t (&ti)[SLOTS]) {
-    for (uint32_t k = k0; k < e; k += CELL_BATCH * STRIDE) {
-        uint32_t ii[CELL_BATCH];
-        float2 pp[CELL_BATCH];
-#pragma unroll
-        for (int u = 0; u < CELL_BATCH; ++u) {
-            const uint32_t kk = min(k + u * STRIDE, e - 1);
-            ii[u] = sv[kk];
-            pp[u] = sorted_pos[kk];
-        }
-#pragma unroll
-        for (int u = 0; u < CELL_BATCH; ++u) {
-            const uint32_t kk = k + u * STRIDE;
-            if (kk < e) {
-                sx += fixed(pp[u].x, d.fx_scale);
-                sy += fixed(pp[u].y, d.fx_scale);
-                top8_insert(tb, ti, ticket_bits(ii[u] - d.p0, d.seed), ii[u]);
-            }
-        }
-    }
-}
-
-// One lane for finest cell f.  Empty cells write nothing: the pyramid reads
-// their count, and their slots are never read.  Cells above lane_max points go to k_cells_mid: up to group_max
-// points MID_GROUP lanes each (listed from mid_cells[0]), above it one wave
-// each (listed from mid_cells[mid_cap]); cells above BIG_CHUNK points are
-// queued in chunks of BIG_CHUNK points for k_cells_chunk and k_cells_merge.
-__device__ __forceinline__ void cell_lane(uint32_t f, const uint32_t* __restrict__ end_,
-                                                      const uint32_t* __restrict__ begin,
-                                                      const uint32_t* __restrict__ sv,
-                                                      const float2* __restrict__ pos0,
-                                                      const float2* __restrict__ pos1,
-                                                      const float2* __restrict__ sorted_pos,
-                                                      long long* __restrict__ sums,
-                                                      int32_t* __restrict__ slot_idx,
-                                                      float2* __restrict__ slot_pos,
-                                                      uint32_t* __restrict__ mid_cells, uint32_t mid_cap,
-                                                      uint4* __restrict__ big_cells,
-                                                      uint2* __restrict__ big_chunks,
-                                                      uint32_t* __restrict__ big_count,
-                                                      const uint32_t* __restrict__ blk_graph,
-                                                      const GDesc* __restrict__ desc, uint32_t lane_max,
-                                                      uint32_t group_max) {
-    // most cells are empty: their lanes leave after this one load (alignment
-    // padding cells are never filled, so they read 0 as well)
-    const uint32_t e = end_[f];
-    if (e == 0) return;
-    const GDesc& d = desc[blk_graph[f / CELL_BLOCK]];
-    if (!(d.passes & (1u << P_MANYBODY))) return;
+__device__ inline uint32_t seg_rm(const SegGraph& d, uint32_t f) {
     const uint32_t mc = f - d.f0;
-    if (mc >= d.gf * d.gf) return;  // alignment padding
-    const uint32_t b = begin[f], n = e - b;
-    const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
-    if (n > lane_max && n <= group_max) {
-        mid_cells[atomicAdd(&big_count[2], 1u)] = f;  // MID_GROUP lanes each (k_cells_mid)
-        return;
-    }
-    if (n > group_max && n <= BIG_CHUNK) {
-        mid_cells[mid_cap + atomicAdd(&big_count[3], 1u)] = f;  // one wave each (k_cells_mid)
-        return;
-    }
-    if (n > BIG_CHUNK) {
-        // crowded: sums are added by the chunk blocks, slots merged after them
-        const size_t c3 = 3 * ((size_t)d.c0 + level_off(d.L - 1) + rm);
-        sums[c3] = 0;
-        sums[c3 + 1] = 0;
-        sums[c3 + 2] = (long long)n;
-        const uint32_t nch = (n + BIG_CHUNK - 1) / BIG_CHUNK;
-        const uint32_t e0 = atomicAdd(&big_count[1], nch);
-        big_cells[atomicAdd(&big_count[0], 1u)] = make_uint4(f, e0, nch, rm);
-        for (uint32_t q = 0; q < nch; ++q) big_chunks[e0 + q] = make_uint2(f, q);
-        return;
-    }
-    const float2* __restrict__ pin = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1 : pos0;
-    uint32_t tb[SLOTS], ti[SLOTS];
-#pragma unroll
-    for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
-    long long sx = 0, sy = 0;
-    cell_points<1>(b, e, sv, sorted_pos, d, sx, sy, tb, ti);
-    write_cell(d, rm, sx, sy, n, ti, pin, sums, slot_idx, slot_pos);
+    return compact1by1(mc >> 1) * d.gf + compact1by1(mc);
 }
 
-// One lane per finest cell, one workgroup per block of CELL_BLOCK cells (all
-// of one graph); a block with no occupied cell (cell_occ clear) leaves at once.
-__global__ void __launch_bounds__(CELL_BLOCK) k_cells(const uint8_t* __restrict__ cell_occ,
-                                                   const uint32_t* __restrict__ end_,
-                                                   const uint32_t* __restrict__ begin,
+// Cross-lane moves by DPP (no LDS): row_shr:k = 0x110 + k, row_bcast:15 =
+// 0x142 (rows 1 and 3 take lane 15 of the row before), row_bcast:31 = 0x143
+// (rows 2 and 3 take lane 31), wave_shr:1 = 0x138.  A lane whose source is
+// outside its row, or whose row is masked off, takes `old`.
+template <int CTRL, int RM>
+__device__ inline uint32_t dpp32(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, RM, 0xf, false);
+}
+template <int CTRL, int RM>
+__device__ inline unsigned long long dpp64(unsigned long long old, unsigned long long v) {
+    const uint32_t lo = dpp32<CTRL, RM>((uint32_t)old, (uint32_t)v);
+    const uint32_t hi = dpp32<CTRL, RM>((uint32_t)(old >> 32), (uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ inline unsigned long long readlane64(unsigned long long v, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Segmented inclusive scans over the wave (Hillis-Steele inside 16-lane rows,
+// then the row broadcasts); f = 1 opens a segment at the lane.
+template <int CTRL, int RM>
+__device__ inline void segmin_step(unsigned long long& v, uint32_t& f) {
+    const unsigned long long vo = dpp64<CTRL, RM>(TOP_NONE, v);
+    const uint32_t fo = dpp32<CTRL, RM>(0u, f);
+    if (!f) v = vo < v ? vo : v;
+    f |= fo;
+}
+__device__ inline unsigned long long segmin_scan(unsigned long long v, uint32_t f) {
+    segmin_step<0x111, 0xf>(v, f);
+    segmin_step<0x112, 0xf>(v, f);
+    segmin_step<0x114, 0xf>(v, f);
+    segmin_step<0x118, 0xf>(v, f);
+    segmin_step<0x142, 0xa>(v, f);
+    segmin_step<0x143, 0xc>(v, f);
+    return v;
+}
+template <int CTRL, int RM>
+__device__ inline void segsum_step(long long& x, long long& y, uint32_t& n, uint32_t& f) {
+    const long long xo = (long long)dpp64<CTRL, RM>(0ull, (unsigned long long)x);
+    const long long yo = (long long)dpp64<CTRL, RM>(0ull, (unsigned long long)y);
+    const uint32_t no = dpp32<CTRL, RM>(0u, n);
+    const uint32_t fo = dpp32<CTRL, RM>(0u, f);
+    if (!f) { x += xo; y += yo; n += no; }
+    f |= fo;
+}
+__device__ inline void segsum_scan(long long& x, long long& y, uint32_t& n, uint32_t f) {
+    segsum_step<0x111, 0xf>(x, y, n, f);
+    segsum_step<0x112, 0xf>(x, y, n, f);
+    segsum_step<0x114, 0xf>(x, y, n, f);
+    segsum_step<0x118, 0xf>(x, y, n, f);
+    segsum_step<0x142, 0xa>(x, y, n, f);
+    segsum_step<0x143, 0xc>(x, y, n, f);
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_cells_seg(const uint32_t* __restrict__ skey,
                                                    const uint32_t* __restrict__ sv,
-                                                   const float2* __restrict__ pos0,
-                                                   const float2* __restrict__ pos1,
-                                                   const float2* __restrict__ sorted_pos,
+                                                   const float2* __restrict__ spos, uint32_t np,
+                                                   const uint32_t* __restrict__ blk_graph,
+                                                   const GDesc* __restrict__ desc, uint32_t one_graph,
                                                    long long* __restrict__ sums,
                                                    int32_t* __restrict__ slot_idx,
                                                    float2* __restrict__ slot_pos,
-                                                   uint32_t* __restrict__ mid_cells, uint32_t mid_cap,
-                                                   uint4* __restrict__ big_cells,
-                                                   uint2* __restrict__ big_chunks,
-                                                   uint32_t* __restrict__ big_count,
-                                                   const uint32_t* __restrict__ blk_graph,
-                                                   const GDesc* __restrict__ desc, uint32_t lane_max,
-                                                   uint32_t group_max) {
-    if (!cell_occ[blockIdx.x]) return;  // uniform over the block
-    cell_lane(blockIdx.x * CELL_BLOCK + threadIdx.x, end_, begin, sv, pos0, pos1, sorted_pos, sums, slot_idx,
-              slot_pos, mid_cells, mid_cap, big_cells, big_chunks, big_count, blk_graph, desc, lane_max,
-              group_max);
-}
-
-// The same for sets whose finest cells far outnumber their points (many small
-// graphs: 4-16 cells per point, layouts that collapse into a few clumps): one
-// wave per block of CELL_BLOCK cells, a quarter of the waves, which leaves at
-// once when no cell of the block is occupied and otherwise queues the
-// occupied cells in LDS (a ballot per 64 cells) and runs them 64 at a time,
-// one lane per cell.  Which lane takes a cell changes no result.
-__global__ void __launch_bounds__(64) k_cells_q(const uint8_t* __restrict__ cell_occ,
-                                                const uint32_t* __restrict__ end_,
-                                                const uint32_t* __restrict__ begin,
-                                                const uint32_t* __restrict__ sv,
-                                                const float2* __restrict__ pos0,
-                                                const float2* __restrict__ pos1,
-                                                const float2* __restrict__ sorted_pos,
-                                                long long* __restrict__ sums,
-                                                int32_t* __restrict__ slot_idx,
-                                                float2* __restrict__ slot_pos,
-                                                uint32_t* __restrict__ mid_cells, uint32_t mid_cap,
-                                                uint4* __restrict__ big_cells,
-                                                uint2* __restrict__ big_chunks,
-                                                uint32_t* __restrict__ big_count,
-                                                const uint32_t* __restrict__ blk_graph,
-                                                const GDesc* __restrict__ desc, uint32_t lane_max,
-                                                uint32_t group_max) {
-    if (!cell_occ[blockIdx.x]) return;  // uniform over the wave
-    __shared__ uint32_t q[128];
-    const uint32_t lane = threadIdx.x;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    uint32_t qn = 0;  // queued cells (uniform over the wave)
-#pragma unroll 1
-    for (uint32_t j = 0; j < CELL_BLOCK; j += 64) {
-        const uint32_t f = blockIdx.x * CELL_BLOCK + j + lane;
-        const bool ne = end_[f] != 0u;
-        const unsigned long long m = __ballot(ne);
-        if (ne) q[qn + (uint32_t)__popcll(m & below)] = f;
-        qn += (uint32_t)__popcll(m);
-        if (qn >= 64) {
-            __syncthreads();
-            const uint32_t f0 = q[lane];
-            const uint32_t rest = qn - 64;
-            const uint32_t moved = lane < rest ? q[64 + lane] : 0u;
-            __syncthreads();
-            if (lane < rest) q[lane] = moved;
-            qn = rest;
-            cell_lane(f0, end_, begin, sv, pos0, pos1, sorted_pos, sums, slot_idx, slot_pos, mid_cells, mid_cap,
-                      big_cells, big_chunks, big_count, blk_graph, desc, lane_max, group_max);
+                                                   CellPart* __restrict__ parts) {
+    constexpr uint32_t WAVE_PTS = 64u * N;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t w0 = w * WAVE_PTS;
+    if (w0 >= np) return;  // uniform over the wave
+    const uint32_t base = w0 + lane * N;
+    uint32_t key[N], idx[N];
+    float2 p[N];
+    if (base + N <= np) {
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q) {
+            const uint4 a = reinterpret_cast<const uint4*>(skey + base)[q];
+            const uint4 b = reinterpret_cast<const uint4*>(sv + base)[q];
+            key[4 * q] = a.x; key[4 * q + 1] = a.y; key[4 * q + 2] = a.z; key[4 * q + 3] = a.w;
+            idx[4 * q] = b.x; idx[4 * q + 1] = b.y; idx[4 * q + 2] = b.z; idx[4 * q + 3] = b.w;
+        }
+#pragma unroll
+        for (int q = 0; q < N / 2; ++q) {
+            const float4 a = reinterpret_cast<const float4*>(spos + base)[q];
+            p[2 * q] = make_float2(a.x, a.y);
+            p[2 * q + 1] = make_float2(a.z, a.w);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < N; ++u) {
+            const uint32_t k = base + u;
+            const bool in = k < np;
+            key[u] = in ? skey[k] : KEY_NONE;
+            idx[u] = in ? sv[k] : 0u;
+            p[u] = in ? spos[k] : make_float2(0.f, 0.f);
         }
     }
-    __syncthreads();
-    if (lane < qn)
-        cell_lane(q[lane], end_, begin, sv, pos0, pos1, sorted_pos, sums, slot_idx, slot_pos, mid_cells, mid_cap,
-                  big_cells, big_chunks, big_count, blk_graph, desc, lane_max, group_max);
-}
-
-// The 8 smallest distinct tickets over the private lists of each aligned
-// group of G lanes (smallest index per ticket), as (ticket << 32 | index),
-// all-ones past the end.
-template <int G>
-__device__ inline void wave_top8(uint32_t (&tb)[SLOTS], uint32_t (&ti)[SLOTS],
-                                 unsigned long long (&out)[SLOTS]) {
+    const uint32_t kprev = w0 > 0 ? skey[w0 - 1] : KEY_NONE;
+    const uint32_t knext = w0 + WAVE_PTS < np ? skey[w0 + WAVE_PTS] : KEY_NONE;
+    // graphs: keys ascend and every graph owns one key range, so a lane's
+    // items belong to the graph of its first item or of its last one, except
+    // in sets of graphs smaller than a lane
+    uint32_t g[N];
 #pragma unroll
-    for (int r = 0; r < SLOTS; ++r) {
-        unsigned long long mn = ((unsigned long long)tb[0] << 32) | ti[0];
+    for (int u = 0; u < N; ++u)
+        g[u] = (one_graph || key[u] == KEY_NONE) ? 0u : blk_graph[key[u] / CELL_BLOCK];
+    uint32_t glast = g[0];
 #pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            unsigned long long other = __shfl_xor(mn, o, 64);
-            mn = other < mn ? other : mn;
-        }
-        const uint32_t wt = (uint32_t)(mn >> 32);
-        out[r] = (wt == TMAX) ? ~0ull : mn;
-        if (wt != TMAX && tb[0] == wt) {
+    for (int u = 1; u < N; ++u) glast = key[u] != KEY_NONE ? g[u] : glast;
+    const SegGraph ga = seg_graph(desc, g[0]);
+    const SegGraph gb = seg_graph(desc, glast);
+    auto gof = [&](int u) -> SegGraph {
+        return g[u] == g[0] ? ga : (g[u] == glast ? gb : seg_graph(desc, g[u]));
+    };
+    uint32_t tk[N];
+    long long fx[N], fy[N];
 #pragma unroll
-            for (int s = 0; s < SLOTS - 1; ++s) { tb[s] = tb[s + 1]; ti[s] = ti[s + 1]; }
-            tb[SLOTS - 1] = TMAX;
-            ti[SLOTS - 1] = TMAX;
-        }
+    for (int u = 0; u < N; ++u) {
+        const SegGraph gu = gof(u);
+        if (key[u] != KEY_NONE && !((gu.passes >> P_MANYBODY) & 1u)) key[u] = KEY_NONE;
+        tk[u] = ticket_bits(idx[u] - gu.p0, gu.seed);
+        fx[u] = fixed(p[u].x, gu.fxs);
+        fy[u] = fixed(p[u].y, gu.fxs);
     }
-}
-
-// G lanes per listed cell (MID_GROUP for cells of cell_lane_max+1 ..
-// cell_group_max points, a wave above, to BIG_CHUNK): the group's lanes stride the
-// cell's segment with private top-8 lists and integer sums, then merge inside
-// the group.  Every lane of a group runs the same iterations, so the group
-// shuffles never read a lane that has left the loop.
-template <int G>
-__global__ void __launch_bounds__(256) k_cells_mid(const uint32_t* __restrict__ end_,
-                                                   const uint32_t* __restrict__ begin,
-                                                   const uint32_t* __restrict__ sv,
-                                                   const float2* __restrict__ pos0,
-                                                   const float2* __restrict__ pos1,
-                                                   const float2* __restrict__ sorted_pos,
-                                                   long long* __restrict__ sums,
-                                                   int32_t* __restrict__ slot_idx,
-                                                   float2* __restrict__ slot_pos,
-                                                   const uint32_t* __restrict__ cells,
-                                                   const uint32_t* __restrict__ count,
-                                                   const uint32_t* __restrict__ blk_graph,
-                                                   const GDesc* __restrict__ desc) {
-    const int gl = threadIdx.x & (G - 1);
-    const uint32_t group = (blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const uint32_t ngroups = (gridDim.x * blockDim.x) / G;
-    const uint32_t nmid = *count;
-    for (uint32_t w = group; w < nmid; w += ngroups) {
-        const uint32_t f = cells[w];
-        const GDesc& d = desc[blk_graph[f / CELL_BLOCK]];
-        const float2* __restrict__ pin = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1 : pos0;
-        const uint32_t b = begin[f], e = end_[f];
-        uint32_t tb[SLOTS], ti[SLOTS];
+    uint32_t kl = __shfl_up(key[N - 1], 1, 64);
+    uint32_t kn = __shfl_down(key[0], 1, 64);
+    if (lane == 0) kl = kprev;
+    if (lane == 63) kn = knext;
+    // runs inside the lane
+    uint32_t rid[N];
+    rid[0] = 0;
 #pragma unroll
-        for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
-        long long sx = 0, sy = 0;
-        cell_points<G>(b + gl, e, sv, sorted_pos, d, sx, sy, tb, ti);
+    for (int u = 1; u < N; ++u) rid[u] = rid[u - 1] + (key[u] != key[u - 1] ? 1u : 0u);
+    const uint32_t nrun = rid[N - 1] + 1;
+    const bool cl = key[0] != KEY_NONE && key[0] == kl;      // the first run began before the lane
+    const bool cr = key[N - 1] != KEY_NONE && key[N - 1] == kn;  // the last run goes on after it
+    const bool whole = cl && cr && nrun == 1;
+    const bool hasL = cr;            // L: the last run, continuing right
+    const bool hasF = cl && !whole;  // F: the first run, from the left, ending in the lane
+    // a ticket repeated inside a run keeps its smallest index, the others are
+    // duplicates; rank = the run's distinct tickets below the point's
+    bool dup[N];
+    uint32_t rank[N];
+    bool anyeq = false;
 #pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            sx += __shfl_xor(sx, o, 64);
-            sy += __shfl_xor(sy, o, 64);
-        }
-        unsigned long long out[SLOTS];
-        wave_top8<G>(tb, ti, out);
-        const uint32_t mc = f - d.f0;
-        const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
-        const size_t sb = ((size_t)d.f0 + rm) * SLOTS;
-        if (gl == 0) {
-            const size_t c3 = 3 * ((size_t)d.c0 + level_off(d.L - 1) + rm);
-            sums[c3] = sx;
-            sums[c3 + 1] = sy;
-            sums[c3 + 2] = (long long)(e - b);
-        }
+    for (int u = 0; u < N; ++u) {
+        dup[u] = false;
+        rank[u] = 0;
 #pragma unroll
-        for (int s = 0; s < SLOTS; ++s) {
-            if (gl == s) {
-                const bool ok = out[s] != ~0ull;
-                const uint32_t i = (uint32_t)out[s];
-                slot_idx[sb + s] = ok ? (int32_t)i : -1;
-                slot_pos[sb + s] = ok ? pin[i] : make_float2(0.f, 0.f);
-            }
-        }
+        for (int v = u + 1; v < N; ++v) anyeq |= tk[u] == tk[v];
     }
-}
-
-// One block per chunk of BIG_CHUNK points of a crowded cell: integer sums
-// added to the cell with 64-bit atomics (exact, so the order of the chunks is
-// immaterial) and the chunk's top-8 list for the merge.
-__global__ void __launch_bounds__(256) k_cells_chunk(const uint32_t* __restrict__ end_,
-                                                     const uint32_t* __restrict__ begin,
-                                                     const uint32_t* __restrict__ sv,
-                                                     const float2* __restrict__ pos0,
-                                                     const float2* __restrict__ pos1,
-                                                     const float2* __restrict__ sorted_pos,
-                                                     long long* __restrict__ sums,
-                                                     const uint2* __restrict__ big_chunks,
-                                                     const uint32_t* __restrict__ big_count,
-                                                     unsigned long long* __restrict__ chunk_top,
-                                                     const uint32_t* __restrict__ blk_graph,
-                                                     const GDesc* __restrict__ desc) {
-    __shared__ long long s_sum[2][4];
-    __shared__ unsigned long long s_top[4][SLOTS];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const uint32_t nchunks = big_count[1];
-    for (uint32_t e = blockIdx.x; e < nchunks; e += gridDim.x) {
-        const uint2 ce = big_chunks[e];
-        const uint32_t f = ce.x;
-        const GDesc& d = desc[blk_graph[f / CELL_BLOCK]];
-        const float2* __restrict__ pin = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1 : pos0;
-        const uint32_t cb = begin[f], ce_ = end_[f];
-        const uint32_t b = cb + ce.y * BIG_CHUNK, end = min(ce_, b + BIG_CHUNK);
-        uint32_t tb[SLOTS], ti[SLOTS];
+    if (__ballot(anyeq) != 0ull) {  // a repeated ticket somewhere in the wave: the full rule
 #pragma unroll
-        for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
-        long long sx = 0, sy = 0;
-        cell_points<256>(b + t, end, sv, sorted_pos, d, sx, sy, tb, ti);
+        for (int u = 0; u < N; ++u) {
+            bool dd = false;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            sx += __shfl_xor(sx, o, 64);
-            sy += __shfl_xor(sy, o, 64);
+            for (int v = 0; v < N; ++v)
+                if (v != u) dd |= rid[v] == rid[u] && tk[v] == tk[u] && idx[v] < idx[u];
+            dup[u] = dd;
         }
-        unsigned long long out[SLOTS];
-        wave_top8<64>(tb, ti, out);
-        if (lane == 0) {
-            s_sum[0][wave] = sx;
-            s_sum[1][wave] = sy;
 #pragma unroll
-            for (int s = 0; s < SLOTS; ++s) s_top[wave][s] = out[s];
+        for (int u = 0; u < N; ++u) {
+            uint32_t r = 0;
+#pragma unroll
+            for (int v = 0; v < N; ++v)
+                if (v != u) r += (rid[v] == rid[u] && !dup[v] && tk[v] < tk[u]) ? 1u : 0u;
+            rank[u] = r;
         }
-        __syncthreads();
-        if (t == 0) {
-            const uint32_t mc = f - d.f0;
-            const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
-            const size_t c3 = 3 * ((size_t)d.c0 + level_off(d.L - 1) + rm);
-            const long long bx = s_sum[0][0] + s_sum[0][1] + s_sum[0][2] + s_sum[0][3];
-            const long long by = s_sum[1][0] + s_sum[1][1] + s_sum[1][2] + s_sum[1][3];
-            atomicAdd(reinterpret_cast<unsigned long long*>(&sums[c3]), (unsigned long long)bx);
-            atomicAdd(reinterpret_cast<unsigned long long*>(&sums[c3 + 1]), (unsigned long long)by);
-            uint32_t mb[SLOTS], mi[SLOTS];
+    } else {
 #pragma unroll
-            for (int s = 0; s < SLOTS; ++s) { mb[s] = TMAX; mi[s] = TMAX; }
-            for (int w = 0; w < 4; ++w)
-                for (int s = 0; s < SLOTS; ++s) {
-                    const unsigned long long v = s_top[w][s];
-                    if (v != ~0ull) top8_insert(mb, mi, (uint32_t)(v >> 32), (uint32_t)v);
+        for (int u = 0; u < N; ++u)
+#pragma unroll
+            for (int v = u + 1; v < N; ++v)
+                if (rid[u] == rid[v]) {
+                    if (tk[v] < tk[u]) ++rank[u];
+                    else ++rank[v];
                 }
+    }
+    // runs finished inside the lane; sums of F and L
+    long long Fx = 0, Fy = 0, Lx = 0, Ly = 0;
+    uint32_t Fn = 0, Ln = 0;
+    {
+        long long ax = 0, ay = 0;
+        uint32_t an = 0, ad = 0;
 #pragma unroll
-            for (int s = 0; s < SLOTS; ++s)
-                chunk_top[(size_t)e * SLOTS + s] =
-                    mb[s] == TMAX ? ~0ull : (((unsigned long long)mb[s] << 32) | mi[s]);
+        for (int u = 0; u < N; ++u) {
+            if (u == 0 || rid[u] != rid[u - 1]) { ax = 0; ay = 0; an = 0; ad = 0; }
+            ax += fx[u];
+            ay += fy[u];
+            an += 1u;
+            ad += dup[u] ? 0u : 1u;
+            const bool isF = hasF && rid[u] == 0;
+            const bool isL = hasL && rid[u] == nrun - 1;
+            const bool last = (u == N - 1) || rid[u + 1] != rid[u];
+            if (isF || isL || key[u] == KEY_NONE) {
+                if (last && isF) { Fx = ax; Fy = ay; Fn = an; }
+                if (last && isL) { Lx = ax; Ly = ay; Ln = an; }
+                continue;
+            }
+            const SegGraph gu = gof(u);
+            const uint32_t rm = seg_rm(gu, key[u]);
+            const size_t sb = ((size_t)gu.f0 + rm) * SLOTS;
+            if (!dup[u] && rank[u] < (uint32_t)SLOTS) {
+                slot_idx[sb + rank[u]] = (int32_t)idx[u];
+                slot_pos[sb + rank[u]] = p[u];
+            }
+            if (last) {
+                const size_t c3 = 3 * ((size_t)gu.c0 + level_off(gu.L - 1) + rm);
+                sums[c3] = ax;
+                sums[c3 + 1] = ay;
+                sums[c3 + 2] = (long long)an;
+                for (uint32_t s = ad; s < (uint32_t)SLOTS; ++s) {
+                    slot_idx[sb + s] = -1;
+                    slot_pos[sb + s] = make_float2(0.f, 0.f);
+                }
+            }
         }
-        __syncthreads();
+    }
+    const unsigned long long mF = __ballot(hasF);
+    const unsigned long long mL = __ballot(hasL);
+    if ((mF | mL) == 0ull) {
+        if (lane == 0) { parts[2 * w].key = KEY_NONE; parts[2 * w + 1].key = KEY_NONE; }
+        return;
+    }
+    // segments: a lane whose L run starts in it (or that has none) opens one
+    const uint32_t open = whole ? 0u : 1u;
+    const unsigned long long above = lane < 63 ? (mF >> (lane + 1)) : 0ull;
+    const uint32_t E = above ? lane + (uint32_t)__ffsll((long long)above) : 64u;  // the segment's F lane
+    // the segment opened before the wave: lane 0 inside a run from the last wave
+    const bool o0 = (__ballot(whole) & 1ull) != 0ull;
+    const uint32_t E0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)E);
+    const bool originL = o0 && lane < E0;
+    const bool originF = lane == 0 || (o0 && lane == E0);
+    const bool doneL = !originL && E < 64u;  // the L run began in this wave and ends in it
+    const bool doneF = !originF;
+    const unsigned long long mPF = __ballot(hasF && originF);  // a run from an earlier wave ends here
+    // the partial an unfinished segment of this lane fills
+    const uint32_t partL = E < 64u ? 2 * w : 2 * w + 1;
+    // sums: segmented scan over the L parts, then the F lane adds its own
+    {
+        long long vx = hasL ? Lx : 0, vy = hasL ? Ly : 0;
+        uint32_t vn = hasL ? Ln : 0u;
+        segsum_scan(vx, vy, vn, open);
+        const long long cx = (long long)dpp64<0x138, 0xf>(0ull, (unsigned long long)vx);
+        const long long cy = (long long)dpp64<0x138, 0xf>(0ull, (unsigned long long)vy);
+        const uint32_t cn = dpp32<0x138, 0xf>(0u, vn);
+        if (hasF) {
+            const long long tx = Fx + cx, ty = Fy + cy;
+            const uint32_t tn = Fn + cn;
+            if (doneF) {
+                const uint32_t rm = seg_rm(ga, key[0]);
+                const size_t c3 = 3 * ((size_t)ga.c0 + level_off(ga.L - 1) + rm);
+                sums[c3] = tx;
+                sums[c3 + 1] = ty;
+                sums[c3 + 2] = (long long)tn;
+            } else {
+                parts[2 * w].key = key[0];
+                parts[2 * w].sx = tx;
+                parts[2 * w].sy = ty;
+                parts[2 * w].cnt = tn;
+            }
+        }
+        if (lane == 63) {
+            parts[2 * w + 1].key = hasL ? key[N - 1] : KEY_NONE;
+            if (hasL) {
+                parts[2 * w + 1].sx = vx;
+                parts[2 * w + 1].sy = vy;
+                parts[2 * w + 1].cnt = vn;
+            }
+        }
+        if (lane == 0 && mPF == 0ull) parts[2 * w].key = KEY_NONE;
+    }
+    // top-8: 8 rounds of a segmented minimum of the lanes' next candidates
+    uint32_t cF8[N], cL8[N];  // each point's place in the F or L list (0xff: none)
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+        cF8[u] = (hasF && rid[u] == 0 && !dup[u]) ? rank[u] : 0xffu;
+        cL8[u] = (hasL && rid[u] == nrun - 1 && !dup[u]) ? rank[u] : 0xffu;
+    }
+    const uint32_t rmF = seg_rm(ga, key[0]), rmL = seg_rm(gb, key[N - 1]);
+    const size_t sbF = ((size_t)ga.f0 + rmF) * SLOTS, sbL = ((size_t)gb.f0 + rmL) * SLOTS;
+    uint32_t hF = 0, hL = 0;
+#pragma unroll 1
+    for (int r = 0; r < SLOTS; ++r) {
+        uint32_t cFt = 0xffffffffu, cFi = 0xffffffffu, cLt = 0xffffffffu, cLi = 0xffffffffu;
+        float2 pF = make_float2(0.f, 0.f), pL = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < N; ++u) {
+            const bool sf = cF8[u] == hF, sl = cL8[u] == hL;
+            cFt = sf ? tk[u] : cFt; cFi = sf ? idx[u] : cFi; pF = sf ? p[u] : pF;
+            cLt = sl ? tk[u] : cLt; cLi = sl ? idx[u] : cLi; pL = sl ? p[u] : pL;
+        }
+        const unsigned long long cF = ((unsigned long long)cFt << 32) | cFi;
+        const unsigned long long cL = ((unsigned long long)cLt << 32) | cLi;
+        const unsigned long long v = segmin_scan(cL, open);
+        const unsigned long long carry = dpp64<0x138, 0xf>(TOP_NONE, v);
+        const unsigned long long m = hasF ? (carry < cF ? carry : cF) : TOP_NONE;
+        const unsigned long long mE = __shfl(m, (int)(E & 63u), 64);
+        const unsigned long long m63 = readlane64(v, 63);
+        const unsigned long long seg = E < 64u ? mE : m63;
+        if (hasL && cL != TOP_NONE && seg != TOP_NONE && cLt == (uint32_t)(seg >> 32)) {
+            if (cL == seg) {
+                if (doneL) {
+                    slot_idx[sbL + r] = (int32_t)cLi;
+                    slot_pos[sbL + r] = pL;
+                } else {
+                    parts[partL].pos[r] = pL;
+                }
+            }
+            ++hL;
+        }
+        if (hasF) {
+            if (cF != TOP_NONE && m != TOP_NONE && cFt == (uint32_t)(m >> 32)) {
+                if (cF == m) {
+                    if (doneF) {
+                        slot_idx[sbF + r] = (int32_t)cFi;
+                        slot_pos[sbF + r] = pF;
+                    } else {
+                        parts[2 * w].pos[r] = pF;
+                    }
+                }
+                ++hF;
+            }
+            if (doneF && m == TOP_NONE) {
+                slot_idx[sbF + r] = -1;
+                slot_pos[sbF + r] = make_float2(0.f, 0.f);
+            }
+            if (!doneF) parts[2 * w].top[r] = m;
+        }
+        if (lane == 63 && hasL) parts[2 * w + 1].top[r] = v;
     }
 }
 
-// One wave per crowded cell: the top-8 over its chunks' lists, then its slots.
-__global__ void __launch_bounds__(256) k_cells_merge(const uint4* __restrict__ big_cells,
-                                                     const uint32_t* __restrict__ big_count,
-                                                     const unsigned long long* __restrict__ chunk_top,
-                                                     const float2* __restrict__ pos0,
-                                                     const float2* __restrict__ pos1,
-                                                     int32_t* __restrict__ slot_idx,
-                                                     float2* __restrict__ slot_pos,
-                                                     const uint32_t* __restrict__ blk_graph,
-                                                     const GDesc* __restrict__ desc) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t ncells = big_count[0];
-    for (uint32_t w = wave; w < ncells; w += nwaves) {
-        const uint4 bc = big_cells[w];  // {cell, first chunk, chunks, row-major cell}
-        const GDesc& d = desc[blk_graph[bc.x / CELL_BLOCK]];
-        const float2* __restrict__ pin = ((d.inbuf >> P_MANYBODY) & 1u) ? pos1 : pos0;
-        uint32_t tb[SLOTS], ti[SLOTS];
+// One wave per wave of k_cells_seg whose first partial names a cell (a run
+// that began in an earlier wave and ends in this one): merges the run's
+// partials — the last one of each earlier wave it crosses, then this one —
+// one per lane (lists merged in the lane above 64 of them), then 8 rounds of
+// the wave's minimum.
+template <int N>
+__global__ void __launch_bounds__(256) k_cells_join(const CellPart* __restrict__ parts, uint32_t nw,
+                                                    const uint32_t* __restrict__ cell_begin,
+                                                    const uint32_t* __restrict__ blk_graph,
+                                                    const GDesc* __restrict__ desc,
+                                                    long long* __restrict__ sums,
+                                                    int32_t* __restrict__ slot_idx,
+                                                    float2* __restrict__ slot_pos) {
+    constexpr uint32_t WAVE_PTS = 64u * N;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (w >= nw) return;
+    const uint32_t f = parts[2 * w].key;
+    if (f == KEY_NONE) return;
+    const uint32_t wa = cell_begin[f] / WAVE_PTS;
+    unsigned long long t8[SLOTS];
+    float2 p8[SLOTS];
 #pragma unroll
-        for (int s = 0; s < SLOTS; ++s) { tb[s] = TMAX; ti[s] = TMAX; }
-        for (uint32_t e = bc.y + lane; e < bc.y + bc.z; e += 64) {
-            // the chunk's whole list in flight at once (4 vector loads)
-            const ulonglong2* __restrict__ ct = reinterpret_cast<const ulonglong2*>(chunk_top + (size_t)e * SLOTS);
-            ulonglong2 v2[SLOTS / 2];
+    for (int s = 0; s < SLOTS; ++s) { t8[s] = TOP_NONE; p8[s] = make_float2(0.f, 0.f); }
+    long long sx = 0, sy = 0;
+    uint32_t cnt = 0;
+    for (uint32_t q = wa + lane; q <= w; q += 64) {
+        const CellPart& P = q == w ? parts[2 * w] : parts[2 * q + 1];
+        sx += P.sx;
+        sy += P.sy;
+        cnt += P.cnt;
+        if (q < wa + 64) {
 #pragma unroll
-            for (int q = 0; q < SLOTS / 2; ++q) v2[q] = ct[q];
+            for (int s = 0; s < SLOTS; ++s) { t8[s] = P.top[s]; p8[s] = P.pos[s]; }
+        } else {
+            // a second list in this lane: merge (8 smallest distinct tickets)
+#pragma unroll 1
+            for (int s = 0; s < SLOTS; ++s) {
+                unsigned long long x = P.top[s];
+                float2 xp = P.pos[s];
+                if (x == TOP_NONE) break;
+                bool dupl = false;
 #pragma unroll
-            for (int q = 0; q < SLOTS / 2; ++q) {
-                if (v2[q].x != ~0ull) top8_insert(tb, ti, (uint32_t)(v2[q].x >> 32), (uint32_t)v2[q].x);
-                if (v2[q].y != ~0ull) top8_insert(tb, ti, (uint32_t)(v2[q].y >> 32), (uint32_t)v2[q].y);
+                for (int k = 0; k < SLOTS; ++k) {
+                    if ((uint32_t)(t8[k] >> 32) == (uint32_t)(x >> 32) && t8[k] != TOP_NONE) {
+                        dupl = true;
+                        if (x < t8[k]) { t8[k] = x; p8[k] = xp; }
+                    }
+                }
+                if (dupl) continue;
+#pragma unroll
+                for (int k = 0; k < SLOTS; ++k) {
+                    const bool sw = x < t8[k];
+                    const unsigned long long a = t8[k];
+                    const float2 ap = p8[k];
+                    t8[k] = sw ? x : a;
+                    p8[k] = sw ? xp : ap;
+                    x = sw ? a : x;
+                    xp = sw ? ap : xp;
+                }
             }
         }
-        unsigned long long out[SLOTS];
-        wave_top8<64>(tb, ti, out);
-        const size_t sb = ((size_t)d.f0 + bc.w) * SLOTS;
+    }
+    const uint32_t open = lane == 0 ? 1u : 0u;
+    {
+        segsum_scan(sx, sy, cnt, open);
+        sx = (long long)readlane64((unsigned long long)sx, 63);
+        sy = (long long)readlane64((unsigned long long)sy, 63);
+        cnt = (uint32_t)__builtin_amdgcn_readlane((int)cnt, 63);
+    }
+    const GDesc& d = desc[blk_graph[f / CELL_BLOCK]];
+    const uint32_t mc = f - d.f0;
+    const uint32_t rm = compact1by1(mc >> 1) * d.gf + compact1by1(mc);
+    const size_t sb = ((size_t)d.f0 + rm) * SLOTS;
+    if (lane == 0) {
+        const size_t c3 = 3 * ((size_t)d.c0 + level_off(d.L - 1) + rm);
+        sums[c3] = sx;
+        sums[c3 + 1] = sy;
+        sums[c3 + 2] = (long long)cnt;
+    }
+    uint32_t h = 0;
+#pragma unroll 1
+    for (int r = 0; r < SLOTS; ++r) {
+        unsigned long long c = TOP_NONE;
+        float2 cp = make_float2(0.f, 0.f);
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) {
-            if (lane == s) {
-                const bool ok = out[s] != ~0ull;
-                const uint32_t i = (uint32_t)out[s];
-                slot_idx[sb + s] = ok ? (int32_t)i : -1;
-                slot_pos[sb + s] = ok ? pin[i] : make_float2(0.f, 0.f);
+            const bool sel = (uint32_t)s == h;
+            c = sel ? t8[s] : c;
+            cp = sel ? p8[s] : cp;
+        }
+        const unsigned long long mn = readlane64(segmin_scan(c, open), 63);
+        if (mn == TOP_NONE) {
+            if (lane == 0) {
+                slot_idx[sb + r] = -1;
+                slot_pos[sb + r] = make_float2(0.f, 0.f);
             }
+            continue;
+        }
+        if (c != TOP_NONE && (uint32_t)(c >> 32) == (uint32_t)(mn >> 32)) {
+            if (c == mn) {
+                slot_idx[sb + r] = (int32_t)(uint32_t)c;
+                slot_pos[sb + r] = cp;
+            }
+            ++h;
         }
     }
 }
@@ -1500,14 +1558,9 @@ int mb_alloc(Ctx* c, uint32_t single_gf) {
     c->cell_dirty = (uint8_t*)dalloc(c, nblk, &rc); if (rc) return rc;
     c->slot_idx = (int32_t*)dalloc(c, F * SLOTS * 4, &rc); if (rc) return rc;
     c->slot_pos = (float2*)dalloc(c, F * SLOTS * 8, &rc); if (rc) return rc;
-    // crowded cells: at most NP / (cell_lane_max + 1) of them, and their chunks
-    const size_t ncells = np / (c->tune.cell_lane_max + 1) + 1, nchunks = np / BIG_CHUNK + ncells + 1;
-    c->mid_cap = (uint32_t)ncells;  // two lists: small mid cells, then wave-sized ones
-    c->mid_cells = (uint32_t*)dalloc(c, 2 * ncells * 4, &rc); if (rc) return rc;
-    c->big_cells = (uint4*)dalloc(c, ncells * 16, &rc); if (rc) return rc;
-    c->big_chunks = (uint2*)dalloc(c, nchunks * 8, &rc); if (rc) return rc;
-    c->chunk_top = (unsigned long long*)dalloc(c, nchunks * SLOTS * 8, &rc); if (rc) return rc;
-    c->big_count = (uint32_t*)dalloc(c, 64, &rc); if (rc) return rc;
+    // the partial cells at the two ends of each wave of the cell pass (waves of
+    // 64 x 4 points at the least)
+    c->cell_parts = (uint4*)dalloc(c, (np / 256 + 2) * 2 * sizeof(CellPart), &rc); if (rc) return rc;
     c->sums = (long long*)dalloc(c, C * 3 * sizeof(long long), &rc); if (rc) return rc;
     c->cent = (float4*)dalloc(c, C * sizeof(float4), &rc); if (rc) return rc;
     // the first tick visits the points in index order
@@ -1551,7 +1604,7 @@ int mb_run(Ctx* c, float2* vout) {
                        c->cell_dirty, c->cell_count);
     hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(np, CB_TILE)), dim3(256), 0, c->stream, c->sorted, c->pos[0],
                        c->pos[1], c->gid, c->desc, np, c->cell_count, c->rk[0], c->rv[0], c->rp[0], c->bin_rank,
-                       c->big_count, c->cell_occ, c->cell_dirty, c->G == 1 ? 1u : 0u);
+                       c->cell_occ, c->cell_dirty, c->G == 1 ? 1u : 0u);
     // cells per scan thread: the fewest (4 .. 256) that keep the ranges to ~4096
     uint32_t cpt = SCAN_CPT_MIN;
     while (cpt < SCAN_CPT_MAX && (uint64_t)F > 4096ull * 256u * cpt) cpt *= 2;
@@ -1571,41 +1624,27 @@ int mb_run(Ctx* c, float2* vout) {
         while (sp * 2 <= items / 4096u) sp *= 2;
         return sp;
     };
-    // a workgroup of CELL_BLOCK lanes per block of cells, or one wave per block
-    // when the set has more than 4M finest cells
-    const uint32_t nblocks = F / CELL_BLOCK;
-    if (nblocks > 16384u)
-        hipLaunchKernelGGL(k_cells_q, dim3(nblocks), dim3(64), 0, c->stream, c->cell_occ, c->cell_end,
-                           c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->slot_idx,
-                           c->slot_pos, c->mid_cells, c->mid_cap, c->big_cells, c->big_chunks, c->big_count,
-                           c->blk_graph, c->desc, c->tune.cell_lane_max, c->tune.cell_group_max);
-    else
-        hipLaunchKernelGGL(k_cells, dim3(nblocks), dim3(CELL_BLOCK), 0, c->stream, c->cell_occ, c->cell_end,
-                           c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->slot_idx,
-                           c->slot_pos, c->mid_cells, c->mid_cap, c->big_cells, c->big_chunks, c->big_count,
-                           c->blk_graph, c->desc, c->tune.cell_lane_max, c->tune.cell_group_max);
-    // the mid tiers and the crowded-cell chunks touch disjoint cells: the
-    // mid tiers run on the side stream beside the other two
-    const bool fork = c->side != nullptr;
-    if (fork) {
-        CG_HIP(c, hipEventRecord(c->ev_fork, c->stream));
-        CG_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    // the cells from runs of the sorted points: one wave per 64 N of them,
+    // then the merge of the runs that cross a wave's end
+    {
+        const uint32_t items = c->tune.cell_items;
+        const uint32_t nw = (np + 64u * items - 1) / (64u * items);
+        CellPart* parts = reinterpret_cast<CellPart*>(c->cell_parts);
+        const uint32_t one = c->G == 1 ? 1u : 0u;
+        if (items == 4) {
+            hipLaunchKernelGGL(k_cells_seg<4>, dim3((nw + 3) / 4), dim3(256), 0, c->stream, c->rk[1], c->sorted,
+                               c->sorted_pos, np, c->blk_graph, c->desc, one, c->sums, c->slot_idx, c->slot_pos,
+                               parts);
+            hipLaunchKernelGGL(k_cells_join<4>, dim3((nw + 3) / 4), dim3(256), 0, c->stream, parts, nw,
+                               c->cell_begin, c->blk_graph, c->desc, c->sums, c->slot_idx, c->slot_pos);
+        } else {
+            hipLaunchKernelGGL(k_cells_seg<8>, dim3((nw + 3) / 4), dim3(256), 0, c->stream, c->rk[1], c->sorted,
+                               c->sorted_pos, np, c->blk_graph, c->desc, one, c->sums, c->slot_idx, c->slot_pos,
+                               parts);
+            hipLaunchKernelGGL(k_cells_join<8>, dim3((nw + 3) / 4), dim3(256), 0, c->stream, parts, nw,
+                               c->cell_begin, c->blk_graph, c->desc, c->sums, c->slot_idx, c->slot_pos);
+        }
     }
-    hipLaunchKernelGGL(k_cells_mid<64>, dim3(MID_BLOCKS), dim3(256), 0, fork ? c->side : c->stream,
-                       c->cell_end, c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums,
-                       c->slot_idx, c->slot_pos, c->mid_cells + c->mid_cap, c->big_count + 3, c->blk_graph,
-                       c->desc);
-    hipLaunchKernelGGL(k_cells_mid<MID_GROUP>, dim3(MID_BLOCKS), dim3(256), 0, fork ? c->side : c->stream,
-                       c->cell_end, c->cell_begin, c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums,
-                       c->slot_idx, c->slot_pos, c->mid_cells, c->big_count + 2, c->blk_graph, c->desc);
-    if (fork) CG_HIP(c, hipEventRecord(c->ev_join, c->side));
-    hipLaunchKernelGGL(k_cells_chunk, dim3(1024), dim3(256), 0, c->stream, c->cell_end, c->cell_begin,
-                       c->sorted, c->pos[0], c->pos[1], c->sorted_pos, c->sums, c->big_chunks,
-                       c->big_count, c->chunk_top, c->blk_graph, c->desc);
-    hipLaunchKernelGGL(k_cells_merge, dim3(64), dim3(256), 0, c->stream, c->big_cells, c->big_count,
-                       c->chunk_top, c->pos[0], c->pos[1], c->slot_idx, c->slot_pos, c->blk_graph,
-                       c->desc);
-    if (fork) CG_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     prof_end(c, CG_STAGE_MB_CELLS);
     prof_begin(c, CG_STAGE_MB_PYRAMID);
     // pyramid: finest -> tiles of up to 32x32 -> remaining top levels

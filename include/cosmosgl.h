@@ -248,8 +248,7 @@ int cg_debug_link_stats(cg_ctx* ctx, uint32_t* out16);
  * a change takes effect on the next step (the link and cell tiers repack). */
 typedef struct cg_tuning {
     uint32_t graph_ticks;      /* ticks per captured HIP graph, 1..256 (default 8) */
-    uint32_t cell_lane_max;    /* finest cells of up to this many points: one lane each (16) */
-    uint32_t cell_group_max;   /* up to this: 16 lanes each; above, to 2048: a wave each (64) */
+    uint32_t cell_items;       /* sorted points per lane of the many-body cell pass: 4 or 8 (8) */
     uint32_t force_tile;       /* level cells staged in LDS by the force kernel: 512 or 1024 (1024) */
     uint32_t link_lane_max;    /* link entries per direction up to which a point runs one lane (64) */
     uint32_t link_group_max;   /* up to which it runs one 64-lane workgroup, <= 256 (256) */

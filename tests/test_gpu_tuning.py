@@ -1,5 +1,5 @@
 """The free implementation parameters (cg_tuning) through the C ABI: the
-bit-neutral ones (ticks per captured graph, crowded-cell tiers, the force
+bit-neutral ones (ticks per captured graph, points per lane of the cell pass, the force
 kernel's LDS window) leave every result bit unchanged over whole ticks; the
 link tiers keep every link force within 1e-5 * spaceSize of the oracle
 (hubs split into segments included); invalid values are refused.
@@ -41,7 +41,7 @@ def test_bit_neutral_tuning_keeps_bits():
     p = params(S, repulsion=0.5)
     outs = []
     for t in ({}, {"graph_ticks": 1}, {"graph_ticks": 32, "force_tile": 512},
-              {"cell_lane_max": 8, "cell_group_max": 32}, {"cell_lane_max": 32, "cell_group_max": 256}):
+              {"cell_items": 4}, {"cell_items": 4, "graph_ticks": 16, "force_tile": 512}):
         with _lib.Context() as c:
             c.set_tuning(t)
             c.set_positions(xy)
@@ -85,8 +85,8 @@ def test_link_tiers_stay_within_tolerance(tiers):
 
 def test_invalid_tuning_refused():
     with _lib.Context() as c:
-        for bad in ({"graph_ticks": 0}, {"force_tile": 700}, {"cell_lane_max": 2},
-                    {"cell_lane_max": 100, "cell_group_max": 64}, {"link_lane_max": 256},
+        for bad in ({"graph_ticks": 0}, {"force_tile": 700}, {"cell_items": 2},
+                    {"cell_items": 16}, {"link_lane_max": 256},
                     {"link_group_max": 512}, {"link_hub_entries": 5}):
             with pytest.raises(_lib.CosmosGLError) as e:
                 c.set_tuning(bad)

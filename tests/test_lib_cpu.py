@@ -147,9 +147,9 @@ def test_tuning_struct_and_table_lookup(tmp_path):
     from graph_amd import autotune
 
     d = _lib.default_tuning()
-    assert d == {"graph_ticks": 8, "cell_lane_max": 16, "cell_group_max": 64, "force_tile": 1024,
+    assert d == {"graph_ticks": 8, "cell_items": 8, "force_tile": 1024,
                  "link_lane_max": 64, "link_group_max": 256, "link_hub_entries": 6}
-    assert ctypes.sizeof(_lib.CgTuning) == 7 * 4
+    assert ctypes.sizeof(_lib.CgTuning) == 6 * 4
     table = {"entries": [
         {"points": 10_000, "mean_degree": 4.0, "tuning": dict(d, graph_ticks=32, link_lane_max=32)},
         {"points": 1_000_000, "mean_degree": 10.0, "tuning": dict(d, force_tile=512, link_hub_entries=8)}]}
@@ -159,5 +159,5 @@ def test_tuning_struct_and_table_lookup(tmp_path):
     t = autotune.lookup(2_000_000, 9.0, bit_exact=False, table=table)
     assert t["force_tile"] == 512 and t["link_hub_entries"] == 8
     assert autotune.lookup(5, 1.0, table={"entries": []}) == d
-    assert all(autotune.valid(dict(d, **{k: v})) for k, vs in autotune.BIT_NEUTRAL.items() for v in vs
-               if not (k == "cell_lane_max" and v > d["cell_group_max"]))
+    assert all(autotune.valid(dict(d, **{k: v})) for k, vs in autotune.BIT_NEUTRAL.items() for v in vs)
+    assert not autotune.valid(dict(d, cell_items=16))
