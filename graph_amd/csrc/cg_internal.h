@@ -154,6 +154,7 @@ struct LinkDir {
     uint32_t* soff = nullptr;    // [n_sched+1]
     uint32_t n_sched = 0;
     uint32_t n_lane = 0;         // lane-tier points (first in the schedule)
+    uint32_t* sgraph = nullptr;  // [n_lane] graph slot of each lane-tier point (sets of several graphs)
     // chunked-tier work units, in schedule order: one per 64-lane point, then
     // one per segment of each 256-lane point (one segment up to 256 x hub_entries)
     uint32_t n_units = 0, n_u64 = 0;
@@ -466,6 +467,17 @@ constexpr uint32_t XCDS = 8;
 __device__ inline uint32_t xcd_block(uint32_t i, uint32_t nb) {
     const uint32_t q = nb / XCDS, r = nb % XCDS, x = i % XCDS, j = i / XCDS;
     return x * q + min(x, r) + j;
+}
+
+// The same in runs of C blocks: of each group of XCDS * C consecutive logical
+// blocks, XCD x runs the x-th run of C, so every XCD keeps a share of each
+// part of [0, nb) (work ordered by cost stays balanced) while nearby blocks
+// still share an L2.  A last partial group keeps the round-robin order.
+__device__ inline uint32_t xcd_run_block(uint32_t i, uint32_t nb, uint32_t C) {
+    const uint32_t full = nb / (XCDS * C) * (XCDS * C);
+    if (i >= full) return i;
+    const uint32_t x = i % XCDS, j = i / XCDS;
+    return ((j / C) * XCDS + x) * C + j % C;
 }
 
 }  // namespace cg

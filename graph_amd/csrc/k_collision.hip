@@ -281,6 +281,134 @@ __global__ void __launch_bounds__(256) k_col_crowd(const uint32_t* __restrict__ 
     }
 }
 
+// Sets whose base cells outnumber their points (many small graphs, each
+// with a grid of up to 512^2 cells) leave most cells empty: there the cells
+// are found from the points.  A cell of grid o gathers base cells
+// (cx - sx .. cx, cy - sy .. cy); an occupied base cell (a run of the sorted
+// keys) is a source of the cells (x .. x + sx, y .. y + sy) of each grid, and
+// each cell is computed once, by its first occupied source in the order
+// k_col_cells reads them, with the same sums.  The cells a pass writes are
+// those of its occupied base cells, so the next pass first clears exactly
+// them (and their base cells' ends), from the keys this pass sorted.
+
+// Clears the cells (and base-cell ends) written by the pass whose sorted
+// keys and points are `old` and `ov`: every cell of every grid that an
+// occupied base cell is a source of.
+__global__ void __launch_bounds__(256) k_col_clear(const uint32_t* __restrict__ old,
+                                                   const uint32_t* __restrict__ ov, uint32_t npc,
+                                                   const uint32_t* __restrict__ gid,
+                                                   const GDesc* __restrict__ desc, uint32_t CB,
+                                                   uint32_t one_graph, uint32_t* __restrict__ end,
+                                                   float4* __restrict__ cells) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npc) return;
+    const uint32_t key = old[k];
+    if ((k > 0 && old[k - 1] == key) || key >= CB) return;
+    end[key] = 0u;
+    const GDesc& d = desc[one_graph ? 0u : gid[ov[k]]];
+    const int Gc = (int)d.col_G;
+    const uint32_t local = key - d.col_c0 / 4u, GG = d.col_G * d.col_G;
+    const int y = (int)udiv24(local, d.col_G), x = (int)local - y * Gc;
+    for (int o = 0; o < 4; ++o) {
+        const int sx = c_off[o][0] != 0.f ? 1 : 0, sy = c_off[o][1] != 0.f ? 1 : 0;
+        for (int dy = 0; dy <= sy; ++dy)
+            for (int dx = 0; dx <= sx; ++dx) {
+                const int cx = x + dx, cy = y + dy;
+                if (cx < Gc && cy < Gc)
+                    cells[d.col_c0 + (uint32_t)o * GG + (uint32_t)(cy * Gc + cx)] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+    }
+}
+
+// Nine lanes per sorted point; at the head of a run (an occupied base cell)
+// lane j takes the j-th cell of the four grids the base cell is a source of
+// (1 + 2 + 2 + 4: grid o's cells x .. x + sx by y .. y + sy) and computes it
+// if the base cell is its first occupied source, summing it in one lane over
+// its sources' points in order, COL_BATCH loads at a time (a cell whose
+// sources hold more than COL_CROWD_MIN points goes to k_col_crowd, as in
+// k_col_cells).
+__constant__ int8_t c_cand[9][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {2, 0, 0}, {2, 0, 1},
+                                    {3, 0, 0}, {3, 1, 0}, {3, 0, 1}, {3, 1, 1}};  // {grid, dx, dy}
+__global__ void __launch_bounds__(256) k_col_cells_sparse(const uint32_t* __restrict__ sk,
+                                                          const uint32_t* __restrict__ sv, uint32_t npc,
+                                                          const uint32_t* __restrict__ begin,
+                                                          const uint32_t* __restrict__ end,
+                                                          const float2* __restrict__ spos,
+                                                          const float* __restrict__ ssize,
+                                                          const uint32_t* __restrict__ gid,
+                                                          const GDesc* __restrict__ desc,
+                                                          float4* __restrict__ cells,
+                                                          uint32_t* __restrict__ crowd, uint32_t crowd_cap,
+                                                          uint32_t one_graph) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = t / 9u, j = t - k * 9u;
+    if (k >= npc) return;
+    const uint32_t key = sk[k];
+    if (k > 0 && sk[k - 1] == key) return;
+    const GDesc& d = desc[one_graph ? 0u : gid[sv[k]]];
+    const int Gc = (int)d.col_G;
+    const uint32_t b0 = d.col_c0 / 4u, local = key - b0;
+    const int y = (int)udiv24(local, d.col_G), x = (int)local - y * Gc;
+    const int o = c_cand[j][0], dx = c_cand[j][1], dy = c_cand[j][2];
+    const int cx = x + dx, cy = y + dy;
+    if (cx >= Gc || cy >= Gc) return;
+    const float ox = c_off[o][0], oy = c_off[o][1];
+    const int sx = ox != 0.f ? 1 : 0, sy = oy != 0.f ? 1 : 0;
+    // the cell's sources in k_col_cells' order; this base cell is source
+    // (sy - dy, sx - dx) of them
+    const int mine = (sy - dy) * 2 + (sx - dx);
+    uint32_t rb[4], re[4], total = 0;
+    bool first = true;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int by = cy - sy + (q >> 1), bx = cx - sx + (q & 1);
+        const bool in = (q >> 1) <= sy && (q & 1) <= sx && by >= 0 && bx >= 0;
+        const uint32_t bc = b0 + (uint32_t)(max(by, 0) * Gc + max(bx, 0));
+        const uint32_t e = in ? end[bc] : 0u;
+        const uint32_t bb = in ? begin[bc] : 0u;
+        first &= !(q < mine && e != 0u);
+        rb[q] = e ? bb : 0u;
+        re[q] = e;
+        total += e ? e - bb : 0u;
+    }
+    if (!first) return;
+    const uint32_t c = d.col_c0 + (uint32_t)o * (uint32_t)(Gc * Gc) + (uint32_t)(cy * Gc + cx);
+    if (total > COL_CROWD_MIN) {
+        const uint32_t idx = atomicAdd(crowd, 1u);
+        if (idx < crowd_cap) crowd[1 + idx] = c;
+        return;
+    }
+    const float h = d.col_h;
+    const float x0 = (float)cx * h, y0 = (float)cy * h;
+    double ax = 0.0, ay = 0.0, as = 0.0;
+    uint32_t n = 0;
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t e = re[q];
+        for (uint32_t kk = rb[q]; kk < e; kk += COL_BATCH) {
+            float2 pp[COL_BATCH];
+            float sz[COL_BATCH];
+#pragma unroll
+            for (int u = 0; u < COL_BATCH; ++u) {
+                const uint32_t ku = min(kk + u, e - 1);
+                pp[u] = spos[ku];
+                sz[u] = ssize[ku];
+            }
+#pragma unroll
+            for (int u = 0; u < COL_BATCH; ++u) {
+                if (kk + u >= e) break;
+                const float2 p = pp[u];
+                if ((sx && col_cell(p.x, ox, h, Gc) != cx) || (sy && col_cell(p.y, oy, h, Gc) != cy)) continue;
+                ax += (double)p.x - (double)x0;
+                ay += (double)p.y - (double)y0;
+                as += (double)sz[u];
+                ++n;
+            }
+        }
+    }
+    cells[c] = make_float4((float)ax, (float)ay, (float)as, (float)n);
+}
+
 // One lane per present collision point, in base-cell order.
 __global__ void __launch_bounds__(256) k_col_force(const float4* __restrict__ cells,
                                                    float2* __restrict__ pos0,
@@ -437,19 +565,33 @@ int col_prepare_graph(Ctx* c, Graph& g) {
 int col_run(Ctx* c, float2* vout) {
     const uint32_t npc = c->NPC, CG = c->CG, CB = c->CG / 4;
     if (npc == 0 || CG == 0) return CG_OK;
+    const int fin = c->col_rs.passes & 1;
+    // more base cells than points: cells from the occupied base cells, after
+    // clearing those of the last pass (whose sorted keys the key kernel is
+    // about to overwrite)
+    const bool sparse = CB > npc;
+    if (sparse)
+        hipLaunchKernelGGL(k_col_clear, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->col_keys[fin],
+                           c->col_vals[fin], npc, c->gid, c->desc, CB, c->G == 1 ? 1u : 0u, c->col_end,
+                           c->col_cell);
     hipLaunchKernelGGL(k_col_keys, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->pos[0], c->pos[1],
                        c->cpresent, npc, c->gid, c->desc, c->col_keys[0], c->col_vals[0], c->col_pos[0],
                        c->G == 1 ? 1u : 0u);
     radix_run(c, c->col_rs, c->col_keys, c->col_vals, false, c->col_pos);
-    const int fin = c->col_rs.passes & 1;
-    hipLaunchKernelGGL(k_fill32, dim3(blocks_for(CB, 256)), dim3(256), 0, c->stream, c->col_end, CB, 0u);
+    if (!sparse)
+        hipLaunchKernelGGL(k_fill32, dim3(blocks_for(CB, 256)), dim3(256), 0, c->stream, c->col_end, CB, 0u);
     hipLaunchKernelGGL(k_fill32, dim3(1), dim3(64), 0, c->stream, c->col_crowd, 1u, 0u);
     hipLaunchKernelGGL(k_col_bounds, dim3(blocks_for(npc, 256)), dim3(256), 0, c->stream, c->col_keys[fin],
                        c->col_vals[fin], npc, c->size, c->col_begin, c->col_end, c->col_ssize);
+    if (sparse)
+        hipLaunchKernelGGL(k_col_cells_sparse, dim3(blocks_for((uint64_t)npc * 9, 256)), dim3(256), 0, c->stream,
+                           c->col_keys[fin], c->col_vals[fin], npc, c->col_begin, c->col_end, c->col_pos[fin],
+                           c->col_ssize, c->gid, c->desc, c->col_cell, c->col_crowd, c->col_crowd_cap,
+                           c->G == 1 ? 1u : 0u);
     // lanes per cell from the points per base cell: 16 where cells are crowded
     // (each lane then strides over a few points), 4 where most are sparse;
     // crowded cells then take a wave each
-    if (npc >= 8u * CB)
+    else if (npc >= 8u * CB)
         hipLaunchKernelGGL(k_col_cells<16>, dim3(blocks_for((uint64_t)CG * 16, 256)), dim3(256), 0, c->stream,
                            c->col_begin, c->col_end, c->col_pos[fin], c->col_ssize, c->colc0s, c->desc, c->G, CG,
                            c->col_cell, c->col_crowd, c->col_crowd_cap);
@@ -513,7 +655,17 @@ int col_alloc(Ctx* c) {
         k0 += d.np;
     }
     const bool one = blk.size() && blk.front().z == 0 && blk.front().w == blk.size() && key0.front() == 0;
-    return one ? radix_alloc(c, c->col_rs, npc, range) : radix_alloc(c, c->col_rs, npc, range, &blk, &key0);
+    rc = one ? radix_alloc(c, c->col_rs, npc, range) : radix_alloc(c, c->col_rs, npc, range, &blk, &key0);
+    if (rc) return rc;
+    // every cell and base-cell end starts empty; the first pass's clear reads
+    // sorted keys of 0 (base cell 0, cleared again: always safe) for points
+    // of the set's first collision graph
+    CG_HIP(c, hipMemsetAsync(c->col_cell, 0, (size_t)c->CG * 16, c->stream));
+    CG_HIP(c, hipMemsetAsync(c->col_end, 0, (size_t)(c->CG / 4) * 4, c->stream));
+    CG_HIP(c, hipMemsetAsync(c->col_keys[c->col_rs.passes & 1], 0, (size_t)npc * 4, c->stream));
+    CG_HIP(c, hipMemcpyAsync(c->col_vals[c->col_rs.passes & 1], c->cpresent, (size_t)npc * 4,
+                             hipMemcpyDeviceToDevice, c->stream));
+    return CG_OK;
 }
 
 }  // namespace cg

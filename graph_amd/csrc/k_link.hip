@@ -173,6 +173,13 @@ __global__ void k_tier_bounds(const uint32_t* __restrict__ skeys, uint32_t ns, u
     if (d > group_max && (k == 0 || prev <= group_max)) counts[1] = k;
 }
 
+// Graph slot of each lane-tier point, in schedule order.
+__global__ void k_sched_graph(const uint32_t* __restrict__ sched, uint32_t n, const uint32_t* __restrict__ gid,
+                              uint32_t* __restrict__ sgraph) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) sgraph[j] = gid[sched[j]];
+}
+
 __global__ void k_slot_of(const uint32_t* __restrict__ sched, uint32_t ns, uint32_t* __restrict__ slot_of) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < ns) slot_of[sched[j]] = j;
@@ -248,37 +255,55 @@ __device__ inline float2 lane_chain(const GDesc& d, const uint4* __restrict__ en
     return v;
 }
 
+constexpr uint32_t LANE_RUN = 16;  // lane-tier blocks per XCD run
+
+// Largest value over the wave, by DPP row shifts and row broadcasts (no LDS
+// round trips), read from lane 63.
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// One lane per lane-tier point.  Every load that the entry loads and the
+// neighbour gathers wait on comes from the schedule itself (the point, its
+// entry range and, for sets of several graphs, its graph slot), so a wave
+// waits on three memory latencies (schedule, entries and descriptor,
+// neighbours) before its first spring step; a point of a graph without the
+// pass runs the chain and stores nothing.
 __global__ void __launch_bounds__(256) k_link_lane(int pass, const uint32_t* __restrict__ sched,
                                                    const uint32_t* __restrict__ soff,
+                                                   const uint32_t* __restrict__ sgraph,
                                                    const uint4* __restrict__ ent, uint32_t count,
-                                                   const uint32_t* __restrict__ gid,
                                                    const GDesc* __restrict__ desc,
                                                    float2* __restrict__ pos0,
                                                    float2* __restrict__ pos1,
                                                    const uint8_t* __restrict__ flags,
                                                    float2* __restrict__ vout, uint32_t one_graph) {
-    // the longer-chain classes start first
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    // the longer-chain classes start first; each XCD takes runs of LANE_RUN
+    // consecutive blocks of the schedule (graph-major inside a class, so in a
+    // set of many graphs an XCD's L2 holds the positions of fewer graphs)
+    const uint32_t r = xcd_run_block(blockIdx.x, gridDim.x, LANE_RUN) * blockDim.x + threadIdx.x;
     const bool in = r < count;
     const uint32_t j = in ? count - 1 - r : 0u;
     const uint32_t p = sched[j];
     const uint32_t b = soff[j], e1 = soff[j + 1];
-    // a set of one graph reads its descriptor without the graph-slot load,
-    // one dependent memory latency less before the entry loads
-    const GDesc& d = desc[one_graph ? 0u : gid[p]];
-    const bool active = in && (d.passes & (1u << pass));
-    const uint32_t e = active ? e1 : b;
+    const GDesc& d = desc[one_graph ? 0u : sgraph[j]];
+    const uint32_t e = in ? e1 : b;
     // batch depth from the wave's longest chain (every lane is still here)
-    uint32_t mx = in ? e1 - b : 0u;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
-    if (!active) return;
+    const uint32_t mx = wave_max_u32(e - b);
+    if (!in) return;
     const bool bsel = (d.inbuf >> pass) & 1u;
     const float2* __restrict__ pin = bsel ? pos1 : pos0;
     const float2 pp = pin[p];
     const float2 v = mx <= 2 ? lane_chain<2>(d, ent, b, e, pin, pp)
                              : (mx <= 4 ? lane_chain<4>(d, ent, b, e, pin, pp)
                                         : (mx <= 16 ? lane_chain<8>(d, ent, b, e, pin, pp) : lane_chain<16>(d, ent, b, e, pin, pp)));
+    if (!(d.passes & (1u << pass))) return;
     if (vout) {
         vout[p] = v;
         return;
@@ -833,7 +858,7 @@ int link_build_graph(Ctx* c, Graph& gr) {
 // order, so their layout depends only on the graph; a graph whose links did
 // not change carries its region over from the previous packing (old).
 void link_free_dir(Ctx* c, LinkDir& L) {
-    dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff); dfree(c, L.unit_a); dfree(c, L.unit_b);
+    dfree(c, L.ent); dfree(c, L.sched); dfree(c, L.soff); dfree(c, L.sgraph); dfree(c, L.unit_a); dfree(c, L.unit_b);
     dfree(c, L.hubs); dfree(c, L.seg_out); dfree(c, L.hub_done); dfree(c, L.warm); dfree(c, L.wjac);
     dfree(c, L.terms);
     L = LinkDir();
@@ -916,6 +941,12 @@ int link_pack(Ctx* c, const LinkDir (&old)[2]) {
         CG_HIP(c, hipStreamSynchronize(c->stream));
         L.n_lane = h[0];
         const uint32_t n64 = h[1] - h[0];
+        if (G > 1 && L.n_lane) {
+            L.sgraph = (uint32_t*)dalloc(c, (size_t)L.n_lane * 4, &rc);
+            if (rc) { cleanup(); return rc; }
+            hipLaunchKernelGGL(k_sched_graph, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, L.sched,
+                               L.n_lane, c->gid, L.sgraph);
+        }
         L.ent = (uint4*)dalloc(c, (size_t)(L.E_live ? L.E_live : 1) * 16, &rc);
         if (rc) { cleanup(); return rc; }
         hipLaunchKernelGGL(k_slot_of, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.sched, ns,
@@ -1080,7 +1111,7 @@ int link_run_e(Ctx* c, int dir, float2* vout) {
                            L.seg_out, L.n_units, L.hubs, L.n_hubs, L.hub_done, stats, vout, nullptr);
     if (L.n_lane)
         hipLaunchKernelGGL(k_link_lane, dim3(blocks_for(L.n_lane, 256)), dim3(256), 0, c->stream, pass,
-                           L.sched, L.soff, L.ent, L.n_lane, c->gid, c->desc, c->pos[0], c->pos[1],
+                           L.sched, L.soff, L.sgraph, L.ent, L.n_lane, c->desc, c->pos[0], c->pos[1],
                            c->flags, vout, c->G == 1 ? 1u : 0u);
     if (fork) CG_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     CG_HIP(c, hipGetLastError());

@@ -201,17 +201,25 @@ def test_link_forces_after_ticks_vs_oracle():
         check(got, want, S, f"link dir {d} after 40 ticks", idx=absent == 0)
 
 
-def test_collision_vs_oracle():
-    S, n = 4096.0, 20_000
+@pytest.mark.parametrize("S,radius,path", [(4096.0, 5.0, "sparse"), (16384.0, 0.0, "sparse"),
+                                          (4096.0, 0.0, "dense4"), (4096.0, 80.0, "dense16")])
+def test_collision_vs_oracle(S, radius, path):
+    """Each cell pass of the collision force: from the occupied base cells
+    (more base cells than points), or over every cell with 4 or 16 lanes;
+    radius 0 takes each point's radius from its size."""
+    n = 20_000
     rng = np.random.default_rng(4)
     xy = np.clip(rng.normal(2048, 300, 2 * n), 0, S).astype(np.float32)
     sizes = 2 * datagen.lognormal(n, np.log(2), 0.5, 9)
+    mx = max(4.0, float(sizes.max()))
+    G, _ = O.collision_grid(S, radius, 0.0, mx)
+    assert path == ("sparse" if G * G > n else ("dense16" if n >= 8 * G * G else "dense4"))
     c = ctx_for(xy)
     c.set_point_sizes(sizes)
-    p = params(S, collision=0.7)
+    p = params(S, collision=0.7, collision_radius=radius)
     got = c.eval_force(p, _lib.FORCE_COLLISION, 0.8)
-    want = O.collision(xy, sizes, S, 0.8, 0.7, max_size=max(4.0, float(sizes.max())))
-    check(got, want, S, "collision")
+    want = O.collision(xy, sizes, S, 0.8, 0.7, radius=radius, max_size=mx)
+    check(got, want, S, f"collision ({path}, grid {G})")
 
 
 def test_ticks_deterministic_alpha_and_masks():
