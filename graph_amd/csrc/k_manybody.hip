@@ -113,9 +113,10 @@ constexpr uint32_t SCAN_CPT_MAX = 256;      // .. 256, so that at most ~4096 blo
 // Keys of the points in the previous pass's cell order (positions as payload)
 // and their ranks in their cells.
 __global__ void __launch_bounds__(256) k_bin_count(const uint32_t* __restrict__ visit,
+                                                   const uint32_t* __restrict__ old_keys,
                                                    const float2* __restrict__ pos0,
                                                    const float2* __restrict__ pos1,
-                                                   const uint32_t* __restrict__ gid,
+                                                   const uint32_t* __restrict__ blk_graph,
                                                    const GDesc* __restrict__ desc, uint32_t np,
                                                    uint32_t* __restrict__ cell_count,
                                                    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -129,14 +130,16 @@ __global__ void __launch_bounds__(256) k_bin_count(const uint32_t* __restrict__ 
         hk[q] = CB_EMPTY;
         hc[q] = 0u;
     }
-    // the lane's items go through each dependent load together (index, graph,
-    // position); a set of one graph skips the graph loads
+    // the lane's items go through each dependent load together (index and
+    // graph, position); a point's graph is that of its previous key (read in
+    // the same order as the visit), a set of one graph skips it
     uint32_t ii[CB_ITEMS], gg[CB_ITEMS];
     float2 qq[CB_ITEMS];
 #pragma unroll
     for (int j = 0; j < CB_ITEMS; ++j) ii[j] = visit[min(bb + j * 256 + t, np - 1)];
 #pragma unroll
-    for (int j = 0; j < CB_ITEMS; ++j) gg[j] = one_graph ? 0u : gid[ii[j]];
+    for (int j = 0; j < CB_ITEMS; ++j)
+        gg[j] = one_graph ? 0u : blk_graph[old_keys[min(bb + j * 256 + t, np - 1)] / CELL_BLOCK];
 #pragma unroll
     for (int j = 0; j < CB_ITEMS; ++j) {
         const GDesc& d = desc[gg[j]];
@@ -216,6 +219,14 @@ __global__ void __launch_bounds__(256) k_bin_count(const uint32_t* __restrict__ 
             rank[k] = hc[slot[j]] + r[j];
         }
     }
+}
+
+// Keys standing in for a pass before the first: each point's graph's first
+// finest cell, in the first pass's visiting order (the present list).
+__global__ void k_first_keys(const uint32_t* __restrict__ present, uint32_t np, const uint32_t* __restrict__ gid,
+                             const uint32_t* __restrict__ f0s, uint32_t* __restrict__ keys) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < np) keys[k] = f0s[gid[present[k]]];
 }
 
 // Block-exclusive scan of one value per thread over 256 threads (4 waves).
@@ -1367,13 +1378,14 @@ __device__ inline void mb_finish(const GDesc& d, uint32_t i, float2 p, float2 r,
 // (88 unbounded, five waves).
 template <int MB_TILE>
 __global__ void __launch_bounds__(256, 8) k_mb_force(const uint32_t* __restrict__ sv,
+                                                  const uint32_t* __restrict__ skey,
                                                   const float2* __restrict__ sorted_pos,
                                                   const float4* __restrict__ cent,
                                                   const int32_t* __restrict__ slot_idx,
                                                   const float2* __restrict__ slot_pos,
                                                   const float2* __restrict__ jitter,
                                                   const uint8_t* __restrict__ flags,
-                                                  const uint32_t* __restrict__ gid,
+                                                  const uint32_t* __restrict__ blk_graph,
                                                   const GDesc* __restrict__ desc, uint32_t np,
                                                   float2* __restrict__ pos0,
                                                   float2* __restrict__ pos1,
@@ -1386,7 +1398,7 @@ __global__ void __launch_bounds__(256, 8) k_mb_force(const uint32_t* __restrict_
     const bool valid = k < np;
     const uint32_t kk = valid ? k : np - 1;
     const uint32_t i = sv[kk];
-    const uint32_t g = one_graph ? 0u : gid[i];
+    const uint32_t g = one_graph ? 0u : blk_graph[skey[kk] / CELL_BLOCK];  // the graph of the point's cell
     const bool run = (desc[g].passes >> P_MANYBODY) & 1u;
     if (t == 0) s_g = g;
     __syncthreads();
@@ -1574,7 +1586,13 @@ int mb_alloc(Ctx* c, uint32_t single_gf) {
     CG_HIP(c, hipMemsetAsync(c->cell_count, 0, F * 4, c->stream));
     CG_HIP(c, hipMemsetAsync(c->cell_occ, 0, nblk, c->stream));
     CG_HIP(c, hipMemsetAsync(c->cell_dirty, 1, nblk, c->stream));
-    CG_HIP(c, hipMemsetAsync(c->rk[c->mb_rs.passes & 1], 0, np * 4, c->stream));
+    // the first pass's previous keys: a cell of each point's own graph (its
+    // first), which names the point's graph to the count and is safe to clear
+    if (c->G > 1 && c->NP)
+        hipLaunchKernelGGL(k_first_keys, dim3(blocks_for(c->NP, 256)), dim3(256), 0, c->stream, c->present,
+                           c->NP, c->gid, c->f0s, c->rk[c->mb_rs.passes & 1]);
+    else
+        CG_HIP(c, hipMemsetAsync(c->rk[c->mb_rs.passes & 1], 0, np * 4, c->stream));
     return rc;
 }
 
@@ -1602,8 +1620,9 @@ int mb_run(Ctx* c, float2* vout) {
     // this pass counts into them
     hipLaunchKernelGGL(k_mb_clear, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->rk[1], np, c->cell_end,
                        c->cell_dirty, c->cell_count);
-    hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(np, CB_TILE)), dim3(256), 0, c->stream, c->sorted, c->pos[0],
-                       c->pos[1], c->gid, c->desc, np, c->cell_count, c->rk[0], c->rv[0], c->rp[0], c->bin_rank,
+    hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(np, CB_TILE)), dim3(256), 0, c->stream, c->sorted, c->rk[1],
+                       c->pos[0], c->pos[1], c->blk_graph, c->desc, np, c->cell_count, c->rk[0], c->rv[0], c->rp[0],
+                       c->bin_rank,
                        c->cell_occ, c->cell_dirty, c->G == 1 ? 1u : 0u);
     // cells per scan thread: the fewest (4 .. 256) that keep the ranges to ~4096
     uint32_t cpt = SCAN_CPT_MIN;
@@ -1661,12 +1680,12 @@ int mb_run(Ctx* c, float2* vout) {
     prof_begin(c, CG_STAGE_MB_FORCE);
     if (c->tune.force_tile == 512)
         hipLaunchKernelGGL(k_mb_force<512>, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->sorted,
-                           c->sorted_pos, c->cent, c->slot_idx, c->slot_pos, c->jitter, c->flags, c->gid,
-                           c->desc, np, c->pos[0], c->pos[1], vout, c->G == 1 ? 1u : 0u);
+                           c->rk[1], c->sorted_pos, c->cent, c->slot_idx, c->slot_pos, c->jitter, c->flags,
+                           c->blk_graph, c->desc, np, c->pos[0], c->pos[1], vout, c->G == 1 ? 1u : 0u);
     else
         hipLaunchKernelGGL(k_mb_force<1024>, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->sorted,
-                           c->sorted_pos, c->cent, c->slot_idx, c->slot_pos, c->jitter, c->flags, c->gid,
-                           c->desc, np, c->pos[0], c->pos[1], vout, c->G == 1 ? 1u : 0u);
+                           c->rk[1], c->sorted_pos, c->cent, c->slot_idx, c->slot_pos, c->jitter, c->flags,
+                           c->blk_graph, c->desc, np, c->pos[0], c->pos[1], vout, c->G == 1 ? 1u : 0u);
     prof_end(c, CG_STAGE_MB_FORCE);
     CG_HIP(c, hipGetLastError());
     return CG_OK;
