@@ -47,9 +47,14 @@ __device__ inline float2 pw_force(float2 p, const GDesc& d, const double* __rest
     return v;
 }
 
-// One lane per present point: force -> integrate.  vout: evaluation mode,
-// velocity only.
-template <int PASS>
+// PW_ITEMS present points per lane (strided by the block, so each load is
+// coalesced), each through force -> integrate; the lane's loads go out
+// together, one memory latency per dependent step for all its points (4 for
+// sets of at least PW_MANY points: the serving mix's gravity pass 101 -> 86
+// us; 1 below, where fewer waves would leave the chip idler: 1M points 14 ->
+// 15 us at 4).  vout: evaluation mode, velocity only.
+constexpr uint32_t PW_MANY = 1u << 22;
+template <int PASS, int PW_ITEMS>
 __global__ void __launch_bounds__(256) k_pointwise(float2* __restrict__ pos0,
                                                    float2* __restrict__ pos1,
                                                    float2* __restrict__ vout,
@@ -58,22 +63,33 @@ __global__ void __launch_bounds__(256) k_pointwise(float2* __restrict__ pos0,
                                                    const uint8_t* __restrict__ flags,
                                                    const GDesc* __restrict__ desc, uint32_t np,
                                                    const double* __restrict__ csum, uint32_t one_graph) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= np) return;
-    const uint32_t i = present[k];
-    const uint32_t g = one_graph ? 0u : gid[i];  // a set of one graph skips the slot load
-    const GDesc& d = desc[g];
-    if (!(d.passes & (1u << PASS))) return;
-    const bool b = (d.inbuf >> PASS) & 1u;
-    const float2* __restrict__ pin = b ? pos1 : pos0;
-    float2* __restrict__ pout = b ? pos0 : pos1;
-    const float2 p = pin[i];
-    const float2 v = pw_force<PASS>(p, d, csum, g);
-    if (vout) {
-        vout[i] = v;
-        return;
+    const uint32_t k0 = blockIdx.x * blockDim.x * PW_ITEMS + threadIdx.x;
+    uint32_t ii[PW_ITEMS], gg[PW_ITEMS];
+#pragma unroll
+    for (int u = 0; u < PW_ITEMS; ++u) ii[u] = present[min(k0 + u * blockDim.x, np - 1)];
+#pragma unroll
+    for (int u = 0; u < PW_ITEMS; ++u) gg[u] = one_graph ? 0u : gid[ii[u]];  // a set of one graph skips the slot load
+    float2 pp[PW_ITEMS];
+    uint8_t fl[PW_ITEMS];
+#pragma unroll
+    for (int u = 0; u < PW_ITEMS; ++u) {
+        const GDesc& d = desc[gg[u]];
+        pp[u] = ((d.inbuf >> PASS) & 1u) ? pos1[ii[u]] : pos0[ii[u]];
+        fl[u] = vout ? 0 : flags[ii[u]];
     }
-    pout[i] = (flags[i] & F_PINNED) ? p : integrate(p, v, d.friction, d.S);
+#pragma unroll
+    for (int u = 0; u < PW_ITEMS; ++u) {
+        if (k0 + u * blockDim.x >= np) break;
+        const GDesc& d = desc[gg[u]];
+        if (!(d.passes & (1u << PASS))) continue;
+        const float2 v = pw_force<PASS>(pp[u], d, csum, gg[u]);
+        if (vout) {
+            vout[ii[u]] = v;
+            continue;
+        }
+        float2* __restrict__ pout = ((d.inbuf >> PASS) & 1u) ? pos0 : pos1;
+        pout[ii[u]] = (fl[u] & F_PINNED) ? pp[u] : integrate(pp[u], v, d.friction, d.S);
+    }
 }
 
 // Centroid of each graph: one block per chunk of CENTER_CHUNK present points of
@@ -251,14 +267,19 @@ int center_run(Ctx* c) {
 
 int pw_run(Ctx* c, int pass, float2* vout) {
     if (c->NP == 0) return CG_OK;
-    const unsigned nb = blocks_for(c->NP, 256);
     if (pass == P_CENTER) {
         int rc = center_run(c);
         if (rc) return rc;
     }
-#define CG_PW(P)                                                                                   \
-    hipLaunchKernelGGL(k_pointwise<P>, dim3(nb), dim3(256), 0, c->stream, c->pos[0], c->pos[1], vout, \
-                       c->present, c->gid, c->flags, c->desc, c->NP, c->center_sum, c->G == 1 ? 1u : 0u)
+#define CG_PW(P)                                                                                         \
+    if (c->NP >= PW_MANY)                                                                                \
+        hipLaunchKernelGGL((k_pointwise<P, 4>), dim3(blocks_for(c->NP, 1024)), dim3(256), 0, c->stream,   \
+                           c->pos[0], c->pos[1], vout, c->present, c->gid, c->flags, c->desc, c->NP,      \
+                           c->center_sum, c->G == 1 ? 1u : 0u);                                           \
+    else                                                                                                 \
+        hipLaunchKernelGGL((k_pointwise<P, 1>), dim3(blocks_for(c->NP, 256)), dim3(256), 0, c->stream,    \
+                           c->pos[0], c->pos[1], vout, c->present, c->gid, c->flags, c->desc, c->NP,      \
+                           c->center_sum, c->G == 1 ? 1u : 0u)
     switch (pass) {
         case P_GRAVITY: CG_PW(P_GRAVITY); break;
         case P_CENTER: CG_PW(P_CENTER); break;
