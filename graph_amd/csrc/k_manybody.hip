@@ -199,15 +199,25 @@ __global__ void __launch_bounds__(256) k_bin_count(const uint32_t* __restrict__ 
     }
     __syncthreads();
     // one global atomic per cell of the block: its old value is the block's
-    // base; the cell's block of CELL_BLOCK cells is marked occupied and dirty
-    for (uint32_t q = t; q < CB_HASH; q += 256) {
-        const uint32_t kk = hk[q];
-        if (kk != CB_EMPTY) {
-            hc[q] = atomicAdd(&cell_count[kk], hc[q]);
-            cell_occ[kk / CELL_BLOCK] = 1u;
-            cell_dirty[kk / CELL_BLOCK] = 1u;
-        }
+    // base; the cell's block of CELL_BLOCK cells is marked occupied and dirty.
+    // A lane's slots go out together (its atomics in flight at once).
+    constexpr int HS = (int)(CB_HASH / 256);
+    uint32_t hkk[HS], hcc[HS];
+#pragma unroll
+    for (int u = 0; u < HS; ++u) {
+        hkk[u] = hk[t + u * 256];
+        hcc[u] = hc[t + u * 256];
     }
+#pragma unroll
+    for (int u = 0; u < HS; ++u)
+        if (hkk[u] != CB_EMPTY) hcc[u] = atomicAdd(&cell_count[hkk[u]], hcc[u]);
+#pragma unroll
+    for (int u = 0; u < HS; ++u)
+        if (hkk[u] != CB_EMPTY) {
+            hc[t + u * 256] = hcc[u];
+            cell_occ[hkk[u] / CELL_BLOCK] = 1u;
+            cell_dirty[hkk[u] / CELL_BLOCK] = 1u;
+        }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < CB_ITEMS; ++j) {
