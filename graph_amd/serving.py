@@ -84,9 +84,10 @@ def footprint_bytes(n: int, m: int, collision: bool) -> int:
     repack makes (positions and link schedules exist twice while it runs)."""
     gf = finest_grid(n)
     cells = gf * gf
-    per_point = 16 + 1 + 4 + 8 + 4 + 4 + 16 + 8 + 32 + 13  # positions x2, flags, gid, jitter, size,
-    #        present, keys/values x2, sorted, link schedule, staged flags/jitter/present
-    per_cell = 8 + 96 + 4 + 54                          # bounds, 8 slots, crowded list, levels
+    per_point = 16 + 1 + 4 + 8 + 4 + 4 + 16 + 8 + 32 + 13 + 6  # positions x2, flags, gid, jitter, size,
+    #        present, keys/values x2, sorted, link schedule, staged flags/jitter/present, ranks and
+    #        the cell pass's partials
+    per_cell = 8 + 96 + 4 + 54                          # bounds, 8 slots, counts, levels
     per_link = 2 * (16 + 16 + 16 + 16) + 24             # CSR + schedule + unit/warm state + hub terms, build
     col = (n * 40 + 4 * 512 * 512 * 24) if collision else 0   # sort keys/values/positions x2, sizes x2, cells
     return int(1.3 * (per_point * n + per_cell * cells + per_link * m + col)) + (1 << 20)
