@@ -165,6 +165,8 @@ def tune(w: Workload, ticks: int, space: dict, rounds: int = 1, log=print) -> di
                 if b < 0.98 * min(a, a2):
                     log(f"{w.name}: {name} {cur[name]} -> {v}: {min(a, a2):.4f} -> {b:.4f} ms/tick")
                     cur = cand
+                else:
+                    log(f"{w.name}: {name} {v} kept {cur[name]}: {min(a, a2):.4f} vs {b:.4f} ms/tick")
     return cur
 
 
