@@ -133,6 +133,39 @@ def test_graph_without_present_points_in_a_mix():
         assert same
 
 
+def tiny(n: int, seed: int) -> dict:
+    """A graph of n points (1 .. 12): a chain of links from 2 points on, every
+    third one with collision, one with a pinned point."""
+    S = 64.0
+    links = None
+    if n >= 2:
+        links = np.stack([np.arange(n - 1), np.arange(1, n)], axis=1).astype(np.float32).reshape(-1)
+    p = workloads.sim_params(S, collision=1.0 if seed % 3 == 0 else 0.0)
+    r_in, r_out = (None, None) if links is None else workloads._draws(links.size // 2, seed)
+    xy = datagen.uniform_square(n, S, seed)
+    return dict(kind=f"tiny{n}", n=n, xy=xy, links=links, r_in=r_in, r_out=r_out, params=p, sizes=None,
+                pinned=np.array([0], np.uint32) if seed % 5 == 0 else np.zeros(0, np.uint32), clusters=None)
+
+
+def test_many_tiny_graphs_in_a_mix():
+    """Forty graphs of 1 to 12 points packed around two larger ones: lanes of
+    the cell pass then hold points of three or more graphs, cell runs cross
+    wave ends next to graph ends, and the collision cells of tiny graphs come
+    from their few occupied base cells.  Every graph matches its run alone
+    bit for bit."""
+    specs = [spec("ba", 31)] + [tiny(1 + i % 12, 40 + i) for i in range(40)] + [spec("collide", 32)]
+    with _lib.Context() as ctx:
+        ids = [load(ctx, s) for s in specs]
+        ctx.step_graphs(ids, 15)
+        got = [ctx.graph_get_positions(g) for g in ids]
+    for s, x in zip(specs, got):
+        same = np.array_equal(bits(alone(s, 15)), bits(x))
+        if not same:
+            say(f"{s['kind']} (n={s['n']}): differs from its run alone")
+        assert same, f"{s['kind']}: mixed result differs from the graph run alone"
+    say(f"{len(specs)} graphs bit-identical to their runs alone")
+
+
 def test_unstepped_graphs_keep_state_and_alpha():
     a, b = spec("ba", 7), spec("nolinks", 8)
     with _lib.Context() as ctx:
