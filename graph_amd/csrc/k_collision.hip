@@ -10,9 +10,11 @@
 // a fixed order (base cells ascending, points by index within each), the sums
 // of (x - x0, y - y0, size) in f64 and the count; relative to the cell's
 // origin so the force pass can remove the point's own contribution without
-// f32 cancellation.  The force pass visits the points in base-cell order (a wave
-// reads the same few cells), sweeps the 3x3 cell aggregates of all four grids
-// and integrates.
+// f32 cancellation.  Where base cells outnumber points, the cells are found from
+// the occupied base cells instead of visiting every cell (k_col_cells_sparse),
+// after clearing the last pass's ones (k_col_clear).  The force pass visits the
+// points in base-cell order (a wave reads the same few cells), sweeps the 3x3
+// cell aggregates of all four grids and integrates.
 //
 // Reference: ForceCollision/index.ts:61-164 (grid geometry, sizes),
 // :264-345 (run); build-grid.vert:28-62; force-collision-spatial.frag:41-168.
