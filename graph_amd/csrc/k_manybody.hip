@@ -109,6 +109,7 @@ constexpr uint32_t CB_HASH = 4096;           // LDS hash slots (at most half of 
 constexpr uint32_t CB_EMPTY = 0xffffffffu;
 constexpr uint32_t SCAN_CPT_MIN = 4;        // cells per thread of the count scan: 4 ..
 constexpr uint32_t SCAN_CPT_MAX = 256;      // .. 256, so that at most ~4096 blocks cover the cells
+constexpr uint32_t COUNT_OWN_PREFIX = 1024; // ranges up to which the bounds blocks sum their own prefix
 
 // Keys of the points in the previous pass's cell order (positions as payload)
 // and their ranks in their cells.
@@ -306,11 +307,22 @@ __global__ void __launch_bounds__(256) k_count_scan(uint32_t* __restrict__ sums,
 // Count scan, step 3: every occupied cell's bounds (begin = the points of the
 // cells before it, end = begin + count).  Empty cells keep end 0 (k_mb_clear
 // cleared the last pass's cells); unoccupied blocks of cells are not read.
+// With `own_prefix` (few ranges) each block adds up the range sums before its
+// own, and step 2 is not launched.
 __global__ void __launch_bounds__(256) k_count_bounds(const uint32_t* __restrict__ cell_count, uint32_t F,
                                                       uint32_t cpt, const uint8_t* __restrict__ cell_occ,
-                                                      const uint32_t* __restrict__ sums,
+                                                      const uint32_t* __restrict__ sums, uint32_t own_prefix,
                                                       uint32_t* __restrict__ begin, uint32_t* __restrict__ end) {
     __shared__ uint32_t s_w[4];
+    uint32_t base;
+    if (own_prefix) {
+        uint32_t part = 0u;
+        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256u) part += sums[b];
+        (void)block_exscan256(part, s_w, base);
+        __syncthreads();  // s_w is rewritten below
+    } else {
+        base = sums[blockIdx.x];
+    }
     const uint32_t c0 = (blockIdx.x * 256u + threadIdx.x) * cpt;
     const uint32_t c1 = min(F, c0 + cpt), step = min(cpt, CELL_BLOCK);
     uint32_t v = 0u;
@@ -323,7 +335,7 @@ __global__ void __launch_bounds__(256) k_count_bounds(const uint32_t* __restrict
         }
     }
     uint32_t total;
-    uint32_t run = sums[blockIdx.x] + block_exscan256(v, s_w, total);
+    uint32_t run = base + block_exscan256(v, s_w, total);
     for (uint32_t cb = c0; cb < c1; cb += step) {
         if (!cell_occ[cb / CELL_BLOCK]) continue;
         const uint4* __restrict__ p = reinterpret_cast<const uint4*>(cell_count + cb);
@@ -1640,9 +1652,11 @@ int mb_run(Ctx* c, float2* vout) {
     const uint32_t nr = (uint32_t)(((uint64_t)F + 256u * cpt - 1) / (256u * cpt));
     hipLaunchKernelGGL(k_count_sums, dim3(nr), dim3(256), 0, c->stream, c->cell_count, F, cpt, c->cell_occ,
                        c->scan_sums);
-    hipLaunchKernelGGL(k_count_scan, dim3(1), dim3(256), 0, c->stream, c->scan_sums, nr);
+    // up to COUNT_OWN_PREFIX ranges each bounds block adds up the sums before it
+    const uint32_t own = nr <= COUNT_OWN_PREFIX ? 1u : 0u;
+    if (!own) hipLaunchKernelGGL(k_count_scan, dim3(1), dim3(256), 0, c->stream, c->scan_sums, nr);
     hipLaunchKernelGGL(k_count_bounds, dim3(nr), dim3(256), 0, c->stream, c->cell_count, F, cpt, c->cell_occ,
-                       c->scan_sums, c->cell_begin, c->cell_end);
+                       c->scan_sums, own, c->cell_begin, c->cell_end);
     hipLaunchKernelGGL(k_bin_place, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, c->rk[0], c->rv[0],
                        c->rp[0], c->bin_rank, np, c->cell_begin, c->rk[1], c->rv[1], c->rp[1]);
     prof_end(c, CG_STAGE_MB_SORT);
