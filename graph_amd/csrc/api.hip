@@ -188,20 +188,6 @@ void prof_end(Ctx* c, int stage) {
 
 namespace {
 
-// Copies the descriptor table of the current tick (ring entry *ctr) to `cur`
-// and advances the counter.  One block, so the counter is read before it moves.
-__global__ void __launch_bounds__(1024) k_desc_advance(const uint32_t* __restrict__ ring,
-                                                       uint32_t* __restrict__ ctr, uint32_t words,
-                                                       uint32_t* __restrict__ cur) {
-    __shared__ uint32_t s_t;
-    if (threadIdx.x == 0) s_t = *ctr;
-    __syncthreads();
-    const uint32_t* __restrict__ src = ring + (size_t)s_t * words;
-    for (uint32_t w = threadIdx.x; w < words; w += 1024) cur[w] = src[w];
-    __syncthreads();
-    if (threadIdx.x == 0) *ctr = s_t + 1;
-}
-
 __global__ void k_set_word(uint32_t* __restrict__ a, uint32_t v) { *a = v; }
 
 // drag-point.frag:28-31: listed points (graph-local indices of present points,
@@ -395,7 +381,7 @@ void drop_graphs(Ctx* c) {
 // Frees the packed arrays except the position buffers and the link schedules.
 void free_packed(Ctx* c) {
     drop_graphs(c);
-    release(c, c->desc_cur); release(c, c->tick_ctr);
+    release(c, c->desc_win);
     release(c, c->flags); release(c, c->present); release(c, c->gid); release(c, c->jitter);
     release(c, c->size); release(c, c->desc_base); c->desc = nullptr;
     release(c, c->f0s); release(c, c->colc0s); release(c, c->blk_graph);
@@ -783,8 +769,7 @@ int pack_body(Ctx* c, const PackSnap& old) {
     // descriptor tables (device and pinned staging)
     c->desc_ticks = std::max<uint32_t>(1, std::min<uint32_t>(DESC_MAX_TICKS, DESC_ENTRIES / (uint32_t)gg));
     c->desc_base = (GDesc*)dalloc(c, (size_t)c->desc_ticks * gg * sizeof(GDesc), &rc);
-    c->desc_cur = (GDesc*)dalloc(c, gg * sizeof(GDesc), &rc);
-    c->tick_ctr = (uint32_t*)dalloc(c, 64, &rc);
+    c->desc_win = (GDesc*)dalloc(c, (size_t)c->desc_ticks * gg * sizeof(GDesc), &rc);
     if (rc) return rc;
     if (c->h_desc_pinned) (void)hipHostFree(c->h_desc_pinned);
     c->h_desc_pinned = nullptr;
@@ -792,7 +777,6 @@ int pack_body(Ctx* c, const PackSnap& old) {
     c->h_desc = st;
     c->desc = c->desc_base;
     rc = upload(c, c->desc_base, st); if (rc) return rc;
-    rc = upload(c, c->desc_cur, st); if (rc) return rc;
 
     rc = mb_alloc(c); if (rc) return rc;
     rc = col_alloc(c); if (rc) return rc;
@@ -937,22 +921,20 @@ int launch_tick(Ctx* c, const TickPlan& tp) {
     return CG_OK;
 }
 
-// One tick as launched (or captured): descriptor advance, then the passes.
+// n ticks as launched (or captured): tick q reads its descriptor table at
+// window slot q, which the caller fills before the launch.
 int launch_tick_seq(Ctx* c, const TickPlan& tp, uint32_t n) {
-    static_assert(sizeof(GDesc) % 4 == 0, "descriptor words");
-    const uint32_t words = c->G * (uint32_t)(sizeof(GDesc) / 4);
     for (uint32_t q = 0; q < n; ++q) {
-        hipLaunchKernelGGL(k_desc_advance, dim3(1), dim3(1024), 0, c->stream, (const uint32_t*)c->desc_base,
-                           c->tick_ctr, words, (uint32_t*)c->desc_cur);
+        c->desc = c->desc_win + (size_t)q * c->G;
         int rc = launch_tick(c, tp);
         if (rc) return rc;
     }
     return CG_OK;
 }
 
-// The executable graph of n ticks of plan tp, captured on first use.  Every
-// tick kernel reads desc_cur and the device tick counter, so one graph serves
-// any tick of any chunk while the packing stands.
+// The executable graph of n ticks of plan tp, captured on first use.  Tick q
+// of it reads descriptor window slot q, which run_ticks fills before every
+// launch, so one graph serves any n ticks of any chunk while the packing stands.
 int tick_graph(Ctx* c, const TickPlan& tp, uint32_t n, hipGraphExec_t* out) {
     for (auto& e : c->graphs_exec)
         if (e.first.first == tp.any && e.first.second == n) {
@@ -1037,8 +1019,6 @@ int run_ticks(Ctx* c, const std::vector<uint32_t>& ids, uint32_t nticks, int64_t
                                  hipMemcpyHostToDevice, c->stream));
         CG_HIP(c, hipEventRecord(c->ev_desc, c->stream));
         c->desc_pending = true;
-        hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, c->stream, c->tick_ctr, 0u);
-        c->desc = c->desc_cur;
         // stage timing brackets single launches, so it runs uncaptured
         const bool cap = c->capture && c->prof_mask == 0;
         for (uint32_t tt = 0; tt < T;) {
@@ -1048,6 +1028,9 @@ int run_ticks(Ctx* c, const std::vector<uint32_t>& ids, uint32_t nticks, int64_t
                 // graph_ticks ticks per captured graph, one-tick graphs for the rest
                 const uint32_t gt = c->tune.graph_ticks;
                 const uint32_t n = (cap && r - done >= gt) ? gt : 1u;
+                // the n ticks' descriptor tables into the window their kernels read
+                CG_HIP(c, hipMemcpyAsync(c->desc_win, c->desc_base + (size_t)(tt + done) * G,
+                                         (size_t)n * G * sizeof(GDesc), hipMemcpyDeviceToDevice, c->stream));
                 if (cap) {
                     hipGraphExec_t x = nullptr;
                     rc = tick_graph(c, plan[tt], n, &x);
@@ -1595,7 +1578,6 @@ int cg_prepare(cg_ctx* x, const cg_sim_params* p) {
     // the plan cg_step runs for graph 0 alone (the other graphs are not stepped)
     const TickPlan tp{graph_passes(c->graphs[0])};
     if (tp.any == 0) return CG_OK;
-    c->desc = c->desc_cur;
     hipGraphExec_t e = nullptr;
     int rc = tick_graph(c, tp, c->tune.graph_ticks, &e);
     if (!rc) rc = tick_graph(c, tp, 1, &e);
@@ -1829,8 +1811,8 @@ int cg_eval_manybody_noalloc(cg_ctx* x, const cg_sim_params* p, float alpha, flo
     s.flags = (uint8_t*)dalloc(&s, n, &rc);
     s.gid = (uint32_t*)dalloc(&s, (size_t)n * 4, &rc);
     s.present = (uint32_t*)dalloc(&s, (size_t)np * 4, &rc);
-    s.desc_cur = (GDesc*)dalloc(&s, sizeof(GDesc), &rc);
-    s.desc = s.desc_cur;
+    s.desc_win = (GDesc*)dalloc(&s, sizeof(GDesc), &rc);
+    s.desc = s.desc_win;
     s.f0s = (uint32_t*)dalloc(&s, 4, &rc);
     s.blk_graph = (uint32_t*)dalloc(&s, (size_t)(s.F / CELL_BLOCK) * 4, &rc);
     uint4 w0[PYR_W0_MAX], w1[PYR_W1_MAX];
@@ -1856,7 +1838,7 @@ int cg_eval_manybody_noalloc(cg_ctx* x, const cg_sim_params* p, float alpha, flo
     CG_HIP(c, hipMemsetAsync(s.f0s, 0, 4, c->stream));
     CG_HIP(c, hipMemsetAsync(s.blk_graph, 0, (size_t)(s.F / CELL_BLOCK) * 4, c->stream));
     CG_HIP(c, hipMemsetAsync(dv, 0, (size_t)n * 8, c->stream));
-    CG_HIP(c, hipMemcpyAsync(s.desc_cur, &d, sizeof(GDesc), hipMemcpyHostToDevice, c->stream));
+    CG_HIP(c, hipMemcpyAsync(s.desc_win, &d, sizeof(GDesc), hipMemcpyHostToDevice, c->stream));
     CG_HIP(c, hipMemcpyAsync(s.pyr[0], w0, (size_t)s.n_pyr[0] * 16, hipMemcpyHostToDevice, c->stream));
     if (s.n_pyr[1])
         CG_HIP(c, hipMemcpyAsync(s.pyr[1], w1, (size_t)s.n_pyr[1] * 16, hipMemcpyHostToDevice, c->stream));

@@ -312,10 +312,9 @@ struct Ctx {
     uint32_t desc_ticks = 0;
     hipEvent_t ev_desc = nullptr;
     bool desc_pending = false;
-    // tick kernels read desc_cur, which k_desc_advance fills from the ring at
-    // desc_base (entry *tick_ctr) at the start of each tick
-    GDesc* desc_cur = nullptr;
-    uint32_t* tick_ctr = nullptr;
+    // tick q of a launch of n ticks reads window slot q (desc_win + q G),
+    // copied from the ring before the launch
+    GDesc* desc_win = nullptr;
 
     // staging of cg_drag_points / cg_get_positions_at (grown on demand)
     uint32_t* io_idx = nullptr;
