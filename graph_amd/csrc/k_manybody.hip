@@ -545,6 +545,7 @@ __global__ void __launch_bounds__(256) k_scan_rows(uint32_t* __restrict__ hist, 
 // lanes' next candidates (each lane pops its candidate when the round's
 // minimum has its ticket).  A run across waves leaves a partial record at
 // each wave's end it crosses, and k_cells_join merges them.
+
 // Position as an integer multiple of 2^-k (exact for |p| >= 2^(23-k)).
 __device__ inline long long fixed(float v, float scale) {
     return __double2ll_rn((double)v * (double)scale);
