@@ -32,6 +32,9 @@ HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
     # no packed-fp32 instructions in any kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no SLP vectors: with packed fp32 off they are split again, and in the
+    # force kernel their splats of the query point spill past its 64 VGPRs
+    "-fno-slp-vectorize",
     "-Wall",
     "-Wno-unused-function",
 ]

@@ -1291,9 +1291,11 @@ __device__ __forceinline__ float2 near_field(const float4* fin, int x0, int y0, 
                                     const float2* __restrict__ slot_pos) {
     const int gf = (int)d.gf;
     float wx = 0.f, wy = 0.f;
+#pragma unroll 1
     for (int dj = -1; dj <= 1; ++dj) {
         const int cy = qy + dj;
         if (cy < 0 || cy >= gf) continue;
+#pragma unroll 1
         for (int di = -1; di <= 1; ++di) {
             const int cx = qx + di;
             if (cx < 0 || cx >= gf) continue;
@@ -1304,54 +1306,108 @@ __device__ __forceinline__ float2 near_field(const float4* fin, int x0, int y0, 
             const size_t sb = ((size_t)d.f0 + cy * gf + cx) * SLOTS;
             const int4* __restrict__ si = reinterpret_cast<const int4*>(slot_idx + sb);
             const float4* __restrict__ sp = reinterpret_cast<const float4*>(slot_pos + sb);
-            const int4 ia = si[0], ib = si[1];
-            const float4 q01 = sp[0], q23 = sp[1], q45 = sp[2], q67 = sp[3];
-            const int js[SLOTS] = {ia.x, ia.y, ia.z, ia.w, ib.x, ib.y, ib.z, ib.w};
-            const float qs[2 * SLOTS] = {q01.x, q01.y, q01.z, q01.w, q23.x, q23.y, q23.z, q23.w,
-                                         q45.x, q45.y, q45.z, q45.w, q67.x, q67.y, q67.z, q67.w};
-            float sxp = 0.f, syp = 0.f, sampled = 0.f;
-            float dxs[SLOTS], dys[SLOTS], ls[SLOTS];
-            bool uses[SLOTS];
-            bool open = true, small = false;
             // coincident points share a cell: only the point's own cell can hold one
             const bool own = di == 0 && dj == 0;  // uniform over the wave
+            const int4 ib = si[1];
+            if (!own && __all(ib.w >= 0)) {
+                // A neighbour cell with all 8 slots filled (8 distinct tickets:
+                // a cell of 8 or more points may hold fewer) on every active
+                // lane: every slot is sampled, none is the point itself (it lies
+                // in its own cell) and none coincides with it (a different cell
+                // holds a different position), so no term needs a mask.  1/l for
+                // every term; if some lane of the wave has a pair below unit
+                // distance, the cell is summed again with the clamped falloff,
+                // which is the same value for every pair at or above it, so a
+                // lane's bits do not depend on the other lanes of its wave.
+                const float4 q[4] = {sp[0], sp[1], sp[2], sp[3]};
+                float sxp = 0.f, syp = 0.f;
+                bool small = false;
 #pragma unroll
-            for (int s = 0; s < SLOTS; ++s) {
+                for (int s = 0; s < SLOTS / 2; ++s) {
+                    const float ax = p.x - q[s].x, ay = p.y - q[s].y, bx = p.x - q[s].z, by = p.y - q[s].w;
+                    const float la = ax * ax + ay * ay, lb = bx * bx + by * by;
+                    small = small || la < 1.f || lb < 1.f;
+                    const float ia = __builtin_amdgcn_rsqf(la), ib2 = __builtin_amdgcn_rsqf(lb);
+                    const float wa = ia * ia, wb = ib2 * ib2;
+                    sxp += wa * ax;
+                    syp += wa * ay;
+                    sxp += wb * bx;
+                    syp += wb * by;
+                }
+                if (__ballot(small) != 0ull) {
+                    sxp = syp = 0.f;
+#pragma unroll
+                    for (int s = 0; s < SLOTS / 2; ++s) {
+                        const float ax = p.x - q[s].x, ay = p.y - q[s].y, bx = p.x - q[s].z, by = p.y - q[s].w;
+                        const float wa = falloff_coef(ax * ax + ay * ay), wb = falloff_coef(bx * bx + by * by);
+                        sxp += wa * ax;
+                        syp += wa * ay;
+                        sxp += wb * bx;
+                        syp += wb * by;
+                    }
+                }
+                const float hw = others * __builtin_amdgcn_rcpf((float)SLOTS);  // as the masked path's
+                wx += hw * sxp;
+                wy += hw * syp;
+                continue;
+            }
+            const int4 ia = si[0];
+            const float4 q[4] = {sp[0], sp[1], sp[2], sp[3]};
+            const int js[SLOTS] = {ia.x, ia.y, ia.z, ia.w, ib.x, ib.y, ib.z, ib.w};
+            // One slot's pair: whether it is sampled and not the point itself,
+            // and its offset (the point's jitter for a coincident point,
+            // force-nearfield.frag:131-136: only the own cell can hold one).
+            auto pair = [&](int s, bool& use, float& dx, float& dy, float& l) {
                 const int j = js[s];
-                open = open && j >= 0;
-                const bool use = open && (uint32_t)j != i;
-                sampled += use ? 1.f : 0.f;
-                float dx = p.x - qs[2 * s], dy = p.y - qs[2 * s + 1];
-                float l = dx * dx + dy * dy;
+                use = (uint32_t)j != i;
+                dx = p.x - q[s >> 1].x;
+                dy = p.y - q[s >> 1].y;
+                if (s & 1) {
+                    dx = p.x - q[s >> 1].z;
+                    dy = p.y - q[s >> 1].w;
+                }
+                l = dx * dx + dy * dy;
                 if (own) {
-                    // coincident points push along the point's jitter (force-nearfield.frag:131-136)
                     const bool same = !(l > 0.f);
                     dx = same ? r.x : dx;
                     dy = same ? r.y : dy;
                     l = same ? r.x * r.x + r.y * r.y : l;
                 }
-                dxs[s] = dx;
-                dys[s] = dy;
-                ls[s] = l;
-                uses[s] = use && l > 0.f;
-                small = small || (uses[s] && l < 1.f);
+            };
+            // filled slots lead: those up to the first empty one are sampled
+            float sxp = 0.f, syp = 0.f, sampled = 0.f;
+            bool open = true, small = false;
+#pragma unroll
+            for (int s = 0; s < SLOTS; ++s) {
+                bool use;
+                float dx, dy, l;
+                pair(s, use, dx, dy, l);
+                open = open && js[s] >= 0;
+                use = use && open;
+                sampled += use ? 1.f : 0.f;
+                use = use && l > 0.f;
+                small = small || (use && l < 1.f);
+                const float inv = __builtin_amdgcn_rsqf(l);
+                const float wgt = use ? inv * inv : 0.f;
+                sxp += wgt * dx;
+                syp += wgt * dy;
             }
-            // below unit distance the falloff is l^-3/4 (a square root); when no
-            // lane of the wave has such a pair, 1/l is inv * inv, the same value
+            // below unit distance the falloff is l^-3/4 (a square root): if a
+            // lane of the wave has such a pair, the cell is summed again with
+            // it (1/l = inv * inv above it, the same value)
             if (__ballot(small) != 0ull) {
+                sxp = syp = 0.f;
+                open = true;
 #pragma unroll
                 for (int s = 0; s < SLOTS; ++s) {
-                    const float wgt = uses[s] ? falloff_coef(ls[s]) : 0.f;
-                    sxp += wgt * dxs[s];
-                    syp += wgt * dys[s];
-                }
-            } else {
-#pragma unroll
-                for (int s = 0; s < SLOTS; ++s) {
-                    const float inv = __builtin_amdgcn_rsqf(ls[s]);
-                    const float wgt = uses[s] ? inv * inv : 0.f;
-                    sxp += wgt * dxs[s];
-                    syp += wgt * dys[s];
+                    bool use;
+                    float dx, dy, l;
+                    pair(s, use, dx, dy, l);
+                    open = open && js[s] >= 0;
+                    use = use && open && l > 0.f;
+                    const float wgt = use ? falloff_coef(l) : 0.f;
+                    sxp += wgt * dx;
+                    syp += wgt * dy;
                 }
             }
             if (sampled > 0.f) {
