@@ -667,6 +667,7 @@ int pack_body(Ctx* c, const PackSnap& old) {
     rc = upload(c, c->blk_graph, hblk); if (rc) return rc;
 
     // positions: host input for newly set graphs, device copy for the others
+    std::vector<uint4> psegs;  // resident graphs: one launch writes both buffers
     for (uint32_t s = 0; s < G; ++s) {
         Graph& g = c->graphs[c->order[s]];
         if (g.n == 0) {
@@ -675,15 +676,16 @@ int pack_body(Ctx* c, const PackSnap& old) {
             continue;
         }
         const size_t bytes = (size_t)g.n * 8;
-        for (int b = 0; b < 2; ++b) {
-            if (g.xy_pending || !old_pos[0])
+        if (g.xy_pending || !old_pos[0]) {
+            for (int b = 0; b < 2; ++b)
                 CG_HIP(c, hipMemcpyAsync(c->pos[b] + g.p0, g.xy.data(), bytes, hipMemcpyHostToDevice, c->stream));
-            else
-                CG_HIP(c, hipMemcpyAsync(c->pos[b] + g.p0, old_pos[g.cur] + old_p0[s], bytes,
-                                         hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            psegs.push_back(make_uint4(old_p0[s] | (g.cur ? 0x80000000u : 0u), g.p0, g.n, 0u));
         }
         g.cur = 0;
     }
+    rc = seg_copy<float2>(c, old_pos[0], old_pos[1], c->pos[0], c->pos[1], psegs);
+    if (rc) return rc;
 
     // centroid chunks and pyramid work lists
     {

@@ -363,6 +363,11 @@ void pyramid_work(uint32_t slot, uint32_t gf, uint32_t L, uint4* w0, uint32_t* n
 int link_build_graph(Ctx* c, Graph& g);                           // k_link.hip
 void link_free_graph(Ctx* c, Graph& g);
 int link_pack(Ctx* c, const LinkDir (&old)[2]);
+// Copies every segment {source offset (bit 31: srcB), destination offset,
+// count, -} of `segs` into dst0 (and dst1 unless null) in one launch on the
+// library stream; `segs` must stay alive until the stream has run it.
+template <typename T>
+int seg_copy(Ctx* c, const T* srcA, const T* srcB, T* dst0, T* dst1, std::vector<uint4>& segs);
 void link_free_dir(Ctx* c, LinkDir& L);
 int link_run(Ctx* c, int dir, float2* vout);
 int col_prepare_graph(Ctx* c, Graph& g);                          // k_collision.hip

@@ -724,17 +724,57 @@ __global__ void __launch_bounds__(B) k_link_chunk(int pass, const uint4* __restr
     }
 }
 
-__global__ void k_copy_f2(const float2* __restrict__ src, float2* __restrict__ dst, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
-}
-
-__global__ void k_copy_f4(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
+// Segmented copies: segment k = {source offset (bit 31: from the second
+// source), destination offset, count, first block}, blocks laid out segment
+// after segment; every element goes to both destinations (a null second one
+// is skipped).  One launch moves every graph's slice of a packed array.
+template <typename T>
+__global__ void k_seg_copy(const T* __restrict__ srcA, const T* __restrict__ srcB, T* __restrict__ dst0,
+                           T* __restrict__ dst1, const uint4* __restrict__ seg, uint32_t nseg) {
+    uint32_t lo = 0, hi = nseg;  // the last segment whose first block is <= blockIdx.x
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (seg[mid].w <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const uint4 sg = seg[lo];
+    const uint32_t i = (blockIdx.x - sg.w) * blockDim.x + threadIdx.x;
+    if (i >= sg.z) return;
+    const T v = ((sg.x >> 31) ? srcB : srcA)[(sg.x & 0x7fffffffu) + i];
+    dst0[sg.y + i] = v;
+    if (dst1) dst1[sg.y + i] = v;
 }
 
 }  // namespace
+
+template <typename T>
+int seg_copy(Ctx* c, const T* srcA, const T* srcB, T* dst0, T* dst1, std::vector<uint4>& segs) {
+    uint32_t blk = 0;
+    std::vector<uint4> kept;
+    kept.reserve(segs.size());
+    for (uint4 sg : segs) {
+        if (sg.z == 0) continue;
+        sg.w = blk;
+        blk += blocks_for(sg.z, 256);
+        kept.push_back(sg);
+    }
+    segs.swap(kept);
+    if (segs.empty()) return CG_OK;
+    int rc = CG_OK;
+    uint4* d = (uint4*)dalloc(c, segs.size() * 16, &rc);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(d, segs.data(), segs.size() * 16, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_seg_copy<T>, dim3(blk), dim3(256), 0, c->stream, srcA, srcB, dst0, dst1, d,
+                           (uint32_t)segs.size());
+        e = hipGetLastError();
+    }
+    dfree(c, d);  // reused only once the stream is idle
+    if (e != hipSuccess) return hip_fail(c, e, "segmented copy");
+    return CG_OK;
+}
+template int seg_copy<float2>(Ctx*, const float2*, const float2*, float2*, float2*, std::vector<uint4>&);
+template int seg_copy<float4>(Ctx*, const float4*, const float4*, float4*, float4*, std::vector<uint4>&);
 
 void link_free_graph(Ctx* c, Graph& g) {
     for (int d = 0; d < 2; ++d) {
@@ -941,6 +981,16 @@ int link_pack(Ctx* c, const LinkDir (&old)[2]) {
         CG_HIP(c, hipStreamSynchronize(c->stream));
         L.n_lane = h[0];
         const uint32_t n64 = h[1] - h[0];
+        // the chunked tiers' schedule, read back before the entry assembly is
+        // queued, so the host builds their work units while it runs
+        const uint32_t nch = ns - L.n_lane;
+        std::vector<uint32_t> hs(nch + 1), hsched(nch ? nch : 1);
+        CG_HIP(c, hipMemcpyAsync(hs.data(), L.soff + L.n_lane, (size_t)(nch + 1) * 4,
+                                 hipMemcpyDeviceToHost, c->stream));
+        if (nch)
+            CG_HIP(c, hipMemcpyAsync(hsched.data(), L.sched + L.n_lane, (size_t)nch * 4,
+                                     hipMemcpyDeviceToHost, c->stream));
+        CG_HIP(c, hipStreamSynchronize(c->stream));
         if (G > 1 && L.n_lane) {
             L.sgraph = (uint32_t*)dalloc(c, (size_t)L.n_lane * 4, &rc);
             if (rc) { cleanup(); return rc; }
@@ -955,15 +1005,7 @@ int link_pack(Ctx* c, const LinkDir (&old)[2]) {
         if (etot)
             hipLaunchKernelGGL(k_assemble, dim3(blocks_for(etot, 256)), dim3(256), 0, c->stream, ebase, G,
                                etot, tab, c->desc, slot_of, L.soff, c->flags, L.ent);
-        // chunked tiers: work units, warm-start offsets
-        const uint32_t nch = ns - L.n_lane;
-        std::vector<uint32_t> hs(nch + 1), hsched(nch ? nch : 1);
-        CG_HIP(c, hipMemcpyAsync(hs.data(), L.soff + L.n_lane, (size_t)(nch + 1) * 4,
-                                 hipMemcpyDeviceToHost, c->stream));
-        if (nch)
-            CG_HIP(c, hipMemcpyAsync(hsched.data(), L.sched + L.n_lane, (size_t)nch * 4,
-                                     hipMemcpyDeviceToHost, c->stream));
-        CG_HIP(c, hipStreamSynchronize(c->stream));
+        // chunked tiers: work units, warm-start offsets.
         // A 64-lane point is one unit; a 256-lane point of d entries is
         // ceil(d / seg) near-equal segments of at most seg = 256 x hub_entries,
         // one unit each, in chain order.  Every unit holds at most 4 (64 lanes)
@@ -1044,17 +1086,19 @@ int link_pack(Ctx* c, const LinkDir (&old)[2]) {
         CG_HIP(c, hipMemsetAsync(L.warm, 0, (wpos ? wpos : 1) * 8, c->stream));
         CG_HIP(c, hipMemsetAsync(L.wjac, 0, (wpos ? wpos : 1) * 16, c->stream));
         CG_HIP(c, hipMemsetAsync(L.seg_out, 0, nu * 64, c->stream));
+        // warm starts of graphs whose regions kept their size, in one launch per array
+        std::vector<uint4> wsegs;
         for (uint32_t s = 0; s < G; ++s) {
             Graph& g = c->graphs[c->order[s]];
-            if (old[d].warm && g.wn[d] && g.wn[d] == region[s].second) {
-                hipLaunchKernelGGL(k_copy_f2, dim3(blocks_for(g.wn[d], 256)), dim3(256), 0, c->stream,
-                                   old[d].warm + g.w0[d], L.warm + region[s].first, (size_t)g.wn[d]);
-                hipLaunchKernelGGL(k_copy_f4, dim3(blocks_for(g.wn[d], 256)), dim3(256), 0, c->stream,
-                                   old[d].wjac + g.w0[d], L.wjac + region[s].first, (size_t)g.wn[d]);
-            }
+            if (old[d].warm && g.wn[d] && g.wn[d] == region[s].second)
+                wsegs.push_back(make_uint4(g.w0[d], (uint32_t)region[s].first, g.wn[d], 0u));
             g.w0[d] = (uint32_t)region[s].first;
             g.wn[d] = (uint32_t)region[s].second;
         }
+        std::vector<uint4> jsegs = wsegs;
+        rc = seg_copy<float2>(c, old[d].warm, nullptr, L.warm, nullptr, wsegs);
+        if (!rc) rc = seg_copy<float4>(c, old[d].wjac, nullptr, L.wjac, nullptr, jsegs);
+        if (rc) { cleanup(); return rc; }
         CG_HIP(c, hipGetLastError());
         // keep the staging vectors alive until the copies above have run
         CG_HIP(c, hipStreamSynchronize(c->stream));
