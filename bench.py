@@ -467,7 +467,8 @@ def bench_serving(window_s: float, budget_gib: float, segment: int, stride: int,
                     f"({serving.alpha_ticks(5000.0)} ticks)",
         "jobs": len(jobs), "points": pts, "links": lnk, "hbm_budget_gib": budget_gib, "segment_ticks": segment,
         "batched": b,
-        "batched_breakdown_s": {k: st[k] for k in ("ingest_s", "pack_s", "tick_s", "retire_s", "idle_s", "wall_s")},
+        "batched_breakdown_s": {k: st[k] for k in ("ingest_s", "overlap_s", "pack_s", "tick_s", "retire_s",
+                                                   "idle_s", "wall_s")},
         "repacks": st["repacks"], "max_active": st["max_active"], "segments": st["segments"],
         "point_ticks_per_s": st["point_ticks"] / st["wall_s"],
         "sequential": {"sample_jobs": len(sample), "stride": stride, "mean_service_s": service,

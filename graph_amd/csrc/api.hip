@@ -153,10 +153,31 @@ void dfree(Ctx* c, void* p) {
     (void)hipFree(p);
 }
 
+void dfree_idle(Ctx* c, void* p) {
+    if (!p || c->arena) return;
+    for (size_t k = 0; k < c->allocs.size(); ++k) {
+        if (c->allocs[k].first == p) {
+            const size_t sz = c->allocs[k].second;
+            c->allocs[k] = c->allocs.back();
+            c->allocs.pop_back();
+            if (c->cached + sz <= CACHE_MAX) {
+                c->cache.emplace(sz, p);
+                c->cached += sz;
+            } else {
+                c->held -= sz;
+                (void)hipFree(p);
+            }
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
+
 void cache_settle(Ctx* c) {
     if (c->pending.empty()) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ingest) (void)hipStreamSynchronize(c->ingest);
     for (auto& b : c->pending) c->cache.emplace(b.second, b.first);
     c->pending.clear();
 }
@@ -528,6 +549,7 @@ void pack_rollback(Ctx* c, const PackSnap& s) {
 int pack_body(Ctx* c, const PackSnap& old) {
     CG_HIP(c, hipStreamSynchronize(c->stream));
     if (c->side) CG_HIP(c, hipStreamSynchronize(c->side));
+    if (c->ingest) CG_HIP(c, hipStreamSynchronize(c->ingest));
     c->desc_pending = false;
     int rc = CG_OK;
     c->order.clear();
@@ -1457,6 +1479,7 @@ int cg_create(cg_ctx** out, const cg_create_params* params) {
     hipError_t e = hipSetDevice(c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->ingest, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_desc, hipEventDisableTiming);
@@ -1478,12 +1501,14 @@ int cg_destroy(cg_ctx* x) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ingest) (void)hipStreamSynchronize(c->ingest);
     drop_graphs(c);
     for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_desc) (void)hipEventDestroy(c->ev_desc);
     if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->ingest) (void)hipStreamDestroy(c->ingest);
     for (auto& a : c->allocs) (void)hipFree(a.first);
     c->allocs.clear();
     for (auto& a : c->pending) (void)hipFree(a.first);

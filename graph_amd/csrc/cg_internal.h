@@ -199,6 +199,7 @@ struct Ctx {
     size_t arena_cap = 0, arena_used = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;     // second stream (default priority) for the hub link tier
+    hipStream_t ingest = nullptr;   // per-graph link builds (default priority): they run beside ticks
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint64_t budget = 0;
     uint64_t held = 0;              // live, pending and cached bytes (cg_device_bytes)
@@ -334,6 +335,7 @@ int fail(Ctx* c, int code, const std::string& msg);
 int hip_fail(Ctx* c, hipError_t e, const char* what);
 void* dalloc(Ctx* c, size_t bytes, int* rc);
 void dfree(Ctx* c, void* p);
+void dfree_idle(Ctx* c, void* p);  // a block no queued work uses: reusable at once
 void cache_settle(Ctx* c);     // synchronises the streams; freed blocks become reusable
 
 #define CG_HIP(c, expr)                                  \

@@ -797,6 +797,9 @@ int link_build_graph(Ctx* c, Graph& gr) {
     if (!gr.links_set || gr.n == 0) return CG_OK;
     const uint32_t n = gr.n, m = gr.m;
     int rc = CG_OK;
+    // its own stream: reads only this graph's inputs and writes only its own
+    // buffers, so it may run while ticks of the packed set are in flight
+    hipStream_t st = c->ingest ? c->ingest : c->stream;
     GraphLinks& lk = gr.lk;
     lk.degree = (uint32_t*)dalloc(c, (size_t)n * 4, &rc);
     if (rc) return rc;
@@ -807,6 +810,8 @@ int link_build_graph(Ctx* c, Graph& gr) {
         return p;
     };
     auto cleanup = [&]() { for (void* p : tmp) dfree(c, p); };
+    // after the stream has run: only this build used them
+    auto release_idle = [&]() { for (void* p : tmp) dfree_idle(c, p); };
     const size_t mm = m ? m : 1;
     float2* d_links = (float2*)talloc(mm * 8);
     float* d_str = gr.strength.empty() ? nullptr : (float*)talloc(mm * 4);
@@ -824,30 +829,30 @@ int link_build_graph(Ctx* c, Graph& gr) {
     uint32_t* lflag = (uint32_t*)talloc((mm + 1) * 4);
     uint32_t* lrank = (uint32_t*)talloc((mm + 1) * 4);  // scan-size probe only
     if (rc) { cleanup(); return rc; }
-    CG_HIP(c, hipMemcpyAsync(d_flags, gr.flags.data(), n, hipMemcpyHostToDevice, c->stream));
-    CG_HIP(c, hipMemsetAsync(outdeg, 0, ((size_t)n + 1) * 4, c->stream));
-    CG_HIP(c, hipMemsetAsync(indeg, 0, ((size_t)n + 1) * 4, c->stream));
-    CG_HIP(c, hipMemsetAsync(lk.degree, 0, (size_t)n * 4, c->stream));
+    CG_HIP(c, hipMemcpyAsync(d_flags, gr.flags.data(), n, hipMemcpyHostToDevice, st));
+    CG_HIP(c, hipMemsetAsync(outdeg, 0, ((size_t)n + 1) * 4, st));
+    CG_HIP(c, hipMemsetAsync(indeg, 0, ((size_t)n + 1) * 4, st));
+    CG_HIP(c, hipMemsetAsync(lk.degree, 0, (size_t)n * 4, st));
     if (m) {
-        CG_HIP(c, hipMemcpyAsync(d_links, gr.links.data(), (size_t)m * 8, hipMemcpyHostToDevice, c->stream));
+        CG_HIP(c, hipMemcpyAsync(d_links, gr.links.data(), (size_t)m * 8, hipMemcpyHostToDevice, st));
         if (d_str)
-            CG_HIP(c, hipMemcpyAsync(d_str, gr.strength.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
-        CG_HIP(c, hipMemcpyAsync(d_rin, gr.rin.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
-        CG_HIP(c, hipMemcpyAsync(d_rout, gr.rout.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_validate, dim3(blocks_for(m, 256)), dim3(256), 0, c->stream, d_links, m,
+            CG_HIP(c, hipMemcpyAsync(d_str, gr.strength.data(), (size_t)m * 4, hipMemcpyHostToDevice, st));
+        CG_HIP(c, hipMemcpyAsync(d_rin, gr.rin.data(), (size_t)m * 4, hipMemcpyHostToDevice, st));
+        CG_HIP(c, hipMemcpyAsync(d_rout, gr.rout.data(), (size_t)m * 4, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_validate, dim3(blocks_for(m, 256)), dim3(256), 0, st, d_links, m,
                            n, src, tgt, outdeg, indeg);
-        hipLaunchKernelGGL(k_degree, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, outdeg, indeg,
+        hipLaunchKernelGGL(k_degree, dim3(blocks_for(n, 256)), dim3(256), 0, st, outdeg, indeg,
                            n, lk.degree);
     }
     int nbits = 1;
     while (nbits < 32 && (1ull << nbits) <= (unsigned long long)n) ++nbits;  // n itself must fit
     size_t sort_bytes = 0, scan_bytes = 0, scan2_bytes = 0;
     CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, v0, v1, (int)mm, 0,
-                                                 nbits, c->stream));
+                                                 nbits, st));
     CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, outdeg, outdeg, (int)n + 1,
-                                               c->stream));
+                                               st));
     CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan2_bytes, lflag, lrank, (int)mm + 1,
-                                               c->stream));
+                                               st));
     size_t tbytes = std::max(sort_bytes, std::max(scan_bytes, scan2_bytes));
     void* d_tmp = talloc(tbytes);
     if (rc) { cleanup(); return rc; }
@@ -859,35 +864,34 @@ int link_build_graph(Ctx* c, Graph& gr) {
         lk.live[d] = (uint32_t*)dalloc(c, (size_t)n * 4, &rc);
         if (rc) { cleanup(); return rc; }
         size_t sb = tbytes;
-        CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, cnt, lk.off_pt[d], (int)n + 1, c->stream));
+        CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, cnt, lk.off_pt[d], (int)n + 1, st));
         uint32_t E = 0;
-        CG_HIP(c, hipMemcpyAsync(&E, lk.off_pt[d] + n, 4, hipMemcpyDeviceToHost, c->stream));
-        CG_HIP(c, hipStreamSynchronize(c->stream));
+        CG_HIP(c, hipMemcpyAsync(&E, lk.off_pt[d] + n, 4, hipMemcpyDeviceToHost, st));
+        CG_HIP(c, hipStreamSynchronize(st));
         lk.E[d] = E;
         lk.ent_pt[d] = (uint4*)dalloc(c, (size_t)(E ? E : 1) * 16, &rc);
         lk.lrank[d] = (uint32_t*)dalloc(c, ((size_t)E + 1) * 4, &rc);
         if (rc) { cleanup(); return rc; }
-        CG_HIP(c, hipMemsetAsync(lflag, 0, (mm + 1) * 4, c->stream));
+        CG_HIP(c, hipMemsetAsync(lflag, 0, (mm + 1) * 4, st));
         if (m) {
-            hipLaunchKernelGGL(k_iota_keys, dim3(blocks_for(m, 256)), dim3(256), 0, c->stream, owner,
+            hipLaunchKernelGGL(k_iota_keys, dim3(blocks_for(m, 256)), dim3(256), 0, st, owner,
                                m, k0, v0);
             sb = tbytes;
             CG_HIP(c, hipcub::DeviceRadixSort::SortPairs(d_tmp, sb, k0, k1, v0, v1, (int)m, 0, nbits,
-                                                         c->stream));
+                                                         st));
             if (E)
-                hipLaunchKernelGGL(k_entries, dim3(blocks_for(E, 256)), dim3(256), 0, c->stream, k1,
+                hipLaunchKernelGGL(k_entries, dim3(blocks_for(E, 256)), dim3(256), 0, st, k1,
                                    v1, E, other, lk.degree, d_str, d == 0 ? d_rin : d_rout, d_flags,
                                    lk.ent_pt[d], lflag);
         }
         sb = tbytes;
-        CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, lflag, lk.lrank[d], (int)E + 1, c->stream));
-        hipLaunchKernelGGL(k_live_count, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream,
+        CG_HIP(c, hipcub::DeviceScan::ExclusiveSum(d_tmp, sb, lflag, lk.lrank[d], (int)E + 1, st));
+        hipLaunchKernelGGL(k_live_count, dim3(blocks_for(n, 256)), dim3(256), 0, st,
                            lk.off_pt[d], lk.lrank[d], n, lk.live[d]);
         CG_HIP(c, hipGetLastError());
     }
-    CG_HIP(c, hipStreamSynchronize(c->stream));
-    cleanup();
-    cache_settle(c);  // the temporaries are reusable by the next build
+    CG_HIP(c, hipStreamSynchronize(st));
+    release_idle();  // the temporaries are reusable by the next build
     if (rc) return rc;
     gr.links_ready = true;
     return CG_OK;

@@ -5,8 +5,10 @@ point where the reference's frame loop ends the simulation
 (src/index.ts:2164-2169, Store/index.ts:8).  ``BatchServer`` keeps every admitted
 job resident in one context's graph set (positions, velocities and alpha stay
 on the device between ticks), advances all of them together, one launch per
-force pass per tick, admits arrivals at segment boundaries while the HBM
-budget allows, and retires each job on the tick its alpha crosses the cutoff.
+force pass per tick, ingests arrivals while the HBM budget allows (also while
+a segment runs: their link builds go on the library's ingest stream), adds
+them to the set at the next segment boundary, and retires each job on the tick
+its alpha crosses the cutoff.
 ``serve_sequential`` is the plain path: one job at a time through the
 single-graph entry points, in arrival order.
 
@@ -149,29 +151,23 @@ class BatchServer:
 
     def run(self, jobs: list[Job], log=None, on_segment=None) -> tuple[list[Result], dict]:
         """Serves `jobs` (arrival times relative to the call).  on_segment(ctx,
-        {graph id: job}) runs after every segment, before retirements."""
+        {graph id: job}) runs after every segment with the jobs it advanced,
+        before retirements."""
         ctx = self.ctx
         pending = sorted(jobs, key=lambda j: j.arrival)
         active: dict[int, tuple[Job, int, int, float, int]] = {}  # gid -> job, left, total, start, bytes
-        held = 0
         results: list[Result] = []
         stats = dict(segments=0, ticks=0, max_active=0, point_ticks=0, rejected=[], cancelled=[],
-                     ingest_s=0.0, pack_s=0.0, tick_s=0.0, retire_s=0.0, idle_s=0.0, repacks=0)
+                     ingest_s=0.0, pack_s=0.0, tick_s=0.0, retire_s=0.0, idle_s=0.0, repacks=0,
+                     overlap_s=0.0)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
-        changed = True
-        while pending or active:
-            t = now()
-            if self._cancel:
-                for gid in [g for g, v in active.items() if v[0].jid in self._cancel]:
-                    ctx.graph_remove(gid)
-                    held -= active[gid][4]
-                    stats["cancelled"].append(active.pop(gid)[0].jid)
-                    changed = True
-                for j in [j for j in pending if j.jid in self._cancel]:
-                    pending.remove(j)
-                    stats["cancelled"].append(j.jid)
-            # admit every arrived job that fits, in arrival order (no overtaking)
+        held_box = [0]  # bytes of the admitted jobs' footprints
+
+        def admit(t: float) -> bool:
+            """Ingests every job arrived by `t` that fits, in arrival order (no
+            overtaking); it joins the set at the next segment boundary."""
+            got = False
             while pending and pending[0].arrival <= t:
                 job = pending[0]
                 need = footprint_bytes(job.n, job.m, collides(job.params))
@@ -182,7 +178,7 @@ class BatchServer:
                     if log:
                         log(f"job {job.jid} n={job.n} refused: footprint {need} B exceeds the budget")
                     continue
-                if held + need > self.budget:
+                if held_box[0] + need > self.budget:
                     break
                 if self.max_points and active and sum(v[0].n for v in active.values()) + job.n > self.max_points:
                     break
@@ -193,8 +189,23 @@ class BatchServer:
                 stats["ingest_s"] += time.perf_counter() - ti
                 total = job.ticks if job.ticks is not None else alpha_ticks(job.params.decay)
                 active[gid] = (job, total, total, t, need)
-                held += need
-                changed = True
+                held_box[0] += need
+                got = True
+            return got
+
+        changed = True
+        while pending or active:
+            t = now()
+            if self._cancel:
+                for gid in [g for g, v in active.items() if v[0].jid in self._cancel]:
+                    ctx.graph_remove(gid)
+                    held_box[0] -= active[gid][4]
+                    stats["cancelled"].append(active.pop(gid)[0].jid)
+                    changed = True
+                for j in [j for j in pending if j.jid in self._cancel]:
+                    pending.remove(j)
+                    stats["cancelled"].append(j.jid)
+            changed |= admit(t)
             if not active:
                 if not pending:
                     break
@@ -216,13 +227,18 @@ class BatchServer:
                 changed = False
             ti = time.perf_counter()
             ctx.step_graphs(ids, k)
+            # while the segment runs: ingest arrivals (their link builds run on
+            # the library's ingest stream beside the ticks)
+            tb = time.perf_counter()
+            changed |= admit(now())
+            stats["overlap_s"] += time.perf_counter() - tb
             ctx.sync()
             stats["tick_s"] += time.perf_counter() - ti
             stats["segments"] += 1
             stats["ticks"] += k
-            stats["point_ticks"] += k * sum(v[0].n for v in active.values())
+            stats["point_ticks"] += k * sum(active[g][0].n for g in ids)
             if on_segment:
-                on_segment(ctx, {gid: v[0] for gid, v in active.items()})
+                on_segment(ctx, {gid: active[gid][0] for gid in ids})
             ti = time.perf_counter()
             for gid in ids:
                 job, left, total, start, need = active[gid]
@@ -234,7 +250,7 @@ class BatchServer:
                 fin = now()
                 ctx.graph_remove(gid)
                 del active[gid]
-                held -= need
+                held_box[0] -= need
                 changed = True
                 results.append(Result(job.jid, job.arrival, start, fin, total, xy))
                 if log:
