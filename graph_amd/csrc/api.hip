@@ -125,7 +125,8 @@ void* dalloc(Ctx* c, size_t bytes, int* rc) {
         e = hipMalloc(&p, want);
     }
     if (e != hipSuccess) {
-        *rc = hip_fail(c, e, "hipMalloc");
+        *rc = hip_fail(c, e, ("hipMalloc of " + std::to_string(want) + " bytes (" + std::to_string(c->held) +
+                              " held)").c_str());
         return nullptr;
     }
     c->held += want;
@@ -398,6 +399,16 @@ void drop_graphs(Ctx* c) {
     for (auto& e : c->graphs_exec) (void)hipGraphExecDestroy(e.second);
     c->graphs_exec.clear();
 }
+
+// Whether the set's ticks replay captured HIP graphs.  Only below
+// CAPTURE_MAX_POINTS: above it a tick's kernels are long enough that launching
+// them directly costs nothing measurable (11.5M points: 62.9 against 62.4 ms per
+// 16 ticks), and the executable graphs of a large set, destroyed at each
+// repack, did not give back the freed packing: with captured ticks every repack
+// of a 20 GB serving set (links with split hubs and collision) lost ~7 GB of
+// device memory until the device ran out; with direct launches none.
+constexpr uint32_t CAPTURE_MAX_POINTS = 8u << 20;
+bool captures(const Ctx* c) { return c->capture && c->prof_mask == 0 && c->N < CAPTURE_MAX_POINTS; }
 
 // Frees the packed arrays except the position buffers and the link schedules.
 void free_packed(Ctx* c) {
@@ -1044,7 +1055,7 @@ int run_ticks(Ctx* c, const std::vector<uint32_t>& ids, uint32_t nticks, int64_t
         CG_HIP(c, hipEventRecord(c->ev_desc, c->stream));
         c->desc_pending = true;
         // stage timing brackets single launches, so it runs uncaptured
-        const bool cap = c->capture && c->prof_mask == 0;
+        const bool cap = captures(c);
         for (uint32_t tt = 0; tt < T;) {
             uint32_t r = 1;
             while (tt + r < T && plan[tt + r].any == plan[tt].any) ++r;
@@ -1601,7 +1612,7 @@ int cg_prepare(cg_ctx* x, const cg_sim_params* p) {
         int rc = pack(c);
         if (rc) return rc;
     }
-    if (!c->capture || c->prof_mask || !p->enable_simulation) return CG_OK;
+    if (!captures(c) || !p->enable_simulation) return CG_OK;
     // the plan cg_step runs for graph 0 alone (the other graphs are not stepped)
     const TickPlan tp{graph_passes(c->graphs[0])};
     if (tp.any == 0) return CG_OK;
