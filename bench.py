@@ -436,7 +436,8 @@ def bench_batched(count: int, ticks: int, sample: int, log) -> dict:
     }
 
 
-def bench_serving(window_s: float, budget_gib: float, segment: int, stride: int, log) -> dict:
+def bench_serving(window_s: float, budget_gib: float, segment: int, stride: int, log,
+                  max_points: int | None = None) -> dict:
     """Continuous batching: Poisson arrivals at 40/s over `window_s` seconds,
     each job laid out until alpha < 0.001, all of them through the batch
     server; the plain first-come first-served path runs every `stride`-th job
@@ -449,7 +450,8 @@ def bench_serving(window_s: float, budget_gib: float, segment: int, stride: int,
     pts, lnk = sum(j.n for j in jobs), sum(j.m for j in jobs)
     log(f"serving: {len(jobs)} jobs, {pts} points, {lnk} links")
     tuning = autotune.lookup(pts, 2.0 * lnk / pts)
-    srv = serving.BatchServer(budget=int(budget_gib * 2**30), segment=segment, tuning=tuning)
+    srv = serving.BatchServer(budget=int(budget_gib * 2**30), segment=segment, tuning=tuning,
+                              max_points=max_points)
     res, st = srv.run(jobs)
     srv.close()
     log(f"serving: batched done in {st['wall_s']:.1f}s")
@@ -466,6 +468,7 @@ def bench_serving(window_s: float, budget_gib: float, segment: int, stride: int,
                     f"Chung-Lu links 2-8x points, 10% with collision, until alpha < 0.001 "
                     f"({serving.alpha_ticks(5000.0)} ticks)",
         "jobs": len(jobs), "points": pts, "links": lnk, "hbm_budget_gib": budget_gib, "segment_ticks": segment,
+        "resident_point_cap": max_points,
         "batched": b,
         "batched_breakdown_s": {k: st[k] for k in ("ingest_s", "overlap_s", "pack_s", "tick_s", "retire_s",
                                                    "idle_s", "wall_s")},
@@ -528,6 +531,8 @@ def main() -> int:
     ap.add_argument("--serve-stride", type=int, default=5, help="sequential path on every k-th request")
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
     ap.add_argument("--serve-segment", type=int, default=100)
+    ap.add_argument("--serve-max-points", type=int, default=0,
+                    help="resident points cap of the batch server (0: the HBM budget alone)")
     ap.add_argument("--error-ticks", type=int, default=1000)
     ap.add_argument("--cfg2-ticks", type=int, default=500)
     ap.add_argument("--cfg2-every", type=int, default=50, help="ticks between force-error checkpoints")
@@ -680,7 +685,7 @@ def main() -> int:
         out["batched"] = bench_batched(args.batch_graphs, args.batch_ticks, args.batch_sample, log)
     if "serving" not in skip:
         out["serving"] = bench_serving(args.serve_seconds, args.serve_budget_gib, args.serve_segment,
-                                       args.serve_stride, log)
+                                       args.serve_stride, log, args.serve_max_points or None)
     if "cfg0" not in skip:
         out["cfg0"] = bench_cfg0(300, 20)
     if not args.no_cpu_baseline:
