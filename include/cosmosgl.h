@@ -281,8 +281,9 @@ uint64_t cg_device_bytes(const cg_ctx* ctx);
 #define CG_PROFILE_LINK_STATS (1u << 31) /* also count link sweeps (cg_debug_link_stats) */
 int cg_set_profiling(cg_ctx* ctx, uint32_t stage_mask);
 /* Tick sequences are captured once into HIP graphs and replayed (on by
- * default; off while stage timing is on).  enable = 0 launches every kernel
- * directly; results are identical either way. */
+ * default; off while stage timing is on, and for sets of 8M points or more,
+ * whose kernels are long enough to launch directly at the same speed).
+ * enable = 0 launches every kernel directly; results are identical either way. */
 int cg_set_capture(cg_ctx* ctx, int32_t enable);
 int cg_get_profile(cg_ctx* ctx, double* ms, uint32_t* launches, uint32_t nstages);
 
