@@ -134,7 +134,10 @@ void* dalloc(Ctx* c, size_t bytes, int* rc) {
     return p;
 }
 
-void dfree(Ctx* c, void* p) {
+namespace {
+// Returns a block to the cache: pending until the streams have been
+// synchronised, or reusable at once when no queued work uses it (`idle`).
+void release_block(Ctx* c, void* p, bool idle) {
     if (!p || c->arena) return;
     for (size_t k = 0; k < c->allocs.size(); ++k) {
         if (c->allocs[k].first == p) {
@@ -142,7 +145,8 @@ void dfree(Ctx* c, void* p) {
             c->allocs[k] = c->allocs.back();
             c->allocs.pop_back();
             if (c->cached + sz <= CACHE_MAX) {
-                c->pending.emplace_back(p, sz);
+                if (idle) c->cache.emplace(sz, p);
+                else c->pending.emplace_back(p, sz);
                 c->cached += sz;
             } else {
                 c->held -= sz;
@@ -153,26 +157,11 @@ void dfree(Ctx* c, void* p) {
     }
     (void)hipFree(p);
 }
+}  // namespace
 
-void dfree_idle(Ctx* c, void* p) {
-    if (!p || c->arena) return;
-    for (size_t k = 0; k < c->allocs.size(); ++k) {
-        if (c->allocs[k].first == p) {
-            const size_t sz = c->allocs[k].second;
-            c->allocs[k] = c->allocs.back();
-            c->allocs.pop_back();
-            if (c->cached + sz <= CACHE_MAX) {
-                c->cache.emplace(sz, p);
-                c->cached += sz;
-            } else {
-                c->held -= sz;
-                (void)hipFree(p);
-            }
-            return;
-        }
-    }
-    (void)hipFree(p);
-}
+void dfree(Ctx* c, void* p) { release_block(c, p, false); }
+
+void dfree_idle(Ctx* c, void* p) { release_block(c, p, true); }
 
 void cache_settle(Ctx* c) {
     if (c->pending.empty()) return;

@@ -162,11 +162,12 @@ class BatchServer:
                      overlap_s=0.0)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
-        held_box = [0]  # bytes of the admitted jobs' footprints
+        held = 0  # bytes of the admitted jobs' footprints
 
         def admit(t: float) -> bool:
             """Ingests every job arrived by `t` that fits, in arrival order (no
             overtaking); it joins the set at the next segment boundary."""
+            nonlocal held
             got = False
             while pending and pending[0].arrival <= t:
                 job = pending[0]
@@ -178,7 +179,7 @@ class BatchServer:
                     if log:
                         log(f"job {job.jid} n={job.n} refused: footprint {need} B exceeds the budget")
                     continue
-                if held_box[0] + need > self.budget:
+                if held + need > self.budget:
                     break
                 if self.max_points and active and sum(v[0].n for v in active.values()) + job.n > self.max_points:
                     break
@@ -189,7 +190,7 @@ class BatchServer:
                 stats["ingest_s"] += time.perf_counter() - ti
                 total = job.ticks if job.ticks is not None else alpha_ticks(job.params.decay)
                 active[gid] = (job, total, total, t, need)
-                held_box[0] += need
+                held += need
                 got = True
             return got
 
@@ -199,7 +200,7 @@ class BatchServer:
             if self._cancel:
                 for gid in [g for g, v in active.items() if v[0].jid in self._cancel]:
                     ctx.graph_remove(gid)
-                    held_box[0] -= active[gid][4]
+                    held -= active[gid][4]
                     stats["cancelled"].append(active.pop(gid)[0].jid)
                     changed = True
                 for j in [j for j in pending if j.jid in self._cancel]:
@@ -250,7 +251,7 @@ class BatchServer:
                 fin = now()
                 ctx.graph_remove(gid)
                 del active[gid]
-                held_box[0] -= need
+                held -= need
                 changed = True
                 results.append(Result(job.jid, job.arrival, start, fin, total, xy))
                 if log:
