@@ -539,9 +539,13 @@ def main() -> int:
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc: calibration kernels of known traffic (256 MiB), then 5 "
                          "direct-launch ticks of the 1M/5M workload; prints nothing else (= --probe cfg1)")
-    ap.add_argument("--probe", choices=["cfg1", "cfg2"], default=None,
+    ap.add_argument("--probe", choices=["cfg1", "cfg2", "serving"], default=None,
                     help="for rocprofv3: the calibration kernels, then --probe-ticks direct-launch ticks of "
-                         "one workload (cfg1: 1M/5M; cfg2: 4M SBM with collision); prints nothing else")
+                         "one workload (cfg1: 1M/5M; cfg2: 4M SBM with collision; serving: the 202 requests "
+                         "of the serving leg's 5 s window packed in one set, after --probe-warm ticks); "
+                         "prints nothing else")
+    ap.add_argument("--probe-warm", type=int, default=2000,
+                    help="serving probe: ticks run before the traced ones, so the layouts have collapsed")
     ap.add_argument("--probe-ticks", type=int, default=5)
     ap.add_argument("--gather-probe", action="store_true",
                     help="the random-gather ceiling of the link passes: 5M gathers of 8-byte positions from "
@@ -569,6 +573,24 @@ def main() -> int:
         dst = src * 2.0
         torch.cuda.synchronize()
         del dst, src
+        if args.probe == "serving":
+            from graph_amd import serving
+
+            jobs = workloads.cfg4_jobs(duration=5.0)
+            pts, lnk = sum(j.n for j in jobs), sum(j.m for j in jobs)
+            ctx = _lib.Context(device=0, hbm_budget=200 << 30)
+            ctx.set_tuning(autotune.lookup(pts, 2.0 * lnk / pts))
+            ids = []
+            for j in jobs:
+                g = ctx.graph_add()
+                serving._load(ctx, g, j)
+                ids.append(g)
+            ctx.step_graphs(ids, args.probe_warm)
+            ctx.set_capture(False)
+            ctx.step_graphs(ids, args.probe_ticks)
+            ctx.sync()
+            ctx.close()
+            return 0
         pw = workloads.cfg2() if args.probe == "cfg2" else workloads.cfg1()
         ctx = _lib.Context(device=0)
         autotune.apply(ctx, pw["n"], pw["links"].size / pw["n"])
