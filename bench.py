@@ -96,6 +96,28 @@ def stage_rooflines(breakdown: dict, info: dict) -> dict:
     return out
 
 
+def link_tiers(links: np.ndarray, n: int, tuning: dict) -> dict:
+    """Points and entries of each link tier per direction, from the link list and
+    the tier bounds (every point present): a point's entries are its valid links
+    as the owner of the direction (IN: the source, OUT: the target,
+    ForceLink/index.ts:49); the lane tier also holds the points without any.
+    Algorithmic bytes as for the whole pass: 24 per entry and 16 per point."""
+    lk = np.asarray(links, dtype=np.float32).reshape(-1, 2)
+    ok = ((lk >= 0) & (lk < n) & (lk == np.floor(lk))).all(axis=1)
+    lane, group = tuning["link_lane_max"], tuning["link_group_max"]
+    hub = 256 * tuning["link_hub_entries"]
+    out = {}
+    for name, col in (("in", 0), ("out", 1)):
+        deg = np.bincount(lk[ok, col].astype(np.int64), minlength=n)
+        tiers = {"lane": deg <= lane, "lanes64": (deg > lane) & (deg <= group),
+                 "lanes256": (deg > group) & (deg <= hub), "split": deg > hub}
+        out[name] = {}
+        for k, m in tiers.items():
+            pts, ent = int(m.sum()), int(deg[m].sum())
+            out[name][k] = {"points": pts, "entries": ent, "algorithmic_bytes": 24 * ent + 16 * pts}
+    return out
+
+
 def profiled_traffic(kernel: str) -> tuple[float | None, str | None]:
     """HBM bytes per pass of a stage (or launch of a kernel) from the committed
     rocprofv3 --pmc summary (read requests by size and WRITE_SIZE over
@@ -695,6 +717,7 @@ def main() -> int:
         "stage_ms_per_tick": {st: v[0] / 50.0 for st, v in breakdown.items()},
         "stages": stage_rooflines(breakdown, info),
         "roofline": roof,
+        "link_tiers": link_tiers(wl["links"], n, tuning),
     }
     ctx.close()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
